@@ -1,0 +1,153 @@
+/*
+ * mkhip.h -- C ABI of the MI355X (gfx950) multi-key mul+relin library.
+ *
+ * Drop-in boundary for the reference's evaluator calls (ZhangChuanxin1/MKF):
+ *   KKLSS MK-CKKS  mkckks.Evaluator.MulRelinNew   software/kklss/mkckks/evaluator.go:419
+ *   KKLSS MK-BFV   mkbfv.Evaluator.MulRelinNew    software/kklss/mkbfv/evaluator.go:85
+ *   MKF-CKKS       (*CKKSCase).Run                software/mkf/mkf_cpu/ckks.go:102
+ *   MKF-BFV        (*BFVCase).Run                 software/mkf/mkf_cpu/bfv.go:212
+ * A Go host binds these through cgo; a C++ host through
+ * mkf_amd/host/mkhip.hpp.  Plain pointers and sizes only.
+ *
+ * Data layout (the reference's fixture/dump layout: little-endian uint64,
+ * modulus-major then coefficient-major, mkckks/fpga_fixture_test.go:125-148):
+ *   poly        [limbs][N]
+ *   ciphertext  [1 + k][level+1][N]: component "0" first, then one component per
+ *               party in the order of the accompanying id array
+ *   switching key (KKLSS) [beta][L+K][N]: beta PolyQP, Q limbs then P limbs,
+ *               NTT + Montgomery form (x * 2^64 mod q)
+ *   switching key (MKF)   [beta][2][L+K][N]: output_poly[0..1] per digit
+ * All residues are canonical ([0, q)).  KKLSS ciphertexts are in the coefficient
+ * domain, MKF-CKKS ciphertexts in the NTT domain (as in the reference).
+ *
+ * Ownership: the caller owns every host buffer; the library owns device memory
+ * (keys stay resident in HBM until unregistered or the context is destroyed).
+ * Threading: a context is not re-entrant (the reference's evaluators share
+ * pools, mkrlwe/keyswitch.go:9-18); use one context per host thread.
+ */
+#ifndef MKHIP_H
+#define MKHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (0 = OK).  The Go binding turns them into panics for the
+ * KKLSS evaluators and into `error` values for MKF, as the reference does. ---- */
+#define MKHIP_OK 0
+#define MKHIP_ERR_ARG (-1)           /* null pointer / bad size / bad count          */
+#define MKHIP_ERR_PARAMS (-2)        /* moduli not NTT friendly, unsupported N        */
+#define MKHIP_ERR_GADGET (-3)        /* alpha = K/gamma is 0 (e.g. K=1, gamma=2)      */
+#define MKHIP_ERR_UNKNOWN_PARTY (-4) /* keys.go:190-198 "there is no ... key"         */
+#define MKHIP_ERR_LEVEL (-5)         /* keyswitch_hoisted.go:53-59 level mismatch     */
+#define MKHIP_ERR_NO_KEY (-6)        /* CRS u / MKF keys not uploaded                 */
+#define MKHIP_ERR_OOM (-7)           /* device memory exhausted (admission refused)   */
+#define MKHIP_ERR_DEVICE (-8)        /* HIP runtime error                             */
+#define MKHIP_ERR_SCALE (-9)         /* Rescale: minScale <= 0, scale 0, level 0      */
+#define MKHIP_ERR_SCHEME (-10)       /* call not valid for this context's scheme      */
+#define MKHIP_ERR_TOO_MANY (-11)     /* more parties than MKHIP_MAX_PARTIES           */
+
+#define MKHIP_MAX_PARTIES 32
+
+/* context flags */
+#define MKHIP_ROOT_RULE_SMALLEST2 1  /* primitive root search from 2 instead of 3  */
+
+typedef struct mkhip_ctx mkhip_ctx;
+
+const char *mkhip_status_string(int status);
+
+/*
+ * Parameter set and device tables.  Replaces the parameter/ring construction of
+ * mkrlwe.NewParameters (mkrlwe/params.go:16-61) and mkckks.NewParameters
+ * (mkckks/params.go:17-25): N = 2^logN, Q[0..L), P[0..K), gamma = 2 for the KKLSS
+ * evaluators (alpha = K/gamma) and 1 for MKF (Lattigo rlwe, alpha = K).
+ * logN in {13, 14}.  t and qmul are for BFV contexts (0 / NULL otherwise).
+ */
+int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, int L, const uint64_t *p, int K,
+                     int gamma, uint64_t t, const uint64_t *qmul, int flags);
+void mkhip_ctx_destroy(mkhip_ctx *ctx);
+/* alpha, beta (at the top level), and the word count of one switching key */
+int mkhip_ctx_info(const mkhip_ctx *ctx, int *alpha, int *beta, size_t *swk_words);
+
+/*
+ * Resident evaluation keys (HBM).  CRS index -1 is the KKLSS `u` (params.go:38).
+ * A party's relinearization key (b, d, v) replaces mkrlwe.RelinearizationKey
+ * (keys.go:133-136) and RelinearizationKeySet.AddRelinearizationKey (keys.go:277).
+ */
+int mkhip_set_crs(mkhip_ctx *ctx, int index, const uint64_t *swk);
+int mkhip_register_party(mkhip_ctx *ctx, int party, const uint64_t *b, const uint64_t *d, const uint64_t *v);
+int mkhip_unregister_party(mkhip_ctx *ctx, int party);
+
+/*
+ * KKLSS MK-CKKS mul + relin + rescale, host buffers
+ * (mkckks.Evaluator.MulRelinNew, evaluator.go:419-446 and :561-586).
+ *   op0: [1+k0][level0+1][N] coefficient domain, parties ids0[k0]; op1 likewise.
+ *   out: [1+ku][level_out+1][N]; ids_out receives the union of ids in ascending
+ *        order (ku entries, ku <= k0+k1); level_out = min(level0, level1) - nres
+ *        where nres comes from the reference's float64 scale loop with
+ *        scale = scale0 * scale1 and minScale = min_scale (evaluator.go:383-388).
+ * out must hold (1 + ku) * (min(level0,level1)+1) * N words.
+ */
+int mkhip_kklss_ckks_mulrelin(mkhip_ctx *ctx, const uint64_t *op0, int level0, double scale0, const int *ids0,
+                              int k0, const uint64_t *op1, int level1, double scale1, const int *ids1, int k1,
+                              double min_scale, uint64_t *out, int *ids_out, int *ku_out, int *level_out,
+                              double *scale_out);
+
+/* ---- device buffers (for callers that keep operands resident in HBM) ---- */
+int mkhip_dev_alloc(mkhip_ctx *ctx, size_t bytes, void **dptr);
+int mkhip_dev_free(mkhip_ctx *ctx, void *dptr);
+int mkhip_h2d(mkhip_ctx *ctx, void *dst, const void *src, size_t bytes);
+int mkhip_d2h(mkhip_ctx *ctx, void *dst, const void *src, size_t bytes);
+int mkhip_sync(mkhip_ctx *ctx);
+
+/*
+ * Batched KKLSS MK-CKKS mul + relin over device-resident operands.  A plan fixes
+ * the batch shape (ops, parties, levels) and owns the descriptors and the
+ * workspace; launching it replays one hipGraph of the whole batch.
+ */
+typedef struct {
+    const uint64_t *op0, *op1; /* device pointers, layout as above           */
+    uint64_t *out;             /* device pointer                              */
+    int ids0[MKHIP_MAX_PARTIES], ids1[MKHIP_MAX_PARTIES];
+    int k0, k1, level;         /* both operands at `level` (same limb count)  */
+    int nres;                  /* rescales to apply (from mkhip_rescale_count) */
+} mkhip_kklss_op;
+
+typedef struct mkhip_plan mkhip_plan;
+
+int mkhip_plan_kklss_ckks(mkhip_ctx *ctx, int nops, const mkhip_kklss_op *ops, int use_graph, mkhip_plan **out);
+int mkhip_plan_launch(mkhip_plan *plan);
+/* average device time of the plan's kernels over `iters` launches (HIP events) */
+int mkhip_plan_time(mkhip_plan *plan, int iters, double *ms_per_launch);
+void mkhip_plan_destroy(mkhip_plan *plan);
+size_t mkhip_plan_workspace_bytes(const mkhip_plan *plan);
+
+/* reference scale loop (mkckks/evaluator.go:383-388): number of rescales */
+int mkhip_rescale_count(const mkhip_ctx *ctx, int level, double scale, double min_scale, int *nres,
+                        double *new_scale);
+
+/*
+ * RNS primitives on host buffers -- the per-primitive operations the reference
+ * times in software/mkf/cmd/rnsbench/main.go:179-201.
+ *   mkhip_ntt_limbs        ring NTTLvl / InvNTTLvl of nlimbs rows; ring_idx[i] selects
+ *                          the modulus (0..L-1 = Q, L..L+K-1 = P) of row i
+ *   mkhip_decompose        mkrlwe KeySwitcher.Decompose (mkrlwe/keyswitch.go:50-76):
+ *                          a [level+1][N] coefficient -> [beta][level+1+K][N] NTT
+ *   mkhip_moddown          FastBasisExtender.ModDownQPtoQ (basis_extension.go:192-232):
+ *                          x [level+1+K][N] coefficient -> [level+1][N]
+ *   mkhip_external_product KeySwitcher.ExternalProductHoisted (keyswitch_hoisted.go:11-45)
+ */
+int mkhip_ntt_limbs(mkhip_ctx *ctx, const uint64_t *in, uint64_t *out, const int *ring_idx, int nlimbs,
+                    int inverse);
+int mkhip_decompose(mkhip_ctx *ctx, const uint64_t *a, int level, uint64_t *out);
+int mkhip_moddown(mkhip_ctx *ctx, const uint64_t *x, int level, uint64_t *out);
+int mkhip_external_product(mkhip_ctx *ctx, const uint64_t *a_hoisted, const uint64_t *key, int level,
+                           uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MKHIP_H */

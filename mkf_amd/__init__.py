@@ -1,0 +1,285 @@
+"""mkf_amd: MI355X-native multi-key mul+relin (MK-CKKS / MK-BFV).
+
+Python ctypes bindings over the C ABI in include/mkhip.h (library
+mkf_amd/libmkhip.so, built by mkf_amd/build.py).  There is no CPU fallback:
+if the HIP library is missing or no GPU is present, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmkhip.so")
+MAX_PARTIES = 32
+
+_lib = None
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+intp = ctypes.POINTER(ctypes.c_int)
+dblp = ctypes.POINTER(ctypes.c_double)
+
+STATUS = {
+    0: "MKHIP_OK", -1: "MKHIP_ERR_ARG", -2: "MKHIP_ERR_PARAMS", -3: "MKHIP_ERR_GADGET",
+    -4: "MKHIP_ERR_UNKNOWN_PARTY", -5: "MKHIP_ERR_LEVEL", -6: "MKHIP_ERR_NO_KEY", -7: "MKHIP_ERR_OOM",
+    -8: "MKHIP_ERR_DEVICE", -9: "MKHIP_ERR_SCALE", -10: "MKHIP_ERR_SCHEME", -11: "MKHIP_ERR_TOO_MANY",
+}
+
+# every symbol the header declares (checked by the CPU test suite)
+EXPORTS = [
+    "mkhip_status_string", "mkhip_ctx_create", "mkhip_ctx_destroy", "mkhip_ctx_info", "mkhip_set_crs",
+    "mkhip_register_party", "mkhip_unregister_party", "mkhip_kklss_ckks_mulrelin", "mkhip_dev_alloc",
+    "mkhip_dev_free", "mkhip_h2d", "mkhip_d2h", "mkhip_sync", "mkhip_plan_kklss_ckks", "mkhip_plan_launch",
+    "mkhip_plan_time", "mkhip_plan_destroy", "mkhip_plan_workspace_bytes", "mkhip_rescale_count",
+    "mkhip_ntt_limbs", "mkhip_decompose", "mkhip_moddown", "mkhip_external_product",
+]
+
+
+class MkhipError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = lib().mkhip_status_string(status).decode()
+        super().__init__("%s: %s (%s)" % (what, STATUS.get(status, status), msg))
+
+
+class KKLSSOp(ctypes.Structure):
+    _fields_ = [
+        ("op0", ctypes.c_void_p), ("op1", ctypes.c_void_p), ("out", ctypes.c_void_p),
+        ("ids0", ctypes.c_int * MAX_PARTIES), ("ids1", ctypes.c_int * MAX_PARTIES),
+        ("k0", ctypes.c_int), ("k1", ctypes.c_int), ("level", ctypes.c_int), ("nres", ctypes.c_int),
+    ]
+
+
+def lib():
+    """Load libmkhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("%s is not built; run `python -c 'import __graft_entry__ as g; g.build()'`"
+                               % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    I, V, S = ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t
+    U, D = ctypes.c_uint64, ctypes.c_double
+    L.mkhip_status_string.restype = ctypes.c_char_p
+    L.mkhip_status_string.argtypes = [I]
+    L.mkhip_ctx_create.argtypes = [ctypes.POINTER(V), I, u64p, I, u64p, I, I, U, u64p, I]
+    L.mkhip_ctx_destroy.argtypes = [V]
+    L.mkhip_ctx_destroy.restype = None
+    L.mkhip_ctx_info.argtypes = [V, intp, intp, ctypes.POINTER(S)]
+    L.mkhip_set_crs.argtypes = [V, I, u64p]
+    L.mkhip_register_party.argtypes = [V, I, u64p, u64p, u64p]
+    L.mkhip_unregister_party.argtypes = [V, I]
+    L.mkhip_kklss_ckks_mulrelin.argtypes = [V, u64p, I, D, intp, I, u64p, I, D, intp, I, D, u64p, intp,
+                                            intp, intp, dblp]
+    L.mkhip_dev_alloc.argtypes = [V, S, ctypes.POINTER(V)]
+    L.mkhip_dev_free.argtypes = [V, V]
+    L.mkhip_h2d.argtypes = [V, V, V, S]
+    L.mkhip_d2h.argtypes = [V, V, V, S]
+    L.mkhip_sync.argtypes = [V]
+    L.mkhip_plan_kklss_ckks.argtypes = [V, I, ctypes.POINTER(KKLSSOp), I, ctypes.POINTER(V)]
+    L.mkhip_plan_launch.argtypes = [V]
+    L.mkhip_plan_time.argtypes = [V, I, dblp]
+    L.mkhip_plan_destroy.argtypes = [V]
+    L.mkhip_plan_destroy.restype = None
+    L.mkhip_plan_workspace_bytes.argtypes = [V]
+    L.mkhip_plan_workspace_bytes.restype = S
+    L.mkhip_rescale_count.argtypes = [V, I, D, D, intp, dblp]
+    L.mkhip_ntt_limbs.argtypes = [V, u64p, u64p, intp, I, I]
+    L.mkhip_decompose.argtypes = [V, u64p, I, u64p]
+    L.mkhip_moddown.argtypes = [V, u64p, I, u64p]
+    L.mkhip_external_product.argtypes = [V, u64p, u64p, I, u64p]
+
+
+def _p(a):
+    if a.dtype != np.uint64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous uint64 array")
+    return a.ctypes.data_as(u64p)
+
+
+def _u64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.uint64))
+
+
+def check(status, what):
+    if status != 0:
+        raise MkhipError(status, what)
+
+
+class Context:
+    """Parameter set + device tables + resident keys (mkhip_ctx)."""
+
+    def __init__(self, logN, Q, P, gamma=2, t=0, qmul=None, flags=0):
+        self.logN, self.N = logN, 1 << logN
+        self.Q, self.P = [int(q) for q in Q], [int(p) for p in P]
+        self.L, self.K, self.gamma = len(self.Q), len(self.P), gamma
+        h = ctypes.c_void_p()
+        q = _u64(self.Q)
+        p = _u64(self.P or [0])
+        check(lib().mkhip_ctx_create(ctypes.byref(h), logN, _p(q), self.L, _p(p), self.K, gamma, t,
+                                     None, flags), "mkhip_ctx_create")
+        self._h = h
+        a, b, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        check(lib().mkhip_ctx_info(h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(w)), "mkhip_ctx_info")
+        self.alpha, self.beta, self.swk_words = a.value, b.value, w.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mkhip_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def beta_at(self, level):
+        return (level + 1 + self.alpha - 1) // self.alpha
+
+    # ---- keys ----
+    def set_crs_u(self, u):
+        u = _u64(u)
+        assert u.size == self.swk_words
+        check(lib().mkhip_set_crs(self._h, -1, _p(u)), "mkhip_set_crs")
+
+    def register_party(self, pid, b, d, v):
+        b, d, v = _u64(b), _u64(d), _u64(v)
+        for x in (b, d, v):
+            assert x.size == self.swk_words, (x.size, self.swk_words)
+        check(lib().mkhip_register_party(self._h, pid, _p(b), _p(d), _p(v)), "mkhip_register_party")
+
+    def unregister_party(self, pid):
+        check(lib().mkhip_unregister_party(self._h, pid), "mkhip_unregister_party")
+
+    # ---- evaluator ----
+    def rescale_count(self, level, scale, min_scale):
+        n, s = ctypes.c_int(), ctypes.c_double()
+        check(lib().mkhip_rescale_count(self._h, level, scale, min_scale, ctypes.byref(n), ctypes.byref(s)),
+              "mkhip_rescale_count")
+        return n.value, s.value
+
+    def kklss_ckks_mulrelin(self, op0, ids0, op1, ids1, scale0=1.0, scale1=1.0, min_scale=1.0):
+        """KKLSS MK-CKKS MulRelinNew with rescale.  op: (1+k, level+1, N) uint64.
+
+        Returns (out, ids_out, level_out, scale_out)."""
+        op0, op1 = _u64(op0), _u64(op1)
+        k0, k1 = len(ids0), len(ids1)
+        assert op0.shape[0] == 1 + k0 and op1.shape[0] == 1 + k1
+        lvl0, lvl1 = op0.shape[1] - 1, op1.shape[1] - 1
+        ku = len(set(ids0) | set(ids1))
+        nl = min(lvl0, lvl1) + 1
+        out = np.zeros((1 + ku) * nl * self.N, np.uint64)
+        i0 = (ctypes.c_int * max(k0, 1))(*ids0)
+        i1 = (ctypes.c_int * max(k1, 1))(*ids1)
+        iout = (ctypes.c_int * max(ku, 1))()
+        kuo, lo, so = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib().mkhip_kklss_ckks_mulrelin(self._h, _p(op0), lvl0, scale0, i0, k0, _p(op1), lvl1, scale1, i1,
+                                              k1, min_scale, _p(out), iout, ctypes.byref(kuo), ctypes.byref(lo),
+                                              ctypes.byref(so)), "mkhip_kklss_ckks_mulrelin")
+        nlo = lo.value + 1
+        out = out[:(1 + kuo.value) * nlo * self.N].reshape(1 + kuo.value, nlo, self.N)
+        return out, list(iout)[:kuo.value], lo.value, so.value
+
+    # ---- primitives ----
+    def ntt(self, rows, ring_idx, inverse=False):
+        rows = _u64(rows)
+        ri = (ctypes.c_int * len(ring_idx))(*ring_idx)
+        out = np.zeros_like(rows)
+        check(lib().mkhip_ntt_limbs(self._h, _p(rows), _p(out), ri, len(ring_idx), int(inverse)),
+              "mkhip_ntt_limbs")
+        return out
+
+    def decompose(self, a, level=None):
+        a = _u64(a)
+        level = a.shape[0] - 1 if level is None else level
+        out = np.zeros((self.beta_at(level), level + 1 + self.K, self.N), np.uint64)
+        check(lib().mkhip_decompose(self._h, _p(a), level, _p(out)), "mkhip_decompose")
+        return out
+
+    def moddown(self, x, level=None):
+        x = _u64(x)
+        level = x.shape[0] - 1 - self.K if level is None else level
+        out = np.zeros((level + 1, self.N), np.uint64)
+        check(lib().mkhip_moddown(self._h, _p(x), level, _p(out)), "mkhip_moddown")
+        return out
+
+    def external_product(self, a_hoisted, key, level=None):
+        a_hoisted, key = _u64(a_hoisted), _u64(key)
+        level = a_hoisted.shape[1] - 1 - self.K if level is None else level
+        out = np.zeros((level + 1, self.N), np.uint64)
+        check(lib().mkhip_external_product(self._h, _p(a_hoisted), _p(key), level, _p(out)),
+              "mkhip_external_product")
+        return out
+
+    # ---- device memory ----
+    def alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        check(lib().mkhip_dev_alloc(self._h, nbytes, ctypes.byref(p)), "mkhip_dev_alloc")
+        return p.value
+
+    def free(self, ptr):
+        check(lib().mkhip_dev_free(self._h, ptr), "mkhip_dev_free")
+
+    def h2d(self, dptr, arr):
+        arr = np.ascontiguousarray(arr)
+        check(lib().mkhip_h2d(self._h, dptr, arr.ctypes.data, arr.nbytes), "mkhip_h2d")
+
+    def d2h(self, arr, dptr):
+        check(lib().mkhip_d2h(self._h, arr.ctypes.data, dptr, arr.nbytes), "mkhip_d2h")
+        return arr
+
+    def sync(self):
+        check(lib().mkhip_sync(self._h), "mkhip_sync")
+
+
+class Plan:
+    """A batched KKLSS MK-CKKS mul+relin over device-resident operands."""
+
+    def __init__(self, ctx, ops, use_graph=True):
+        arr = (KKLSSOp * len(ops))()
+        for i, o in enumerate(ops):
+            a = arr[i]
+            a.op0, a.op1, a.out = o["op0"], o["op1"], o["out"]
+            for j, v in enumerate(o["ids0"]):
+                a.ids0[j] = v
+            for j, v in enumerate(o["ids1"]):
+                a.ids1[j] = v
+            a.k0, a.k1, a.level, a.nres = len(o["ids0"]), len(o["ids1"]), o["level"], o["nres"]
+        h = ctypes.c_void_p()
+        check(lib().mkhip_plan_kklss_ckks(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)),
+              "mkhip_plan_kklss_ckks")
+        self._h = h
+        self.ctx = ctx
+
+    def launch(self):
+        check(lib().mkhip_plan_launch(self._h), "mkhip_plan_launch")
+
+    def time(self, iters):
+        ms = ctypes.c_double()
+        check(lib().mkhip_plan_time(self._h, iters, ctypes.byref(ms)), "mkhip_plan_time")
+        return ms.value
+
+    @property
+    def workspace_bytes(self):
+        return lib().mkhip_plan_workspace_bytes(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mkhip_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,379 @@
+// CDNA4 kernels of the multi-key mul+relin path.
+//
+//   k_ntt_batch      batched limb NTT / INTT (tensor inputs, MKF switches)
+//   k_decomp         gadget decomposition + NTT of every digit row, fused with the
+//                    KKLSS x/y accumulation (mkrlwe/keyswitch.go:22-76,
+//                    keyswitch_hoisted.go:84-123)
+//   k_tensor         tensor product and INTT (keyswitch_hoisted.go:126-153)
+//   k_ext            external-product MAC over the digits + INTT of every row
+//                    (keyswitch_hoisted.go:11-45)
+//   k_moddown        ModDown Q u P -> Q, accumulation into the output, and the
+//                    fused rescale (basis_extension.go:192-232, mkckks/evaluator.go:360-401)
+//
+// Every kernel keeps one RNS limb of one polynomial per workgroup (N/16 threads,
+// 16 coefficients per thread) or one coefficient per thread, reads and writes
+// limb-major rows coalesced, and produces canonical residues.
+#include "ntt.hpp"
+#include "tables.hpp"
+
+namespace mkhip {
+
+extern __shared__ u64 g_lds[];
+
+template <int LOGN>
+__device__ __forceinline__ void load_row(u64 (&v)[16], const u64 *__restrict__ src, int tid) {
+    constexpr int T = (1 << LOGN) / 16;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = __builtin_nontemporal_load(src + tid + i * T);
+}
+
+template <int LOGN>
+__device__ __forceinline__ void load_row_cached(u64 (&v)[16], const u64 *__restrict__ src, int tid) {
+    constexpr int T = (1 << LOGN) / 16;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = src[tid + i * T];
+}
+
+template <int LOGN>
+__device__ __forceinline__ void store_row(const u64 (&v)[16], u64 *__restrict__ dst, int tid) {
+    constexpr int T = (1 << LOGN) / 16;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) dst[tid + i * T] = v[i];
+}
+
+__device__ __forceinline__ int ring_index(int r, int level, int L) { return r <= level ? r : L + (r - level - 1); }
+
+// 128-bit accumulate of a * b.
+__device__ __forceinline__ void mac128(u64 &hi, u64 &lo, u64 a, u64 b) {
+    u64 l = a * b;
+    u64 h = mulhi64(a, b);
+    lo += l;
+    hi += h + (lo < l ? 1 : 0);
+}
+
+// Lazy Montgomery reduction of a 128-bit accumulator as in multSum
+// (basis_extension.go:623-645): hi - hi(lo * pinv * p) + p.
+__device__ __forceinline__ u64 mont_reduce128(u64 hi, u64 lo, u64 p, u64 pinv) {
+    return hi - mulhi64(lo * pinv, p) + p;
+}
+
+// ---------------------------------------------------------------------------
+// batched NTT
+
+template <int LOGN, bool INV>
+__global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_batch(DevTables T, const NttJob *__restrict__ jobs) {
+    constexpr int N = 1 << LOGN;
+    const int tid = threadIdx.x;
+    const NttJob j = jobs[blockIdx.x];
+    const ModConst m = T.mod[j.mod];
+    u64 v[16];
+    load_row_cached<LOGN>(v, j.src, tid);
+    if (j.reduce) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = reduce64(v[i], m.q, m.mu);
+    }
+    if (INV)
+        ntt_inv<LOGN>(v, g_lds, tid, T.itw + (size_t)j.mod * N, m.q, m.ninv, m.ninvp);
+    else
+        ntt_fwd<LOGN>(v, g_lds, tid, T.tw + (size_t)j.mod * N, m.q);
+    store_row<LOGN>(v, j.dst, tid);
+}
+
+// ---------------------------------------------------------------------------
+// gadget digit of one coefficient into modulus `m` (ring index ri)
+
+// Exact basis extension of the digit's nd residues to modulus m: y_l = MRed(x_l,
+// (D/q_l)^-1 R), v = trunc(sum_l float64(y_l)/float64(q_l)) accumulated in
+// ascending l, out = sum_l y_l (D/q_l) - v D mod m  (basis_extension.go:471-533).
+__device__ __forceinline__ u64 digit_modup(const DevTables &T, const u64 *__restrict__ src, int c, int start,
+                                           int nd, int ebase, int ri, const ModConst &m) {
+    const int N = T.N;
+    const u64 *yc = T.dc_y + (size_t)ebase * kMaxAlpha;
+    const u64 *qs = T.dc_qstar + ((size_t)ebase * (T.L + T.K) + ri) * kMaxAlpha;
+    const u64 *vt = T.dc_vt + ((size_t)ebase * (T.L + T.K) + ri) * (kMaxAlpha + 1);
+    u64 hi = 0, lo = 0;
+    double vi = 0.0;
+    for (int l = 0; l < nd; ++l) {
+        const ModConst ml = T.mod[start + l];
+        const u64 x = src[(size_t)(start + l) * N + c];
+        const u64 y = mred(x, yc[l], ml.q, ml.qinv);
+        vi += (double)y / (double)ml.q;
+        mac128(hi, lo, y, qs[l]);
+    }
+    const u64 v = (u64)vi;
+    u64 r = mont_reduce128(hi, lo, m.q, m.qinv) + vt[v];
+    return reduce64(r, m.q, m.mu);
+}
+
+// blockIdx.x = (desc * beta + digit) * rows + row
+template <int LOGN>
+__global__ void __launch_bounds__(1 << (LOGN - 4))
+    k_decomp(DevTables T, const DecompDesc *__restrict__ descs, int beta, int rows) {
+    constexpr int N = 1 << LOGN;
+    constexpr int TH = N / 16;
+    const int tid = threadIdx.x;
+    const int bid = blockIdx.x;
+    const int r = bid % rows;
+    const int i = (bid / rows) % beta;
+    const int d = bid / (rows * beta);
+    const DecompDesc &Dd = descs[d];
+    const int level = Dd.level, k = Dd.k;
+    const int ri = ring_index(r, level, T.L);
+    const ModConst m = T.mod[ri];
+    const int alpha = T.alpha;
+    const int start = i * alpha;
+    const int nd = (level > alpha * (i + 1) - 1) ? alpha : (level % alpha) + 1;
+    const int ebase = i * (kMaxAlpha + 1) + nd;
+    const bool own_row = (r <= level) && (r >= start) && (r < start + nd);
+    const ulonglong2 *tw = T.tw + (size_t)ri * N;
+    const size_t kstride = (size_t)(T.L + T.K) * N;
+    u64 acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0;
+    for (int p = 0; p < k; ++p) {
+        const u64 *src = Dd.comp[p];
+        u64 v[16];
+        if (nd == 1 || own_row) {
+            // copy path / the digit's own rows: the residue itself (reduced mod m)
+            const u64 *row = src + (size_t)(own_row ? r : start) * N;
+            load_row_cached<LOGN>(v, row, tid);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[e] = digit_modup(T, src, tid + e * TH, start, nd, ebase, ri, m);
+        }
+        ntt_fwd<LOGN>(v, g_lds, tid, tw, m.q);
+        store_row<LOGN>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N, tid);
+        const u64 *key = Dd.key[p];
+        if (key) {
+            const u64 *kr = key + (size_t)i * kstride + (size_t)ri * N;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[e] = add_mod(acc[e], mred(kr[tid + e * TH], v[e], m.q, m.qinv), m.q);
+        }
+    }
+    if (Dd.xy) {
+        u64 *xr = Dd.xy + ((size_t)i * rows + r) * N;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) xr[tid + e * TH] = mred(acc[e], m.r2, m.q, m.qinv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tensor product + INTT ; blockIdx.x = desc * nl + limb
+
+__device__ __forceinline__ u64 mulmod_m(u64 a, u64 b, const ModConst &m) {
+    return mred(mred(a, m.r2, m.q, m.qinv), b, m.q, m.qinv);
+}
+
+template <int LOGN>
+__global__ void __launch_bounds__(1 << (LOGN - 4)) k_tensor(DevTables T, const TensorDesc *__restrict__ descs, int nl) {
+    constexpr int N = 1 << LOGN;
+    constexpr int TH = N / 16;
+    const int tid = threadIdx.x;
+    const int l = blockIdx.x % nl;
+    const TensorDesc &Dd = descs[blockIdx.x / nl];
+    const ModConst m = T.mod[l];
+    const size_t off = (size_t)l * N;
+    u64 v[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const size_t c = off + tid + e * TH;
+        u64 x;
+        if (Dd.is_c0) {
+            x = mulmod_m(Dd.a0[c], Dd.b0[c], m);
+        } else {
+            x = 0;
+            if (Dd.au) x = mulmod_m(Dd.b0[c], Dd.au[c], m);
+            if (Dd.bu) x = add_mod(x, mulmod_m(Dd.a0[c], Dd.bu[c], m), m.q);
+        }
+        v[e] = x;
+    }
+    ntt_inv<LOGN>(v, g_lds, tid, T.itw + (size_t)l * N, m.q, m.ninv, m.ninvp);
+    store_row<LOGN>(v, Dd.out + off, tid);
+}
+
+// ---------------------------------------------------------------------------
+// external product MAC + INTT ; blockIdx.x = desc * rows + row
+
+template <int LOGN>
+__global__ void __launch_bounds__(1 << (LOGN - 4))
+    k_ext(DevTables T, const ExtDesc *__restrict__ descs, int beta, int rows) {
+    constexpr int N = 1 << LOGN;
+    constexpr int TH = N / 16;
+    const int tid = threadIdx.x;
+    const int r = blockIdx.x % rows;
+    const ExtDesc &Dd = descs[blockIdx.x / rows];
+    const int ri = ring_index(r, Dd.level, T.L);
+    const ModConst m = T.mod[ri];
+    const size_t kstride = (size_t)(T.L + T.K) * N;
+    const int nkey = Dd.nkey;
+    u64 a0[16], a1[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) a0[e] = a1[e] = 0;
+    for (int i = 0; i < beta; ++i) {
+        const u64 *dr = Dd.D + ((size_t)i * rows + r) * N;
+        const u64 *k0 = Dd.key[0] + (size_t)i * kstride + (size_t)ri * N;
+        const u64 *k1 = nkey > 1 ? Dd.key[1] + (size_t)i * kstride + (size_t)ri * N : nullptr;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const u64 dv = dr[tid + e * TH];
+            a0[e] = add_mod(a0[e], mred(k0[tid + e * TH], dv, m.q, m.qinv), m.q);
+            if (nkey > 1) a1[e] = add_mod(a1[e], mred(k1[tid + e * TH], dv, m.q, m.qinv), m.q);
+        }
+    }
+    const ulonglong2 *itw = T.itw + (size_t)ri * N;
+    ntt_inv<LOGN>(a0, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
+    store_row<LOGN>(a0, Dd.acc[0] + (size_t)r * N, tid);
+    if (nkey > 1) {
+        ntt_inv<LOGN>(a1, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
+        store_row<LOGN>(a1, Dd.acc[1] + (size_t)r * N, tid);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ModDown (+ accumulate) (+ rescale) ; one coefficient per thread, grid (N/256, ndesc)
+
+constexpr int kMaxQ = 16;
+constexpr int kMaxK = 4;
+
+__global__ void __launch_bounds__(256) k_moddown(DevTables T, const ModDownDesc *__restrict__ descs) {
+    const int N = T.N, K = T.K, L = T.L;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const ModDownDesc &Dd = descs[blockIdx.y];
+    const int nl = Dd.level + 1;
+    u64 res[kMaxQ];
+#pragma unroll
+    for (int j = 0; j < kMaxQ; ++j) res[j] = (Dd.accumulate && j < nl) ? Dd.dst[(size_t)j * N + c] : 0;
+    for (int s = 0; s < Dd.nsrc; ++s) {
+        const u64 *src = Dd.src[s];
+        u64 y[kMaxK];
+        double vi = 0.0;
+#pragma unroll
+        for (int k = 0; k < kMaxK; ++k) {
+            if (k < K) {
+                const ModConst mp = T.mod[L + k];
+                y[k] = mred(src[(size_t)(nl + k) * N + c], T.md_y[k], mp.q, mp.qinv);
+                vi += (double)y[k] / (double)mp.q;
+            }
+        }
+        const u64 v = (u64)vi;
+#pragma unroll
+        for (int j = 0; j < kMaxQ; ++j) {
+            if (j < nl) {
+                const ModConst mq = T.mod[j];
+                u64 hi = 0, lo = 0;
+#pragma unroll
+                for (int k = 0; k < kMaxK; ++k)
+                    if (k < K) mac128(hi, lo, y[k], T.md_qstar[j * K + k]);
+                u64 z = mont_reduce128(hi, lo, mq.q, mq.qinv) + T.md_vt[j * (K + 1) + v];
+                z = reduce64(z, mq.q, mq.mu);
+                const u64 x = src[(size_t)j * N + c];
+                const u64 md = mred(z + 2 * mq.q - x, T.md_pinv[j], mq.q, mq.qinv);
+                res[j] = add_mod(res[j], md, mq.q);
+            }
+        }
+    }
+    if (Dd.base) {
+#pragma unroll
+        for (int j = 0; j < kMaxQ; ++j)
+            if (j < nl) res[j] = add_mod(res[j], Dd.base[(size_t)j * N + c], T.mod[j].q);
+    }
+    if (!Dd.final_) {
+#pragma unroll
+        for (int j = 0; j < kMaxQ; ++j)
+            if (j < nl) Dd.dst[(size_t)j * N + c] = res[j];
+        return;
+    }
+    // DivRoundByLastModulus nres times: out_j = (x_j + h - [x_l + h]_{q_l}) q_l^-1
+    int top = nl;
+    for (int n = 0; n < Dd.nres; ++n) {
+        const int l = top - 1;
+        const ModConst ml = T.mod[l];
+        u64 xl = 0;
+#pragma unroll
+        for (int j = 0; j < kMaxQ; ++j)
+            if (j == l) xl = res[j];
+        const u64 t = add_mod(xl, ml.q >> 1, ml.q);
+#pragma unroll
+        for (int j = 0; j < kMaxQ; ++j) {
+            if (j < l) {
+                const ModConst mj = T.mod[j];
+                const u64 tj = reduce64(t, mj.q, mj.mu);
+                const u64 num = sub_mod(add_mod(res[j], T.rs_half[l * L + j], mj.q), tj, mj.q);
+                res[j] = mred(num, T.rs_inv[l * L + j], mj.q, mj.qinv);
+            }
+        }
+        top = l;
+    }
+#pragma unroll
+    for (int j = 0; j < kMaxQ; ++j)
+        if (j < top) Dd.out[(size_t)j * N + c] = res[j];
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+
+template <class F>
+static hipError_t allow_lds(F kernel, size_t bytes) {
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
+}
+
+template <int LOGN>
+static hipError_t launch_all(int what, const DevTables &T, const void *descs, int n, int a, int b,
+                             hipStream_t st) {
+    constexpr int N = 1 << LOGN;
+    constexpr size_t lds = (size_t)N * sizeof(u64);
+    static bool init = false;
+    if (!init) {
+        hipError_t e;
+        if ((e = allow_lds(k_ntt_batch<LOGN, false>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_ntt_batch<LOGN, true>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_decomp<LOGN>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_tensor<LOGN>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_ext<LOGN>, lds)) != hipSuccess) return e;
+        init = true;
+    }
+    const dim3 block(N / 16);
+    switch (what) {
+    case 0:
+        hipLaunchKernelGGL((k_ntt_batch<LOGN, false>), dim3(n), block, lds, st, T, (const NttJob *)descs);
+        break;
+    case 1:
+        hipLaunchKernelGGL((k_ntt_batch<LOGN, true>), dim3(n), block, lds, st, T, (const NttJob *)descs);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_decomp<LOGN>), dim3(n * a * b), block, lds, st, T, (const DecompDesc *)descs, a, b);
+        break;
+    case 3:
+        hipLaunchKernelGGL((k_tensor<LOGN>), dim3(n * a), block, lds, st, T, (const TensorDesc *)descs, a);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_ext<LOGN>), dim3(n * b), block, lds, st, T, (const ExtDesc *)descs, a, b);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// what: 0 ntt fwd, 1 ntt inv (n jobs); 2 decomp (n descs, a = beta, b = rows);
+// 3 tensor (n descs, a = limbs); 4 ext (n descs, a = beta, b = rows); 5 moddown (n descs)
+hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (what == 5) {
+        hipLaunchKernelGGL(k_moddown, dim3(T.N / 256, n), dim3(256), 0, st, T, (const ModDownDesc *)descs);
+        return hipGetLastError();
+    }
+    switch (T.logN) {
+    case 13:
+        return launch_all<13>(what, T, descs, n, a, b, st);
+    case 14:
+        return launch_all<14>(what, T, descs, n, a, b, st);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace mkhip

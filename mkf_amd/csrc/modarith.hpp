@@ -1,0 +1,98 @@
+// 64-bit modular arithmetic for moduli q < 2^61 on CDNA4 (host + device).
+//
+// Conventions follow the reference's ring arithmetic (Lattigo v2.3.0 as used by
+// software/kklss/mkrlwe and software/mkf/mkf_cpu):
+//   mred(x, y)   = x * y * 2^-64 mod q        (Lattigo ring.MRed; keys are stored
+//                                               "in Montgomery form", x * 2^64 mod q)
+//   shoup(a, w)  = a * w mod q, lazily in [0, 2q), with the precomputed companion
+//                  w' = floor(w * 2^64 / q); used for NTT twiddles and constants.
+// All functions are exact integer arithmetic.  The NTT kernels keep values in
+// [0, 4q) (Harvey's lazy butterflies), which needs 4q < 2^63.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mkhip {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+#define MKHIP_HD __host__ __device__ __forceinline__
+
+MKHIP_HD u64 mulhi64(u64 a, u64 b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (u64)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// a * b * 2^-64 mod q, canonical, for a * b < q * 2^64 (Lattigo MRed).
+MKHIP_HD u64 mred(u64 a, u64 b, u64 q, u64 qinv) {
+    u64 lo = a * b;
+    u64 hi = mulhi64(a, b);
+    u64 m = lo * qinv;
+    u64 mqhi = mulhi64(m, q);
+    u64 r = hi - mqhi + q;
+    return r >= q ? r - q : r;
+}
+
+// a * w mod q in [0, 2q) for any a < 2^64 and w < q, w' = floor(w 2^64 / q).
+MKHIP_HD u64 shoup_lazy(u64 a, u64 w, u64 wp, u64 q) {
+    u64 qh = mulhi64(a, wp);
+    return a * w - qh * q;
+}
+
+MKHIP_HD u64 shoup(u64 a, u64 w, u64 wp, u64 q) {
+    u64 r = shoup_lazy(a, w, wp, q);
+    return r >= q ? r - q : r;
+}
+
+// x mod q for any x < 2^64, mu = floor(2^64 / q) (Barrett with one correction).
+MKHIP_HD u64 reduce64(u64 x, u64 q, u64 mu) {
+    u64 r = x - mulhi64(x, mu) * q;  // r < 2q + q for q < 2^63
+    r = r >= q ? r - q : r;
+    return r >= q ? r - q : r;
+}
+
+MKHIP_HD u64 add_mod(u64 a, u64 b, u64 q) {
+    u64 s = a + b;
+    return s >= q ? s - q : s;
+}
+
+MKHIP_HD u64 sub_mod(u64 a, u64 b, u64 q) { return a >= b ? a - b : a + q - b; }
+
+// Per-modulus constants resident in device memory.
+struct ModConst {
+    u64 q;      // modulus
+    u64 qinv;   // q^-1 mod 2^64 (Montgomery)
+    u64 mu;     // floor(2^64 / q) (Barrett)
+    u64 r2;     // 2^128 mod q: mred(x, r2) = x * 2^64 mod q (MForm)
+    u64 ninv;   // N^-1 mod q (canonical) for this modulus at the context's N
+    u64 ninvp;  // Shoup companion of ninv
+    u64 pad0, pad1;
+};
+
+// host helpers (exact, 128-bit); host-only functions, parsed in both compilation passes
+inline u64 h_mulmod(u64 a, u64 b, u64 q) { return (u64)(((unsigned __int128)a * b) % q); }
+inline u64 h_powmod(u64 x, u64 e, u64 q) {
+    u64 r = 1 % q, b = x % q;
+    while (e) {
+        if (e & 1) r = h_mulmod(r, b, q);
+        b = h_mulmod(b, b, q);
+        e >>= 1;
+    }
+    return r;
+}
+inline u64 h_inv(u64 a, u64 q) { return h_powmod(a % q, q - 2, q); }
+inline u64 h_mform(u64 a, u64 q) { return (u64)((((unsigned __int128)(a % q)) << 64) % q); }
+inline u64 h_shoup_companion(u64 w, u64 q) { return (u64)((((unsigned __int128)w) << 64) / q); }
+inline u64 h_qinv(u64 q) {
+    u64 x = q;  // Newton iteration for q^-1 mod 2^64
+    for (int i = 0; i < 6; ++i) x *= 2 - q * x;
+    return x;
+}
+
+
+}  // namespace mkhip

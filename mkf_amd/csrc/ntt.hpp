@@ -1,0 +1,212 @@
+// LDS-resident negacyclic NTT / INTT of one RNS limb for N = 2^13 and 2^14 (and the
+// register-split N = 2^15 variant) on one workgroup of N/16 threads.
+//
+// Transform definition (what must match bit for bit): the reference's ring NTT
+// (Lattigo v2.3.0 NTTLazy / InvNTTLazy, restated in
+// hardware/u55c/hls/kernels/par1_mkf_ckks_kernel.cpp:92-157): forward stage s
+// (t = N >> (s+1)) pairs j and j+t of group g = j / 2t with twiddle psi[2^s + g]
+// (Cooley-Tukey, natural order in, bit-reversed order out); inverse stage u
+// (t = 2^u) uses psiinv[(N >> (u+1)) + g] (Gentleman-Sande) and finishes with a
+// multiplication by N^-1.  psi[brv(j)] = psi^j for a primitive 2N-th root psi.
+//
+// Implementation: every thread holds 16 coefficients in registers.  Stages are
+// processed in passes of up to 4 (radix-16 in registers); between passes the
+// limb is exchanged through LDS (N * 8 bytes: 128 KiB at N = 2^14).  Butterflies
+// are Harvey's lazy forms with Shoup twiddles: values stay in [0, 4q) forward
+// and [0, 2q) inverse and are reduced to canonical residues on the way out.
+#pragma once
+
+#include "modarith.hpp"
+
+namespace mkhip {
+
+// ---- butterflies ---------------------------------------------------------
+
+__device__ __forceinline__ void fwd_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q2, u64 q) {
+    u64 u = x >= q2 ? x - q2 : x;          // [0, 2q)
+    u64 t = shoup_lazy(y, w, wp, q);       // [0, 2q)
+    x = u + t;                             // [0, 4q)
+    y = u + q2 - t;                        // (0, 4q)
+}
+
+__device__ __forceinline__ void inv_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q2, u64 q) {
+    u64 u = x, v = y;                      // both in [0, 2q)
+    u64 s = u + v;
+    x = s >= q2 ? s - q2 : s;              // [0, 2q)
+    y = shoup_lazy(u + q2 - v, w, wp, q);  // [0, 2q)
+}
+
+// ---- pass geometry -------------------------------------------------------
+// A forward pass (S0, NS) covers stages S0..S0+NS-1.  Its "columns" are the sets
+// of 2^NS coefficients that interact within the pass: column col has base
+// b * 2^NS * TL + r (b = col / TL, r = col % TL, TL = t of the pass's last stage)
+// and stride TL.  Thread tid owns columns tid + c * T (c < 16 / 2^NS).
+
+template <int LOGN, int S0, int NS>
+struct FwdPass {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int T = N / 16;
+    static constexpr int TL = N >> (S0 + NS);
+    static constexpr int R = 1 << NS;
+    static constexpr int C = 16 / R;
+    __device__ static __forceinline__ int elem(int tid, int i) {
+        const int c = i / R, k = i % R;
+        const int col = tid + c * T;
+        const int b = col / TL, r = col % TL;
+        return b * R * TL + r + k * TL;
+    }
+    __device__ static __forceinline__ void run(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ tw,
+                                               u64 q) {
+        const u64 q2 = q << 1;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int col = tid + c * T;
+            const int b = col / TL;
+#pragma unroll
+            for (int u = 0; u < NS; ++u) {
+                const int half = R >> (u + 1);
+#pragma unroll
+                for (int grp = 0; grp < (1 << u); ++grp) {
+                    const ulonglong2 w = tw[(1 << (S0 + u)) + b * (1 << u) + grp];
+#pragma unroll
+                    for (int kk = 0; kk < half; ++kk) {
+                        const int k0 = grp * 2 * half + kk;
+                        fwd_bfly(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, q2, q);
+                    }
+                }
+            }
+        }
+    }
+};
+
+// An inverse pass (U0, NS) covers inverse stages U0..U0+NS-1 (t = 2^U0 .. 2^(U0+NS-1)).
+template <int LOGN, int U0, int NS>
+struct InvPass {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int T = N / 16;
+    static constexpr int TF = 1 << U0;
+    static constexpr int R = 1 << NS;
+    static constexpr int C = 16 / R;
+    __device__ static __forceinline__ int elem(int tid, int i) {
+        const int c = i / R, k = i % R;
+        const int col = tid + c * T;
+        const int b = col / TF, r = col % TF;
+        return b * R * TF + r + k * TF;
+    }
+    __device__ static __forceinline__ void run(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ itw,
+                                               u64 q) {
+        const u64 q2 = q << 1;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int col = tid + c * T;
+            const int b = col / TF;
+#pragma unroll
+            for (int u = 0; u < NS; ++u) {
+                const int half = 1 << u;
+                const int m = N >> (U0 + u + 1);
+                const int ngrp = R / (2 * half);
+#pragma unroll
+                for (int grp = 0; grp < ngrp; ++grp) {
+                    const ulonglong2 w = itw[m + b * ngrp + grp];
+#pragma unroll
+                    for (int kk = 0; kk < half; ++kk) {
+                        const int k0 = grp * 2 * half + kk;
+                        inv_bfly(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, q2, q);
+                    }
+                }
+            }
+        }
+    }
+};
+
+template <class From, class To>
+__device__ __forceinline__ void lds_exchange(u64 (&v)[16], u64 *s, int tid) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s[From::elem(tid, i)] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = s[To::elem(tid, i)];
+    __syncthreads();
+}
+
+// Natural coalesced layout: element tid + i * T.
+template <int LOGN>
+struct Linear {
+    static constexpr int T = (1 << LOGN) / 16;
+    __device__ static __forceinline__ int elem(int tid, int i) { return tid + i * T; }
+};
+
+// ---- whole transforms ------------------------------------------------------
+// Input: v[i] = coefficient tid + i*T (values < 4q).  Output: v[i] = NTT value at
+// index tid + i*T, canonical.  s: N words of LDS.
+
+template <int LOGN>
+struct NttPlan;
+
+template <>
+struct NttPlan<14> {
+    typedef FwdPass<14, 0, 4> F0;
+    typedef FwdPass<14, 4, 4> F1;
+    typedef FwdPass<14, 8, 4> F2;
+    typedef FwdPass<14, 12, 2> F3;
+    typedef InvPass<14, 0, 2> I0;
+    typedef InvPass<14, 2, 4> I1;
+    typedef InvPass<14, 6, 4> I2;
+    typedef InvPass<14, 10, 4> I3;
+};
+
+template <>
+struct NttPlan<13> {
+    typedef FwdPass<13, 0, 4> F0;
+    typedef FwdPass<13, 4, 4> F1;
+    typedef FwdPass<13, 8, 4> F2;
+    typedef FwdPass<13, 12, 1> F3;
+    typedef InvPass<13, 0, 1> I0;
+    typedef InvPass<13, 1, 4> I1;
+    typedef InvPass<13, 5, 4> I2;
+    typedef InvPass<13, 9, 4> I3;
+};
+
+// Forward NTT; v in Linear layout in and out; returns canonical values.
+template <int LOGN>
+__device__ __forceinline__ void ntt_fwd(u64 (&v)[16], u64 *s, int tid, const ulonglong2 *__restrict__ tw,
+                                        u64 q) {
+    typedef NttPlan<LOGN> P;
+    static_assert(P::F0::TL == Linear<LOGN>::T, "first pass must be the linear layout");
+    P::F0::run(v, tid, tw, q);
+    lds_exchange<typename P::F0, typename P::F1>(v, s, tid);
+    P::F1::run(v, tid, tw, q);
+    lds_exchange<typename P::F1, typename P::F2>(v, s, tid);
+    P::F2::run(v, tid, tw, q);
+    lds_exchange<typename P::F2, typename P::F3>(v, s, tid);
+    P::F3::run(v, tid, tw, q);
+    lds_exchange<typename P::F3, Linear<LOGN> >(v, s, tid);
+    const u64 q2 = q << 1;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        u64 x = v[i];
+        x = x >= q2 ? x - q2 : x;
+        v[i] = x >= q ? x - q : x;
+    }
+}
+
+// Inverse NTT including the N^-1 scaling; v canonical (or < 2q) in Linear layout;
+// returns canonical values in Linear layout.
+template <int LOGN>
+__device__ __forceinline__ void ntt_inv(u64 (&v)[16], u64 *s, int tid, const ulonglong2 *__restrict__ itw,
+                                        u64 q, u64 ninv, u64 ninvp) {
+    typedef NttPlan<LOGN> P;
+    static_assert(P::I3::TF * 16 == (1 << LOGN), "last inverse pass must be the linear layout");
+    lds_exchange<Linear<LOGN>, typename P::I0>(v, s, tid);
+    P::I0::run(v, tid, itw, q);
+    lds_exchange<typename P::I0, typename P::I1>(v, s, tid);
+    P::I1::run(v, tid, itw, q);
+    lds_exchange<typename P::I1, typename P::I2>(v, s, tid);
+    P::I2::run(v, tid, itw, q);
+    lds_exchange<typename P::I2, typename P::I3>(v, s, tid);
+    P::I3::run(v, tid, itw, q);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = shoup(v[i], ninv, ninvp, q);
+}
+
+}  // namespace mkhip

@@ -1,0 +1,91 @@
+// Device-resident constant tables of one parameter set, and the work descriptors
+// the kernels consume.  Shared by host (mkhip.hip) and device code.
+#pragma once
+
+#include "modarith.hpp"
+
+namespace mkhip {
+
+constexpr int kMaxParties = 32;   // components per ciphertext side handled by one descriptor
+constexpr int kMaxModuli = 48;    // Q + P (+ BFV auxiliary) moduli of one context
+constexpr int kMaxAlpha = 4;      // primes per gadget digit
+
+// Pointers into one device allocation holding every table of a context.
+struct DevTables {
+    const ModConst *mod;      // [nmod] : Q[0..L) then P[0..K) then (BFV) QMul[0..L)
+    const ulonglong2 *tw;     // [nmod][N] forward twiddles (w, Shoup w'), Lattigo index order
+    const ulonglong2 *itw;    // [nmod][N] inverse twiddles
+    // ModDown Q u P -> Q (basis_extension.go:192-232 with paramsPtoQ[K-1]):
+    const u64 *md_y;          // [K]       MForm((P/p_i)^-1 mod p_i)
+    const u64 *md_qstar;      // [L][K]    MForm((P/p_i) mod q_j)
+    const u64 *md_vt;         // [L][K+1]  v * (-P) mod q_j
+    const u64 *md_pinv;       // [L]       MForm(-P^-1 mod q_j)
+    // gadget digits with nd >= 2 primes (Decomposer, basis_extension.go:368-424):
+    // index e = (digit * kMaxAlpha + nd) ; rows r over Q_all u P_all (L + K)
+    const u64 *dc_y;          // [beta][kMaxAlpha+1][kMaxAlpha]        MForm((D/q_l)^-1 mod q_l)
+    const u64 *dc_qstar;      // [beta][kMaxAlpha+1][L+K][kMaxAlpha]    MForm((D/q_l) mod m_r)
+    const u64 *dc_vt;         // [beta][kMaxAlpha+1][L+K][kMaxAlpha+1]  v * (-D) mod m_r
+    // rescale (DivRoundByLastModulus) : for dropped modulus index l and j < l
+    const u64 *rs_inv;        // [L][L]  MForm(q_l^-1 mod q_j)
+    const u64 *rs_half;       // [L][L]  (q_l >> 1) mod q_j
+    int logN, N, L, K, alpha, beta_max, nmod;
+    int pad;
+};
+
+__device__ __forceinline__ int dc_base(const DevTables &T, int digit, int nd) {
+    return digit * (kMaxAlpha + 1) + nd;
+}
+
+// ---- descriptors -----------------------------------------------------------
+
+// One hoisted gadget decomposition side: k component polys (coefficient domain,
+// level+1 limbs each, row stride N) decomposed into D[k][beta][rows][N]; when key
+// pointers are given the kernel also accumulates xy[beta][rows] = MForm(sum_p key_p (.) D_p).
+struct DecompDesc {
+    const u64 *comp[kMaxParties];
+    const u64 *key[kMaxParties];   // KKLSS d_i / b_j (MForm, beta x (L+K) x N); null => no MAC
+    u64 *D;                        // [k][beta][rows][N]
+    u64 *xy;                       // [beta][rows][N] (MForm), may be null
+    int k, level;
+    int pad[2];
+};
+
+// Generic batched limb NTT job.
+struct NttJob {
+    const u64 *src;
+    u64 *dst;
+    int mod;       // ring index
+    int reduce;    // 1: src values may exceed the modulus (reduce mod q first)
+};
+
+// Tensor output component: out = a0 (.) b_u + a_u (.) b0 (either term may be absent),
+// or out = a0 (.) b0 for the c0 component.  Inputs NTT domain, output coefficient domain.
+struct TensorDesc {
+    const u64 *a0, *b0, *au, *bu;  // au / bu null when the party is absent on that side
+    u64 *out;
+    int is_c0;
+    int pad;
+};
+
+// External-product MAC over beta digits followed by an INTT of every row:
+// acc_o[r] = INTT( sum_i key_o[i][r] (.) D[i][r] ) for o < nkey.
+struct ExtDesc {
+    const u64 *D;          // [beta][rows][N]
+    const u64 *key[2];     // [beta][L+K][N] each (key row index = ring index)
+    u64 *acc[2];           // [rows][N]
+    int nkey, level;
+};
+
+// ModDown of nsrc accumulators (rows = level+1+K, coefficient domain) into dst
+// (level+1 limbs): dst (=|+=) sum_s ModDown(src_s).  With `final`, the result
+// plus `base` is rescaled nres times and written to out (level+1-nres limbs).
+struct ModDownDesc {
+    const u64 *src[kMaxParties];
+    u64 *dst;              // level+1 limbs
+    const u64 *base;       // added before writing (may be null)
+    u64 *out;              // final output (when nres >= 0 and final)
+    int nsrc, level, accumulate, final_, nres;
+    int pad[3];
+};
+
+}  // namespace mkhip

@@ -1,0 +1,331 @@
+"""CPU oracle for multi-key mul+relin -- TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings over ``oracle/libmkref.so`` (built from mkref.c by ``make -C
+oracle``), a plain-C restatement of the reference's KKLSS/MKF mul+relin path
+(see mkref.c for the reference file:line each function follows).  Only the
+tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py``
+use this module, and only as the checker; the shipped library never does.
+
+Parity status of each restated function (what pins it):
+  mred / mred_params        64 RTL known-answer vectors (tb_mred64_pipeline.sv)
+  butterfly                 128 RTL known-answer vectors (tb_ntt_butterfly_x8.sv)
+  ntt / intt                invariants: INTT(NTT(x)) = x, NTT product = negacyclic
+                            schoolbook product; N^-1 table vs HLS N_INV_*.  The
+                            choice of primitive root (Lattigo ring.NewRing) is
+                            pinned by no in-tree file: parity unpinned for the
+                            absolute NTT-domain values of keys.
+  modup params              HLS tables P2Q_*, DECOMP_*, P_INV_Q, Q2M_*, M2Q_*
+  modup_exact / moddown     big-integer CRT closed form (exact when the float64
+                            estimate is exact) and brute force on small moduli
+  decompose / ext. product  gadget identity: Ext(D(a), g*s) ~ a*s*P/P, noise bound
+  kklss / mkf mul+relin     decrypt-and-compare tests (mkckks_test.go:352-358 bound)
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libmkref.so")
+_lib = None
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+i64p = ctypes.POINTER(ctypes.c_int64)
+intp = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    U = ctypes.c_uint64
+    I = ctypes.c_int
+    L.mkref_mred_params.restype = U
+    L.mkref_mred_params.argtypes = [U]
+    L.mkref_mred.restype = U
+    L.mkref_mred.argtypes = [U, U, U, U]
+    L.mkref_mred_lazy.restype = U
+    L.mkref_mred_lazy.argtypes = [U, U, U, U]
+    L.mkref_mform.restype = U
+    L.mkref_mform.argtypes = [U, U]
+    L.mkref_primitive_root.restype = U
+    L.mkref_primitive_root.argtypes = [U, I]
+    L.mkref_butterfly.argtypes = [U, U, U, U, U, u64p, u64p]
+    L.mkref_gen_primes.restype = I
+    L.mkref_gen_primes.argtypes = [I, I, I, I, u64p, u64p, I]
+    L.mkref_ctx_new.restype = ctypes.c_void_p
+    L.mkref_ctx_new.argtypes = [I, u64p, I, u64p, I, I, I]
+    L.mkref_ctx_delete.argtypes = [ctypes.c_void_p]
+    L.mkref_ctx_tables.argtypes = [ctypes.c_void_p, I, u64p, u64p, u64p, u64p]
+    L.mkref_ctx_ntt.argtypes = [ctypes.c_void_p, I, u64p, I]
+    L.mkref_modup_tables.argtypes = [u64p, I, u64p, I, u64p, u64p, u64p]
+    L.mkref_modup_exact_simple.argtypes = [I, u64p, u64p, I, u64p, u64p, I]
+    L.mkref_moddown.argtypes = [I, u64p, u64p, I, u64p, u64p, I, u64p]
+    L.mkref_decompose.argtypes = [ctypes.c_void_p, I, u64p, u64p]
+    L.mkref_decompose_digit.argtypes = [ctypes.c_void_p, I, I, u64p, u64p]
+    L.mkref_external_product_hoisted.argtypes = [ctypes.c_void_p, I, u64p, u64p, u64p]
+    L.mkref_kklss_mulrelin.restype = I
+    L.mkref_kklss_mulrelin.argtypes = [ctypes.c_void_p, I, u64p, intp, I, u64p, intp, I,
+                                       ctypes.POINTER(u64p), ctypes.POINTER(u64p),
+                                       ctypes.POINTER(u64p), u64p, u64p, intp]
+    L.mkref_rescale_count.restype = I
+    L.mkref_rescale_count.argtypes = [u64p, I, ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(ctypes.c_double)]
+    L.mkref_div_round_last.argtypes = [ctypes.c_void_p, I, I, u64p]
+    L.mkref_div_round_last_ntt.argtypes = [ctypes.c_void_p, I, I, u64p]
+    L.mkref_mkf_switch_one.argtypes = [ctypes.c_void_p, I, u64p, I, u64p, I, u64p, u64p]
+    L.mkref_mkf_ckks_mulrelin.argtypes = [ctypes.c_void_p, I, I, u64p, u64p,
+                                          ctypes.POINTER(u64p), u64p, u64p, u64p]
+    L.mkref_prng_seed.argtypes = [ctypes.c_void_p, U]
+    L.mkref_prng_next.restype = U
+    L.mkref_prng_next.argtypes = [ctypes.c_void_p]
+    L.mkref_sample_uniform.argtypes = [ctypes.c_void_p, u64p, I, I, u64p]
+    L.mkref_sample_ternary.argtypes = [ctypes.c_void_p, I, ctypes.c_double, i64p]
+    L.mkref_sample_gaussian.argtypes = [ctypes.c_void_p, I, ctypes.c_double, I, i64p]
+    L.mkref_small_to_rns.argtypes = [i64p, I, u64p, I, u64p]
+
+
+def _p(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(u64p)
+
+
+def _u64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.uint64))
+
+
+# ---- scalar helpers -------------------------------------------------------
+
+def mred_params(q):
+    return int(lib().mkref_mred_params(q))
+
+
+def mred(x, y, q, qinv=None):
+    return int(lib().mkref_mred(x, y, q, mred_params(q) if qinv is None else qinv))
+
+
+def mform(a, q):
+    return int(lib().mkref_mform(a, q))
+
+
+def primitive_root(q, rule=0):
+    return int(lib().mkref_primitive_root(q, rule))
+
+
+def butterfly(u, v, w, q, qinv):
+    s = ctypes.c_uint64()
+    d = ctypes.c_uint64()
+    lib().mkref_butterfly(u, v, w, q, qinv, ctypes.byref(s), ctypes.byref(d))
+    return s.value, d.value
+
+
+def gen_primes(bits, logN, count, exclude=()):
+    out = np.zeros(count, dtype=np.uint64)
+    ex = _u64(list(exclude) or [0])
+    rc = lib().mkref_gen_primes(bits, logN, count, 1, _p(out), _p(ex), len(exclude))
+    if rc != 0:
+        raise ValueError("not enough primes")
+    return [int(v) for v in out]
+
+
+def modup_tables(Q, P):
+    Q = _u64(Q)
+    P = _u64(P)
+    a = np.zeros(len(Q), np.uint64)
+    b = np.zeros(len(Q) * len(P), np.uint64)
+    v = np.zeros((len(Q) + 1) * len(P), np.uint64)
+    lib().mkref_modup_tables(_p(Q), len(Q), _p(P), len(P), _p(a), _p(b), _p(v))
+    return a, b.reshape(len(P), len(Q)), v.reshape(len(P), len(Q) + 1)
+
+
+def modup_exact(x, Q, P):
+    x = _u64(x)
+    N = x.shape[-1]
+    Qa, Pa = _u64(Q), _u64(P)
+    out = np.zeros((len(P), N), np.uint64)
+    lib().mkref_modup_exact_simple(N, _p(x), _p(Qa), len(Q), _p(out), _p(Pa), len(P))
+    return out
+
+
+def moddown(xq, xp, Q, P):
+    xq, xp = _u64(xq), _u64(xp)
+    N = xq.shape[-1]
+    Qa, Pa = _u64(Q), _u64(P)
+    out = np.zeros((len(Q), N), np.uint64)
+    lib().mkref_moddown(N, _p(xq), _p(Qa), len(Q), _p(xp), _p(Pa), len(P), _p(out))
+    return out
+
+
+class Prng:
+    """xoshiro256** seeded by splitmix64 (the build's deterministic key/input source)."""
+
+    def __init__(self, seed):
+        self._state = (ctypes.c_uint64 * 4)()
+        lib().mkref_prng_seed(ctypes.cast(self._state, ctypes.c_void_p), seed)
+
+    @property
+    def _h(self):
+        return ctypes.cast(self._state, ctypes.c_void_p)
+
+    def uniform(self, moduli, N):
+        q = _u64(moduli)
+        out = np.zeros((len(moduli), N), np.uint64)
+        lib().mkref_sample_uniform(self._h, _p(q), len(moduli), N, _p(out))
+        return out
+
+    def ternary(self, N, p_zero=0.5):
+        out = np.zeros(N, np.int64)
+        lib().mkref_sample_ternary(self._h, N, p_zero, out.ctypes.data_as(i64p))
+        return out
+
+    def gaussian(self, N, sigma=3.2, bound=19):
+        out = np.zeros(N, np.int64)
+        lib().mkref_sample_gaussian(self._h, N, sigma, bound, out.ctypes.data_as(i64p))
+        return out
+
+
+def small_to_rns(s, moduli):
+    s = np.ascontiguousarray(np.asarray(s, np.int64))
+    q = _u64(moduli)
+    out = np.zeros((len(moduli), s.shape[0]), np.uint64)
+    lib().mkref_small_to_rns(s.ctypes.data_as(i64p), s.shape[0], _p(q), len(moduli), _p(out))
+    return out
+
+
+class Ctx:
+    """RNS context: Q chain, P chain, gamma (2 = KKLSS, 1 = MKF / Lattigo rlwe)."""
+
+    def __init__(self, logN, Q, P, gamma=2, root_rule=0):
+        self.logN, self.N = logN, 1 << logN
+        self.Q, self.P = [int(q) for q in Q], [int(p) for p in P]
+        self.L, self.K, self.gamma = len(Q), len(P), gamma
+        q, p = _u64(self.Q), _u64(self.P or [0])
+        self._h = lib().mkref_ctx_new(logN, _p(q), self.L, _p(p), self.K, gamma, root_rule)
+        if not self._h:
+            raise ValueError("mkref_ctx_new failed (moduli not NTT friendly?)")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mkref_ctx_delete(self._h)
+            self._h = None
+
+    @property
+    def QP(self):
+        return self.Q + self.P
+
+    @property
+    def alpha(self):
+        return self.K // self.gamma
+
+    def beta(self, level=None):
+        level = self.L - 1 if level is None else level
+        a = self.alpha
+        return (level + 1 + a - 1) // a
+
+    def tables(self, limb):
+        psi = np.zeros(self.N, np.uint64)
+        psiinv = np.zeros(self.N, np.uint64)
+        root = ctypes.c_uint64()
+        ninv = ctypes.c_uint64()
+        lib().mkref_ctx_tables(self._h, limb, _p(psi), _p(psiinv), ctypes.byref(root), ctypes.byref(ninv))
+        return psi, psiinv, root.value, ninv.value
+
+    def ntt(self, a, limb, inverse=False):
+        a = _u64(a).copy()
+        lib().mkref_ctx_ntt(self._h, limb, _p(a), 1 if inverse else 0)
+        return a
+
+    def ntt_poly(self, a, inverse=False, rows=None):
+        """NTT every limb of a (nlimbs, N) poly; rows gives ring indices per limb."""
+        a = _u64(a).copy()
+        rows = list(range(a.shape[0])) if rows is None else rows
+        for i, r in enumerate(rows):
+            lib().mkref_ctx_ntt(self._h, r, _p(a[i]), 1 if inverse else 0)
+        return a
+
+    def decompose(self, a, level=None):
+        level = self.L - 1 if level is None else level
+        a = _u64(a)
+        out = np.zeros((self.beta(level), level + 1 + self.K, self.N), np.uint64)
+        lib().mkref_decompose(self._h, level, _p(a), _p(out))
+        return out
+
+    def decompose_digit(self, a, digit, level=None):
+        level = self.L - 1 if level is None else level
+        a = _u64(a)
+        out = np.zeros((level + 1 + self.K, self.N), np.uint64)
+        lib().mkref_decompose_digit(self._h, level, digit, _p(a), _p(out))
+        return out
+
+    def external_product(self, a_hoisted, key, level=None):
+        level = self.L - 1 if level is None else level
+        out = np.zeros((level + 1, self.N), np.uint64)
+        lib().mkref_external_product_hoisted(self._h, level, _p(_u64(a_hoisted)), _p(_u64(key)), _p(out))
+        return out
+
+    def kklss_mulrelin(self, op0, ids0, op1, ids1, rlk_b, rlk_d, rlk_v, crs_u, level=None):
+        """KKLSS MK-CKKS mul+relin before rescale; op = (1+k, level+1, N). Returns (out, ids)."""
+        level = op0.shape[1] - 1 if level is None else level
+        op0, op1 = _u64(op0[:, :level + 1]), _u64(op1[:, :level + 1])
+        keys = [_u64(k) for k in (list(rlk_b) + list(rlk_d) + list(rlk_v))]
+        n = len(rlk_b)
+        arr = (u64p * n)
+        b = arr(*[_p(k) for k in keys[:n]])
+        d = arr(*[_p(k) for k in keys[n:2 * n]])
+        v = arr(*[_p(k) for k in keys[2 * n:]])
+        u = _u64(crs_u)
+        k0, k1 = len(ids0), len(ids1)
+        i0 = (ctypes.c_int * max(k0, 1))(*ids0)
+        i1 = (ctypes.c_int * max(k1, 1))(*ids1)
+        ku = len(set(ids0) | set(ids1))
+        out = np.zeros((1 + ku, level + 1, self.N), np.uint64)
+        iout = (ctypes.c_int * ku)()
+        nc = lib().mkref_kklss_mulrelin(self._h, level, _p(op0), i0, k0, _p(op1), i1, k1,
+                                        b, d, v, _p(u), _p(out), iout)
+        assert nc == ku + 1
+        return out, list(iout)
+
+    def rescale_count(self, level, scale, min_scale):
+        q = _u64(self.Q)
+        ns = ctypes.c_double()
+        nb = lib().mkref_rescale_count(_p(q), level, scale, min_scale, ctypes.byref(ns))
+        return nb, ns.value
+
+    def div_round_last(self, poly, n=1, ntt=False):
+        poly = _u64(poly).copy()
+        level = poly.shape[0] - 1
+        fn = lib().mkref_div_round_last_ntt if ntt else lib().mkref_div_round_last
+        fn(self._h, level, n, _p(poly))
+        return poly[:level + 1 - n].copy()
+
+    def mkf_switch_one(self, a, swk, in_ntt=True, ntt_output=True, level=None):
+        level = a.shape[0] - 1 if level is None else level
+        o0 = np.zeros((level + 1, self.N), np.uint64)
+        o1 = np.zeros((level + 1, self.N), np.uint64)
+        lib().mkref_mkf_switch_one(self._h, level, _p(_u64(a)), int(in_ntt), _p(_u64(swk)),
+                                   int(ntt_output), _p(o0), _p(o1))
+        return o0, o1
+
+    def mkf_ckks_mulrelin(self, left, right, fold_keys, relin_key):
+        k = left.shape[0] - 1
+        level = left.shape[1] - 1
+        keys = [_u64(f) for f in fold_keys]
+        arr = (u64p * k)(*[_p(f) for f in keys])
+        o0 = np.zeros((level + 1, self.N), np.uint64)
+        o1 = np.zeros((level + 1, self.N), np.uint64)
+        lib().mkref_mkf_ckks_mulrelin(self._h, level, k, _p(_u64(left)), _p(_u64(right)), arr,
+                                      _p(_u64(relin_key)), _p(o0), _p(o1))
+        return o0, o1

@@ -1,0 +1,209 @@
+"""Test-side key generation, encryption and decryption over the CPU oracle.
+
+The reference samples keys with a crypto PRNG (mkrlwe/keygen.go:26), so "keys
+from seed s" is this build's convention: the oracle's xoshiro256** stream.
+The key equations follow mkrlwe/keygen.go:
+  secret  s  : ternary, P(0)=1/2 (:58-75), stored as MForm(NTT(s)) over QP (:44-55)
+  b_i = MForm(e_i - a_i s)                              (:161-169)
+  d_i = MForm(e_i + g_i P s - a_i r)                    (:171-176, GenSwitchingKey :270-327)
+  v_i = MForm(-(e_i + g_i P r) - u_i s)                 (:178-185)
+where a = CRS[0], u = CRS[-1] are uniform MForm PolyQP vectors (params.go:16-61)
+and g_i P s adds P*s on the Q limbs of digit i only.
+"""
+import numpy as np
+
+import oracle as O
+
+
+def _mulmod_rows(a, b, moduli):
+    out = np.empty_like(a)
+    for i, q in enumerate(moduli):
+        out[i] = np.array([(int(x) * int(y)) % q for x, y in zip(a[i], b[i])], np.uint64) \
+            if a.shape[-1] <= 64 else _mulmod_vec(a[i], b[i], q)
+    return out
+
+
+def _mulmod_vec(x, y, q):
+    # exact 64x64 mod q via Python ints on object arrays
+    xo = x.astype(object)
+    yo = y.astype(object)
+    return np.array((xo * yo) % q, dtype=np.uint64)
+
+
+def mulmod(a, b, moduli):
+    a = np.asarray(a, np.uint64)
+    b = np.asarray(b, np.uint64)
+    return np.stack([_mulmod_vec(a[i], b[i], q) for i, q in enumerate(moduli)])
+
+
+def addmod(a, b, moduli):
+    return np.stack([np.array((a[i].astype(object) + b[i].astype(object)) % q, np.uint64)
+                     for i, q in enumerate(moduli)])
+
+
+def submod(a, b, moduli):
+    return np.stack([np.array((a[i].astype(object) - b[i].astype(object)) % q, np.uint64)
+                     for i, q in enumerate(moduli)])
+
+
+def mform(a, moduli):
+    return np.stack([np.array((a[i].astype(object) * (1 << 64)) % q, np.uint64) for i, q in enumerate(moduli)])
+
+
+def inv_mform(a, moduli):
+    return np.stack([np.array((a[i].astype(object) * pow(1 << 64, -1, q)) % q, np.uint64)
+                     for i, q in enumerate(moduli)])
+
+
+class KKLSSKeys:
+    """CRS + per-party (b, d, v) relinearization keys + secrets for a KKLSS context."""
+
+    def __init__(self, ctx, nparties, seed=0):
+        self.ctx = ctx
+        g = O.Prng(seed)
+        N, QP = ctx.N, ctx.QP
+        beta = ctx.beta()
+        self.beta = beta
+        # CRS[0] = a, CRS[-1] = u (uniform, MForm)
+        self.a = np.stack([mform(g.uniform(QP, N), QP) for _ in range(beta)])
+        self.u = np.stack([mform(g.uniform(QP, N), QP) for _ in range(beta)])
+        self.sk, self.sk_small, self.b, self.d, self.v, self.pk = [], [], [], [], [], []
+        for _ in range(nparties):
+            s_small = g.ternary(N)
+            r_small = g.ternary(N)
+            s = self._secret(s_small)
+            r = self._secret(r_small)
+            self.sk.append(s)
+            self.sk_small.append(s_small)
+            self.b.append(self._gen_b(g, s))
+            self.d.append(self._gen_d(g, s, r))
+            self.v.append(self._gen_v(g, s, r))
+            self.pk.append(self._gen_pk(g, s))
+
+    def _secret(self, small):
+        ctx = self.ctx
+        return mform(ctx.ntt_poly(O.small_to_rns(small, ctx.QP)), ctx.QP)
+
+    def _err_ntt(self, g):
+        ctx = self.ctx
+        return ctx.ntt_poly(O.small_to_rns(g.gaussian(ctx.N), ctx.QP))
+
+    def _plain(self, x_mform):
+        return inv_mform(x_mform, self.ctx.QP)
+
+    def _gen_b(self, g, s):
+        QP = self.ctx.QP
+        out = []
+        for i in range(self.beta):
+            as_ = mulmod(self._plain(self.a[i]), self._plain(s), QP)
+            out.append(mform(submod(self._err_ntt(g), as_, QP), QP))
+        return np.stack(out)
+
+    def _gadget(self, g, s, i):
+        """MForm(e + g_i P s): P*s added on the Q limbs of digit i (keygen.go:270-327)."""
+        ctx = self.ctx
+        QP = ctx.QP
+        e = self._err_ntt(g)
+        Pprod = 1
+        for p in ctx.P:
+            Pprod *= p
+        splain = self._plain(s)
+        alpha = ctx.alpha
+        for j in range(alpha):
+            idx = i * alpha + j
+            if idx >= ctx.L:
+                break
+            q = QP[idx]
+            e[idx] = np.array((e[idx].astype(object) + splain[idx].astype(object) * (Pprod % q)) % q, np.uint64)
+        return e
+
+    def _gen_d(self, g, s, r):
+        QP = self.ctx.QP
+        out = []
+        for i in range(self.beta):
+            x = self._gadget(g, s, i)
+            ar = mulmod(self._plain(self.a[i]), self._plain(r), QP)
+            out.append(mform(submod(x, ar, QP), QP))
+        return np.stack(out)
+
+    def _gen_v(self, g, s, r):
+        QP = self.ctx.QP
+        out = []
+        for i in range(self.beta):
+            x = self._gadget(g, r, i)
+            us = mulmod(self._plain(self.u[i]), self._plain(s), QP)
+            neg = submod(np.zeros_like(x), addmod(x, us, QP), QP)
+            out.append(mform(neg, QP))
+        return np.stack(out)
+
+    def _gen_pk(self, g, s):
+        """pk = (-s a_0 + e, a_0) over Q (keygen.go:88-109), NTT form, not MForm."""
+        ctx = self.ctx
+        Q = ctx.Q
+        L = ctx.L
+        a0 = self._plain(self.a[0])[:L]
+        e = self._err_ntt(g)[:L]
+        pk0 = submod(e, mulmod(a0, self._plain(s)[:L], Q), Q)
+        return pk0, a0
+
+    def encrypt(self, party, m_coeff, g):
+        """Public-key encryption (mkrlwe/encryptor.go:55-119), coefficient-domain output."""
+        ctx = self.ctx
+        Q, N = ctx.Q, ctx.N
+        pk0, pk1 = self.pk[party]
+        vv = ctx.ntt_poly(O.small_to_rns(g.ternary(N), Q))
+        c0 = ctx.ntt_poly(mulmod(vv, pk0, Q), inverse=True)
+        c1 = ctx.ntt_poly(mulmod(vv, pk1, Q), inverse=True)
+        c0 = addmod(c0, O.small_to_rns(g.gaussian(N), Q), Q)
+        c1 = addmod(c1, O.small_to_rns(g.gaussian(N), Q), Q)
+        c0 = addmod(c0, m_coeff, Q)
+        return c0, c1
+
+    def decrypt(self, ct, ids, nlimbs=None):
+        """c0 + sum_i c_i s_i (mkrlwe/decryptor.go:34-66) on the first nlimbs limbs."""
+        ctx = self.ctx
+        nl = ct.shape[1] if nlimbs is None else nlimbs
+        Q = ctx.Q[:nl]
+        acc = ct[0][:nl].copy()
+        for j, pid in enumerate(ids):
+            s = self._plain(self.sk[pid])[:nl]
+            cj = ctx.ntt_poly(ct[1 + j][:nl])
+            acc = addmod(acc, ctx.ntt_poly(mulmod(cj, s, Q), inverse=True), Q)
+        return acc
+
+
+def centered_crt(x, moduli):
+    """Lift an RNS poly (nlimbs, N) to centered integers via CRT (Python ints)."""
+    Qp = 1
+    for q in moduli:
+        Qp *= q
+    out = []
+    N = x.shape[1]
+    coefs = [0] * N
+    for i, q in enumerate(moduli):
+        Mi = Qp // q
+        inv = pow(Mi, -1, q)
+        for c in range(N):
+            coefs[c] += int(x[i][c]) * inv % q * Mi
+    for c in range(N):
+        v = coefs[c] % Qp
+        if v > Qp // 2:
+            v -= Qp
+        out.append(v)
+    return out
+
+
+def negacyclic_mul(a, b):
+    """Schoolbook product in Z[X]/(X^N+1) of Python-int lists."""
+    N = len(a)
+    out = [0] * N
+    for i, ai in enumerate(a):
+        if ai == 0:
+            continue
+        for j, bj in enumerate(b):
+            k = i + j
+            if k < N:
+                out[k] += ai * bj
+            else:
+                out[k - N] -= ai * bj
+    return out

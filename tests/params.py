@@ -1,0 +1,32 @@
+"""Parameter sets used by the tests and the benchmark.
+
+The BASELINE workloads name synthetic NTT-friendly primes (~50-bit Q, ~60-bit P,
+q = 1 mod 2^16 so that every N up to 2^15 is supported); they are generated
+deterministically by scanning down from 2^bits + 1.  P1 is the reference's own
+KKLSS/MKF parameter literal (software/mkf/mkf_cpu/params.go:15-28,
+software/kklss/mkckks/mkckks_test.go:75-89).
+"""
+import oracle as O
+
+_cache = {}
+
+
+def primes(bits, count, exclude=()):
+    key = (bits, count, tuple(exclude))
+    if key not in _cache:
+        _cache[key] = O.gen_primes(bits, 15, count, exclude)
+    return _cache[key]
+
+
+def config(name):
+    """(logN, Q, P) of a named workload."""
+    if name == "c1":  # plain MK-CKKS, k=2, N=2^13, 4 Q + 1 P (MKF only)
+        return 13, primes(50, 4), primes(60, 1)
+    if name == "c2":  # batched MK-CKKS, k=4, N=2^14, 8 Q + 2 P
+        return 14, primes(50, 8), primes(60, 2)
+    if name == "small":  # 4 Q + 2 P at N=2^13 (KKLSS-compatible variant of c1)
+        return 13, primes(50, 4), primes(60, 2)
+    if name == "P1":
+        return 14, [0x7ffffffffe70001, 0xffffffff00001, 0xfffffffe40001, 0xfffffffe20001,
+                    0xfffffffbe0001, 0xfffffffa60001], [0xffffffffffc0001, 0xfffffffff840001]
+    raise KeyError(name)
