@@ -105,8 +105,10 @@ __device__ __forceinline__ u64 digit_modup(const DevTables &T, const u64 *__rest
     return reduce64(r, m.q, m.mu);
 }
 
-// blockIdx.x = (desc * beta + digit) * rows + row
-template <int LOGN>
+// blockIdx.x = (desc * beta + digit) * rows + row.  With MAC the x/y accumulator
+// row lives in the output buffer (read-modify-write by the owning thread), so the
+// NTT keeps the whole register budget.
+template <int LOGN, bool MAC, bool CRT>
 __global__ void __launch_bounds__(1 << (LOGN - 4))
     k_decomp(DevTables T, const DecompDesc *__restrict__ descs, int beta, int rows) {
     constexpr int N = 1 << LOGN;
@@ -127,35 +129,36 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
     const bool own_row = (r <= level) && (r >= start) && (r < start + nd);
     const ulonglong2 *tw = T.tw + (size_t)ri * N;
     const size_t kstride = (size_t)(T.L + T.K) * N;
-    u64 acc[16];
+    u64 *xr = MAC ? Dd.xy + ((size_t)i * rows + r) * N : nullptr;
+    if (MAC && k == 0) {  // empty party set: the accumulator is the zero polynomial
 #pragma unroll
-    for (int e = 0; e < 16; ++e) acc[e] = 0;
+        for (int e = 0; e < 16; ++e) xr[tid + e * TH] = 0;
+    }
     for (int p = 0; p < k; ++p) {
         const u64 *src = Dd.comp[p];
         u64 v[16];
-        if (nd == 1 || own_row) {
+        if (!CRT || nd == 1 || own_row) {
             // copy path / the digit's own rows: the residue itself (reduced mod m)
             const u64 *row = src + (size_t)(own_row ? r : start) * N;
             load_row_cached<LOGN>(v, row, tid);
 #pragma unroll
             for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
         } else {
-#pragma unroll
             for (int e = 0; e < 16; ++e) v[e] = digit_modup(T, src, tid + e * TH, start, nd, ebase, ri, m);
         }
         ntt_fwd<LOGN>(v, g_lds, tid, tw, m.q);
         store_row<LOGN>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N, tid);
-        const u64 *key = Dd.key[p];
-        if (key) {
-            const u64 *kr = key + (size_t)i * kstride + (size_t)ri * N;
+        if (MAC) {
+            // x += MRed(key, D) (MulCoeffsMontgomeryAndAddLvl), MForm after the last party
+            const u64 *kr = Dd.key[p] + (size_t)i * kstride + (size_t)ri * N;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) acc[e] = add_mod(acc[e], mred(kr[tid + e * TH], v[e], m.q, m.qinv), m.q);
+            for (int e = 0; e < 16; ++e) {
+                const int c = tid + e * TH;
+                u64 t = mred(kr[c], v[e], m.q, m.qinv);
+                if (p > 0) t = add_mod(t, xr[c], m.q);
+                xr[c] = (p == k - 1) ? mred(t, m.r2, m.q, m.qinv) : t;
+            }
         }
-    }
-    if (Dd.xy) {
-        u64 *xr = Dd.xy + ((size_t)i * rows + r) * N;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) xr[tid + e * TH] = mred(acc[e], m.r2, m.q, m.qinv);
     }
 }
 
@@ -196,7 +199,9 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_tensor(DevTables T, const T
 // ---------------------------------------------------------------------------
 // external product MAC + INTT ; blockIdx.x = desc * rows + row
 
-template <int LOGN>
+// NKEY keys share one hoisted decomposition D (the nested products with v_i and
+// u, keyswitch_hoisted.go:177-193); each key's MAC is a separate pass over D.
+template <int LOGN, int NKEY>
 __global__ void __launch_bounds__(1 << (LOGN - 4))
     k_ext(DevTables T, const ExtDesc *__restrict__ descs, int beta, int rows) {
     constexpr int N = 1 << LOGN;
@@ -207,27 +212,21 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
     const int ri = ring_index(r, Dd.level, T.L);
     const ModConst m = T.mod[ri];
     const size_t kstride = (size_t)(T.L + T.K) * N;
-    const int nkey = Dd.nkey;
-    u64 a0[16], a1[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) a0[e] = a1[e] = 0;
-    for (int i = 0; i < beta; ++i) {
-        const u64 *dr = Dd.D + ((size_t)i * rows + r) * N;
-        const u64 *k0 = Dd.key[0] + (size_t)i * kstride + (size_t)ri * N;
-        const u64 *k1 = nkey > 1 ? Dd.key[1] + (size_t)i * kstride + (size_t)ri * N : nullptr;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const u64 dv = dr[tid + e * TH];
-            a0[e] = add_mod(a0[e], mred(k0[tid + e * TH], dv, m.q, m.qinv), m.q);
-            if (nkey > 1) a1[e] = add_mod(a1[e], mred(k1[tid + e * TH], dv, m.q, m.qinv), m.q);
-        }
-    }
     const ulonglong2 *itw = T.itw + (size_t)ri * N;
-    ntt_inv<LOGN>(a0, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
-    store_row<LOGN>(a0, Dd.acc[0] + (size_t)r * N, tid);
-    if (nkey > 1) {
-        ntt_inv<LOGN>(a1, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
-        store_row<LOGN>(a1, Dd.acc[1] + (size_t)r * N, tid);
+#pragma unroll
+    for (int o = 0; o < NKEY; ++o) {
+        u64 a[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) a[e] = 0;
+        for (int i = 0; i < beta; ++i) {
+            const u64 *dr = Dd.D + ((size_t)i * rows + r) * N;
+            const u64 *kr = Dd.key[o] + (size_t)i * kstride + (size_t)ri * N;
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                a[e] = add_mod(a[e], mred(kr[tid + e * TH], dr[tid + e * TH], m.q, m.qinv), m.q);
+        }
+        ntt_inv<LOGN>(a, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
+        store_row<LOGN>(a, Dd.acc[o] + (size_t)r * N, tid);
     }
 }
 
@@ -237,6 +236,7 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
 constexpr int kMaxQ = 16;
 constexpr int kMaxK = 4;
 
+template <bool FINAL>
 __global__ void __launch_bounds__(256) k_moddown(DevTables T, const ModDownDesc *__restrict__ descs) {
     const int N = T.N, K = T.K, L = T.L;
     const int c = blockIdx.x * 256 + threadIdx.x;
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(256) k_moddown(DevTables T, const ModDownDesc 
         for (int j = 0; j < kMaxQ; ++j)
             if (j < nl) res[j] = add_mod(res[j], Dd.base[(size_t)j * N + c], T.mod[j].q);
     }
-    if (!Dd.final_) {
+    if (!FINAL) {
 #pragma unroll
         for (int j = 0; j < kMaxQ; ++j)
             if (j < nl) Dd.dst[(size_t)j * N + c] = res[j];
@@ -330,27 +330,47 @@ static hipError_t launch_all(int what, const DevTables &T, const void *descs, in
         hipError_t e;
         if ((e = allow_lds(k_ntt_batch<LOGN, false>, lds)) != hipSuccess) return e;
         if ((e = allow_lds(k_ntt_batch<LOGN, true>, lds)) != hipSuccess) return e;
-        if ((e = allow_lds(k_decomp<LOGN>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_decomp<LOGN, true, false>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_decomp<LOGN, false, false>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_decomp<LOGN, true, true>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_decomp<LOGN, false, true>, lds)) != hipSuccess) return e;
         if ((e = allow_lds(k_tensor<LOGN>, lds)) != hipSuccess) return e;
-        if ((e = allow_lds(k_ext<LOGN>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_ext<LOGN, 1>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_ext<LOGN, 2>, lds)) != hipSuccess) return e;
         init = true;
     }
     const dim3 block(N / 16);
     switch (what) {
-    case 0:
+    case KNTT_FWD:
         hipLaunchKernelGGL((k_ntt_batch<LOGN, false>), dim3(n), block, lds, st, T, (const NttJob *)descs);
         break;
-    case 1:
+    case KNTT_INV:
         hipLaunchKernelGGL((k_ntt_batch<LOGN, true>), dim3(n), block, lds, st, T, (const NttJob *)descs);
         break;
-    case 2:
-        hipLaunchKernelGGL((k_decomp<LOGN>), dim3(n * a * b), block, lds, st, T, (const DecompDesc *)descs, a, b);
+    case KDECOMP_MAC:
+        if (T.alpha > 1)
+            hipLaunchKernelGGL((k_decomp<LOGN, true, true>), dim3(n * a * b), block, lds, st, T,
+                               (const DecompDesc *)descs, a, b);
+        else
+            hipLaunchKernelGGL((k_decomp<LOGN, true, false>), dim3(n * a * b), block, lds, st, T,
+                               (const DecompDesc *)descs, a, b);
         break;
-    case 3:
+    case KDECOMP:
+        if (T.alpha > 1)
+            hipLaunchKernelGGL((k_decomp<LOGN, false, true>), dim3(n * a * b), block, lds, st, T,
+                               (const DecompDesc *)descs, a, b);
+        else
+            hipLaunchKernelGGL((k_decomp<LOGN, false, false>), dim3(n * a * b), block, lds, st, T,
+                               (const DecompDesc *)descs, a, b);
+        break;
+    case KTENSOR:
         hipLaunchKernelGGL((k_tensor<LOGN>), dim3(n * a), block, lds, st, T, (const TensorDesc *)descs, a);
         break;
-    case 4:
-        hipLaunchKernelGGL((k_ext<LOGN>), dim3(n * b), block, lds, st, T, (const ExtDesc *)descs, a, b);
+    case KEXT1:
+        hipLaunchKernelGGL((k_ext<LOGN, 1>), dim3(n * b), block, lds, st, T, (const ExtDesc *)descs, a, b);
+        break;
+    case KEXT2:
+        hipLaunchKernelGGL((k_ext<LOGN, 2>), dim3(n * b), block, lds, st, T, (const ExtDesc *)descs, a, b);
         break;
     default:
         return hipErrorInvalidValue;
@@ -358,12 +378,17 @@ static hipError_t launch_all(int what, const DevTables &T, const void *descs, in
     return hipGetLastError();
 }
 
-// what: 0 ntt fwd, 1 ntt inv (n jobs); 2 decomp (n descs, a = beta, b = rows);
-// 3 tensor (n descs, a = limbs); 4 ext (n descs, a = beta, b = rows); 5 moddown (n descs)
+// what: KernelId (tables.hpp).  NTT: n jobs; decomp: n descs x a (beta) x b (rows);
+// tensor: n descs x a (limbs); ext: n descs x b (rows), a = beta; moddown: n descs.
 hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    if (what == 5) {
-        hipLaunchKernelGGL(k_moddown, dim3(T.N / 256, n), dim3(256), 0, st, T, (const ModDownDesc *)descs);
+    if (what == KMODDOWN || what == KMODDOWN_FINAL) {
+        if (what == KMODDOWN)
+            hipLaunchKernelGGL((k_moddown<false>), dim3(T.N / 256, n), dim3(256), 0, st, T,
+                               (const ModDownDesc *)descs);
+        else
+            hipLaunchKernelGGL((k_moddown<true>), dim3(T.N / 256, n), dim3(256), 0, st, T,
+                               (const ModDownDesc *)descs);
         return hipGetLastError();
     }
     switch (T.logN) {

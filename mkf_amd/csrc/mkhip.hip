@@ -500,18 +500,26 @@ int find_id(const int *ids, int k, int id) {
     return -1;
 }
 
-int plan_launch_raw(mkhip_plan *pl) {
+// stage s (0..7) of a KKLSS plan
+hipError_t plan_stage(mkhip_plan *pl, int s) {
     mkhip_ctx *c = pl->ctx;
     const DevTables &T = c->T;
     hipStream_t st = c->stream;
-    HIP_TRY(launch_kernel(2, T, pl->dec1, pl->n_dec1, pl->beta, pl->rows, st));
-    HIP_TRY(launch_kernel(0, T, pl->ntt, pl->n_ntt, 0, 0, st));
-    HIP_TRY(launch_kernel(3, T, pl->ten, pl->n_ten, pl->nl, 0, st));
-    HIP_TRY(launch_kernel(4, T, pl->ext1, pl->n_ext1, pl->beta, pl->rows, st));
-    HIP_TRY(launch_kernel(5, T, pl->md1, pl->n_md1, 0, 0, st));
-    HIP_TRY(launch_kernel(2, T, pl->dec2, pl->n_dec2, pl->beta, pl->rows, st));
-    HIP_TRY(launch_kernel(4, T, pl->ext2, pl->n_ext2, pl->beta, pl->rows, st));
-    HIP_TRY(launch_kernel(5, T, pl->md2, pl->n_md2, 0, 0, st));
+    switch (s) {
+    case 0: return launch_kernel(KDECOMP_MAC, T, pl->dec1, pl->n_dec1, pl->beta, pl->rows, st);
+    case 1: return launch_kernel(KNTT_FWD, T, pl->ntt, pl->n_ntt, 0, 0, st);
+    case 2: return launch_kernel(KTENSOR, T, pl->ten, pl->n_ten, pl->nl, 0, st);
+    case 3: return launch_kernel(KEXT1, T, pl->ext1, pl->n_ext1, pl->beta, pl->rows, st);
+    case 4: return launch_kernel(KMODDOWN, T, pl->md1, pl->n_md1, 0, 0, st);
+    case 5: return launch_kernel(KDECOMP, T, pl->dec2, pl->n_dec2, pl->beta, pl->rows, st);
+    case 6: return launch_kernel(KEXT2, T, pl->ext2, pl->n_ext2, pl->beta, pl->rows, st);
+    case 7: return launch_kernel(KMODDOWN_FINAL, T, pl->md2, pl->n_md2, 0, 0, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int plan_launch_raw(mkhip_plan *pl) {
+    for (int s = 0; s < 8; ++s) HIP_TRY(plan_stage(pl, s));
     return MKHIP_OK;
 }
 
@@ -843,6 +851,49 @@ extern "C" int mkhip_plan_time(mkhip_plan *pl, int iters, double *ms) {
 
 extern "C" size_t mkhip_plan_workspace_bytes(const mkhip_plan *pl) { return pl ? pl->bytes : 0; }
 
+// Average device time of each of the plan's eight launches (HIP events on the
+// context stream around each kernel, eager launches), in ms.
+extern "C" int mkhip_plan_stage_times(mkhip_plan *pl, int iters, double *ms8) {
+    if (!pl || iters <= 0 || !ms8) return MKHIP_ERR_ARG;
+    hipStream_t st = pl->ctx->stream;
+    hipEvent_t ev[9];
+    for (int i = 0; i < 9; ++i) HIP_TRY(hipEventCreate(&ev[i]));
+    double acc[8] = {0};
+    for (int it = 0; it < iters; ++it) {
+        HIP_TRY(hipEventRecord(ev[0], st));
+        for (int s = 0; s < 8; ++s) {
+            HIP_TRY(plan_stage(pl, s));
+            HIP_TRY(hipEventRecord(ev[s + 1], st));
+        }
+        HIP_TRY(hipEventSynchronize(ev[8]));
+        for (int s = 0; s < 8; ++s) {
+            float f = 0;
+            HIP_TRY(hipEventElapsedTime(&f, ev[s], ev[s + 1]));
+            acc[s] += f;
+        }
+    }
+    for (int i = 0; i < 9; ++i) (void)hipEventDestroy(ev[i]);
+    for (int s = 0; s < 8; ++s) ms8[s] = acc[s] / iters;
+    return MKHIP_OK;
+}
+
+// Launch-unit counts of the plan's stages (for the roofline denominators):
+// units[s] = workgroups of stage s (limb transforms for the NTT-shaped stages,
+// coefficient blocks of 256 for the ModDown stages).
+extern "C" int mkhip_plan_stage_units(const mkhip_plan *pl, long long *units8) {
+    if (!pl || !units8) return MKHIP_ERR_ARG;
+    const int br = pl->beta * pl->rows;
+    units8[0] = (long long)pl->n_dec1 * br;
+    units8[1] = pl->n_ntt;
+    units8[2] = (long long)pl->n_ten * pl->nl;
+    units8[3] = (long long)pl->n_ext1 * pl->rows;
+    units8[4] = (long long)pl->n_md1 * (pl->ctx->N / 256);
+    units8[5] = (long long)pl->n_dec2 * br;
+    units8[6] = (long long)pl->n_ext2 * pl->rows;
+    units8[7] = (long long)pl->n_md2 * (pl->ctx->N / 256);
+    return MKHIP_OK;
+}
+
 // ---------------------------------------------------------------------------
 // host-pointer entry point
 
@@ -970,7 +1021,7 @@ extern "C" int mkhip_ntt_limbs(mkhip_ctx *c, const uint64_t *in, uint64_t *out, 
     }
     HIP_TRY(hipMemcpyAsync(din, in, words * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(djobs, jobs.data(), sizeof(NttJob) * nlimbs, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(launch_kernel(inverse ? 1 : 0, c->T, djobs, nlimbs, 0, 0, c->stream));
+    HIP_TRY(launch_kernel(inverse ? KNTT_INV : KNTT_FWD, c->T, djobs, nlimbs, 0, 0, c->stream));
     HIP_TRY(hipMemcpyAsync(out, dout, words * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MKHIP_OK;
@@ -995,7 +1046,7 @@ extern "C" int mkhip_decompose(mkhip_ctx *c, const uint64_t *a, int level, uint6
     h.level = level;
     HIP_TRY(hipMemcpyAsync(din, a, inw * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(dd, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(launch_kernel(2, c->T, dd, 1, beta, rows, c->stream));
+    HIP_TRY(launch_kernel(KDECOMP, c->T, dd, 1, beta, rows, c->stream));
     HIP_TRY(hipMemcpyAsync(out, dout, outw * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MKHIP_OK;
@@ -1020,7 +1071,7 @@ extern "C" int mkhip_moddown(mkhip_ctx *c, const uint64_t *x, int level, uint64_
     h.level = level;
     HIP_TRY(hipMemcpyAsync(din, x, inw * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(dd, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(launch_kernel(5, c->T, dd, 1, 0, 0, c->stream));
+    HIP_TRY(launch_kernel(KMODDOWN, c->T, dd, 1, 0, 0, c->stream));
     HIP_TRY(hipMemcpyAsync(out, dout, outw * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MKHIP_OK;
@@ -1055,8 +1106,8 @@ extern "C" int mkhip_external_product(mkhip_ctx *c, const uint64_t *a_hoisted, c
     HIP_TRY(hipMemcpyAsync(dk, key, kw * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(de, &he, sizeof(he), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(dm, &hm, sizeof(hm), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(launch_kernel(4, c->T, de, 1, beta, rows, c->stream));
-    HIP_TRY(launch_kernel(5, c->T, dm, 1, 0, 0, c->stream));
+    HIP_TRY(launch_kernel(KEXT1, c->T, de, 1, beta, rows, c->stream));
+    HIP_TRY(launch_kernel(KMODDOWN, c->T, dm, 1, 0, 0, c->stream));
     HIP_TRY(hipMemcpyAsync(out, dout, outw * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MKHIP_OK;

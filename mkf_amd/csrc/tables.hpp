@@ -36,6 +36,20 @@ __device__ __forceinline__ int dc_base(const DevTables &T, int digit, int nd) {
     return digit * (kMaxAlpha + 1) + nd;
 }
 
+// ---- kernels ---------------------------------------------------------------
+
+enum KernelId {
+    KNTT_FWD = 0,        // k_ntt_batch<LOGN, false>
+    KNTT_INV = 1,        // k_ntt_batch<LOGN, true>
+    KDECOMP_MAC = 2,     // k_decomp<LOGN, true>   decomposition + x/y accumulation
+    KTENSOR = 3,         // k_tensor<LOGN>
+    KEXT1 = 4,           // k_ext<LOGN, 1>
+    KMODDOWN = 5,        // k_moddown<false>
+    KDECOMP = 6,         // k_decomp<LOGN, false>  decomposition only
+    KEXT2 = 7,           // k_ext<LOGN, 2>         two keys sharing one decomposition
+    KMODDOWN_FINAL = 8,  // k_moddown<true>        + accumulation into the output + rescale
+};
+
 // ---- descriptors -----------------------------------------------------------
 
 // One hoisted gadget decomposition side: k component polys (coefficient domain,
