@@ -70,6 +70,28 @@ def pct(xs, p):
     return float(np.percentile(np.asarray(xs), p))
 
 
+def stage_alg_bytes(B, k, L, K, beta, N, W=8):
+    """Algorithmic bytes per launch of each plan stage (each datum counted once).
+
+    Per-unit figures: a limb transform (NTT / INTT) moves 2 N W bytes (read +
+    write); a key-switch MAC moves (digit words + key words + output words) W;
+    a ModDown reads (L + K) rows and writes L rows.  Keys shared by the batch
+    count once per launch."""
+    rows = L + K
+    limb = N * W
+    return [
+        B * 2 * k * beta * rows * 2 * limb,                                     # decompose: limb NTTs
+        (B * 2 * k * beta * rows + 2 * k * beta * rows + B * 2 * beta * rows) * limb,  # x/y MAC
+        B * 2 * (1 + k) * L * 2 * limb,                                         # tensor input NTTs
+        B * (1 + k) * L * 2 * limb,                                             # tensor + INTT
+        B * 2 * k * (2 * beta * rows + rows) * limb,                            # Ext(x|y): D + x|y + out
+        B * 2 * k * (rows + L) * limb,                                          # ModDown
+        B * k * beta * rows * 2 * limb,                                         # decompose t
+        (B * k * (beta * rows + 2 * rows) + (k + 1) * beta * rows) * limb,      # Ext(v,u): D + out, keys once
+        B * (2 * k * (rows + L) + (1 + k) * (L + L - 1)) * limb,                 # final ModDown + rescale
+    ]
+
+
 def cpu_baseline(logN, Q, P, k, keys, u, budget_s):
     """The CPU oracle (C restatement of the reference path, 1 thread) on a bounded sample."""
     import oracle as O
@@ -165,38 +187,27 @@ def main():
     ws_bytes = plan.workspace_bytes
 
     # ---- plain per-op path: one op per launch ------------------------------
-    pops, pbufs = make_ops(1)
-    pplan = mkf_amd.Plan(ctx, pops, use_graph=True)
-    for _ in range(3):
-        pplan.launch()
-    ctx.sync()
-    plat = []
-    t1 = time.perf_counter()
-    for _ in range(args.plain_ops):
-        s = time.perf_counter()
-        pplan.launch()
+    pbufs, pplan, plat, plain_ops = [], None, [float("nan")], float("nan")
+    if args.plain_ops > 0:
+        pops, pbufs = make_ops(1)
+        pplan = mkf_amd.Plan(ctx, pops, use_graph=True)
+        for _ in range(3):
+            pplan.launch()
         ctx.sync()
-        plat.append(time.perf_counter() - s)
-    plain_ops = args.plain_ops / (time.perf_counter() - t1)
+        plat = []
+        t1 = time.perf_counter()
+        for _ in range(args.plain_ops):
+            s = time.perf_counter()
+            pplan.launch()
+            ctx.sync()
+            plat.append(time.perf_counter() - s)
+        plain_ops = args.plain_ops / (time.perf_counter() - t1)
 
     # ---- roofline of the dominant kernel ------------------------------------
-    # per-unit algorithmic bytes: a limb transform moves 2 N W bytes (read + write);
-    # the decomposition stage performs (k0 + k1) * beta * rows limb NTTs per op.
     rows = L + K
-    ntt_units = {
-        0: B * 2 * k * beta * rows,       # decompose + x/y MAC
-        1: B * 2 * (1 + k) * L,           # tensor input NTTs
-        2: B * (1 + k) * L,               # tensor INTTs
-        3: B * 2 * k * rows,              # Ext INTTs (MAC over beta digits read)
-        5: B * k * beta * rows,           # decompose t
-        6: B * 2 * k * rows,              # Ext(v), Ext(u) INTTs
-    }
-    dom = max(range(8), key=lambda s: stage_ms[s])
-    per_unit = 2 * N * 8
-    if dom in ntt_units:
-        alg_bytes = ntt_units[dom] * per_unit
-    else:  # ModDown: reads (level+1+K) rows, writes level+1 rows per accumulator
-        alg_bytes = B * 2 * k * (rows + L) * N * 8
+    alg = stage_alg_bytes(B, k, L, K, beta, N)
+    dom = max(range(len(stage_ms)), key=lambda s: stage_ms[s])
+    alg_bytes = alg[dom]
     achieved = alg_bytes / (stage_ms[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -227,14 +238,15 @@ def main():
         "plain": {"ops_per_s": round(plain_ops, 2), "p50_ms": round(pct(plat, 50) * 1e3, 3),
                   "p99_ms": round(pct(plat, 99) * 1e3, 3)},
         "batched_over_plain": round(batched_ops / plain_ops, 3),
-        "stages_ms": {mkf_amd.STAGES[s]: round(stage_ms[s], 4) for s in range(8)},
+        "stages_ms": {mkf_amd.STAGES[s]: round(stage_ms[s], 4) for s in range(len(stage_ms))},
         "workspace_gb": round(ws_bytes / 1e9, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     plan.close()
-    pplan.close()
+    if pplan is not None:
+        pplan.close()
     for p in bufs + pbufs:
         ctx.free(p)
     ctx.close()

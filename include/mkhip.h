@@ -125,11 +125,11 @@ int mkhip_plan_time(mkhip_plan *plan, int iters, double *ms_per_launch);
 void mkhip_plan_destroy(mkhip_plan *plan);
 size_t mkhip_plan_workspace_bytes(const mkhip_plan *plan);
 /* per-stage device time (ms, averaged over `iters` eager launches, HIP events on
- * the context stream) of the eight launches: decompose+x/y, tensor NTT, tensor,
- * Ext(x/y), ModDown, decompose(t), Ext(v,u), final ModDown+rescale */
-int mkhip_plan_stage_times(mkhip_plan *plan, int iters, double *ms8);
-/* workgroups per stage (limb transforms / 256-coefficient blocks) */
-int mkhip_plan_stage_units(const mkhip_plan *plan, long long *units8);
+ * the context stream) of the MKHIP_PLAN_STAGES launches: decompose, x/y MAC,
+ * tensor NTT, tensor, Ext(x/y), ModDown, decompose(t), Ext(v,u), final
+ * ModDown+rescale.  ms must hold MKHIP_PLAN_STAGES doubles. */
+#define MKHIP_PLAN_STAGES 9
+int mkhip_plan_stage_times(mkhip_plan *plan, int iters, double *ms);
 
 /* reference scale loop (mkckks/evaluator.go:383-388): number of rescales */
 int mkhip_rescale_count(const mkhip_ctx *ctx, int level, double scale, double min_scale, int *nres,

@@ -32,12 +32,14 @@ EXPORTS = [
     "mkhip_dev_free", "mkhip_h2d", "mkhip_d2h", "mkhip_sync", "mkhip_plan_kklss_ckks", "mkhip_plan_launch",
     "mkhip_plan_time", "mkhip_plan_destroy", "mkhip_plan_workspace_bytes", "mkhip_rescale_count",
     "mkhip_ntt_limbs", "mkhip_decompose", "mkhip_moddown", "mkhip_external_product",
-    "mkhip_plan_stage_times", "mkhip_plan_stage_units",
+    "mkhip_plan_stage_times",
 ]
 
-STAGES = ["decompose_xy", "tensor_ntt", "tensor", "ext_xy", "moddown_xy", "decompose_t", "ext_vu",
+# launches of a KKLSS MK-CKKS plan, in order, and the kernel each one runs
+STAGES = ["decompose", "mac_xy", "tensor_ntt", "tensor", "ext_xy", "moddown_xy", "decompose_t", "ext_vu",
           "moddown_rescale"]
-STAGE_KERNELS = ["k_decomp", "k_ntt_batch", "k_tensor", "k_ext", "k_moddown", "k_decomp", "k_ext", "k_moddown"]
+STAGE_KERNELS = ["k_decomp", "k_mac_xy", "k_ntt_batch", "k_tensor", "k_ext<1>", "k_moddown<false>",
+                 "k_decomp", "k_ext<2>", "k_moddown<true>"]
 
 
 class MkhipError(RuntimeError):
@@ -100,7 +102,6 @@ def _declare(L):
     L.mkhip_moddown.argtypes = [V, u64p, I, u64p]
     L.mkhip_external_product.argtypes = [V, u64p, u64p, I, u64p]
     L.mkhip_plan_stage_times.argtypes = [V, I, dblp]
-    L.mkhip_plan_stage_units.argtypes = [V, ctypes.POINTER(ctypes.c_longlong)]
 
 
 def _p(a):
@@ -277,14 +278,9 @@ class Plan:
         return ms.value
 
     def stage_times(self, iters):
-        ms = (ctypes.c_double * 8)()
+        ms = (ctypes.c_double * len(STAGES))()
         check(lib().mkhip_plan_stage_times(self._h, iters, ms), "mkhip_plan_stage_times")
         return list(ms)
-
-    def stage_units(self):
-        u = (ctypes.c_longlong * 8)()
-        check(lib().mkhip_plan_stage_units(self._h, u), "mkhip_plan_stage_units")
-        return list(u)
 
     @property
     def workspace_bytes(self):

@@ -105,60 +105,64 @@ __device__ __forceinline__ u64 digit_modup(const DevTables &T, const u64 *__rest
     return reduce64(r, m.q, m.mu);
 }
 
-// blockIdx.x = (desc * beta + digit) * rows + row.  With MAC the x/y accumulator
-// row lives in the output buffer (read-modify-write by the owning thread), so the
-// NTT keeps the whole register budget.
-template <int LOGN, bool MAC, bool CRT>
+// One workgroup per (desc, component p, digit i, row r): logical block
+// ((desc * kmax + p) * beta + i) * rows + r, XCD-remapped so the rows of one
+// (component, digit) -- which all read the same input limbs -- share an L2.
+template <int LOGN, bool CRT>
 __global__ void __launch_bounds__(1 << (LOGN - 4))
-    k_decomp(DevTables T, const DecompDesc *__restrict__ descs, int beta, int rows) {
+    k_decomp(DevTables T, const DecompDesc *__restrict__ descs, int kmax, int beta, int rows) {
     constexpr int N = 1 << LOGN;
     constexpr int TH = N / 16;
     const int tid = threadIdx.x;
-    const int bid = blockIdx.x;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int r = bid % rows;
     const int i = (bid / rows) % beta;
-    const int d = bid / (rows * beta);
+    const int p = (bid / (rows * beta)) % kmax;
+    const int d = bid / (rows * beta * kmax);
     const DecompDesc &Dd = descs[d];
-    const int level = Dd.level, k = Dd.k;
+    if (p >= Dd.k) return;
+    const int level = Dd.level;
     const int ri = ring_index(r, level, T.L);
     const ModConst m = T.mod[ri];
     const int alpha = T.alpha;
     const int start = i * alpha;
     const int nd = (level > alpha * (i + 1) - 1) ? alpha : (level % alpha) + 1;
-    const int ebase = i * (kMaxAlpha + 1) + nd;
     const bool own_row = (r <= level) && (r >= start) && (r < start + nd);
-    const ulonglong2 *tw = T.tw + (size_t)ri * N;
-    const size_t kstride = (size_t)(T.L + T.K) * N;
-    u64 *xr = MAC ? Dd.xy + ((size_t)i * rows + r) * N : nullptr;
-    if (MAC && k == 0) {  // empty party set: the accumulator is the zero polynomial
+    const u64 *src = Dd.comp[p];
+    u64 v[16];
+    if (!CRT || nd == 1 || own_row) {
+        // copy path / the digit's own rows: the residue itself (reduced mod m)
+        load_row_cached<LOGN>(v, src + (size_t)(own_row ? r : start) * N, tid);
 #pragma unroll
-        for (int e = 0; e < 16; ++e) xr[tid + e * TH] = 0;
+        for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
+    } else {
+        const int ebase = i * (kMaxAlpha + 1) + nd;
+        for (int e = 0; e < 16; ++e) v[e] = digit_modup(T, src, tid + e * TH, start, nd, ebase, ri, m);
     }
-    for (int p = 0; p < k; ++p) {
-        const u64 *src = Dd.comp[p];
-        u64 v[16];
-        if (!CRT || nd == 1 || own_row) {
-            // copy path / the digit's own rows: the residue itself (reduced mod m)
-            const u64 *row = src + (size_t)(own_row ? r : start) * N;
-            load_row_cached<LOGN>(v, row, tid);
-#pragma unroll
-            for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
-        } else {
-            for (int e = 0; e < 16; ++e) v[e] = digit_modup(T, src, tid + e * TH, start, nd, ebase, ri, m);
-        }
-        ntt_fwd<LOGN>(v, g_lds, tid, tw, m.q);
-        store_row<LOGN>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N, tid);
-        if (MAC) {
-            // x += MRed(key, D) (MulCoeffsMontgomeryAndAddLvl), MForm after the last party
-            const u64 *kr = Dd.key[p] + (size_t)i * kstride + (size_t)ri * N;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int c = tid + e * TH;
-                u64 t = mred(kr[c], v[e], m.q, m.qinv);
-                if (p > 0) t = add_mod(t, xr[c], m.q);
-                xr[c] = (p == k - 1) ? mred(t, m.r2, m.q, m.qinv) : t;
-            }
-        }
+    ntt_fwd<LOGN>(v, g_lds, tid, T.tw + (size_t)ri * N, m.q);
+    store_row<LOGN>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N, tid);
+}
+
+// Key-stationary x / y accumulation: grid (N/256, beta*rows, 2 sides); the
+// thread of coefficient c of (digit i, row r) walks all nops ops of the batch.
+__global__ void __launch_bounds__(256)
+    k_mac_xy(DevTables T, const MacDesc *__restrict__ descs, int nops, int beta, int rows) {
+    const int N = T.N;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const int i = blockIdx.y / rows, r = blockIdx.y % rows;
+    const int side = blockIdx.z;
+    const size_t kstride = (size_t)(T.L + T.K) * N;
+    for (int o = 0; o < nops; ++o) {
+        const MacDesc &Md = descs[2 * o + side];
+        const int ri = ring_index(r, Md.level, T.L);
+        const ModConst m = T.mod[ri];
+        const size_t koff = (size_t)i * kstride + (size_t)ri * N + c;
+        const size_t doff = ((size_t)i * rows + r) * N + c;
+        const size_t dstride = (size_t)beta * rows * N;
+        u64 acc = 0;
+        for (int p = 0; p < Md.k; ++p)
+            acc = add_mod(acc, mred(Md.key[p][koff], Md.D[p * dstride + doff], m.q, m.qinv), m.q);
+        Md.xy[doff] = mred(acc, m.r2, m.q, m.qinv);
     }
 }
 
@@ -174,8 +178,9 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_tensor(DevTables T, const T
     constexpr int N = 1 << LOGN;
     constexpr int TH = N / 16;
     const int tid = threadIdx.x;
-    const int l = blockIdx.x % nl;
-    const TensorDesc &Dd = descs[blockIdx.x / nl];
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int l = bid % nl;
+    const TensorDesc &Dd = descs[bid / nl];
     const ModConst m = T.mod[l];
     const size_t off = (size_t)l * N;
     u64 v[16];
@@ -200,33 +205,43 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_tensor(DevTables T, const T
 // external product MAC + INTT ; blockIdx.x = desc * rows + row
 
 // NKEY keys share one hoisted decomposition D (the nested products with v_i and
-// u, keyswitch_hoisted.go:177-193); each key's MAC is a separate pass over D.
+// u, keyswitch_hoisted.go:177-193): one pass over D feeds both accumulators; the
+// second is parked in its output row while the first is transformed.
 template <int LOGN, int NKEY>
 __global__ void __launch_bounds__(1 << (LOGN - 4))
     k_ext(DevTables T, const ExtDesc *__restrict__ descs, int beta, int rows) {
     constexpr int N = 1 << LOGN;
     constexpr int TH = N / 16;
     const int tid = threadIdx.x;
-    const int r = blockIdx.x % rows;
-    const ExtDesc &Dd = descs[blockIdx.x / rows];
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int r = bid % rows;
+    const ExtDesc &Dd = descs[bid / rows];
     const int ri = ring_index(r, Dd.level, T.L);
     const ModConst m = T.mod[ri];
     const size_t kstride = (size_t)(T.L + T.K) * N;
     const ulonglong2 *itw = T.itw + (size_t)ri * N;
+    u64 a0[16], a1[16];
 #pragma unroll
-    for (int o = 0; o < NKEY; ++o) {
-        u64 a[16];
+    for (int e = 0; e < 16; ++e) a0[e] = a1[e] = 0;
+    for (int i = 0; i < beta; ++i) {
+        const u64 *dr = Dd.D + ((size_t)i * rows + r) * N;
+        const u64 *k0 = Dd.key[0] + (size_t)i * kstride + (size_t)ri * N;
+        const u64 *k1 = NKEY > 1 ? Dd.key[1] + (size_t)i * kstride + (size_t)ri * N : nullptr;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) a[e] = 0;
-        for (int i = 0; i < beta; ++i) {
-            const u64 *dr = Dd.D + ((size_t)i * rows + r) * N;
-            const u64 *kr = Dd.key[o] + (size_t)i * kstride + (size_t)ri * N;
-#pragma unroll
-            for (int e = 0; e < 16; ++e)
-                a[e] = add_mod(a[e], mred(kr[tid + e * TH], dr[tid + e * TH], m.q, m.qinv), m.q);
+        for (int e = 0; e < 16; ++e) {
+            const u64 dv = dr[tid + e * TH];
+            a0[e] = add_mod(a0[e], mred(k0[tid + e * TH], dv, m.q, m.qinv), m.q);
+            if (NKEY > 1) a1[e] = add_mod(a1[e], mred(k1[tid + e * TH], dv, m.q, m.qinv), m.q);
         }
-        ntt_inv<LOGN>(a, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
-        store_row<LOGN>(a, Dd.acc[o] + (size_t)r * N, tid);
+    }
+    u64 *o1 = NKEY > 1 ? Dd.acc[1] + (size_t)r * N : nullptr;
+    if (NKEY > 1) store_row<LOGN>(a1, o1, tid);
+    ntt_inv<LOGN>(a0, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
+    store_row<LOGN>(a0, Dd.acc[0] + (size_t)r * N, tid);
+    if (NKEY > 1) {
+        load_row_cached<LOGN>(a1, o1, tid);
+        ntt_inv<LOGN>(a1, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
+        store_row<LOGN>(a1, o1, tid);
     }
 }
 
@@ -321,7 +336,7 @@ static hipError_t allow_lds(F kernel, size_t bytes) {
 }
 
 template <int LOGN>
-static hipError_t launch_all(int what, const DevTables &T, const void *descs, int n, int a, int b,
+static hipError_t launch_all(int what, const DevTables &T, const void *descs, int n, int a, int b, int c,
                              hipStream_t st) {
     constexpr int N = 1 << LOGN;
     constexpr size_t lds = (size_t)N * sizeof(u64);
@@ -330,10 +345,8 @@ static hipError_t launch_all(int what, const DevTables &T, const void *descs, in
         hipError_t e;
         if ((e = allow_lds(k_ntt_batch<LOGN, false>, lds)) != hipSuccess) return e;
         if ((e = allow_lds(k_ntt_batch<LOGN, true>, lds)) != hipSuccess) return e;
-        if ((e = allow_lds(k_decomp<LOGN, true, false>, lds)) != hipSuccess) return e;
-        if ((e = allow_lds(k_decomp<LOGN, false, false>, lds)) != hipSuccess) return e;
-        if ((e = allow_lds(k_decomp<LOGN, true, true>, lds)) != hipSuccess) return e;
-        if ((e = allow_lds(k_decomp<LOGN, false, true>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_decomp<LOGN, false>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds(k_decomp<LOGN, true>, lds)) != hipSuccess) return e;
         if ((e = allow_lds(k_tensor<LOGN>, lds)) != hipSuccess) return e;
         if ((e = allow_lds(k_ext<LOGN, 1>, lds)) != hipSuccess) return e;
         if ((e = allow_lds(k_ext<LOGN, 2>, lds)) != hipSuccess) return e;
@@ -347,21 +360,13 @@ static hipError_t launch_all(int what, const DevTables &T, const void *descs, in
     case KNTT_INV:
         hipLaunchKernelGGL((k_ntt_batch<LOGN, true>), dim3(n), block, lds, st, T, (const NttJob *)descs);
         break;
-    case KDECOMP_MAC:
+    case KDECOMP:  // n descs, a = beta, b = rows, c = kmax
         if (T.alpha > 1)
-            hipLaunchKernelGGL((k_decomp<LOGN, true, true>), dim3(n * a * b), block, lds, st, T,
-                               (const DecompDesc *)descs, a, b);
+            hipLaunchKernelGGL((k_decomp<LOGN, true>), dim3(n * c * a * b), block, lds, st, T,
+                               (const DecompDesc *)descs, c, a, b);
         else
-            hipLaunchKernelGGL((k_decomp<LOGN, true, false>), dim3(n * a * b), block, lds, st, T,
-                               (const DecompDesc *)descs, a, b);
-        break;
-    case KDECOMP:
-        if (T.alpha > 1)
-            hipLaunchKernelGGL((k_decomp<LOGN, false, true>), dim3(n * a * b), block, lds, st, T,
-                               (const DecompDesc *)descs, a, b);
-        else
-            hipLaunchKernelGGL((k_decomp<LOGN, false, false>), dim3(n * a * b), block, lds, st, T,
-                               (const DecompDesc *)descs, a, b);
+            hipLaunchKernelGGL((k_decomp<LOGN, false>), dim3(n * c * a * b), block, lds, st, T,
+                               (const DecompDesc *)descs, c, a, b);
         break;
     case KTENSOR:
         hipLaunchKernelGGL((k_tensor<LOGN>), dim3(n * a), block, lds, st, T, (const TensorDesc *)descs, a);
@@ -378,10 +383,17 @@ static hipError_t launch_all(int what, const DevTables &T, const void *descs, in
     return hipGetLastError();
 }
 
-// what: KernelId (tables.hpp).  NTT: n jobs; decomp: n descs x a (beta) x b (rows);
-// tensor: n descs x a (limbs); ext: n descs x b (rows), a = beta; moddown: n descs.
-hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st) {
+// what: KernelId (tables.hpp).  NTT: n jobs; decomp: n descs x c (kmax) x a (beta) x
+// b (rows); tensor: n descs x a (limbs); ext: n descs x b (rows), a = beta; moddown:
+// n descs; mac_xy: n ops (2 descs each), a = beta, b = rows.
+hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st,
+                         int c) {
     if (n <= 0) return hipSuccess;
+    if (what == KMAC_XY) {
+        hipLaunchKernelGGL(k_mac_xy, dim3(T.N / 256, a * b, 2), dim3(256), 0, st, T, (const MacDesc *)descs, n,
+                           a, b);
+        return hipGetLastError();
+    }
     if (what == KMODDOWN || what == KMODDOWN_FINAL) {
         if (what == KMODDOWN)
             hipLaunchKernelGGL((k_moddown<false>), dim3(T.N / 256, n), dim3(256), 0, st, T,
@@ -393,9 +405,9 @@ hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n,
     }
     switch (T.logN) {
     case 13:
-        return launch_all<13>(what, T, descs, n, a, b, st);
+        return launch_all<13>(what, T, descs, n, a, b, c, st);
     case 14:
-        return launch_all<14>(what, T, descs, n, a, b, st);
+        return launch_all<14>(what, T, descs, n, a, b, c, st);
     default:
         return hipErrorInvalidValue;
     }

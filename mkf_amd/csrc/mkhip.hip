@@ -26,7 +26,8 @@
 #include <vector>
 
 namespace mkhip {
-hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st);
+hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st,
+                         int c = 1);
 }
 
 using namespace mkhip;
@@ -172,9 +173,10 @@ struct mkhip_plan {
     size_t bytes = 0;
     int beta = 0, rows = 0, nl = 0;
     // stage descriptor offsets (device pointers) and counts
-    const void *dec1 = nullptr, *ntt = nullptr, *ten = nullptr, *ext1 = nullptr, *md1 = nullptr,
+    const void *dec1 = nullptr, *mac = nullptr, *ntt = nullptr, *ten = nullptr, *ext1 = nullptr, *md1 = nullptr,
                *dec2 = nullptr, *ext2 = nullptr, *md2 = nullptr;
-    int n_dec1 = 0, n_ntt = 0, n_ten = 0, n_ext1 = 0, n_md1 = 0, n_dec2 = 0, n_ext2 = 0, n_md2 = 0;
+    int n_dec1 = 0, n_mac = 0, n_ntt = 0, n_ten = 0, n_ext1 = 0, n_md1 = 0, n_dec2 = 0, n_ext2 = 0, n_md2 = 0;
+    int kmax1 = 1, kmax2 = 1;  // largest party count of the decompositions of stage 0 / 6
     bool graph = false;
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
@@ -506,20 +508,21 @@ hipError_t plan_stage(mkhip_plan *pl, int s) {
     const DevTables &T = c->T;
     hipStream_t st = c->stream;
     switch (s) {
-    case 0: return launch_kernel(KDECOMP_MAC, T, pl->dec1, pl->n_dec1, pl->beta, pl->rows, st);
-    case 1: return launch_kernel(KNTT_FWD, T, pl->ntt, pl->n_ntt, 0, 0, st);
-    case 2: return launch_kernel(KTENSOR, T, pl->ten, pl->n_ten, pl->nl, 0, st);
-    case 3: return launch_kernel(KEXT1, T, pl->ext1, pl->n_ext1, pl->beta, pl->rows, st);
-    case 4: return launch_kernel(KMODDOWN, T, pl->md1, pl->n_md1, 0, 0, st);
-    case 5: return launch_kernel(KDECOMP, T, pl->dec2, pl->n_dec2, pl->beta, pl->rows, st);
-    case 6: return launch_kernel(KEXT2, T, pl->ext2, pl->n_ext2, pl->beta, pl->rows, st);
-    case 7: return launch_kernel(KMODDOWN_FINAL, T, pl->md2, pl->n_md2, 0, 0, st);
+    case 0: return launch_kernel(KDECOMP, T, pl->dec1, pl->n_dec1, pl->beta, pl->rows, st, pl->kmax1);
+    case 1: return launch_kernel(KMAC_XY, T, pl->mac, pl->n_mac, pl->beta, pl->rows, st);
+    case 2: return launch_kernel(KNTT_FWD, T, pl->ntt, pl->n_ntt, 0, 0, st);
+    case 3: return launch_kernel(KTENSOR, T, pl->ten, pl->n_ten, pl->nl, 0, st);
+    case 4: return launch_kernel(KEXT1, T, pl->ext1, pl->n_ext1, pl->beta, pl->rows, st);
+    case 5: return launch_kernel(KMODDOWN, T, pl->md1, pl->n_md1, 0, 0, st);
+    case 6: return launch_kernel(KDECOMP, T, pl->dec2, pl->n_dec2, pl->beta, pl->rows, st, pl->kmax2);
+    case 7: return launch_kernel(KEXT2, T, pl->ext2, pl->n_ext2, pl->beta, pl->rows, st);
+    case 8: return launch_kernel(KMODDOWN_FINAL, T, pl->md2, pl->n_md2, 0, 0, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 int plan_launch_raw(mkhip_plan *pl) {
-    for (int s = 0; s < 8; ++s) HIP_TRY(plan_stage(pl, s));
+    for (int s = 0; s < MKHIP_PLAN_STAGES; ++s) HIP_TRY(plan_stage(pl, s));
     return MKHIP_OK;
 }
 
@@ -552,6 +555,7 @@ extern "C" int mkhip_plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_o
     // ---- sizes ----
     Carve cv;
     std::vector<DecompDesc> dec1, dec2;
+    std::vector<MacDesc> mac;
     std::vector<NttJob> ntt;
     std::vector<TensorDesc> ten;
     std::vector<ExtDesc> ext1, ext2;
@@ -601,6 +605,7 @@ extern "C" int mkhip_plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_o
     Carve dv;
     dv.off = ws_bytes;
     const size_t o_dec1 = dv.take(sizeof(DecompDesc) * 2 * nops);
+    const size_t o_mac = dv.take(sizeof(MacDesc) * 2 * nops);
     const size_t o_ntt = dv.take(sizeof(NttJob) * n_ntt);
     const size_t o_ten = dv.take(sizeof(TensorDesc) * n_ten);
     const size_t o_ext1 = dv.take(sizeof(ExtDesc) * std::max(n_ext1, 1));
@@ -623,26 +628,37 @@ extern "C" int mkhip_plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_o
     for (int o = 0; o < nops; ++o) {
         const mkhip_kklss_op &op = ops[o];
         const OpWs &w = ws[o];
-        // 1: decompositions of both sides with the x / y accumulations
+        // 1: decompositions of both sides; x / y accumulations (key-stationary)
         DecompDesc s0 = {}, s1 = {};
+        MacDesc m0 = {}, m1 = {};
         for (int p = 0; p < op.k0; ++p) {
             s0.comp[p] = op.op0 + (size_t)(1 + p) * nl * N;
-            s0.key[p] = c->parties[op.ids0[p]].d;
+            m0.key[p] = c->parties[op.ids0[p]].d;
         }
         s0.D = P(w.D0);
-        s0.xy = P(w.x);
         s0.k = op.k0;
         s0.level = level;
+        m0.D = P(w.D0);
+        m0.xy = P(w.x);
+        m0.k = op.k0;
+        m0.level = level;
         for (int p = 0; p < op.k1; ++p) {
             s1.comp[p] = op.op1 + (size_t)(1 + p) * nl * N;
-            s1.key[p] = c->parties[op.ids1[p]].b;
+            m1.key[p] = c->parties[op.ids1[p]].b;
         }
         s1.D = P(w.D1);
-        s1.xy = P(w.y);
         s1.k = op.k1;
         s1.level = level;
+        m1.D = P(w.D1);
+        m1.xy = P(w.y);
+        m1.k = op.k1;
+        m1.level = level;
         dec1.push_back(s0);
         dec1.push_back(s1);
+        mac.push_back(m0);
+        mac.push_back(m1);
+        pl->kmax1 = std::max(pl->kmax1, std::max(op.k0, op.k1));
+        pl->kmax2 = std::max(pl->kmax2, op.k0);
         // 2: NTT of every input component: op0 c0, op0 comps, op1 c0, op1 comps
         u64 *tnt = P(w.tnt);
         const int nc0 = 1 + op.k0, nc1 = 1 + op.k1;
@@ -767,6 +783,7 @@ extern "C" int mkhip_plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_o
         if (bytes) std::memcpy(hd.data() + (off - ws_bytes), src, bytes);
     };
     put(o_dec1, dec1.data(), dec1.size() * sizeof(DecompDesc));
+    put(o_mac, mac.data(), mac.size() * sizeof(MacDesc));
     put(o_ntt, ntt.data(), ntt.size() * sizeof(NttJob));
     put(o_ten, ten.data(), ten.size() * sizeof(TensorDesc));
     put(o_ext1, ext1.data(), ext1.size() * sizeof(ExtDesc));
@@ -784,6 +801,8 @@ extern "C" int mkhip_plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_o
     pl->nl = nl;
     pl->dec1 = B + o_dec1;
     pl->n_dec1 = (int)dec1.size();
+    pl->mac = B + o_mac;
+    pl->n_mac = nops;
     pl->ntt = B + o_ntt;
     pl->n_ntt = (int)ntt.size();
     pl->ten = B + o_ten;
@@ -851,46 +870,30 @@ extern "C" int mkhip_plan_time(mkhip_plan *pl, int iters, double *ms) {
 
 extern "C" size_t mkhip_plan_workspace_bytes(const mkhip_plan *pl) { return pl ? pl->bytes : 0; }
 
-// Average device time of each of the plan's eight launches (HIP events on the
-// context stream around each kernel, eager launches), in ms.
-extern "C" int mkhip_plan_stage_times(mkhip_plan *pl, int iters, double *ms8) {
-    if (!pl || iters <= 0 || !ms8) return MKHIP_ERR_ARG;
+// Average device time of each of the plan's launches (HIP events on the context
+// stream around each kernel, eager launches), in ms.
+extern "C" int mkhip_plan_stage_times(mkhip_plan *pl, int iters, double *ms) {
+    if (!pl || iters <= 0 || !ms) return MKHIP_ERR_ARG;
+    constexpr int S = MKHIP_PLAN_STAGES;
     hipStream_t st = pl->ctx->stream;
-    hipEvent_t ev[9];
-    for (int i = 0; i < 9; ++i) HIP_TRY(hipEventCreate(&ev[i]));
-    double acc[8] = {0};
+    hipEvent_t ev[S + 1];
+    for (int i = 0; i <= S; ++i) HIP_TRY(hipEventCreate(&ev[i]));
+    double acc[S] = {0};
     for (int it = 0; it < iters; ++it) {
         HIP_TRY(hipEventRecord(ev[0], st));
-        for (int s = 0; s < 8; ++s) {
+        for (int s = 0; s < S; ++s) {
             HIP_TRY(plan_stage(pl, s));
             HIP_TRY(hipEventRecord(ev[s + 1], st));
         }
-        HIP_TRY(hipEventSynchronize(ev[8]));
-        for (int s = 0; s < 8; ++s) {
+        HIP_TRY(hipEventSynchronize(ev[S]));
+        for (int s = 0; s < S; ++s) {
             float f = 0;
             HIP_TRY(hipEventElapsedTime(&f, ev[s], ev[s + 1]));
             acc[s] += f;
         }
     }
-    for (int i = 0; i < 9; ++i) (void)hipEventDestroy(ev[i]);
-    for (int s = 0; s < 8; ++s) ms8[s] = acc[s] / iters;
-    return MKHIP_OK;
-}
-
-// Launch-unit counts of the plan's stages (for the roofline denominators):
-// units[s] = workgroups of stage s (limb transforms for the NTT-shaped stages,
-// coefficient blocks of 256 for the ModDown stages).
-extern "C" int mkhip_plan_stage_units(const mkhip_plan *pl, long long *units8) {
-    if (!pl || !units8) return MKHIP_ERR_ARG;
-    const int br = pl->beta * pl->rows;
-    units8[0] = (long long)pl->n_dec1 * br;
-    units8[1] = pl->n_ntt;
-    units8[2] = (long long)pl->n_ten * pl->nl;
-    units8[3] = (long long)pl->n_ext1 * pl->rows;
-    units8[4] = (long long)pl->n_md1 * (pl->ctx->N / 256);
-    units8[5] = (long long)pl->n_dec2 * br;
-    units8[6] = (long long)pl->n_ext2 * pl->rows;
-    units8[7] = (long long)pl->n_md2 * (pl->ctx->N / 256);
+    for (int i = 0; i <= S; ++i) (void)hipEventDestroy(ev[i]);
+    for (int s = 0; s < S; ++s) ms[s] = acc[s] / iters;
     return MKHIP_OK;
 }
 

@@ -48,7 +48,19 @@ enum KernelId {
     KDECOMP = 6,         // k_decomp<LOGN, false>  decomposition only
     KEXT2 = 7,           // k_ext<LOGN, 2>         two keys sharing one decomposition
     KMODDOWN_FINAL = 8,  // k_moddown<true>        + accumulation into the output + rescale
+    KMAC_XY = 9,         // k_mac_xy               key-stationary x / y accumulation
 };
+
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8).  Remapping b to a logical index that gives every XCD one contiguous
+// range keeps workgroups that share data (an input limb decomposed into every
+// row, a key row used by every op) inside one XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    constexpr int kXcd = 8;
+    const int full = n - n % kXcd;
+    if (b >= full) return b;
+    return (b % kXcd) * (full / kXcd) + b / kXcd;
+}
 
 // ---- descriptors -----------------------------------------------------------
 
@@ -79,6 +91,17 @@ struct TensorDesc {
     u64 *out;
     int is_c0;
     int pad;
+};
+
+// Key-stationary x / y accumulation over a batch: for one side of every op,
+// xy[beta][rows] = MForm(sum_p key_p (.) D[p]) (keyswitch_hoisted.go:84-123).
+// One thread owns one coefficient of one (digit, row) and walks every op of the
+// batch, so each key word is fetched once per batch (L1/L2 reuse across ops).
+struct MacDesc {
+    const u64 *D;                   // [k][beta][rows][N]
+    u64 *xy;                        // [beta][rows][N]
+    const u64 *key[kMaxParties];    // [beta][L+K][N] per component (MForm)
+    int k, level;
 };
 
 // External-product MAC over beta digits followed by an INTT of every row:
