@@ -184,6 +184,7 @@ def main():
     batched_ops = B * args.steps / elapsed
     # per-stage device times (HIP events on the stream the kernels run on)
     stage_ms = plan.stage_times(args.stage_iters)
+    stage_names = plan.stages
     ws_bytes = plan.workspace_bytes
 
     # ---- plain per-op path: one op per launch ------------------------------
@@ -211,7 +212,7 @@ def main():
     achieved = alg_bytes / (stage_ms[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": mkf_amd.STAGE_KERNELS[dom], "stage": mkf_amd.STAGES[dom],
+                "kernel": mkf_amd.STAGE_KERNELS[stage_names[dom]], "stage": stage_names[dom],
                 "kernel_ms": round(stage_ms[dom], 4), "alg_bytes_per_launch": alg_bytes}
 
     cpu = None
@@ -238,7 +239,7 @@ def main():
         "plain": {"ops_per_s": round(plain_ops, 2), "p50_ms": round(pct(plat, 50) * 1e3, 3),
                   "p99_ms": round(pct(plat, 99) * 1e3, 3)},
         "batched_over_plain": round(batched_ops / plain_ops, 3),
-        "stages_ms": {mkf_amd.STAGES[s]: round(stage_ms[s], 4) for s in range(len(stage_ms))},
+        "stages_ms": {stage_names[s]: round(stage_ms[s], 4) for s in range(len(stage_ms))},
         "workspace_gb": round(ws_bytes / 1e9, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,

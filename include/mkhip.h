@@ -124,12 +124,39 @@ int mkhip_plan_launch(mkhip_plan *plan);
 int mkhip_plan_time(mkhip_plan *plan, int iters, double *ms_per_launch);
 void mkhip_plan_destroy(mkhip_plan *plan);
 size_t mkhip_plan_workspace_bytes(const mkhip_plan *plan);
-/* per-stage device time (ms, averaged over `iters` eager launches, HIP events on
- * the context stream) of the MKHIP_PLAN_STAGES launches: decompose, x/y MAC,
- * tensor NTT, tensor, Ext(x/y), ModDown, decompose(t), Ext(v,u), final
- * ModDown+rescale.  ms must hold MKHIP_PLAN_STAGES doubles. */
-#define MKHIP_PLAN_STAGES 9
+/* A plan is a fixed sequence of kernel launches ("stages").  stage_times gives
+ * the device time of each (ms, averaged over `iters` eager launches, HIP events
+ * on the context stream); ms must hold mkhip_plan_stage_count(plan) doubles. */
+int mkhip_plan_stage_count(const mkhip_plan *plan);
+const char *mkhip_plan_stage_name(const mkhip_plan *plan, int stage);
 int mkhip_plan_stage_times(mkhip_plan *plan, int iters, double *ms);
+
+/*
+ * MKF MK-CKKS (software/mkf/mkf_cpu/ckks.go:102-151, CKKSCase.Run): fold both
+ * operands to the aggregate secret (common.go:79-126), Karatsuba tensor, one
+ * relinearization, rescale, k-fold materialization of Gamma.  Needs a context
+ * with gamma = 1 (alpha = K, the Lattigo rlwe decomposition).
+ *
+ * A group is one party set with its fold keys s_i -> s_aggregate and the
+ * relinearization key of s_aggregate (common.go:32-51), each a Lattigo
+ * SwitchingKey laid out [beta][2][L+K][N] (mkf_cpu/fpga_fixture_test.go:56-69).
+ */
+int mkhip_register_mkf_group(mkhip_ctx *ctx, int group, int k, const uint64_t *const *fold_keys,
+                             const uint64_t *relin_key);
+int mkhip_unregister_mkf_group(mkhip_ctx *ctx, int group);
+/* host buffers: left/right [1+k][level+1][N] NTT domain; out [1+k][level_out+1][N]
+ * NTT domain = c0 followed by k copies of Gamma.  Rescale errors are returned. */
+int mkhip_mkf_ckks_mulrelin(mkhip_ctx *ctx, int group, const uint64_t *left, const uint64_t *right, int level,
+                            double scale_l, double scale_r, double min_scale, uint64_t *out, int *level_out,
+                            double *scale_out);
+
+typedef struct {
+    const uint64_t *left, *right; /* device, [1+k][level+1][N], NTT domain   */
+    uint64_t *out;                /* device, [1+k][level+1-nres][N]          */
+    int group, level, nres;       /* nres in {0, 1}                           */
+} mkhip_mkf_op;
+
+int mkhip_plan_mkf_ckks(mkhip_ctx *ctx, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out);
 
 /* reference scale loop (mkckks/evaluator.go:383-388): number of rescales */
 int mkhip_rescale_count(const mkhip_ctx *ctx, int level, double scale, double min_scale, int *nres,

@@ -32,14 +32,19 @@ EXPORTS = [
     "mkhip_dev_free", "mkhip_h2d", "mkhip_d2h", "mkhip_sync", "mkhip_plan_kklss_ckks", "mkhip_plan_launch",
     "mkhip_plan_time", "mkhip_plan_destroy", "mkhip_plan_workspace_bytes", "mkhip_rescale_count",
     "mkhip_ntt_limbs", "mkhip_decompose", "mkhip_moddown", "mkhip_external_product",
-    "mkhip_plan_stage_times",
+    "mkhip_plan_stage_times", "mkhip_plan_stage_count", "mkhip_plan_stage_name",
+    "mkhip_register_mkf_group", "mkhip_unregister_mkf_group", "mkhip_mkf_ckks_mulrelin", "mkhip_plan_mkf_ckks",
 ]
 
-# launches of a KKLSS MK-CKKS plan, in order, and the kernel each one runs
-STAGES = ["decompose", "mac_xy", "tensor_ntt", "tensor", "ext_xy", "moddown_xy", "decompose_t", "ext_vu",
-          "moddown_rescale"]
-STAGE_KERNELS = ["k_decomp", "k_mac_xy", "k_ntt_batch", "k_tensor", "k_ext<1>", "k_moddown<false>",
-                 "k_decomp", "k_ext<2>", "k_moddown<true>"]
+# kernel run by each named plan stage
+STAGE_KERNELS = {
+    "decompose": "k_decomp", "mac_xy": "k_mac_xy", "tensor_ntt": "k_ntt_batch<fwd>", "tensor": "k_tensor",
+    "ext_xy": "k_ext<1>", "moddown_xy": "k_moddown<false>", "decompose_t": "k_decomp", "ext_vu": "k_ext<2>",
+    "moddown_rescale": "k_moddown<true>", "intt_in": "k_ntt_batch<inv>", "mac_fold": "k_mac2",
+    "moddown_fold": "k_moddown_ntt", "karatsuba": "k_karatsuba", "intt_mu2": "k_ntt_batch<inv>",
+    "decompose_mu2": "k_decomp", "mac_relin": "k_mac2", "moddown_relin": "k_moddown_ntt",
+    "intt_last": "k_ntt_batch<inv>", "rescale": "k_rescale_ntt",
+}
 
 
 class MkhipError(RuntimeError):
@@ -54,6 +59,13 @@ class KKLSSOp(ctypes.Structure):
         ("op0", ctypes.c_void_p), ("op1", ctypes.c_void_p), ("out", ctypes.c_void_p),
         ("ids0", ctypes.c_int * MAX_PARTIES), ("ids1", ctypes.c_int * MAX_PARTIES),
         ("k0", ctypes.c_int), ("k1", ctypes.c_int), ("level", ctypes.c_int), ("nres", ctypes.c_int),
+    ]
+
+
+class MKFOp(ctypes.Structure):
+    _fields_ = [
+        ("left", ctypes.c_void_p), ("right", ctypes.c_void_p), ("out", ctypes.c_void_p),
+        ("group", ctypes.c_int), ("level", ctypes.c_int), ("nres", ctypes.c_int),
     ]
 
 
@@ -102,6 +114,13 @@ def _declare(L):
     L.mkhip_moddown.argtypes = [V, u64p, I, u64p]
     L.mkhip_external_product.argtypes = [V, u64p, u64p, I, u64p]
     L.mkhip_plan_stage_times.argtypes = [V, I, dblp]
+    L.mkhip_plan_stage_count.argtypes = [V]
+    L.mkhip_plan_stage_name.argtypes = [V, I]
+    L.mkhip_plan_stage_name.restype = ctypes.c_char_p
+    L.mkhip_register_mkf_group.argtypes = [V, I, I, ctypes.POINTER(u64p), u64p]
+    L.mkhip_unregister_mkf_group.argtypes = [V, I]
+    L.mkhip_mkf_ckks_mulrelin.argtypes = [V, I, u64p, u64p, I, D, D, D, u64p, intp, dblp]
+    L.mkhip_plan_mkf_ckks.argtypes = [V, I, ctypes.POINTER(MKFOp), I, ctypes.POINTER(V)]
 
 
 def _p(a):
@@ -198,6 +217,33 @@ class Context:
         out = out[:(1 + kuo.value) * nlo * self.N].reshape(1 + kuo.value, nlo, self.N)
         return out, list(iout)[:kuo.value], lo.value, so.value
 
+    # ---- MKF ----
+    def register_mkf_group(self, group, fold_keys, relin_key):
+        """fold_keys: k switching keys [beta][2][L+K][N]; relin_key likewise."""
+        keys = [_u64(f) for f in fold_keys]
+        relin = _u64(relin_key)
+        for x in keys + [relin]:
+            assert x.size == 2 * self.swk_words, (x.size, 2 * self.swk_words)
+        arr = (u64p * len(keys))(*[_p(f) for f in keys])
+        check(lib().mkhip_register_mkf_group(self._h, group, len(keys), arr, _p(relin)),
+              "mkhip_register_mkf_group")
+
+    def unregister_mkf_group(self, group):
+        check(lib().mkhip_unregister_mkf_group(self._h, group), "mkhip_unregister_mkf_group")
+
+    def mkf_ckks_mulrelin(self, group, left, right, scale_l=1.0, scale_r=1.0, min_scale=1.0):
+        """MKF CKKSCase.Run: left/right (1+k, level+1, N) NTT domain -> (out, level_out, scale_out)."""
+        left, right = _u64(left), _u64(right)
+        k = left.shape[0] - 1
+        level = left.shape[1] - 1
+        out = np.zeros((1 + k) * (level + 1) * self.N, np.uint64)
+        lo, so = ctypes.c_int(), ctypes.c_double()
+        check(lib().mkhip_mkf_ckks_mulrelin(self._h, group, _p(left), _p(right), level, scale_l, scale_r,
+                                            min_scale, _p(out), ctypes.byref(lo), ctypes.byref(so)),
+              "mkhip_mkf_ckks_mulrelin")
+        nlo = lo.value + 1
+        return out[:(1 + k) * nlo * self.N].reshape(1 + k, nlo, self.N), lo.value, so.value
+
     # ---- primitives ----
     def ntt(self, rows, ring_idx, inverse=False):
         rows = _u64(rows)
@@ -251,23 +297,43 @@ class Context:
 
 
 class Plan:
-    """A batched KKLSS MK-CKKS mul+relin over device-resident operands."""
+    """A batched mul+relin over device-resident operands (one hipGraph per launch).
 
-    def __init__(self, ctx, ops, use_graph=True):
-        arr = (KKLSSOp * len(ops))()
-        for i, o in enumerate(ops):
-            a = arr[i]
-            a.op0, a.op1, a.out = o["op0"], o["op1"], o["out"]
-            for j, v in enumerate(o["ids0"]):
-                a.ids0[j] = v
-            for j, v in enumerate(o["ids1"]):
-                a.ids1[j] = v
-            a.k0, a.k1, a.level, a.nres = len(o["ids0"]), len(o["ids1"]), o["level"], o["nres"]
+    scheme "kklss": ops are dicts {op0, op1, out, ids0, ids1, level, nres};
+    scheme "mkf":   ops are dicts {left, right, out, group, level, nres}."""
+
+    def __init__(self, ctx, ops, use_graph=True, scheme="kklss"):
         h = ctypes.c_void_p()
-        check(lib().mkhip_plan_kklss_ckks(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)),
-              "mkhip_plan_kklss_ckks")
+        if scheme == "kklss":
+            arr = (KKLSSOp * len(ops))()
+            for i, o in enumerate(ops):
+                a = arr[i]
+                a.op0, a.op1, a.out = o["op0"], o["op1"], o["out"]
+                for j, v in enumerate(o["ids0"]):
+                    a.ids0[j] = v
+                for j, v in enumerate(o["ids1"]):
+                    a.ids1[j] = v
+                a.k0, a.k1, a.level, a.nres = len(o["ids0"]), len(o["ids1"]), o["level"], o["nres"]
+            check(lib().mkhip_plan_kklss_ckks(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)),
+                  "mkhip_plan_kklss_ckks")
+        elif scheme == "mkf":
+            arr = (MKFOp * len(ops))()
+            for i, o in enumerate(ops):
+                a = arr[i]
+                a.left, a.right, a.out = o["left"], o["right"], o["out"]
+                a.group, a.level, a.nres = o["group"], o["level"], o["nres"]
+            check(lib().mkhip_plan_mkf_ckks(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)),
+                  "mkhip_plan_mkf_ckks")
+        else:
+            raise ValueError(scheme)
         self._h = h
         self.ctx = ctx
+        self.scheme = scheme
+
+    @property
+    def stages(self):
+        n = lib().mkhip_plan_stage_count(self._h)
+        return [lib().mkhip_plan_stage_name(self._h, i).decode() for i in range(n)]
 
     def launch(self):
         check(lib().mkhip_plan_launch(self._h), "mkhip_plan_launch")
@@ -278,7 +344,7 @@ class Plan:
         return ms.value
 
     def stage_times(self, iters):
-        ms = (ctypes.c_double * len(STAGES))()
+        ms = (ctypes.c_double * lib().mkhip_plan_stage_count(self._h))()
         check(lib().mkhip_plan_stage_times(self._h, iters, ms), "mkhip_plan_stage_times")
         return list(ms)
 

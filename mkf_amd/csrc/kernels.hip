@@ -13,49 +13,9 @@
 // Every kernel keeps one RNS limb of one polynomial per workgroup (N/16 threads,
 // 16 coefficients per thread) or one coefficient per thread, reads and writes
 // limb-major rows coalesced, and produces canonical residues.
-#include "ntt.hpp"
-#include "tables.hpp"
+#include "kcommon.hpp"
 
 namespace mkhip {
-
-extern __shared__ u64 g_lds[];
-
-template <int LOGN>
-__device__ __forceinline__ void load_row(u64 (&v)[16], const u64 *__restrict__ src, int tid) {
-    constexpr int T = (1 << LOGN) / 16;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = __builtin_nontemporal_load(src + tid + i * T);
-}
-
-template <int LOGN>
-__device__ __forceinline__ void load_row_cached(u64 (&v)[16], const u64 *__restrict__ src, int tid) {
-    constexpr int T = (1 << LOGN) / 16;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = src[tid + i * T];
-}
-
-template <int LOGN>
-__device__ __forceinline__ void store_row(const u64 (&v)[16], u64 *__restrict__ dst, int tid) {
-    constexpr int T = (1 << LOGN) / 16;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) dst[tid + i * T] = v[i];
-}
-
-__device__ __forceinline__ int ring_index(int r, int level, int L) { return r <= level ? r : L + (r - level - 1); }
-
-// 128-bit accumulate of a * b.
-__device__ __forceinline__ void mac128(u64 &hi, u64 &lo, u64 a, u64 b) {
-    u64 l = a * b;
-    u64 h = mulhi64(a, b);
-    lo += l;
-    hi += h + (lo < l ? 1 : 0);
-}
-
-// Lazy Montgomery reduction of a 128-bit accumulator as in multSum
-// (basis_extension.go:623-645): hi - hi(lo * pinv * p) + p.
-__device__ __forceinline__ u64 mont_reduce128(u64 hi, u64 lo, u64 p, u64 pinv) {
-    return hi - mulhi64(lo * pinv, p) + p;
-}
 
 // ---------------------------------------------------------------------------
 // batched NTT
@@ -169,9 +129,6 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // tensor product + INTT ; blockIdx.x = desc * nl + limb
 
-__device__ __forceinline__ u64 mulmod_m(u64 a, u64 b, const ModConst &m) {
-    return mred(mred(a, m.r2, m.q, m.qinv), b, m.q, m.qinv);
-}
 
 template <int LOGN>
 __global__ void __launch_bounds__(1 << (LOGN - 4)) k_tensor(DevTables T, const TensorDesc *__restrict__ descs, int nl) {
@@ -249,11 +206,10 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
 // ModDown (+ accumulate) (+ rescale) ; one coefficient per thread, grid (N/256, ndesc)
 
 constexpr int kMaxQ = 16;
-constexpr int kMaxK = 4;
 
 template <bool FINAL>
 __global__ void __launch_bounds__(256) k_moddown(DevTables T, const ModDownDesc *__restrict__ descs) {
-    const int N = T.N, K = T.K, L = T.L;
+    const int N = T.N, L = T.L;
     const int c = blockIdx.x * 256 + threadIdx.x;
     const ModDownDesc &Dd = descs[blockIdx.y];
     const int nl = Dd.level + 1;
@@ -262,27 +218,12 @@ __global__ void __launch_bounds__(256) k_moddown(DevTables T, const ModDownDesc 
     for (int j = 0; j < kMaxQ; ++j) res[j] = (Dd.accumulate && j < nl) ? Dd.dst[(size_t)j * N + c] : 0;
     for (int s = 0; s < Dd.nsrc; ++s) {
         const u64 *src = Dd.src[s];
-        u64 y[kMaxK];
-        double vi = 0.0;
-#pragma unroll
-        for (int k = 0; k < kMaxK; ++k) {
-            if (k < K) {
-                const ModConst mp = T.mod[L + k];
-                y[k] = mred(src[(size_t)(nl + k) * N + c], T.md_y[k], mp.q, mp.qinv);
-                vi += (double)y[k] / (double)mp.q;
-            }
-        }
-        const u64 v = (u64)vi;
+        const PUp up = modup_p_prepare(T, src, nl, c);
 #pragma unroll
         for (int j = 0; j < kMaxQ; ++j) {
             if (j < nl) {
                 const ModConst mq = T.mod[j];
-                u64 hi = 0, lo = 0;
-#pragma unroll
-                for (int k = 0; k < kMaxK; ++k)
-                    if (k < K) mac128(hi, lo, y[k], T.md_qstar[j * K + k]);
-                u64 z = mont_reduce128(hi, lo, mq.q, mq.qinv) + T.md_vt[j * (K + 1) + v];
-                z = reduce64(z, mq.q, mq.mu);
+                const u64 z = modup_p_to_row(T, up, j, mq);
                 const u64 x = src[(size_t)j * N + c];
                 const u64 md = mred(z + 2 * mq.q - x, T.md_pinv[j], mq.q, mq.qinv);
                 res[j] = add_mod(res[j], md, mq.q);

@@ -1,0 +1,196 @@
+// CDNA4 kernels of the MKF construction (software/mkf/mkf_cpu): aggregate-secret
+// fold by key switching, Karatsuba tensor, one relinearization, NTT-domain
+// rescale, k-fold materialization of Gamma.  The decomposition and the batched
+// NTT come from kernels.hip.
+//
+//   k_mac2          two-output switching-key MAC (KeyswitchHoistedNoModDown), P rows INTT'd
+//   k_moddown_ntt   ModDownQPtoQNTT of several switches, summed (fold)
+//   k_karatsuba     mu0 = l0 r0, mu2 = l1 r1, mu1 = (l0+l1)(r0+r1) - mu0 - mu2
+//   k_rescale_ntt   DivRoundByLastModulus in the NTT domain + copies
+#include "kcommon.hpp"
+
+namespace mkhip {
+
+// blockIdx.x = desc * rows + row (XCD-remapped)
+template <int LOGN>
+__global__ void __launch_bounds__(1 << (LOGN - 4))
+    k_mac2(DevTables T, const Mac2Desc *__restrict__ descs, int beta, int rows) {
+    constexpr int N = 1 << LOGN;
+    constexpr int TH = N / 16;
+    const int tid = threadIdx.x;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int r = bid % rows;
+    const Mac2Desc &Dd = descs[bid / rows];
+    const int ri = ring_index(r, Dd.level, T.L);
+    const ModConst m = T.mod[ri];
+    const size_t kstride = (size_t)(T.L + T.K) * N;
+    u64 a0[16], a1[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) a0[e] = a1[e] = 0;
+    for (int b = 0; b < beta; ++b) {
+        const u64 *dr = Dd.D + ((size_t)b * rows + r) * N;
+        const u64 *k0 = Dd.swk + (size_t)(2 * b) * kstride + (size_t)ri * N;
+        const u64 *k1 = k0 + kstride;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const u64 dv = dr[tid + e * TH];
+            a0[e] = add_mod(a0[e], mred(k0[tid + e * TH], dv, m.q, m.qinv), m.q);
+            a1[e] = add_mod(a1[e], mred(k1[tid + e * TH], dv, m.q, m.qinv), m.q);
+        }
+    }
+    u64 *o0 = Dd.acc[0] + (size_t)r * N, *o1 = Dd.acc[1] + (size_t)r * N;
+    if (r > Dd.level || Dd.intt_all) {
+        const ulonglong2 *itw = T.itw + (size_t)ri * N;
+        store_row<LOGN>(a1, o1, tid);
+        ntt_inv<LOGN>(a0, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
+        store_row<LOGN>(a0, o0, tid);
+        load_row_cached<LOGN>(a1, o1, tid);
+        ntt_inv<LOGN>(a1, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
+        store_row<LOGN>(a1, o1, tid);
+    } else {
+        store_row<LOGN>(a0, o0, tid);
+        store_row<LOGN>(a1, o1, tid);
+    }
+}
+
+// blockIdx.x = desc * (level+1) + Q row j
+template <int LOGN>
+__global__ void __launch_bounds__(1 << (LOGN - 4)) k_moddown_ntt(DevTables T, const MdNttDesc *__restrict__ descs, int nl) {
+    constexpr int N = 1 << LOGN;
+    constexpr int TH = N / 16;
+    const int tid = threadIdx.x;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = bid % nl;
+    const MdNttDesc &Dd = descs[bid / nl];
+    const ModConst m = T.mod[j];
+    const ulonglong2 *tw = T.tw + (size_t)j * N;
+    const u64 pinv = T.md_pinv[j];
+    u64 res[16];
+    if (Dd.base)
+        load_row_cached<LOGN>(res, Dd.base + (size_t)j * N, tid);
+    else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) res[e] = 0;
+    }
+    for (int a = 0; a < Dd.nacc; ++a) {
+        const u64 *acc = Dd.acc[a];
+        u64 z[16];
+        for (int e = 0; e < 16; ++e) {
+            const PUp up = modup_p_prepare(T, acc, nl, tid + e * TH);
+            z[e] = modup_p_to_row(T, up, j, m);
+        }
+        ntt_fwd<LOGN>(z, g_lds, tid, tw, m.q);
+        const u64 *xr = acc + (size_t)j * N;
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+            res[e] = add_mod(res[e], mred(z[e] + 2 * m.q - xr[tid + e * TH], pinv, m.q, m.qinv), m.q);
+    }
+    store_row<LOGN>(res, Dd.dst + (size_t)j * N, tid);
+}
+
+// grid (N/256, ndesc * nl), one coefficient per thread
+__global__ void __launch_bounds__(256) k_karatsuba(DevTables T, const KaraDesc *__restrict__ descs, int nl) {
+    const int N = T.N;
+    const int l = blockIdx.y % nl;
+    const KaraDesc &Dd = descs[blockIdx.y / nl];
+    const size_t c = (size_t)l * N + blockIdx.x * 256 + threadIdx.x;
+    const ModConst m = T.mod[l];
+    const u64 l0 = Dd.l0[c], l1 = Dd.l1[c], r0 = Dd.r0[c], r1 = Dd.r1[c];
+    const u64 mu0 = mulmod_m(l0, r0, m);
+    const u64 mu2 = mulmod_m(l1, r1, m);
+    u64 mu1 = mulmod_m(add_mod(l0, l1, m.q), add_mod(r0, r1, m.q), m);
+    mu1 = sub_mod(sub_mod(mu1, mu0, m.q), mu2, m.q);
+    Dd.mu0[c] = mu0;
+    Dd.mu1[c] = mu1;
+    Dd.mu2[c] = mu2;
+}
+
+// blockIdx.x = desc * nout_limbs + row j ; nout_limbs = level + 1 - nres
+template <int LOGN>
+__global__ void __launch_bounds__(1 << (LOGN - 4))
+    k_rescale_ntt(DevTables T, const RescaleNttDesc *__restrict__ descs, int nlo) {
+    constexpr int N = 1 << LOGN;
+    constexpr int TH = N / 16;
+    const int tid = threadIdx.x;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = bid % nlo;
+    const RescaleNttDesc &Dd = descs[bid / nlo];
+    const ModConst m = T.mod[j];
+    u64 v[16];
+    load_row_cached<LOGN>(v, Dd.x + (size_t)j * N, tid);
+    if (Dd.nres == 1) {
+        // out_j = (x_j - NTT_j(([x_L + h]_{q_L} - h) mod q_j)) q_L^-1, h = q_L >> 1
+        const int l = Dd.level;
+        const u64 qL = T.mod[l].q;
+        const u64 h = qL >> 1;
+        const u64 hj = T.rs_half[l * T.L + j];
+        u64 w[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const u64 t = add_mod(Dd.xlast[tid + e * TH], h, qL);
+            w[e] = sub_mod(reduce64(t, m.q, m.mu), hj, m.q);
+        }
+        ntt_fwd<LOGN>(w, g_lds, tid, T.tw + (size_t)j * N, m.q);
+        const u64 inv = T.rs_inv[l * T.L + j];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = mred(sub_mod(v[e], w[e], m.q), inv, m.q, m.qinv);
+    }
+    for (int o = 0; o < Dd.nout; ++o) store_row<LOGN>(v, Dd.out[o] + (size_t)j * N, tid);
+}
+
+// ---------------------------------------------------------------------------
+
+template <class F>
+static hipError_t allow_lds_mkf(F kernel, size_t bytes) {
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
+}
+
+template <int LOGN>
+static hipError_t launch_mkf_n(int what, const DevTables &T, const void *descs, int n, int a, int b,
+                               hipStream_t st) {
+    constexpr int N = 1 << LOGN;
+    constexpr size_t lds = (size_t)N * sizeof(u64);
+    static bool init = false;
+    if (!init) {
+        hipError_t e;
+        if ((e = allow_lds_mkf(k_mac2<LOGN>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds_mkf(k_moddown_ntt<LOGN>, lds)) != hipSuccess) return e;
+        if ((e = allow_lds_mkf(k_rescale_ntt<LOGN>, lds)) != hipSuccess) return e;
+        init = true;
+    }
+    const dim3 block(N / 16);
+    switch (what) {
+    case KMAC2:  // n descs, a = beta, b = rows
+        hipLaunchKernelGGL((k_mac2<LOGN>), dim3(n * b), block, lds, st, T, (const Mac2Desc *)descs, a, b);
+        break;
+    case KMD_NTT:  // n descs, a = level + 1
+        hipLaunchKernelGGL((k_moddown_ntt<LOGN>), dim3(n * a), block, lds, st, T, (const MdNttDesc *)descs, a);
+        break;
+    case KRESCALE_NTT:  // n descs, a = output limbs
+        hipLaunchKernelGGL((k_rescale_ntt<LOGN>), dim3(n * a), block, lds, st, T, (const RescaleNttDesc *)descs,
+                           a);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mkf(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (what == KKARA) {  // n descs, a = level + 1
+        hipLaunchKernelGGL(k_karatsuba, dim3(T.N / 256, n * a), dim3(256), 0, st, T, (const KaraDesc *)descs, a);
+        return hipGetLastError();
+    }
+    switch (T.logN) {
+    case 13:
+        return launch_mkf_n<13>(what, T, descs, n, a, b, st);
+    case 14:
+        return launch_mkf_n<14>(what, T, descs, n, a, b, st);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace mkhip

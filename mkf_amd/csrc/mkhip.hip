@@ -1,34 +1,12 @@
-// Host side of the C ABI (include/mkhip.h): parameter tables, resident keys,
-// KKLSS MK-CKKS mul+relin plans (descriptor tables + workspace + hipGraph).
-//
-// The orchestration follows mkrlwe.KeySwitcher.MulAndRelinHoisted
-// (software/kklss/mkrlwe/keyswitch_hoisted.go:49-194) preceded by the hoisted
-// decompositions of mkckks.Evaluator.MulRelinNew (mkckks/evaluator.go:419-446)
-// and followed by Evaluator.Rescale (:360-401), as eight batched launches:
-//
-//   1 k_decomp    D(op0_i), D(op1_j) for every party, x = MForm(sum d_i D0_i),
-//                 y = MForm(sum b_j D1_j)                                 (:84-123)
-//   2 k_ntt       NTT of every input component                           (:126-150)
-//   3 k_tensor    c0 c0', c0 c'_j + c_j c0', INTT                          (:126-153)
-//   4 k_ext       Ext(D1_j, x) and Ext(D0_i, y) MACs + INTT               (:155-170)
-//   5 k_moddown   ctOut_j += ModDown(.), t_i = ModDown(.)
-//   6 k_decomp    D(t_i)                                                  (:175)
-//   7 k_ext       Ext(D(t_i), v_i) and Ext(D(t_i), u) + INTT               (:177-193)
-//   8 k_moddown   ctOut_0 += sum_i ModDown(.), ctOut_i += ModDown(.), rescale
-#include "../../include/mkhip.h"
-#include "tables.hpp"
+// Host side of the C ABI (include/mkhip.h): parameter tables (moduli, twiddles,
+// basis-extension and rescale constants), resident keys, device buffers, and the
+// RNS primitives on host buffers.  Plans live in mkhip_plan.hip, the KKLSS path
+// in mkhip_kklss.hip, the MKF path in mkhip_mkf.hip.
+#include "mkhip_internal.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
-#include <cstring>
-#include <map>
-#include <vector>
-
-namespace mkhip {
-hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st,
-                         int c = 1);
-}
 
 using namespace mkhip;
 
@@ -135,53 +113,7 @@ void modup_params(const std::vector<u64> &Qin, const std::vector<u64> &Pout, std
     }
 }
 
-int hip_status(hipError_t e) { return e == hipSuccess ? MKHIP_OK : (e == hipErrorOutOfMemory ? MKHIP_ERR_OOM : MKHIP_ERR_DEVICE); }
-
-#define HIP_TRY(expr)                                  \
-    do {                                               \
-        hipError_t _e = (expr);                        \
-        if (_e != hipSuccess) return hip_status(_e);   \
-    } while (0)
-
 }  // namespace
-
-// ---------------------------------------------------------------------------
-
-struct PartyKeys {
-    u64 *b = nullptr, *d = nullptr, *v = nullptr;
-};
-
-struct mkhip_ctx {
-    int logN = 0, N = 0, L = 0, K = 0, gamma = 1, alpha = 0, beta_max = 0;
-    u64 t = 0;
-    int flags = 0;
-    std::vector<u64> Q, P;
-    void *d_tables = nullptr;
-    DevTables T;
-    hipStream_t stream = nullptr;
-    u64 *d_crs_u = nullptr;
-    std::map<int, PartyKeys> parties;
-    size_t swk_words = 0;
-    mkhip_plan *single = nullptr;  // cached plan of the host-pointer entry point
-    u64 *single_io = nullptr;      // its device operand buffers
-    size_t single_io_words = 0;
-};
-
-struct mkhip_plan {
-    mkhip_ctx *ctx = nullptr;
-    void *d_mem = nullptr;  // descriptors + workspace
-    size_t bytes = 0;
-    int beta = 0, rows = 0, nl = 0;
-    // stage descriptor offsets (device pointers) and counts
-    const void *dec1 = nullptr, *mac = nullptr, *ntt = nullptr, *ten = nullptr, *ext1 = nullptr, *md1 = nullptr,
-               *dec2 = nullptr, *ext2 = nullptr, *md2 = nullptr;
-    int n_dec1 = 0, n_mac = 0, n_ntt = 0, n_ten = 0, n_ext1 = 0, n_md1 = 0, n_dec2 = 0, n_ext2 = 0, n_md2 = 0;
-    int kmax1 = 1, kmax2 = 1;  // largest party count of the decompositions of stage 0 / 6
-    bool graph = false;
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    int sig[4 + 2 * MKHIP_MAX_PARTIES] = {0};  // shape signature (single-op cache)
-};
 
 extern "C" const char *mkhip_status_string(int s) {
     switch (s) {
@@ -230,6 +162,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     c->Q.assign(q, q + L);
     c->P.assign(p, p + K);
     c->swk_words = (size_t)c->beta_max * (L + K) * N;
+    c->mkf_swk_words = 2 * c->swk_words;
     const int nmod = L + K;
     std::vector<u64> QP(c->Q);
     QP.insert(QP.end(), c->P.begin(), c->P.end());
@@ -348,14 +281,6 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     return MKHIP_OK;
 }
 
-extern "C" void mkhip_plan_destroy(mkhip_plan *plan) {
-    if (!plan) return;
-    if (plan->ge) hipGraphExecDestroy(plan->ge);
-    if (plan->g) hipGraphDestroy(plan->g);
-    if (plan->d_mem) hipFree(plan->d_mem);
-    delete plan;
-}
-
 extern "C" void mkhip_ctx_destroy(mkhip_ctx *c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
@@ -365,6 +290,10 @@ extern "C" void mkhip_ctx_destroy(mkhip_ctx *c) {
         hipFree(kv.second.b);
         hipFree(kv.second.d);
         hipFree(kv.second.v);
+    }
+    for (auto &kv : c->mkf_groups) {
+        for (u64 *f : kv.second.fold) hipFree(f);
+        hipFree(kv.second.relin);
     }
     if (c->d_crs_u) hipFree(c->d_crs_u);
     if (c->d_tables) hipFree(c->d_tables);
@@ -479,512 +408,6 @@ extern "C" int mkhip_d2h(mkhip_ctx *c, void *dst, const void *src, size_t bytes)
 extern "C" int mkhip_sync(mkhip_ctx *c) {
     if (!c) return MKHIP_ERR_ARG;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    return MKHIP_OK;
-}
-
-// ---------------------------------------------------------------------------
-// KKLSS MK-CKKS plan
-
-namespace {
-
-struct Carve {
-    size_t off = 0;
-    size_t take(size_t bytes) {
-        size_t o = (off + 255) & ~(size_t)255;
-        off = o + bytes;
-        return o;
-    }
-};
-
-int find_id(const int *ids, int k, int id) {
-    for (int i = 0; i < k; ++i)
-        if (ids[i] == id) return i;
-    return -1;
-}
-
-// stage s (0..7) of a KKLSS plan
-hipError_t plan_stage(mkhip_plan *pl, int s) {
-    mkhip_ctx *c = pl->ctx;
-    const DevTables &T = c->T;
-    hipStream_t st = c->stream;
-    switch (s) {
-    case 0: return launch_kernel(KDECOMP, T, pl->dec1, pl->n_dec1, pl->beta, pl->rows, st, pl->kmax1);
-    case 1: return launch_kernel(KMAC_XY, T, pl->mac, pl->n_mac, pl->beta, pl->rows, st);
-    case 2: return launch_kernel(KNTT_FWD, T, pl->ntt, pl->n_ntt, 0, 0, st);
-    case 3: return launch_kernel(KTENSOR, T, pl->ten, pl->n_ten, pl->nl, 0, st);
-    case 4: return launch_kernel(KEXT1, T, pl->ext1, pl->n_ext1, pl->beta, pl->rows, st);
-    case 5: return launch_kernel(KMODDOWN, T, pl->md1, pl->n_md1, 0, 0, st);
-    case 6: return launch_kernel(KDECOMP, T, pl->dec2, pl->n_dec2, pl->beta, pl->rows, st, pl->kmax2);
-    case 7: return launch_kernel(KEXT2, T, pl->ext2, pl->n_ext2, pl->beta, pl->rows, st);
-    case 8: return launch_kernel(KMODDOWN_FINAL, T, pl->md2, pl->n_md2, 0, 0, st);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-int plan_launch_raw(mkhip_plan *pl) {
-    for (int s = 0; s < MKHIP_PLAN_STAGES; ++s) HIP_TRY(plan_stage(pl, s));
-    return MKHIP_OK;
-}
-
-}  // namespace
-
-extern "C" int mkhip_plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, int use_graph,
-                                     mkhip_plan **out) {
-    if (!c || !ops || nops <= 0 || !out) return MKHIP_ERR_ARG;
-    *out = nullptr;
-    if (c->alpha == 0) return MKHIP_ERR_GADGET;
-    if (!c->d_crs_u) return MKHIP_ERR_NO_KEY;
-    const int level = ops[0].level;
-    const int N = c->N, K = c->K;
-    for (int o = 0; o < nops; ++o) {
-        const mkhip_kklss_op &op = ops[o];
-        if (!op.op0 || !op.op1 || !op.out) return MKHIP_ERR_ARG;
-        if (op.level != level) return MKHIP_ERR_LEVEL;
-        if (op.level < 0 || op.level >= c->L) return MKHIP_ERR_LEVEL;
-        if (op.k0 < 0 || op.k1 < 0 || op.k0 > MKHIP_MAX_PARTIES || op.k1 > MKHIP_MAX_PARTIES) return MKHIP_ERR_TOO_MANY;
-        if (op.nres < 0 || op.nres > op.level) return MKHIP_ERR_SCALE;
-        for (int i = 0; i < op.k0; ++i)
-            if (!c->parties.count(op.ids0[i])) return MKHIP_ERR_UNKNOWN_PARTY;
-        for (int i = 0; i < op.k1; ++i)
-            if (!c->parties.count(op.ids1[i])) return MKHIP_ERR_UNKNOWN_PARTY;
-    }
-    const int nl = level + 1, rows = nl + K;
-    const int beta = (nl + c->alpha - 1) / c->alpha;
-    const size_t limb = (size_t)N * 8, poly = (size_t)nl * limb, qp = (size_t)rows * limb, hoist = (size_t)beta * qp;
-
-    // ---- sizes ----
-    Carve cv;
-    std::vector<DecompDesc> dec1, dec2;
-    std::vector<MacDesc> mac;
-    std::vector<NttJob> ntt;
-    std::vector<TensorDesc> ten;
-    std::vector<ExtDesc> ext1, ext2;
-    std::vector<ModDownDesc> md1, md2;
-    struct OpWs {
-        size_t D0, D1, x, y, tnt, ct, accx, accy, t, Dt, accv, accu;
-        int ku;
-        int ids[2 * MKHIP_MAX_PARTIES];
-    };
-    std::vector<OpWs> ws(nops);
-    for (int o = 0; o < nops; ++o) {
-        const mkhip_kklss_op &op = ops[o];
-        OpWs &w = ws[o];
-        // union of ids, ascending
-        int ku = 0;
-        for (int i = 0; i < op.k0; ++i)
-            if (find_id(w.ids, ku, op.ids0[i]) < 0) w.ids[ku++] = op.ids0[i];
-        for (int i = 0; i < op.k1; ++i)
-            if (find_id(w.ids, ku, op.ids1[i]) < 0) w.ids[ku++] = op.ids1[i];
-        std::sort(w.ids, w.ids + ku);
-        w.ku = ku;
-        if (ku > MKHIP_MAX_PARTIES) return MKHIP_ERR_TOO_MANY;
-        w.D0 = cv.take(op.k0 * hoist);
-        w.D1 = cv.take(op.k1 * hoist);
-        w.x = cv.take(hoist);
-        w.y = cv.take(hoist);
-        w.tnt = cv.take((2 + op.k0 + op.k1) * poly);
-        w.ct = cv.take((1 + ku) * poly);
-        w.accx = cv.take(op.k1 * qp);
-        w.accy = cv.take(op.k0 * qp);
-        w.t = cv.take(op.k0 * poly);
-        w.Dt = cv.take(op.k0 * hoist);
-        w.accv = cv.take(op.k0 * qp);
-        w.accu = cv.take(op.k0 * qp);
-    }
-    const size_t ws_bytes = cv.off;
-    // descriptor counts
-    int n_ntt = 0, n_ten = 0, n_ext1 = 0, n_md1 = 0, n_ext2 = 0, n_md2 = 0;
-    for (int o = 0; o < nops; ++o) {
-        n_ntt += (2 + ops[o].k0 + ops[o].k1) * nl;
-        n_ten += 1 + ws[o].ku;
-        n_ext1 += ops[o].k0 + ops[o].k1;
-        n_md1 += ops[o].k0 + ops[o].k1;
-        n_ext2 += ops[o].k0;
-        n_md2 += 1 + ws[o].ku;
-    }
-    Carve dv;
-    dv.off = ws_bytes;
-    const size_t o_dec1 = dv.take(sizeof(DecompDesc) * 2 * nops);
-    const size_t o_mac = dv.take(sizeof(MacDesc) * 2 * nops);
-    const size_t o_ntt = dv.take(sizeof(NttJob) * n_ntt);
-    const size_t o_ten = dv.take(sizeof(TensorDesc) * n_ten);
-    const size_t o_ext1 = dv.take(sizeof(ExtDesc) * std::max(n_ext1, 1));
-    const size_t o_md1 = dv.take(sizeof(ModDownDesc) * std::max(n_md1, 1));
-    const size_t o_dec2 = dv.take(sizeof(DecompDesc) * nops);
-    const size_t o_ext2 = dv.take(sizeof(ExtDesc) * std::max(n_ext2, 1));
-    const size_t o_md2 = dv.take(sizeof(ModDownDesc) * n_md2);
-
-    mkhip_plan *pl = new mkhip_plan();
-    pl->ctx = c;
-    pl->bytes = dv.off;
-    hipError_t e = hipMalloc(&pl->d_mem, pl->bytes);
-    if (e != hipSuccess) {
-        delete pl;
-        return hip_status(e);
-    }
-    char *B = (char *)pl->d_mem;
-    auto P = [&](size_t off) { return (u64 *)(B + off); };
-
-    for (int o = 0; o < nops; ++o) {
-        const mkhip_kklss_op &op = ops[o];
-        const OpWs &w = ws[o];
-        // 1: decompositions of both sides; x / y accumulations (key-stationary)
-        DecompDesc s0 = {}, s1 = {};
-        MacDesc m0 = {}, m1 = {};
-        for (int p = 0; p < op.k0; ++p) {
-            s0.comp[p] = op.op0 + (size_t)(1 + p) * nl * N;
-            m0.key[p] = c->parties[op.ids0[p]].d;
-        }
-        s0.D = P(w.D0);
-        s0.k = op.k0;
-        s0.level = level;
-        m0.D = P(w.D0);
-        m0.xy = P(w.x);
-        m0.k = op.k0;
-        m0.level = level;
-        for (int p = 0; p < op.k1; ++p) {
-            s1.comp[p] = op.op1 + (size_t)(1 + p) * nl * N;
-            m1.key[p] = c->parties[op.ids1[p]].b;
-        }
-        s1.D = P(w.D1);
-        s1.k = op.k1;
-        s1.level = level;
-        m1.D = P(w.D1);
-        m1.xy = P(w.y);
-        m1.k = op.k1;
-        m1.level = level;
-        dec1.push_back(s0);
-        dec1.push_back(s1);
-        mac.push_back(m0);
-        mac.push_back(m1);
-        pl->kmax1 = std::max(pl->kmax1, std::max(op.k0, op.k1));
-        pl->kmax2 = std::max(pl->kmax2, op.k0);
-        // 2: NTT of every input component: op0 c0, op0 comps, op1 c0, op1 comps
-        u64 *tnt = P(w.tnt);
-        const int nc0 = 1 + op.k0, nc1 = 1 + op.k1;
-        for (int comp = 0; comp < nc0 + nc1; ++comp) {
-            const u64 *src = comp < nc0 ? op.op0 + (size_t)comp * nl * N : op.op1 + (size_t)(comp - nc0) * nl * N;
-            for (int l = 0; l < nl; ++l) {
-                NttJob j;
-                j.src = src + (size_t)l * N;
-                j.dst = tnt + ((size_t)comp * nl + l) * N;
-                j.mod = l;
-                j.reduce = 0;
-                ntt.push_back(j);
-            }
-        }
-        const u64 *a0 = tnt, *b0 = tnt + (size_t)nc0 * nl * N;
-        // 3: tensor into ct (coefficient domain)
-        u64 *ct = P(w.ct);
-        {
-            TensorDesc t0 = {};
-            t0.a0 = a0;
-            t0.b0 = b0;
-            t0.out = ct;
-            t0.is_c0 = 1;
-            ten.push_back(t0);
-        }
-        for (int u = 0; u < w.ku; ++u) {
-            const int id = w.ids[u];
-            const int p0 = find_id(op.ids0, op.k0, id), p1 = find_id(op.ids1, op.k1, id);
-            TensorDesc td = {};
-            td.a0 = a0;
-            td.b0 = b0;
-            td.au = p0 >= 0 ? tnt + (size_t)(1 + p0) * nl * N : nullptr;
-            td.bu = p1 >= 0 ? tnt + (size_t)(nc0 + 1 + p1) * nl * N : nullptr;
-            td.out = ct + (size_t)(1 + u) * nl * N;
-            td.is_c0 = 0;
-            ten.push_back(td);
-        }
-        // 4/5: Ext(D1_j, x) -> ct_j ; Ext(D0_i, y) -> t_i
-        for (int p = 0; p < op.k1; ++p) {
-            ExtDesc ed = {};
-            ed.D = P(w.D1) + (size_t)p * beta * rows * N;
-            ed.key[0] = P(w.x);
-            ed.acc[0] = P(w.accx) + (size_t)p * rows * N;
-            ed.nkey = 1;
-            ed.level = level;
-            ext1.push_back(ed);
-            ModDownDesc md = {};
-            md.src[0] = ed.acc[0];
-            md.nsrc = 1;
-            md.dst = ct + (size_t)(1 + find_id(w.ids, w.ku, op.ids1[p])) * nl * N;
-            md.level = level;
-            md.accumulate = 1;
-            md1.push_back(md);
-        }
-        for (int p = 0; p < op.k0; ++p) {
-            ExtDesc ed = {};
-            ed.D = P(w.D0) + (size_t)p * beta * rows * N;
-            ed.key[0] = P(w.y);
-            ed.acc[0] = P(w.accy) + (size_t)p * rows * N;
-            ed.nkey = 1;
-            ed.level = level;
-            ext1.push_back(ed);
-            ModDownDesc md = {};
-            md.src[0] = ed.acc[0];
-            md.nsrc = 1;
-            md.dst = P(w.t) + (size_t)p * nl * N;
-            md.level = level;
-            md.accumulate = 0;
-            md1.push_back(md);
-        }
-        // 6: D(t_i)
-        DecompDesc s2 = {};
-        for (int p = 0; p < op.k0; ++p) s2.comp[p] = P(w.t) + (size_t)p * nl * N;
-        s2.D = P(w.Dt);
-        s2.k = op.k0;
-        s2.level = level;
-        dec2.push_back(s2);
-        // 7: Ext(D(t_i), v_i), Ext(D(t_i), u)
-        for (int p = 0; p < op.k0; ++p) {
-            ExtDesc ed = {};
-            ed.D = P(w.Dt) + (size_t)p * beta * rows * N;
-            ed.key[0] = c->parties[op.ids0[p]].v;
-            ed.key[1] = c->d_crs_u;
-            ed.acc[0] = P(w.accv) + (size_t)p * rows * N;
-            ed.acc[1] = P(w.accu) + (size_t)p * rows * N;
-            ed.nkey = 2;
-            ed.level = level;
-            ext2.push_back(ed);
-        }
-        // 8: final ModDowns + rescale into the output
-        {
-            ModDownDesc md = {};
-            for (int p = 0; p < op.k0; ++p) md.src[p] = P(w.accv) + (size_t)p * rows * N;
-            md.nsrc = op.k0;
-            md.dst = ct;
-            md.level = level;
-            md.accumulate = 1;
-            md.final_ = 1;
-            md.nres = op.nres;
-            md.out = op.out;
-            md2.push_back(md);
-        }
-        for (int u = 0; u < w.ku; ++u) {
-            const int p0 = find_id(op.ids0, op.k0, w.ids[u]);
-            ModDownDesc md = {};
-            if (p0 >= 0) {
-                md.src[0] = P(w.accu) + (size_t)p0 * rows * N;
-                md.nsrc = 1;
-            }
-            md.dst = ct + (size_t)(1 + u) * nl * N;
-            md.level = level;
-            md.accumulate = 1;
-            md.final_ = 1;
-            md.nres = op.nres;
-            md.out = op.out + (size_t)(1 + u) * (nl - op.nres) * N;
-            md2.push_back(md);
-        }
-    }
-    // ---- upload descriptors ----
-    std::vector<char> hd(pl->bytes - ws_bytes);
-    auto put = [&](size_t off, const void *src, size_t bytes) {
-        if (bytes) std::memcpy(hd.data() + (off - ws_bytes), src, bytes);
-    };
-    put(o_dec1, dec1.data(), dec1.size() * sizeof(DecompDesc));
-    put(o_mac, mac.data(), mac.size() * sizeof(MacDesc));
-    put(o_ntt, ntt.data(), ntt.size() * sizeof(NttJob));
-    put(o_ten, ten.data(), ten.size() * sizeof(TensorDesc));
-    put(o_ext1, ext1.data(), ext1.size() * sizeof(ExtDesc));
-    put(o_md1, md1.data(), md1.size() * sizeof(ModDownDesc));
-    put(o_dec2, dec2.data(), dec2.size() * sizeof(DecompDesc));
-    put(o_ext2, ext2.data(), ext2.size() * sizeof(ExtDesc));
-    put(o_md2, md2.data(), md2.size() * sizeof(ModDownDesc));
-    e = hipMemcpy(B + ws_bytes, hd.data(), hd.size(), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        mkhip_plan_destroy(pl);
-        return hip_status(e);
-    }
-    pl->beta = beta;
-    pl->rows = rows;
-    pl->nl = nl;
-    pl->dec1 = B + o_dec1;
-    pl->n_dec1 = (int)dec1.size();
-    pl->mac = B + o_mac;
-    pl->n_mac = nops;
-    pl->ntt = B + o_ntt;
-    pl->n_ntt = (int)ntt.size();
-    pl->ten = B + o_ten;
-    pl->n_ten = (int)ten.size();
-    pl->ext1 = B + o_ext1;
-    pl->n_ext1 = (int)ext1.size();
-    pl->md1 = B + o_md1;
-    pl->n_md1 = (int)md1.size();
-    pl->dec2 = B + o_dec2;
-    pl->n_dec2 = (int)dec2.size();
-    pl->ext2 = B + o_ext2;
-    pl->n_ext2 = (int)ext2.size();
-    pl->md2 = B + o_md2;
-    pl->n_md2 = (int)md2.size();
-
-    if (use_graph) {
-        // first launch eagerly (sets kernel attributes), then capture the sequence
-        int s = plan_launch_raw(pl);
-        if (s == MKHIP_OK) s = hip_status(hipStreamSynchronize(c->stream));
-        if (s == MKHIP_OK) s = hip_status(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        if (s == MKHIP_OK) {
-            int s2 = plan_launch_raw(pl);
-            hipError_t e2 = hipStreamEndCapture(c->stream, &pl->g);
-            s = s2 != MKHIP_OK ? s2 : hip_status(e2);
-        }
-        if (s == MKHIP_OK) s = hip_status(hipGraphInstantiate(&pl->ge, pl->g, nullptr, nullptr, 0));
-        if (s != MKHIP_OK) {
-            mkhip_plan_destroy(pl);
-            return s;
-        }
-        pl->graph = true;
-    }
-    *out = pl;
-    return MKHIP_OK;
-}
-
-extern "C" int mkhip_plan_launch(mkhip_plan *pl) {
-    if (!pl) return MKHIP_ERR_ARG;
-    if (pl->graph) {
-        HIP_TRY(hipGraphLaunch(pl->ge, pl->ctx->stream));
-        return MKHIP_OK;
-    }
-    return plan_launch_raw(pl);
-}
-
-extern "C" int mkhip_plan_time(mkhip_plan *pl, int iters, double *ms) {
-    if (!pl || iters <= 0 || !ms) return MKHIP_ERR_ARG;
-    hipEvent_t a, b;
-    HIP_TRY(hipEventCreate(&a));
-    HIP_TRY(hipEventCreate(&b));
-    HIP_TRY(hipEventRecord(a, pl->ctx->stream));
-    for (int i = 0; i < iters; ++i) {
-        int s = mkhip_plan_launch(pl);
-        if (s != MKHIP_OK) return s;
-    }
-    HIP_TRY(hipEventRecord(b, pl->ctx->stream));
-    HIP_TRY(hipEventSynchronize(b));
-    float f = 0;
-    HIP_TRY(hipEventElapsedTime(&f, a, b));
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    *ms = f / iters;
-    return MKHIP_OK;
-}
-
-extern "C" size_t mkhip_plan_workspace_bytes(const mkhip_plan *pl) { return pl ? pl->bytes : 0; }
-
-// Average device time of each of the plan's launches (HIP events on the context
-// stream around each kernel, eager launches), in ms.
-extern "C" int mkhip_plan_stage_times(mkhip_plan *pl, int iters, double *ms) {
-    if (!pl || iters <= 0 || !ms) return MKHIP_ERR_ARG;
-    constexpr int S = MKHIP_PLAN_STAGES;
-    hipStream_t st = pl->ctx->stream;
-    hipEvent_t ev[S + 1];
-    for (int i = 0; i <= S; ++i) HIP_TRY(hipEventCreate(&ev[i]));
-    double acc[S] = {0};
-    for (int it = 0; it < iters; ++it) {
-        HIP_TRY(hipEventRecord(ev[0], st));
-        for (int s = 0; s < S; ++s) {
-            HIP_TRY(plan_stage(pl, s));
-            HIP_TRY(hipEventRecord(ev[s + 1], st));
-        }
-        HIP_TRY(hipEventSynchronize(ev[S]));
-        for (int s = 0; s < S; ++s) {
-            float f = 0;
-            HIP_TRY(hipEventElapsedTime(&f, ev[s], ev[s + 1]));
-            acc[s] += f;
-        }
-    }
-    for (int i = 0; i <= S; ++i) (void)hipEventDestroy(ev[i]);
-    for (int s = 0; s < S; ++s) ms[s] = acc[s] / iters;
-    return MKHIP_OK;
-}
-
-// ---------------------------------------------------------------------------
-// host-pointer entry point
-
-extern "C" int mkhip_kklss_ckks_mulrelin(mkhip_ctx *c, const uint64_t *op0, int level0, double scale0,
-                                         const int *ids0, int k0, const uint64_t *op1, int level1, double scale1,
-                                         const int *ids1, int k1, double min_scale, uint64_t *out, int *ids_out,
-                                         int *ku_out, int *level_out, double *scale_out) {
-    if (!c || !op0 || !op1 || !out || (k0 && !ids0) || (k1 && !ids1)) return MKHIP_ERR_ARG;
-    if (k0 < 0 || k1 < 0) return MKHIP_ERR_ARG;
-    if (k0 > MKHIP_MAX_PARTIES || k1 > MKHIP_MAX_PARTIES) return MKHIP_ERR_TOO_MANY;
-    if (level0 < 0 || level1 < 0 || level0 >= c->L || level1 >= c->L) return MKHIP_ERR_LEVEL;
-    if (c->alpha == 0) return MKHIP_ERR_GADGET;
-    // DropLevel to the common level (evaluator.go:577-581).  With alpha > 1 the
-    // reference decomposes each operand at its own level, which this path does not model.
-    const int level = std::min(level0, level1);
-    if (level0 != level1 && c->alpha != 1) return MKHIP_ERR_LEVEL;
-    // mulRelinHoisted ignores Rescale's error return (evaluator.go:585): a
-    // rejected rescale (level 0, minScale <= 0) leaves the product unrescaled.
-    int nres = 0;
-    double ns = scale0 * scale1;
-    int s = mkhip_rescale_count(c, level, scale0 * scale1, min_scale, &nres, &ns);
-    if (s == MKHIP_ERR_SCALE) {
-        nres = 0;
-        ns = scale0 * scale1;
-    } else if (s != MKHIP_OK) {
-        return s;
-    }
-    const int N = c->N, nl = level + 1;
-    int ids[2 * MKHIP_MAX_PARTIES], ku = 0;
-    for (int i = 0; i < k0; ++i)
-        if (find_id(ids, ku, ids0[i]) < 0) ids[ku++] = ids0[i];
-    for (int i = 0; i < k1; ++i)
-        if (find_id(ids, ku, ids1[i]) < 0) ids[ku++] = ids1[i];
-    if (ku > MKHIP_MAX_PARTIES) return MKHIP_ERR_TOO_MANY;
-    std::sort(ids, ids + ku);
-    const size_t in0 = (size_t)(1 + k0) * nl * N, in1 = (size_t)(1 + k1) * nl * N, ow = (size_t)(1 + ku) * nl * N;
-    const size_t words = in0 + in1 + ow;
-    // shape signature of the cached plan
-    int sig[4 + 2 * MKHIP_MAX_PARTIES] = {0};
-    sig[0] = k0;
-    sig[1] = k1;
-    sig[2] = level;
-    sig[3] = nres;
-    for (int i = 0; i < k0; ++i) sig[4 + i] = ids0[i];
-    for (int i = 0; i < k1; ++i) sig[4 + MKHIP_MAX_PARTIES + i] = ids1[i];
-    if (!c->single || std::memcmp(sig, c->single->sig, sizeof(sig)) != 0 || c->single_io_words < words) {
-        mkhip_plan_destroy(c->single);
-        c->single = nullptr;
-        if (c->single_io_words < words) {
-            if (c->single_io) hipFree(c->single_io);
-            c->single_io = nullptr;
-            c->single_io_words = 0;
-            HIP_TRY(hipMalloc(&c->single_io, words * 8));
-            c->single_io_words = words;
-        }
-        mkhip_kklss_op op = {};
-        op.op0 = c->single_io;
-        op.op1 = c->single_io + in0;
-        op.out = c->single_io + in0 + in1;
-        for (int i = 0; i < k0; ++i) op.ids0[i] = ids0[i];
-        for (int i = 0; i < k1; ++i) op.ids1[i] = ids1[i];
-        op.k0 = k0;
-        op.k1 = k1;
-        op.level = level;
-        op.nres = nres;
-        mkhip_plan *pl = nullptr;
-        s = mkhip_plan_kklss_ckks(c, 1, &op, 0, &pl);
-        if (s != MKHIP_OK) return s;
-        std::memcpy(pl->sig, sig, sizeof(sig));
-        c->single = pl;
-    }
-    // stage the operands at the common level (limbs 0..level of each component)
-    for (int comp = 0; comp <= k0; ++comp)
-        HIP_TRY(hipMemcpyAsync(c->single_io + (size_t)comp * nl * N, op0 + (size_t)comp * (level0 + 1) * N,
-                               (size_t)nl * N * 8, hipMemcpyHostToDevice, c->stream));
-    for (int comp = 0; comp <= k1; ++comp)
-        HIP_TRY(hipMemcpyAsync(c->single_io + in0 + (size_t)comp * nl * N, op1 + (size_t)comp * (level1 + 1) * N,
-                               (size_t)nl * N * 8, hipMemcpyHostToDevice, c->stream));
-    s = mkhip_plan_launch(c->single);
-    if (s != MKHIP_OK) return s;
-    const size_t outw = (size_t)(1 + ku) * (nl - nres) * N;
-    HIP_TRY(hipMemcpyAsync(out, c->single_io + in0 + in1, outw * 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (ids_out)
-        for (int i = 0; i < ku; ++i) ids_out[i] = ids[i];
-    if (ku_out) *ku_out = ku;
-    if (level_out) *level_out = level - nres;
-    if (scale_out) *scale_out = ns;
     return MKHIP_OK;
 }
 

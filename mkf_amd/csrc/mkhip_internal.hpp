@@ -1,0 +1,127 @@
+// Host-side internals shared by the C ABI translation units: the context, the
+// plan (a list of kernel launches over descriptor tables, optionally captured in
+// a hipGraph), status helpers.
+#pragma once
+
+#include "../../include/mkhip.h"
+#include "tables.hpp"
+
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace mkhip {
+// kernels.hip
+hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st,
+                         int c = 1);
+// kernels_mkf.hip
+hipError_t launch_mkf(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st);
+
+inline int hip_status(hipError_t e) {
+    return e == hipSuccess ? MKHIP_OK : (e == hipErrorOutOfMemory ? MKHIP_ERR_OOM : MKHIP_ERR_DEVICE);
+}
+}  // namespace mkhip
+
+#define HIP_TRY(expr)                                          \
+    do {                                                       \
+        hipError_t _e = (expr);                                \
+        if (_e != hipSuccess) return mkhip::hip_status(_e);    \
+    } while (0)
+
+struct PartyKeys {
+    mkhip::u64 *b = nullptr, *d = nullptr, *v = nullptr;
+};
+
+// MKF keys of one party group: fold keys s_i -> s_aggregate and the
+// relinearization key of s_aggregate (mkf_cpu/common.go:32-51), each
+// [beta][2][L+K][N].
+struct MkfGroup {
+    int k = 0;
+    std::vector<mkhip::u64 *> fold;
+    mkhip::u64 *relin = nullptr;
+};
+
+// One launch of a plan: kernel id + descriptor table + launch geometry.
+struct PlanStage {
+    int what = 0;
+    const void *descs = nullptr;
+    int n = 0, a = 0, b = 0, c = 1;
+    std::string name;
+};
+
+struct mkhip_ctx {
+    int logN = 0, N = 0, L = 0, K = 0, gamma = 1, alpha = 0, beta_max = 0;
+    mkhip::u64 t = 0;
+    int flags = 0;
+    std::vector<mkhip::u64> Q, P;
+    void *d_tables = nullptr;
+    mkhip::DevTables T;
+    hipStream_t stream = nullptr;
+    mkhip::u64 *d_crs_u = nullptr;
+    std::map<int, PartyKeys> parties;
+    std::map<int, MkfGroup> mkf_groups;
+    size_t swk_words = 0;       // beta x (L+K) x N
+    size_t mkf_swk_words = 0;   // beta x 2 x (L+K) x N
+    mkhip_plan *single = nullptr;  // cached plan of the host-pointer entry point
+    mkhip::u64 *single_io = nullptr;
+    size_t single_io_words = 0;
+};
+
+struct mkhip_plan {
+    mkhip_ctx *ctx = nullptr;
+    void *d_mem = nullptr;   // workspace
+    void *d_desc = nullptr;  // descriptor tables
+    size_t bytes = 0;        // workspace + descriptor bytes
+    std::vector<PlanStage> stages;
+    bool graph = false;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    std::vector<int> sig;  // shape signature (single-op cache)
+};
+
+namespace mkhip {
+
+// 256-byte aligned bump allocator over one buffer.
+struct Carve {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        size_t o = (off + 255) & ~(size_t)255;
+        off = o + bytes;
+        return o;
+    }
+};
+
+// Host copy of the descriptor tables of a plan, uploaded after the workspace.
+struct DescBlob {
+    std::vector<char> host;
+    template <class D>
+    size_t add(const std::vector<D> &v) {
+        size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + std::max<size_t>(v.size(), 1) * sizeof(D));
+        if (!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(D));
+        return off;
+    }
+};
+
+hipError_t plan_stage(mkhip_plan *pl, int s);
+int plan_launch_raw(mkhip_plan *pl);
+// New plan with its workspace allocated (ws_bytes); *out is null on failure.
+int plan_create(mkhip_ctx *c, size_t ws_bytes, mkhip_plan **out);
+// Upload the descriptor tables, turn each stage's blob offset (stage.descs holds
+// the offset as an integer on entry) into a device pointer, and optionally
+// capture the whole launch sequence into a hipGraph.  Destroys the plan on failure.
+int plan_finish(mkhip_plan *pl, const DescBlob &blob, int use_graph);
+inline PlanStage make_stage(int what, size_t blob_off, int n, int a, int b, int c, const char *name) {
+    PlanStage s;
+    s.what = what;
+    s.descs = reinterpret_cast<const void *>(blob_off);
+    s.n = n;
+    s.a = a;
+    s.b = b;
+    s.c = c;
+    s.name = name;
+    return s;
+}
+
+}  // namespace mkhip

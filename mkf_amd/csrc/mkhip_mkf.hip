@@ -1,0 +1,297 @@
+// MKF MK-CKKS mul + relin (software/mkf/mkf_cpu/ckks.go:102-151): fold each side
+// to the aggregate secret with one hoisted key switch per party
+// (common.go:79-126), Karatsuba tensor, one relinearization of mu2, NTT-domain
+// rescale, and the k-fold materialization of Gamma (common.go:128-139).
+//
+// Launches (each batched over all ops of the plan):
+//   intt_in        components -> coefficient domain (DecomposeNTT's InvNTT)
+//   decompose      gadget digits of every component, NTT'd (alpha = K, CRT when K > 1)
+//   mac_fold       KeyswitchHoistedNoModDown: two accumulators per component
+//   moddown_fold   ModDownQPtoQNTT of every switch, summed with c0 (fold)
+//   karatsuba      mu0, mu1, mu2
+//   intt_mu2, decompose_mu2, mac_relin, moddown_relin   switch(mu2) added to mu0, mu1
+//   intt_last      last limb of the result (rescale)
+//   rescale        DivRoundByLastModulus (NTT domain) into c0 and k copies of Gamma
+#include "mkhip_internal.hpp"
+
+#include <algorithm>
+
+using namespace mkhip;
+
+extern "C" int mkhip_register_mkf_group(mkhip_ctx *c, int group, int k, const uint64_t *const *fold_keys,
+                                        const uint64_t *relin_key) {
+    if (!c || !fold_keys || !relin_key || k <= 0 || group < 0) return MKHIP_ERR_ARG;
+    if (k > MKHIP_MAX_PARTIES) return MKHIP_ERR_TOO_MANY;
+    if (c->alpha == 0) return MKHIP_ERR_GADGET;
+    for (int i = 0; i < k; ++i)
+        if (!fold_keys[i]) return MKHIP_ERR_ARG;
+    mkhip_unregister_mkf_group(c, group);
+    MkfGroup g;
+    g.k = k;
+    const size_t bytes = c->mkf_swk_words * 8;
+    auto upload = [&](const uint64_t *src, u64 **dst) -> int {
+        void *p = nullptr;
+        HIP_TRY(hipMalloc(&p, bytes));
+        *dst = (u64 *)p;
+        HIP_TRY(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+        return MKHIP_OK;
+    };
+    int s = MKHIP_OK;
+    for (int i = 0; i < k && s == MKHIP_OK; ++i) {
+        u64 *p = nullptr;
+        s = upload(fold_keys[i], &p);
+        if (p) g.fold.push_back(p);
+    }
+    if (s == MKHIP_OK) s = upload(relin_key, &g.relin);
+    if (s != MKHIP_OK) {
+        for (u64 *p : g.fold) (void)hipFree(p);
+        if (g.relin) (void)hipFree(g.relin);
+        return s;
+    }
+    c->mkf_groups[group] = g;
+    return MKHIP_OK;
+}
+
+extern "C" int mkhip_unregister_mkf_group(mkhip_ctx *c, int group) {
+    if (!c) return MKHIP_ERR_ARG;
+    auto it = c->mkf_groups.find(group);
+    if (it == c->mkf_groups.end()) return MKHIP_ERR_NO_KEY;
+    (void)hipStreamSynchronize(c->stream);
+    for (u64 *p : it->second.fold) (void)hipFree(p);
+    (void)hipFree(it->second.relin);
+    c->mkf_groups.erase(it);
+    return MKHIP_OK;
+}
+
+extern "C" int mkhip_plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph,
+                                   mkhip_plan **out) {
+    if (!c || !ops || nops <= 0 || !out) return MKHIP_ERR_ARG;
+    *out = nullptr;
+    if (c->alpha == 0) return MKHIP_ERR_GADGET;
+    const int level = ops[0].level, nres = ops[0].nres;
+    const int N = c->N, K = c->K;
+    for (int o = 0; o < nops; ++o) {
+        const mkhip_mkf_op &op = ops[o];
+        if (!op.left || !op.right || !op.out) return MKHIP_ERR_ARG;
+        if (op.level != level || level < 0 || level >= c->L) return MKHIP_ERR_LEVEL;
+        if (op.nres != nres || nres < 0 || nres > 1 || nres > level) return MKHIP_ERR_SCALE;
+        if (!c->mkf_groups.count(op.group)) return MKHIP_ERR_NO_KEY;
+    }
+    const int nl = level + 1, rows = nl + K;
+    const int beta = (nl + c->alpha - 1) / c->alpha;
+    const size_t limb = (size_t)N * 8, poly = (size_t)nl * limb, qp = (size_t)rows * limb, hoist = (size_t)beta * qp;
+
+    struct OpWs {
+        size_t coef, D, facc, fold, mu, mu2c, Dr, racc, fin, last;
+        int k;
+    };
+    std::vector<OpWs> ws(nops);
+    Carve cv;
+    int kmax = 1;
+    for (int o = 0; o < nops; ++o) {
+        OpWs &w = ws[o];
+        const int k = c->mkf_groups[ops[o].group].k;
+        w.k = k;
+        kmax = std::max(kmax, k);
+        w.coef = cv.take(2 * k * poly);
+        w.D = cv.take(2 * k * hoist);
+        w.facc = cv.take(2 * k * 2 * qp);
+        w.fold = cv.take(4 * poly);
+        w.mu = cv.take(3 * poly);
+        w.mu2c = cv.take(poly);
+        w.Dr = cv.take(hoist);
+        w.racc = cv.take(2 * qp);
+        w.fin = cv.take(2 * poly);
+        w.last = cv.take(2 * limb);
+    }
+    mkhip_plan *pl = nullptr;
+    int st = plan_create(c, cv.off, &pl);
+    if (st != MKHIP_OK) return st;
+    char *B = (char *)pl->d_mem;
+    auto P = [&](size_t off) { return (u64 *)(B + off); };
+
+    std::vector<NttJob> intt_in, intt_mu2, intt_last;
+    std::vector<DecompDesc> dec, dec_mu2;
+    std::vector<Mac2Desc> mac_fold, mac_relin;
+    std::vector<MdNttDesc> md_fold, md_relin;
+    std::vector<KaraDesc> kara;
+    std::vector<RescaleNttDesc> resc;
+    for (int o = 0; o < nops; ++o) {
+        const mkhip_mkf_op &op = ops[o];
+        const OpWs &w = ws[o];
+        const MkfGroup &g = c->mkf_groups[op.group];
+        const int k = w.k;
+        const u64 *side[2] = {op.left, op.right};
+        auto coef = [&](int s, int p) { return P(w.coef) + ((size_t)s * k + p) * nl * N; };
+        auto facc = [&](int s, int p, int oo) { return P(w.facc) + (((size_t)s * k + p) * 2 + oo) * rows * N; };
+        auto fold = [&](int s, int oo) { return P(w.fold) + ((size_t)s * 2 + oo) * nl * N; };
+        auto mu = [&](int i) { return P(w.mu) + (size_t)i * nl * N; };
+        for (int s = 0; s < 2; ++s) {
+            DecompDesc dd = {};
+            for (int p = 0; p < k; ++p) {
+                for (int l = 0; l < nl; ++l) {
+                    NttJob j;
+                    j.src = side[s] + ((size_t)(1 + p) * nl + l) * N;
+                    j.dst = coef(s, p) + (size_t)l * N;
+                    j.mod = l;
+                    j.reduce = 0;
+                    intt_in.push_back(j);
+                }
+                dd.comp[p] = coef(s, p);
+                Mac2Desc md = {};
+                md.D = P(w.D) + ((size_t)s * k + p) * beta * rows * N;
+                md.swk = g.fold[p];
+                md.acc[0] = facc(s, p, 0);
+                md.acc[1] = facc(s, p, 1);
+                md.level = level;
+                mac_fold.push_back(md);
+            }
+            dd.D = P(w.D) + (size_t)s * k * beta * rows * N;
+            dd.k = k;
+            dd.level = level;
+            dec.push_back(dd);
+            for (int oo = 0; oo < 2; ++oo) {
+                MdNttDesc m = {};
+                for (int p = 0; p < k; ++p) m.acc[p] = facc(s, p, oo);
+                m.nacc = k;
+                m.base = oo == 0 ? side[s] : nullptr;  // fold: out[0] starts from c0
+                m.dst = fold(s, oo);
+                m.level = level;
+                md_fold.push_back(m);
+            }
+        }
+        KaraDesc kd = {};
+        kd.l0 = fold(0, 0);
+        kd.l1 = fold(0, 1);
+        kd.r0 = fold(1, 0);
+        kd.r1 = fold(1, 1);
+        kd.mu0 = mu(0);
+        kd.mu1 = mu(1);
+        kd.mu2 = mu(2);
+        kara.push_back(kd);
+        for (int l = 0; l < nl; ++l) {
+            NttJob j;
+            j.src = mu(2) + (size_t)l * N;
+            j.dst = P(w.mu2c) + (size_t)l * N;
+            j.mod = l;
+            j.reduce = 0;
+            intt_mu2.push_back(j);
+        }
+        DecompDesc d2 = {};
+        d2.comp[0] = P(w.mu2c);
+        d2.D = P(w.Dr);
+        d2.k = 1;
+        d2.level = level;
+        dec_mu2.push_back(d2);
+        Mac2Desc mr = {};
+        mr.D = P(w.Dr);
+        mr.swk = g.relin;
+        mr.acc[0] = P(w.racc);
+        mr.acc[1] = P(w.racc) + (size_t)rows * N;
+        mr.level = level;
+        mac_relin.push_back(mr);
+        for (int oo = 0; oo < 2; ++oo) {
+            MdNttDesc m = {};
+            m.acc[0] = mr.acc[oo];
+            m.nacc = 1;
+            m.base = mu(oo);
+            m.dst = P(w.fin) + (size_t)oo * nl * N;
+            m.level = level;
+            md_relin.push_back(m);
+            if (nres == 1) {
+                NttJob j;
+                j.src = m.dst + (size_t)level * N;
+                j.dst = P(w.last) + (size_t)oo * N;
+                j.mod = level;
+                j.reduce = 0;
+                intt_last.push_back(j);
+            }
+            RescaleNttDesc rd = {};
+            rd.x = m.dst;
+            rd.xlast = P(w.last) + (size_t)oo * N;
+            rd.level = level;
+            rd.nres = nres;
+            const size_t olimbs = (size_t)(nl - nres) * N;
+            if (oo == 0) {
+                rd.out[0] = op.out;
+                rd.nout = 1;
+            } else {
+                for (int p = 0; p < k; ++p) rd.out[p] = op.out + (size_t)(1 + p) * olimbs;
+                rd.nout = k;
+            }
+            resc.push_back(rd);
+        }
+    }
+    DescBlob blob;
+    auto &S = pl->stages;
+    S.push_back(make_stage(KNTT_INV, blob.add(intt_in), (int)intt_in.size(), 0, 0, 1, "intt_in"));
+    S.push_back(make_stage(KDECOMP, blob.add(dec), (int)dec.size(), beta, rows, kmax, "decompose"));
+    S.push_back(make_stage(KMAC2, blob.add(mac_fold), (int)mac_fold.size(), beta, rows, 1, "mac_fold"));
+    S.push_back(make_stage(KMD_NTT, blob.add(md_fold), (int)md_fold.size(), nl, 0, 1, "moddown_fold"));
+    S.push_back(make_stage(KKARA, blob.add(kara), (int)kara.size(), nl, 0, 1, "karatsuba"));
+    S.push_back(make_stage(KNTT_INV, blob.add(intt_mu2), (int)intt_mu2.size(), 0, 0, 1, "intt_mu2"));
+    S.push_back(make_stage(KDECOMP, blob.add(dec_mu2), (int)dec_mu2.size(), beta, rows, 1, "decompose_mu2"));
+    S.push_back(make_stage(KMAC2, blob.add(mac_relin), (int)mac_relin.size(), beta, rows, 1, "mac_relin"));
+    S.push_back(make_stage(KMD_NTT, blob.add(md_relin), (int)md_relin.size(), nl, 0, 1, "moddown_relin"));
+    if (nres == 1)
+        S.push_back(make_stage(KNTT_INV, blob.add(intt_last), (int)intt_last.size(), 0, 0, 1, "intt_last"));
+    S.push_back(make_stage(KRESCALE_NTT, blob.add(resc), (int)resc.size(), nl - nres, 0, 1, "rescale"));
+    st = plan_finish(pl, blob, use_graph);
+    if (st != MKHIP_OK) return st;
+    *out = pl;
+    return MKHIP_OK;
+}
+
+// MKF CKKSCase.Run on host buffers: left/right [1+k][level+1][N] NTT domain;
+// out [1+k][level_out+1][N]: c0 then k copies of Gamma.  The rescale error of
+// Lattigo's Evaluator.Rescale is returned (ckks.go:141-149).
+extern "C" int mkhip_mkf_ckks_mulrelin(mkhip_ctx *c, int group, const uint64_t *left, const uint64_t *right,
+                                       int level, double scale_l, double scale_r, double min_scale, uint64_t *out,
+                                       int *level_out, double *scale_out) {
+    if (!c || !left || !right || !out) return MKHIP_ERR_ARG;
+    if (level < 0 || level >= c->L) return MKHIP_ERR_LEVEL;
+    auto it = c->mkf_groups.find(group);
+    if (it == c->mkf_groups.end()) return MKHIP_ERR_NO_KEY;
+    const int k = it->second.k;
+    int nres = 0;
+    double ns = scale_l * scale_r;
+    int s = mkhip_rescale_count(c, level, scale_l * scale_r, min_scale, &nres, &ns);
+    if (s != MKHIP_OK) return s;
+    if (nres > 1) return MKHIP_ERR_SCALE;
+    const int N = c->N, nl = level + 1;
+    const size_t in = (size_t)(1 + k) * nl * N, ow = (size_t)(1 + k) * nl * N, words = 2 * in + ow;
+    std::vector<int> sig = {1, group, k, level, nres};
+    if (!c->single || c->single->sig != sig || c->single_io_words < words) {
+        mkhip_plan_destroy(c->single);
+        c->single = nullptr;
+        if (c->single_io_words < words) {
+            if (c->single_io) (void)hipFree(c->single_io);
+            c->single_io = nullptr;
+            c->single_io_words = 0;
+            HIP_TRY(hipMalloc(&c->single_io, words * 8));
+            c->single_io_words = words;
+        }
+        mkhip_mkf_op op = {};
+        op.left = c->single_io;
+        op.right = c->single_io + in;
+        op.out = c->single_io + 2 * in;
+        op.group = group;
+        op.level = level;
+        op.nres = nres;
+        mkhip_plan *pl = nullptr;
+        s = mkhip_plan_mkf_ckks(c, 1, &op, 0, &pl);
+        if (s != MKHIP_OK) return s;
+        pl->sig = sig;
+        c->single = pl;
+    }
+    HIP_TRY(hipMemcpyAsync(c->single_io, left, in * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->single_io + in, right, in * 8, hipMemcpyHostToDevice, c->stream));
+    s = mkhip_plan_launch(c->single);
+    if (s != MKHIP_OK) return s;
+    const size_t outw = (size_t)(1 + k) * (nl - nres) * N;
+    HIP_TRY(hipMemcpyAsync(out, c->single_io + 2 * in, outw * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (level_out) *level_out = level - nres;
+    if (scale_out) *scale_out = ns;
+    return MKHIP_OK;
+}

@@ -1,0 +1,141 @@
+// Plans: a list of kernel launches over descriptor tables in device memory,
+// replayed eagerly or as one hipGraph.  Shared by the KKLSS, MKF and BFV paths.
+#include "mkhip_internal.hpp"
+
+#include <algorithm>
+
+using namespace mkhip;
+
+namespace mkhip {
+
+hipError_t plan_stage(mkhip_plan *pl, int s) {
+    const PlanStage &S = pl->stages[s];
+    mkhip_ctx *c = pl->ctx;
+    if (S.what >= KMAC2 && S.what <= KRESCALE_NTT) return launch_mkf(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream);
+    return launch_kernel(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream, S.c);
+}
+
+int plan_launch_raw(mkhip_plan *pl) {
+    for (int s = 0; s < (int)pl->stages.size(); ++s) HIP_TRY(plan_stage(pl, s));
+    return MKHIP_OK;
+}
+
+int plan_create(mkhip_ctx *c, size_t ws_bytes, mkhip_plan **out) {
+    *out = nullptr;
+    mkhip_plan *pl = new mkhip_plan();
+    pl->ctx = c;
+    hipError_t e = hipMalloc(&pl->d_mem, std::max<size_t>(ws_bytes, 256));
+    if (e != hipSuccess) {
+        delete pl;
+        return hip_status(e);
+    }
+    pl->bytes = ws_bytes;
+    *out = pl;
+    return MKHIP_OK;
+}
+
+int plan_finish(mkhip_plan *pl, const DescBlob &blob, int use_graph) {
+    mkhip_ctx *c = pl->ctx;
+    hipError_t e = hipMalloc(&pl->d_desc, std::max<size_t>(blob.host.size(), 256));
+    if (e == hipSuccess) e = hipMemcpy(pl->d_desc, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        mkhip_plan_destroy(pl);
+        return hip_status(e);
+    }
+    pl->bytes += blob.host.size();
+    for (PlanStage &s : pl->stages) s.descs = (const char *)pl->d_desc + reinterpret_cast<size_t>(s.descs);
+    if (use_graph) {
+        // first launch eagerly (sets kernel attributes), then capture the sequence
+        int st = plan_launch_raw(pl);
+        if (st == MKHIP_OK) st = hip_status(hipStreamSynchronize(c->stream));
+        if (st == MKHIP_OK) st = hip_status(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        if (st == MKHIP_OK) {
+            int s2 = plan_launch_raw(pl);
+            hipError_t e2 = hipStreamEndCapture(c->stream, &pl->g);
+            st = s2 != MKHIP_OK ? s2 : hip_status(e2);
+        }
+        if (st == MKHIP_OK) st = hip_status(hipGraphInstantiate(&pl->ge, pl->g, nullptr, nullptr, 0));
+        if (st != MKHIP_OK) {
+            mkhip_plan_destroy(pl);
+            return st;
+        }
+        pl->graph = true;
+    }
+    return MKHIP_OK;
+}
+
+}  // namespace mkhip
+
+extern "C" int mkhip_plan_launch(mkhip_plan *pl) {
+    if (!pl) return MKHIP_ERR_ARG;
+    if (pl->graph) {
+        HIP_TRY(hipGraphLaunch(pl->ge, pl->ctx->stream));
+        return MKHIP_OK;
+    }
+    return plan_launch_raw(pl);
+}
+
+extern "C" int mkhip_plan_time(mkhip_plan *pl, int iters, double *ms) {
+    if (!pl || iters <= 0 || !ms) return MKHIP_ERR_ARG;
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, pl->ctx->stream));
+    for (int i = 0; i < iters; ++i) {
+        int s = mkhip_plan_launch(pl);
+        if (s != MKHIP_OK) return s;
+    }
+    HIP_TRY(hipEventRecord(b, pl->ctx->stream));
+    HIP_TRY(hipEventSynchronize(b));
+    float f = 0;
+    HIP_TRY(hipEventElapsedTime(&f, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = f / iters;
+    return MKHIP_OK;
+}
+
+extern "C" size_t mkhip_plan_workspace_bytes(const mkhip_plan *pl) { return pl ? pl->bytes : 0; }
+
+extern "C" int mkhip_plan_stage_count(const mkhip_plan *pl) { return pl ? (int)pl->stages.size() : 0; }
+
+extern "C" const char *mkhip_plan_stage_name(const mkhip_plan *pl, int s) {
+    if (!pl || s < 0 || s >= (int)pl->stages.size()) return "";
+    return pl->stages[s].name.c_str();
+}
+
+// Average device time of each launch of the plan (HIP events on the context
+// stream around each kernel, eager launches), in ms; ms holds stage_count doubles.
+extern "C" int mkhip_plan_stage_times(mkhip_plan *pl, int iters, double *ms) {
+    if (!pl || iters <= 0 || !ms) return MKHIP_ERR_ARG;
+    const int S = (int)pl->stages.size();
+    hipStream_t st = pl->ctx->stream;
+    std::vector<hipEvent_t> ev(S + 1);
+    for (int i = 0; i <= S; ++i) HIP_TRY(hipEventCreate(&ev[i]));
+    std::vector<double> acc(S, 0.0);
+    for (int it = 0; it < iters; ++it) {
+        HIP_TRY(hipEventRecord(ev[0], st));
+        for (int s = 0; s < S; ++s) {
+            HIP_TRY(plan_stage(pl, s));
+            HIP_TRY(hipEventRecord(ev[s + 1], st));
+        }
+        HIP_TRY(hipEventSynchronize(ev[S]));
+        for (int s = 0; s < S; ++s) {
+            float f = 0;
+            HIP_TRY(hipEventElapsedTime(&f, ev[s], ev[s + 1]));
+            acc[s] += f;
+        }
+    }
+    for (int i = 0; i <= S; ++i) (void)hipEventDestroy(ev[i]);
+    for (int s = 0; s < S; ++s) ms[s] = acc[s] / iters;
+    return MKHIP_OK;
+}
+
+extern "C" void mkhip_plan_destroy(mkhip_plan *plan) {
+    if (!plan) return;
+    if (plan->ge) (void)hipGraphExecDestroy(plan->ge);
+    if (plan->g) (void)hipGraphDestroy(plan->g);
+    if (plan->d_mem) (void)hipFree(plan->d_mem);
+    if (plan->d_desc) (void)hipFree(plan->d_desc);
+    delete plan;
+}

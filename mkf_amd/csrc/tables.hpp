@@ -49,6 +49,50 @@ enum KernelId {
     KEXT2 = 7,           // k_ext<LOGN, 2>         two keys sharing one decomposition
     KMODDOWN_FINAL = 8,  // k_moddown<true>        + accumulation into the output + rescale
     KMAC_XY = 9,         // k_mac_xy               key-stationary x / y accumulation
+    KMAC2 = 10,          // k_mac2<LOGN>           MKF switch: two-output key MAC (+ INTT)
+    KMD_NTT = 11,        // k_moddown_ntt<LOGN>    NTT-domain ModDown, summed over a fold
+    KKARA = 12,          // k_karatsuba            MKF tensor (Karatsuba, NTT domain)
+    KRESCALE_NTT = 13,   // k_rescale_ntt<LOGN>    NTT-domain rescale + materialized copies
+};
+
+// ---- MKF (mkf_cpu) descriptors ----------------------------------------------
+
+// One hoisted key switch without ModDown (Lattigo KeyswitchHoistedNoModDown, called
+// from mkf_cpu/common.go:82-96): acc_o[r] = sum_b swk[b][o][r] (.) D[b][r]
+// for o = 0, 1; the P rows (every row with intt_all) are returned to the
+// coefficient domain.  swk layout [beta][2][L+K][N].
+struct Mac2Desc {
+    const u64 *D;
+    const u64 *swk;
+    u64 *acc[2];
+    int level, intt_all;
+};
+
+// NTT-domain ModDown summed over nacc switches (ModDownQPtoQNTT,
+// basis_extension.go:239-288, folded as in mkf_cpu/common.go:109-126):
+// dst[j] = base[j] + sum_a (acc_a[j] - NTT_j(ModUp(acc_a P rows))) P^-1.
+struct MdNttDesc {
+    const u64 *acc[kMaxParties];
+    const u64 *base;       // NTT domain, may be null
+    u64 *dst;              // level+1 limbs
+    int nacc, level;
+};
+
+// Karatsuba tensor (mkf_cpu/ckks.go:120-131), NTT domain, level+1 limbs.
+struct KaraDesc {
+    const u64 *l0, *l1, *r0, *r1;
+    u64 *mu0, *mu1, *mu2;
+};
+
+// Rescale by the last modulus in the NTT domain (nres = 1) or a plain copy
+// (nres = 0), written to nout destinations (the k-fold materialization of Gamma,
+// mkf_cpu/common.go:128-139).  xlast: the last limb already in the coefficient domain.
+struct RescaleNttDesc {
+    const u64 *x;
+    const u64 *xlast;
+    u64 *out[kMaxParties + 1];
+    int nout, level, nres;
+    int pad;
 };
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on XCD

@@ -172,6 +172,75 @@ class KKLSSKeys:
         return acc
 
 
+class MKFKeys:
+    """MKF aggregate-secret keys (software/mkf/mkf_cpu/common.go:32-51) over a
+    gamma = 1 context: secrets s_i, s = sum s_i, fold keys s_i -> s and the
+    relinearization key s^2 -> s, each a Lattigo switching key
+    [beta][2][L+K][N] in MForm: ([-a s_out + e + P s_in g_b], [a])."""
+
+    def __init__(self, ctx, k, seed=0):
+        self.ctx = ctx
+        g = O.Prng(seed)
+        N, QP = ctx.N, ctx.QP
+        self.small = [g.ternary(N) for _ in range(k)]
+        agg = np.sum(np.stack(self.small), axis=0)
+        self.agg_small = agg
+        self.s = self._ntt_small(agg)                 # plain NTT(s) over QP
+        s_in = [self._ntt_small(x) for x in self.small]
+        self.fold = [self._swk(g, si, self.s) for si in s_in]
+        s2 = mulmod(self.s, self.s, QP)
+        self.relin = self._swk(g, s2, self.s)
+        self.pk = [self._pk(g, self._ntt_small(x)) for x in self.small]
+
+    def _ntt_small(self, x):
+        return self.ctx.ntt_poly(O.small_to_rns(x, self.ctx.QP))
+
+    def _swk(self, g, s_in, s_out):
+        ctx = self.ctx
+        QP, N = ctx.QP, ctx.N
+        Pprod = 1
+        for p in ctx.P:
+            Pprod *= p
+        out = []
+        for b in range(ctx.beta()):
+            a = g.uniform(QP, N)
+            e = ctx.ntt_poly(O.small_to_rns(g.gaussian(N), QP))
+            v0 = submod(e, mulmod(a, s_out, QP), QP)
+            for j in range(ctx.alpha):
+                idx = b * ctx.alpha + j
+                if idx >= ctx.L:
+                    break
+                q = QP[idx]
+                v0[idx] = np.array((v0[idx].astype(object) + s_in[idx].astype(object) * (Pprod % q)) % q, np.uint64)
+            out.append(np.stack([mform(v0, QP), mform(a, QP)]))
+        return np.stack(out)
+
+    def _pk(self, g, s):
+        ctx = self.ctx
+        Q, N = ctx.Q, ctx.N
+        a = g.uniform(Q, N)
+        e = ctx.ntt_poly(O.small_to_rns(g.gaussian(N), Q))
+        return submod(e, mulmod(a, s[:ctx.L], Q), Q), a
+
+    def encrypt(self, party, m_coeff, g):
+        """NTT-domain public-key encryption (the multiCiphertext inputs of ckks.go:81-100)."""
+        ctx = self.ctx
+        Q, N = ctx.Q, ctx.N
+        pk0, pk1 = self.pk[party]
+        v = ctx.ntt_poly(O.small_to_rns(g.ternary(N), Q))
+        e0 = ctx.ntt_poly(O.small_to_rns(g.gaussian(N), Q))
+        e1 = ctx.ntt_poly(O.small_to_rns(g.gaussian(N), Q))
+        m = ctx.ntt_poly(m_coeff)
+        return addmod(addmod(mulmod(v, pk0, Q), e0, Q), m, Q), addmod(mulmod(v, pk1, Q), e1, Q)
+
+    def decrypt(self, c0, c1):
+        """c0 + c1 s (NTT domain in, coefficient domain out)."""
+        ctx = self.ctx
+        nl = c0.shape[0]
+        Q = ctx.Q[:nl]
+        return ctx.ntt_poly(addmod(c0, mulmod(c1, self.s[:nl], Q), Q), inverse=True)
+
+
 def centered_crt(x, moduli):
     """Lift an RNS poly (nlimbs, N) to centered integers via CRT (Python ints)."""
     Qp = 1

@@ -307,3 +307,55 @@ def test_planted_relin_mistake_is_caught():
     got = G.centered_crt(keys.decrypt(out, ids), Q)
     exp = G.negacyclic_mul([D] * ctx.N, [D] * ctx.N)
     assert max(abs(a - b) for a, b in zip(got, exp)) > 2 ** 80
+
+
+def test_mkf_ckks_mulrelin_decrypts_to_the_product():
+    """MKF CKKSCase.Run restated: fold + Karatsuba + relin + rescale decrypts under
+    the aggregate secret to round(m_left m_right / q_L) (ckks.go:155-165 checks the
+    decoded product within 1e-6; here the coefficient-domain bound)."""
+    logN = 10
+    Q, P = O.gen_primes(50, 15, 4), O.gen_primes(60, 15, 2)
+    ctx = O.Ctx(logN, Q, P, gamma=1)  # Lattigo rlwe decomposition: alpha = K = 2
+    k = 3
+    keys = G.MKFKeys(ctx, k, seed=5)
+    g = O.Prng(6)
+    N = ctx.N
+    D = 1 << 40
+
+    def small():
+        return (np.array([int(v) for v in g.uniform([15], N)[0]]) - 7).astype(np.int64)
+
+    def multi(xs):
+        parts = [keys.encrypt(p, O.small_to_rns(x * D, Q), g) for p, x in enumerate(xs)]
+        c0 = parts[0][0]
+        for p in parts[1:]:
+            c0 = G.addmod(c0, p[0], Q)
+        return np.stack([c0] + [p[1] for p in parts])
+
+    xs, ys = [small() for _ in range(k)], [small() for _ in range(k)]
+    left, right = multi(xs), multi(ys)
+    o0, o1 = ctx.mkf_ckks_mulrelin(left, right, keys.fold, keys.relin)
+    got = G.centered_crt(keys.decrypt(o0, o1), Q)
+    X = [int(v) * D for v in np.sum(np.stack(xs), axis=0)]
+    Y = [int(v) * D for v in np.sum(np.stack(ys), axis=0)]
+    exp = G.negacyclic_mul(X, Y)
+    assert max(abs(a - b) for a, b in zip(got, exp)) < 2 ** 62
+    r0, r1 = ctx.div_round_last(o0, 1, ntt=True), ctx.div_round_last(o1, 1, ntt=True)
+    got2 = G.centered_crt(keys.decrypt(r0, r1), Q[:-1])
+    exp2 = [(e + Q[-1] // 2) // Q[-1] for e in exp]
+    assert max(abs(a - b) for a, b in zip(got2, exp2)) < 2 ** 14
+
+
+def test_mkf_switch_is_a_key_switch():
+    """switchOne(c, fold_i) = (p0, p1) with p0 + p1 s = c s_i + small (common.go:79-107)."""
+    logN = 9
+    Q, P = O.gen_primes(50, 15, 3), O.gen_primes(60, 15, 1)
+    ctx = O.Ctx(logN, Q, P, gamma=1)  # alpha = K = 1: the copy decomposition
+    keys = G.MKFKeys(ctx, 2, seed=7)
+    g = O.Prng(8)
+    c = g.uniform(Q, ctx.N)
+    p0, p1 = ctx.mkf_switch_one(c, keys.fold[1], in_ntt=True, ntt_output=True)
+    got = G.centered_crt(keys.decrypt(p0, p1), Q)
+    si = ctx.ntt_poly(O.small_to_rns(keys.small[1], Q))
+    want = G.centered_crt(ctx.ntt_poly(G.mulmod(c, si, Q), inverse=True), Q)
+    assert max(abs(a - b) for a, b in zip(got, want)) < 2 ** 30
