@@ -64,7 +64,10 @@ const char *mkhip_status_string(int status);
  * mkrlwe.NewParameters (mkrlwe/params.go:16-61) and mkckks.NewParameters
  * (mkckks/params.go:17-25): N = 2^logN, Q[0..L), P[0..K), gamma = 2 for the KKLSS
  * evaluators (alpha = K/gamma) and 1 for MKF (Lattigo rlwe, alpha = K).
- * logN in {13, 14}.  t and qmul are for BFV contexts (0 / NULL otherwise).
+ * logN in {13, 14}.  t and qmul are for BFV contexts (0 / NULL otherwise): a
+ * non-null qmul (L primes, the QMul chain of mkbfv.ParametersLiteral,
+ * mkbfv/params.go:12-81) makes a KKLSS MK-BFV context over R = Q u QMul with
+ * plaintext modulus t; it needs gamma = 2 and alpha = K/2 = 1 (K = 2 or 3).
  */
 int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, int L, const uint64_t *p, int K,
                      int gamma, uint64_t t, const uint64_t *qmul, int flags);
@@ -158,6 +161,34 @@ typedef struct {
 
 int mkhip_plan_mkf_ckks(mkhip_ctx *ctx, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out);
 
+/*
+ * KKLSS MK-BFV (software/kklss/mkbfv).  A party's key is mkbfv.RelinearizationKey
+ * (mkbfv/keys.go:6-9): two mkrlwe relinearization keys, one per gadget branch;
+ * the evaluator reads b and d of both branches and v of branch 0
+ * (keyswitch_hoisted.go:105-229), each [beta][L+K][N] as above.  Replaces
+ * RelinearizationKeySet.AddRelinearizationKey (mkbfv/keys.go:57-67).
+ */
+int mkhip_register_bfv_party(mkhip_ctx *ctx, int party, const uint64_t *b0, const uint64_t *d0,
+                             const uint64_t *v0, const uint64_t *b1, const uint64_t *d1);
+/* mkbfv.Evaluator.MulRelinNew (evaluator.go:85-91): host buffers op0 [1+k0][L][N],
+ * op1 [1+k1][L][N] coefficient domain; out [1+ku][L][N] with the union of ids in
+ * ascending order written to ids_out (ku entries). */
+int mkhip_kklss_bfv_mulrelin(mkhip_ctx *ctx, const uint64_t *op0, const int *ids0, int k0, const uint64_t *op1,
+                             const int *ids1, int k1, uint64_t *out, int *ids_out, int *ku_out);
+/* batched over device operands: every op has level = L-1 and nres = 0 */
+int mkhip_plan_kklss_bfv(mkhip_ctx *ctx, int nops, const mkhip_kklss_op *ops, int use_graph, mkhip_plan **out);
+/* R-basis primitives of mkbfv.FastBasisExtender (mkbfv/basis_extension.go), host buffers:
+ *   mkhip_bfv_modup_q_to_r   ModUpQtoR (:50-66): [L][N] -> [2L][N] (QMul limbs unreduced,
+ *                            congruent, as multSum returns them)
+ *   mkhip_bfv_rescale_q_to_r Rescale (:88-104): floor(QMul x / Q) into R, [L][N] -> [2L][N]
+ *   mkhip_bfv_quantize       Quantize (:69-85): NTT-domain [2L][N] -> [L][N] coefficient
+ *   mkhip_bfv_decompose      KeySwitcher.DecomposeBFV (mkbfv/keyswitch.go:58-83):
+ *                            [2L][N] -> [2L][L+K][N] NTT domain */
+int mkhip_bfv_modup_q_to_r(mkhip_ctx *ctx, const uint64_t *in, uint64_t *out);
+int mkhip_bfv_rescale_q_to_r(mkhip_ctx *ctx, const uint64_t *in, uint64_t *out);
+int mkhip_bfv_quantize(mkhip_ctx *ctx, const uint64_t *in_ntt, uint64_t *out);
+int mkhip_bfv_decompose(mkhip_ctx *ctx, const uint64_t *aR, uint64_t *out);
+
 /* reference scale loop (mkckks/evaluator.go:383-388): number of rescales */
 int mkhip_rescale_count(const mkhip_ctx *ctx, int level, double scale, double min_scale, int *nres,
                         double *new_scale);
@@ -166,7 +197,7 @@ int mkhip_rescale_count(const mkhip_ctx *ctx, int level, double scale, double mi
  * RNS primitives on host buffers -- the per-primitive operations the reference
  * times in software/mkf/cmd/rnsbench/main.go:179-201.
  *   mkhip_ntt_limbs        ring NTTLvl / InvNTTLvl of nlimbs rows; ring_idx[i] selects
- *                          the modulus (0..L-1 = Q, L..L+K-1 = P) of row i
+ *                          the modulus (0..L-1 = Q, L..L+K-1 = P, then BFV QMul) of row i
  *   mkhip_decompose        mkrlwe KeySwitcher.Decompose (mkrlwe/keyswitch.go:50-76):
  *                          a [level+1][N] coefficient -> [beta][level+1+K][N] NTT
  *   mkhip_moddown          FastBasisExtender.ModDownQPtoQ (basis_extension.go:192-232):

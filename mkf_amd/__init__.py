@@ -34,6 +34,8 @@ EXPORTS = [
     "mkhip_ntt_limbs", "mkhip_decompose", "mkhip_moddown", "mkhip_external_product",
     "mkhip_plan_stage_times", "mkhip_plan_stage_count", "mkhip_plan_stage_name",
     "mkhip_register_mkf_group", "mkhip_unregister_mkf_group", "mkhip_mkf_ckks_mulrelin", "mkhip_plan_mkf_ckks",
+    "mkhip_register_bfv_party", "mkhip_kklss_bfv_mulrelin", "mkhip_plan_kklss_bfv", "mkhip_bfv_modup_q_to_r",
+    "mkhip_bfv_rescale_q_to_r", "mkhip_bfv_quantize", "mkhip_bfv_decompose",
 ]
 
 # kernel run by each named plan stage
@@ -44,7 +46,10 @@ STAGE_KERNELS = {
     "moddown_fold": "k_moddown_ntt", "karatsuba": "k_karatsuba", "intt_mu2": "k_ntt_batch<inv>",
     "decompose_mu2": "k_decomp", "mac_relin": "k_mac2", "moddown_relin": "k_moddown_ntt",
     "intt_last": "k_ntt_batch<inv>", "rescale": "k_rescale_ntt",
+    "lift": "k_bfv_lift", "quantize": "k_quantize", "moddown_final": "k_moddown<true>",
 }
+# the BFV plan's tensor stage runs the R-basis kernel
+BFV_STAGE_KERNELS = dict(STAGE_KERNELS, tensor="k_tensor_r")
 
 
 class MkhipError(RuntimeError):
@@ -121,6 +126,11 @@ def _declare(L):
     L.mkhip_unregister_mkf_group.argtypes = [V, I]
     L.mkhip_mkf_ckks_mulrelin.argtypes = [V, I, u64p, u64p, I, D, D, D, u64p, intp, dblp]
     L.mkhip_plan_mkf_ckks.argtypes = [V, I, ctypes.POINTER(MKFOp), I, ctypes.POINTER(V)]
+    L.mkhip_register_bfv_party.argtypes = [V, I, u64p, u64p, u64p, u64p, u64p]
+    L.mkhip_kklss_bfv_mulrelin.argtypes = [V, u64p, intp, I, u64p, intp, I, u64p, intp, intp]
+    L.mkhip_plan_kklss_bfv.argtypes = [V, I, ctypes.POINTER(KKLSSOp), I, ctypes.POINTER(V)]
+    for f in ("mkhip_bfv_modup_q_to_r", "mkhip_bfv_rescale_q_to_r", "mkhip_bfv_quantize", "mkhip_bfv_decompose"):
+        getattr(L, f).argtypes = [V, u64p, u64p]
 
 
 def _p(a):
@@ -142,14 +152,18 @@ class Context:
     """Parameter set + device tables + resident keys (mkhip_ctx)."""
 
     def __init__(self, logN, Q, P, gamma=2, t=0, qmul=None, flags=0):
+        """qmul (L primes) and t make a KKLSS MK-BFV context over R = Q u QMul."""
         self.logN, self.N = logN, 1 << logN
         self.Q, self.P = [int(q) for q in Q], [int(p) for p in P]
         self.L, self.K, self.gamma = len(self.Q), len(self.P), gamma
+        self.QMul = [int(q) for q in qmul] if qmul is not None else None
+        self.t = int(t)
         h = ctypes.c_void_p()
         q = _u64(self.Q)
         p = _u64(self.P or [0])
-        check(lib().mkhip_ctx_create(ctypes.byref(h), logN, _p(q), self.L, _p(p), self.K, gamma, t,
-                                     None, flags), "mkhip_ctx_create")
+        qm = _u64(self.QMul) if self.QMul is not None else None
+        check(lib().mkhip_ctx_create(ctypes.byref(h), logN, _p(q), self.L, _p(p), self.K, gamma, self.t,
+                                     _p(qm) if qm is not None else None, flags), "mkhip_ctx_create")
         self._h = h
         a, b, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
         check(lib().mkhip_ctx_info(h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(w)), "mkhip_ctx_info")
@@ -216,6 +230,52 @@ class Context:
         nlo = lo.value + 1
         out = out[:(1 + kuo.value) * nlo * self.N].reshape(1 + kuo.value, nlo, self.N)
         return out, list(iout)[:kuo.value], lo.value, so.value
+
+    # ---- KKLSS MK-BFV ----
+    def register_bfv_party(self, pid, bkey, dkey, vkey):
+        """bkey / dkey: both gadget branches stacked, (2 beta, L+K, N); vkey (beta, L+K, N)."""
+        b, d, v = _u64(bkey), _u64(dkey), _u64(vkey)
+        w = self.swk_words
+        assert b.size == 2 * w and d.size == 2 * w and v.size == w
+        b, d = b.reshape(2, w), d.reshape(2, w)
+        b0, b1, d0, d1 = (np.ascontiguousarray(x) for x in (b[0], b[1], d[0], d[1]))
+        check(lib().mkhip_register_bfv_party(self._h, pid, _p(b0), _p(d0), _p(v), _p(b1), _p(d1)),
+              "mkhip_register_bfv_party")
+
+    def kklss_bfv_mulrelin(self, op0, ids0, op1, ids1):
+        """mkbfv MulRelinNew: op (1+k, L, N) coefficient domain -> (out, ids_out)."""
+        op0, op1 = _u64(op0), _u64(op1)
+        k0, k1 = len(ids0), len(ids1)
+        assert op0.shape == (1 + k0, self.L, self.N) and op1.shape == (1 + k1, self.L, self.N)
+        ku = len(set(ids0) | set(ids1))
+        out = np.zeros((1 + ku, self.L, self.N), np.uint64)
+        i0 = (ctypes.c_int * max(k0, 1))(*ids0)
+        i1 = (ctypes.c_int * max(k1, 1))(*ids1)
+        iout = (ctypes.c_int * max(ku, 1))()
+        kuo = ctypes.c_int()
+        check(lib().mkhip_kklss_bfv_mulrelin(self._h, _p(op0), i0, k0, _p(op1), i1, k1, _p(out), iout,
+                                             ctypes.byref(kuo)), "mkhip_kklss_bfv_mulrelin")
+        return out, list(iout)[:kuo.value]
+
+    def bfv_modup_q_to_r(self, a):
+        out = np.zeros((2 * self.L, self.N), np.uint64)
+        check(lib().mkhip_bfv_modup_q_to_r(self._h, _p(_u64(a)), _p(out)), "mkhip_bfv_modup_q_to_r")
+        return out
+
+    def bfv_rescale_q_to_r(self, a):
+        out = np.zeros((2 * self.L, self.N), np.uint64)
+        check(lib().mkhip_bfv_rescale_q_to_r(self._h, _p(_u64(a)), _p(out)), "mkhip_bfv_rescale_q_to_r")
+        return out
+
+    def bfv_quantize(self, a_ntt):
+        out = np.zeros((self.L, self.N), np.uint64)
+        check(lib().mkhip_bfv_quantize(self._h, _p(_u64(a_ntt)), _p(out)), "mkhip_bfv_quantize")
+        return out
+
+    def bfv_decompose(self, aR):
+        out = np.zeros((2 * self.L, self.L + self.K, self.N), np.uint64)
+        check(lib().mkhip_bfv_decompose(self._h, _p(_u64(aR)), _p(out)), "mkhip_bfv_decompose")
+        return out
 
     # ---- MKF ----
     def register_mkf_group(self, group, fold_keys, relin_key):
@@ -300,11 +360,12 @@ class Plan:
     """A batched mul+relin over device-resident operands (one hipGraph per launch).
 
     scheme "kklss": ops are dicts {op0, op1, out, ids0, ids1, level, nres};
+    scheme "bfv":   the same dicts, level = L-1 and nres = 0 (KKLSS MK-BFV);
     scheme "mkf":   ops are dicts {left, right, out, group, level, nres}."""
 
     def __init__(self, ctx, ops, use_graph=True, scheme="kklss"):
         h = ctypes.c_void_p()
-        if scheme == "kklss":
+        if scheme in ("kklss", "bfv"):
             arr = (KKLSSOp * len(ops))()
             for i, o in enumerate(ops):
                 a = arr[i]
@@ -314,8 +375,8 @@ class Plan:
                 for j, v in enumerate(o["ids1"]):
                     a.ids1[j] = v
                 a.k0, a.k1, a.level, a.nres = len(o["ids0"]), len(o["ids1"]), o["level"], o["nres"]
-            check(lib().mkhip_plan_kklss_ckks(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)),
-                  "mkhip_plan_kklss_ckks")
+            fn = "mkhip_plan_kklss_ckks" if scheme == "kklss" else "mkhip_plan_kklss_bfv"
+            check(getattr(lib(), fn)(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)), fn)
         elif scheme == "mkf":
             arr = (MKFOp * len(ops))()
             for i, o in enumerate(ops):

@@ -10,7 +10,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmkhip.so")
-SOURCES = ["kernels.hip", "kernels_mkf.hip", "mkhip.hip", "mkhip_plan.hip", "mkhip_kklss.hip", "mkhip_mkf.hip"]
+SOURCES = ["kernels.hip", "kernels_mkf.hip", "kernels_bfv.hip", "mkhip.hip", "mkhip_plan.hip",
+           "mkhip_kklss.hip", "mkhip_mkf.hip", "mkhip_bfv.hip"]
+OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -29,17 +31,34 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l)
-    if res.returncode != 0:
+    return res.returncode, out
+
+
+def build(verbose=False, jobs=None):
+    """Compile every source to an object in parallel, then link the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs = [os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    cmds = [[HIPCC] + compile_flags + ["-c", os.path.join(CSRC, s), "-o", o] for s, o in zip(SOURCES, objs)]
+    jobs = jobs or min(len(cmds), max(1, (os.cpu_count() or 2) - 1), 8)
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(lambda c: _run(c, verbose), cmds))
+    for (rc, out), src in zip(results, SOURCES):
+        if rc != 0:
+            sys.stderr.write(out + "\n")
+            raise RuntimeError("hipcc failed compiling %s" % src)
+    tmp = LIB + ".tmp"
+    rc, out = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp], verbose)
+    if rc != 0:
         sys.stderr.write(out + "\n")
-        raise RuntimeError("hipcc failed building %s" % LIB)
+        raise RuntimeError("hipcc failed linking %s" % LIB)
     os.replace(tmp, LIB)
     return LIB
 

@@ -28,11 +28,13 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_batch(DevTables T, cons
     const ModConst m = T.mod[j.mod];
     u64 v[16];
     load_row_cached<LOGN>(v, j.src, tid);
-    if (j.reduce) {
+    if (j.reduce & 1) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) v[i] = reduce64(v[i], m.q, m.mu);
     }
-    if (INV)
+    if (INV && (j.reduce & 2))
+        ntt_inv<LOGN>(v, g_lds, tid, T.itw + (size_t)j.mod * N, m.q, m.tninv, m.tninvp);
+    else if (INV)
         ntt_inv<LOGN>(v, g_lds, tid, T.itw + (size_t)j.mod * N, m.q, m.ninv, m.ninvp);
     else
         ntt_fwd<LOGN>(v, g_lds, tid, T.tw + (size_t)j.mod * N, m.q);
@@ -105,6 +107,8 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
 
 // Key-stationary x / y accumulation: grid (N/256, beta*rows, 2 sides); the
 // thread of coefficient c of (digit i, row r) walks all nops ops of the batch.
+// x / y are written as full-level switching keys ([beta][L+K][N], row = ring
+// index), the layout k_ext reads keys in.
 __global__ void __launch_bounds__(256)
     k_mac_xy(DevTables T, const MacDesc *__restrict__ descs, int nops, int beta, int rows) {
     const int N = T.N;
@@ -122,7 +126,7 @@ __global__ void __launch_bounds__(256)
         u64 acc = 0;
         for (int p = 0; p < Md.k; ++p)
             acc = add_mod(acc, mred(Md.key[p][koff], Md.D[p * dstride + doff], m.q, m.qinv), m.q);
-        Md.xy[doff] = mred(acc, m.r2, m.q, m.qinv);
+        Md.xy[koff] = mred(acc, m.r2, m.q, m.qinv);  // full-level key layout, like the keys
     }
 }
 

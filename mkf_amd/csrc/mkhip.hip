@@ -135,19 +135,28 @@ extern "C" const char *mkhip_status_string(int s) {
 
 extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, int L, const uint64_t *p, int K,
                                 int gamma, uint64_t t, const uint64_t *qmul, int flags) {
-    (void)qmul;
     if (!out || !q || (K > 0 && !p) || L <= 0 || K < 0 || gamma <= 0) return MKHIP_ERR_ARG;
     *out = nullptr;
     if (logN != 13 && logN != 14) return MKHIP_ERR_PARAMS;
     if (L > 16 || K > 4 || L + K > kMaxModuli) return MKHIP_ERR_PARAMS;
+    const bool bfv = qmul != nullptr;
     const int N = 1 << logN;
-    for (int i = 0; i < L + K; ++i) {
-        u64 m = i < L ? q[i] : p[i - L];
+    const int nmod = L + K + (bfv ? L : 0);
+    std::vector<u64> all(q, q + L);
+    all.insert(all.end(), p, p + K);
+    if (bfv) all.insert(all.end(), qmul, qmul + L);
+    for (int i = 0; i < nmod; ++i) {
+        const u64 m = all[i];
         if (m >= (1ULL << 61) || ((m - 1) % (2 * (u64)N)) != 0 || !is_prime64(m)) return MKHIP_ERR_PARAMS;
+        for (int j = 0; j < i; ++j)
+            if (all[j] == m) return MKHIP_ERR_PARAMS;
     }
     const int alpha = K / gamma;
     if (K > 0 && alpha == 0) return MKHIP_ERR_GADGET;
     if (alpha > kMaxAlpha) return MKHIP_ERR_PARAMS;
+    // KKLSS MK-BFV (mkbfv/params.go:36-81): gamma 2 over Q u P, R = Q u QMul,
+    // plaintext modulus t; the R-poly decomposition is implemented for alpha = 1.
+    if (bfv && (gamma != 2 || alpha != 1 || t < 2)) return MKHIP_ERR_PARAMS;
 
     mkhip_ctx *c = new mkhip_ctx();
     c->logN = logN;
@@ -161,9 +170,10 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     c->flags = flags;
     c->Q.assign(q, q + L);
     c->P.assign(p, p + K);
+    c->bfv = bfv;
+    if (bfv) c->QMul.assign(qmul, qmul + L);
     c->swk_words = (size_t)c->beta_max * (L + K) * N;
     c->mkf_swk_words = 2 * c->swk_words;
-    const int nmod = L + K;
     std::vector<u64> QP(c->Q);
     QP.insert(QP.end(), c->P.begin(), c->P.end());
 
@@ -171,7 +181,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     std::vector<ModConst> mods(nmod);
     std::vector<ulonglong2> tw((size_t)nmod * N), itw((size_t)nmod * N);
     for (int i = 0; i < nmod; ++i) {
-        const u64 qi = QP[i];
+        const u64 qi = all[i];
         ModConst &m = mods[i];
         m.q = qi;
         m.qinv = h_qinv(qi);
@@ -179,6 +189,8 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
         m.r2 = (u64)((((u128)1 << 64) % qi) * (u128)(((u128)1 << 64) % qi) % qi);
         m.ninv = h_inv((u64)N, qi);
         m.ninvp = h_shoup_companion(m.ninv, qi);
+        m.tninv = h_mulmod(t % qi, m.ninv, qi);
+        m.tninvp = h_shoup_companion(m.tninv, qi);
         const u64 g = primitive_root(qi, (flags & MKHIP_ROOT_RULE_SMALLEST2) != 0);
         const u64 psi = h_powmod(g, (qi - 1) / (2 * (u64)N), qi);
         const u64 psiinv = h_inv(psi, qi);
@@ -204,7 +216,8 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     // gadget digits with nd >= 2 primes
     const int beta = c->beta_max > 0 ? c->beta_max : 1;
     const size_t ne = (size_t)beta * (kMaxAlpha + 1);
-    std::vector<u64> dc_y(ne * kMaxAlpha, 0), dc_qstar(ne * nmod * kMaxAlpha, 0), dc_vt(ne * nmod * (kMaxAlpha + 1), 0);
+    const int nqp = L + K;  // digit rows span Q u P
+    std::vector<u64> dc_y(ne * kMaxAlpha, 0), dc_qstar(ne * nqp * kMaxAlpha, 0), dc_vt(ne * nqp * (kMaxAlpha + 1), 0);
     for (int i = 0; i < c->beta_max; ++i) {
         for (int nd = 2; nd <= alpha; ++nd) {
             if (i * alpha + nd > L) continue;
@@ -212,9 +225,9 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
             modup_params(Qin, QP, y, qs, vt);
             const size_t e = (size_t)i * (kMaxAlpha + 1) + nd;
             for (int l = 0; l < nd; ++l) dc_y[e * kMaxAlpha + l] = y[l];
-            for (int r = 0; r < nmod; ++r) {
-                for (int l = 0; l < nd; ++l) dc_qstar[(e * nmod + r) * kMaxAlpha + l] = qs[(size_t)r * nd + l];
-                for (int v = 0; v <= nd; ++v) dc_vt[(e * nmod + r) * (kMaxAlpha + 1) + v] = vt[(size_t)r * (nd + 1) + v];
+            for (int r = 0; r < nqp; ++r) {
+                for (int l = 0; l < nd; ++l) dc_qstar[(e * nqp + r) * kMaxAlpha + l] = qs[(size_t)r * nd + l];
+                for (int v = 0; v <= nd; ++v) dc_vt[(e * nqp + r) * (kMaxAlpha + 1) + v] = vt[(size_t)r * (nd + 1) + v];
             }
         }
     }
@@ -225,6 +238,28 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
             rs_inv[(size_t)l * L + j] = h_mform(h_inv(c->Q[l] % c->Q[j], c->Q[j]), c->Q[j]);
             rs_half[(size_t)l * L + j] = (c->Q[l] >> 1) % c->Q[j];
         }
+
+    // BFV basis extensions between Q and QMul (mkbfv/basis_extension.go:21-104)
+    std::vector<u64> bq_qm, bq_up_y, bq_up_qs, bq_up_vt, bq_dn_y, bq_dn_qs, bq_dn_vt, bq_rsc, bq_qz;
+    if (bfv) {
+        const std::vector<u64> &QM = c->QMul;
+        modup_params(c->Q, QM, bq_up_y, bq_up_qs, bq_up_vt);
+        modup_params(QM, c->Q, bq_dn_y, bq_dn_qs, bq_dn_vt);
+        bq_qm.resize(L);
+        bq_rsc.resize(L);
+        bq_qz.resize(L);
+        for (int i = 0; i < L; ++i) {
+            u64 qm = 1, qzi = 1, rsc = 1;
+            for (int j = 0; j < L; ++j) {
+                qm = h_mulmod(qm, QM[j] % c->Q[i], c->Q[i]);
+                qzi = h_mulmod(qzi, h_inv(QM[j] % c->Q[i], c->Q[i]), c->Q[i]);
+                rsc = h_mulmod(rsc, h_inv(c->Q[j] % QM[i], QM[i]), QM[i]);
+            }
+            bq_qm[i] = h_mform(qm, c->Q[i]);
+            bq_qz[i] = (c->Q[i] - h_mform(qzi, c->Q[i])) % c->Q[i];
+            bq_rsc[i] = (QM[i] - h_mform(rsc, QM[i])) % QM[i];
+        }
+    }
 
     // ---- one device allocation ----
     struct Blob {
@@ -248,6 +283,10 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     size_t o_dcv = blob.add(dc_vt.data(), dc_vt.size() * 8);
     size_t o_rsi = blob.add(rs_inv.data(), rs_inv.size() * 8);
     size_t o_rsh = blob.add(rs_half.data(), rs_half.size() * 8);
+    size_t o_bq[9];
+    const std::vector<u64> *bqv[9] = {&bq_qm, &bq_up_y, &bq_up_qs, &bq_up_vt, &bq_dn_y,
+                                      &bq_dn_qs, &bq_dn_vt, &bq_rsc, &bq_qz};
+    for (int i = 0; i < 9; ++i) o_bq[i] = blob.add(bqv[i]->data(), bqv[i]->size() * 8);
     hipError_t e = hipMalloc(&c->d_tables, blob.host.size());
     if (e == hipSuccess) e = hipMemcpy(c->d_tables, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -270,6 +309,10 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     T.dc_vt = (const u64 *)(base + o_dcv);
     T.rs_inv = (const u64 *)(base + o_rsi);
     T.rs_half = (const u64 *)(base + o_rsh);
+    const u64 **bqt[9] = {&T.bq_qm, &T.bq_up_y, &T.bq_up_qstar, &T.bq_up_vt, &T.bq_dn_y,
+                          &T.bq_dn_qstar, &T.bq_dn_vt, &T.bq_rsc, &T.bq_qz};
+    for (int i = 0; i < 9; ++i) *bqt[i] = bfv ? (const u64 *)(base + o_bq[i]) : nullptr;
+    T.bfv = bfv ? 1 : 0;
     T.logN = logN;
     T.N = N;
     T.L = L;
@@ -334,7 +377,8 @@ extern "C" int mkhip_set_crs(mkhip_ctx *c, int index, const uint64_t *swk) {
 extern "C" int mkhip_register_party(mkhip_ctx *c, int party, const uint64_t *b, const uint64_t *d, const uint64_t *v) {
     if (!c || !b || !d || !v || party < 0) return MKHIP_ERR_ARG;
     if (c->alpha == 0) return MKHIP_ERR_GADGET;
-    mkhip_unregister_party(c, party);
+    if (c->bfv) return MKHIP_ERR_SCHEME;
+    if (c->parties.count(party)) mkhip_unregister_party(c, party);
     PartyKeys k;
     int s;
     if ((s = upload_new(c, b, &k.b)) != MKHIP_OK) return s;
@@ -343,6 +387,43 @@ extern "C" int mkhip_register_party(mkhip_ctx *c, int party, const uint64_t *b, 
         return s;
     }
     if ((s = upload_new(c, v, &k.v)) != MKHIP_OK) {
+        hipFree(k.b);
+        hipFree(k.d);
+        return s;
+    }
+    c->parties[party] = k;
+    return MKHIP_OK;
+}
+
+// Two switching keys laid out back to back (the two BFV gadget branches).
+static int upload_pair(mkhip_ctx *c, const uint64_t *x0, const uint64_t *x1, u64 **dst) {
+    void *p = nullptr;
+    const size_t w = c->swk_words;
+    HIP_TRY(hipMalloc(&p, 2 * w * 8));
+    hipError_t e = hipMemcpy(p, x0, w * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy((u64 *)p + w, x1, w * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(p);
+        return hip_status(e);
+    }
+    *dst = (u64 *)p;
+    return MKHIP_OK;
+}
+
+extern "C" int mkhip_register_bfv_party(mkhip_ctx *c, int party, const uint64_t *b0, const uint64_t *d0,
+                                        const uint64_t *v0, const uint64_t *b1, const uint64_t *d1) {
+    if (!c || !b0 || !d0 || !v0 || !b1 || !d1 || party < 0) return MKHIP_ERR_ARG;
+    if (!c->bfv) return MKHIP_ERR_SCHEME;
+    if (c->parties.count(party)) mkhip_unregister_party(c, party);
+    PartyKeys k;
+    k.bfv = true;
+    int s;
+    if ((s = upload_pair(c, b0, b1, &k.b)) != MKHIP_OK) return s;
+    if ((s = upload_pair(c, d0, d1, &k.d)) != MKHIP_OK) {
+        hipFree(k.b);
+        return s;
+    }
+    if ((s = upload_new(c, v0, &k.v)) != MKHIP_OK) {
         hipFree(k.b);
         hipFree(k.d);
         return s;
@@ -432,7 +513,7 @@ extern "C" int mkhip_ntt_limbs(mkhip_ctx *c, const uint64_t *in, uint64_t *out, 
     if (!c || !in || !out || !ring_idx || nlimbs <= 0) return MKHIP_ERR_ARG;
     const int N = c->N;
     for (int i = 0; i < nlimbs; ++i)
-        if (ring_idx[i] < 0 || ring_idx[i] >= c->L + c->K) return MKHIP_ERR_ARG;
+        if (ring_idx[i] < 0 || ring_idx[i] >= c->T.nmod) return MKHIP_ERR_ARG;
     const size_t words = (size_t)nlimbs * N;
     DevScratch s;
     HIP_TRY(hipMalloc(&s.p, words * 8 * 2 + sizeof(NttJob) * nlimbs + 256));

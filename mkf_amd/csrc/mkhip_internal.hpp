@@ -17,6 +17,8 @@ hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n,
                          int c = 1);
 // kernels_mkf.hip
 hipError_t launch_mkf(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st);
+// kernels_bfv.hip
+hipError_t launch_bfv(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st);
 
 inline int hip_status(hipError_t e) {
     return e == hipSuccess ? MKHIP_OK : (e == hipErrorOutOfMemory ? MKHIP_ERR_OOM : MKHIP_ERR_DEVICE);
@@ -29,8 +31,11 @@ inline int hip_status(hipError_t e) {
         if (_e != hipSuccess) return mkhip::hip_status(_e);    \
     } while (0)
 
+// A party's KKLSS relinearization key (b, d, v), each beta x (L+K) x N; for a
+// BFV party b and d hold both gadget branches (2 beta digits: branch 0, then 1).
 struct PartyKeys {
     mkhip::u64 *b = nullptr, *d = nullptr, *v = nullptr;
+    bool bfv = false;
 };
 
 // MKF keys of one party group: fold keys s_i -> s_aggregate and the
@@ -54,7 +59,8 @@ struct mkhip_ctx {
     int logN = 0, N = 0, L = 0, K = 0, gamma = 1, alpha = 0, beta_max = 0;
     mkhip::u64 t = 0;
     int flags = 0;
-    std::vector<mkhip::u64> Q, P;
+    std::vector<mkhip::u64> Q, P, QMul;  // QMul: BFV auxiliary chain (empty for CKKS)
+    bool bfv = false;
     void *d_tables = nullptr;
     mkhip::DevTables T;
     hipStream_t stream = nullptr;

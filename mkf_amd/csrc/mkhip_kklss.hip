@@ -36,6 +36,7 @@ extern "C" int mkhip_plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_o
     if (!c || !ops || nops <= 0 || !out) return MKHIP_ERR_ARG;
     *out = nullptr;
     if (c->alpha == 0) return MKHIP_ERR_GADGET;
+    if (c->bfv) return MKHIP_ERR_SCHEME;
     if (!c->d_crs_u) return MKHIP_ERR_NO_KEY;
     const int level = ops[0].level;
     const int N = c->N, K = c->K;
@@ -54,6 +55,7 @@ extern "C" int mkhip_plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_o
     const int nl = level + 1, rows = nl + K;
     const int beta = (nl + c->alpha - 1) / c->alpha;
     const size_t limb = (size_t)N * 8, poly = (size_t)nl * limb, qp = (size_t)rows * limb, hoist = (size_t)beta * qp;
+    const size_t keyw = (size_t)beta * (c->L + K) * limb;
 
     // ---- workspace layout ----
     Carve cv;
@@ -76,8 +78,8 @@ extern "C" int mkhip_plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_o
         if (ku > MKHIP_MAX_PARTIES) return MKHIP_ERR_TOO_MANY;
         w.D0 = cv.take(op.k0 * hoist);
         w.D1 = cv.take(op.k1 * hoist);
-        w.x = cv.take(hoist);
-        w.y = cv.take(hoist);
+        w.x = cv.take(keyw);  // full-level switching keys (k_ext reads them like keys)
+        w.y = cv.take(keyw);
         w.tnt = cv.take((2 + op.k0 + op.k1) * poly);
         w.ct = cv.take((1 + ku) * poly);
         w.accx = cv.take(op.k1 * qp);

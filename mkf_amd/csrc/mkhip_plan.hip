@@ -12,6 +12,8 @@ hipError_t plan_stage(mkhip_plan *pl, int s) {
     const PlanStage &S = pl->stages[s];
     mkhip_ctx *c = pl->ctx;
     if (S.what >= KMAC2 && S.what <= KRESCALE_NTT) return launch_mkf(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream);
+    if (S.what >= KBFV_LIFT && S.what <= KBFV_QUANTIZE)
+        return launch_bfv(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream);
     return launch_kernel(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream, S.c);
 }
 

@@ -71,7 +71,8 @@ struct ModConst {
     u64 r2;     // 2^128 mod q: mred(x, r2) = x * 2^64 mod q (MForm)
     u64 ninv;   // N^-1 mod q (canonical) for this modulus at the context's N
     u64 ninvp;  // Shoup companion of ninv
-    u64 pad0, pad1;
+    u64 tninv;  // t * N^-1 mod q (BFV: Quantize's MulScalar(t) folded into the INTT)
+    u64 tninvp; // Shoup companion of tninv
 };
 
 // host helpers (exact, 128-bit); host-only functions, parsed in both compilation passes

@@ -28,9 +28,23 @@ struct DevTables {
     // rescale (DivRoundByLastModulus) : for dropped modulus index l and j < l
     const u64 *rs_inv;        // [L][L]  MForm(q_l^-1 mod q_j)
     const u64 *rs_half;       // [L][L]  (q_l >> 1) mod q_j
+    // BFV (R = Q u QMul, mkbfv/basis_extension.go); null in CKKS contexts.
+    // QMul limb j has ring index L + K + j.
+    const u64 *bq_qm;         // [L]        MForm(QMul mod q_i)
+    const u64 *bq_up_y;       // [L]        Q -> QMul : MForm((Q/q_i)^-1 mod q_i)
+    const u64 *bq_up_qstar;   // [L][L]     [j][i] MForm((Q/q_i) mod qm_j)
+    const u64 *bq_up_vt;      // [L][L+1]   v * (-Q) mod qm_j
+    const u64 *bq_dn_y;       // [L]        QMul -> Q : MForm((QMul/qm_j)^-1 mod qm_j)
+    const u64 *bq_dn_qstar;   // [L][L]     [i][j] MForm((QMul/qm_j) mod q_i)
+    const u64 *bq_dn_vt;      // [L][L+1]   v * (-QMul) mod q_i
+    const u64 *bq_rsc;        // [L]        qm_j - MForm(Q^-1 mod qm_j)   (Rescale)
+    const u64 *bq_qz;         // [L]        q_i - MForm(QMul^-1 mod q_i)  (Quantize)
     int logN, N, L, K, alpha, beta_max, nmod;
-    int pad;
+    int bfv;                  // 1 when the QMul chain is present
 };
+
+// ring index of limb l of an R = Q u QMul polynomial
+__host__ __device__ __forceinline__ int r_index(int l, int L, int K) { return l < L ? l : K + l; }
 
 __device__ __forceinline__ int dc_base(const DevTables &T, int digit, int nd) {
     return digit * (kMaxAlpha + 1) + nd;
@@ -53,6 +67,29 @@ enum KernelId {
     KMD_NTT = 11,        // k_moddown_ntt<LOGN>    NTT-domain ModDown, summed over a fold
     KKARA = 12,          // k_karatsuba            MKF tensor (Karatsuba, NTT domain)
     KRESCALE_NTT = 13,   // k_rescale_ntt<LOGN>    NTT-domain rescale + materialized copies
+    KBFV_LIFT = 14,      // k_bfv_lift             ModUpQtoR / Rescale into R
+    KBFV_TENSOR = 15,    // k_tensor<LOGN, true>   tensor over R, t x, INTT
+    KBFV_QUANTIZE = 16,  // k_quantize             ModDown R -> Q (+ accumulate)
+};
+
+// ---- BFV descriptors ----------------------------------------------------------
+
+// Lift of one coefficient-domain Q poly (L limbs) into R (2L limbs):
+// mode 0 ModUpQtoR, mode 1 Rescale (mkbfv/basis_extension.go:50-66, 88-104).
+struct BfvLiftDesc {
+    const u64 *src;
+    u64 *dst;
+    int mode;
+    int pad;
+};
+
+// Quantize's ModDown of a coefficient-domain R poly into Q (basis_extension.go:69-85,
+// the t x and INTT already applied): dst (=|+=) (x_Q - ModUp(x_QMul)) QMul^-1.
+struct BfvQuantDesc {
+    const u64 *src;
+    u64 *dst;
+    int accumulate;
+    int pad;
 };
 
 // ---- MKF (mkf_cpu) descriptors ----------------------------------------------
@@ -125,7 +162,8 @@ struct NttJob {
     const u64 *src;
     u64 *dst;
     int mod;       // ring index
-    int reduce;    // 1: src values may exceed the modulus (reduce mod q first)
+    int reduce;    // bit 0: src values may exceed the modulus (reduce mod q first);
+                   // bit 1 (inverse only): scale by t N^-1 instead of N^-1
 };
 
 // Tensor output component: out = a0 (.) b_u + a_u (.) b0 (either term may be absent),
@@ -143,7 +181,7 @@ struct TensorDesc {
 // batch, so each key word is fetched once per batch (L1/L2 reuse across ops).
 struct MacDesc {
     const u64 *D;                   // [k][beta][rows][N]
-    u64 *xy;                        // [beta][rows][N]
+    u64 *xy;                        // [beta][L+K][N] (full-level key layout)
     const u64 *key[kMaxParties];    // [beta][L+K][N] per component (MForm)
     int k, level;
 };

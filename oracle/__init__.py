@@ -89,6 +89,18 @@ def _declare(L):
     L.mkref_mkf_switch_one.argtypes = [ctypes.c_void_p, I, u64p, I, u64p, I, u64p, u64p]
     L.mkref_mkf_ckks_mulrelin.argtypes = [ctypes.c_void_p, I, I, u64p, u64p,
                                           ctypes.POINTER(u64p), u64p, u64p, u64p]
+    V = ctypes.c_void_p
+    L.mkref_bfv_new.restype = V
+    L.mkref_bfv_new.argtypes = [I, u64p, I, u64p, u64p, I, U, I]
+    L.mkref_bfv_delete.argtypes = [V]
+    L.mkref_bfv_modup_q_to_r.argtypes = [V, u64p, u64p]
+    L.mkref_bfv_rescale_q_to_r.argtypes = [V, u64p, u64p]
+    L.mkref_bfv_quantize.argtypes = [V, u64p, u64p]
+    L.mkref_bfv_decompose.argtypes = [V, u64p, u64p]
+    L.mkref_bfv_ntt_r.argtypes = [V, I, u64p, I]
+    L.mkref_kklss_bfv_mulrelin.restype = I
+    L.mkref_kklss_bfv_mulrelin.argtypes = [V, u64p, intp, I, u64p, intp, I, ctypes.POINTER(u64p),
+                                           ctypes.POINTER(u64p), ctypes.POINTER(u64p), u64p, u64p, intp]
     L.mkref_prng_seed.argtypes = [ctypes.c_void_p, U]
     L.mkref_prng_next.restype = U
     L.mkref_prng_next.argtypes = [ctypes.c_void_p]
@@ -329,3 +341,86 @@ class Ctx:
         lib().mkref_mkf_ckks_mulrelin(self._h, level, k, _p(_u64(left)), _p(_u64(right)), arr,
                                       _p(_u64(relin_key)), _p(o0), _p(o1))
         return o0, o1
+
+
+class BfvCtx:
+    """KKLSS MK-BFV context: Q u P (gamma 2, alpha must be 1), QMul (len L), t.
+    ``qp`` is the matching Q u P context for the shared primitives."""
+
+    def __init__(self, logN, Q, QMul, P, t, root_rule=0):
+        self.logN, self.N = logN, 1 << logN
+        self.Q, self.QMul, self.P = [int(q) for q in Q], [int(q) for q in QMul], [int(p) for p in P]
+        self.L, self.K, self.t = len(Q), len(P), int(t)
+        if len(QMul) != len(Q):
+            raise ValueError("len(QMul) must equal len(Q)")
+        if self.K // 2 != 1:
+            raise ValueError("the BFV restatement covers alpha = K/2 = 1")
+        self.qp = Ctx(logN, Q, P, gamma=2, root_rule=root_rule)
+        q, qm, p = _u64(self.Q), _u64(self.QMul), _u64(self.P)
+        self._h = lib().mkref_bfv_new(logN, _p(q), self.L, _p(qm), _p(p), self.K, self.t, root_rule)
+        if not self._h:
+            raise ValueError("mkref_bfv_new failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mkref_bfv_delete(self._h)
+            self._h = None
+
+    @property
+    def R(self):
+        return self.Q + self.QMul
+
+    @property
+    def QP(self):
+        return self.Q + self.P
+
+    def beta(self):
+        return self.L
+
+    def modup_q_to_r(self, a):
+        out = np.zeros((2 * self.L, self.N), np.uint64)
+        lib().mkref_bfv_modup_q_to_r(self._h, _p(_u64(a)), _p(out))
+        return out
+
+    def rescale_q_to_r(self, a):
+        out = np.zeros((2 * self.L, self.N), np.uint64)
+        lib().mkref_bfv_rescale_q_to_r(self._h, _p(_u64(a)), _p(out))
+        return out
+
+    def quantize(self, a_ntt):
+        out = np.zeros((self.L, self.N), np.uint64)
+        lib().mkref_bfv_quantize(self._h, _p(_u64(a_ntt)), _p(out))
+        return out
+
+    def decompose(self, aR):
+        out = np.zeros((2 * self.L, self.L + self.K, self.N), np.uint64)
+        lib().mkref_bfv_decompose(self._h, _p(_u64(aR)), _p(out))
+        return out
+
+    def ntt_r(self, a, inverse=False):
+        a = _u64(a).copy()
+        for l in range(a.shape[0]):
+            lib().mkref_bfv_ntt_r(self._h, l, _p(a[l]), 1 if inverse else 0)
+        return a
+
+    def mulrelin(self, op0, ids0, op1, ids1, bkeys, dkeys, vkeys, crs_u):
+        """op = (1+k, L, N) coefficient domain; keys indexed by party id:
+        bkeys/dkeys (2L, L+K, N), vkeys (L, L+K, N).  Returns (out, ids)."""
+        n = len(bkeys)
+        kb = [_u64(k) for k in bkeys]
+        kd = [_u64(k) for k in dkeys]
+        kv = [_u64(k) for k in vkeys]
+        arr = (u64p * n)
+        b = arr(*[_p(k) for k in kb])
+        d = arr(*[_p(k) for k in kd])
+        v = arr(*[_p(k) for k in kv])
+        k0, k1 = len(ids0), len(ids1)
+        i0 = (ctypes.c_int * max(k0, 1))(*ids0)
+        i1 = (ctypes.c_int * max(k1, 1))(*ids1)
+        ku = len(set(ids0) | set(ids1))
+        out = np.zeros((1 + ku, self.L, self.N), np.uint64)
+        iout = (ctypes.c_int * ku)()
+        nc = lib().mkref_kklss_bfv_mulrelin(self._h, _p(_u64(op0)), i0, k0, _p(_u64(op1)), i1, k1,
+                                            b, d, v, _p(_u64(crs_u)), _p(out), iout)
+        assert nc == ku + 1
+        return out, list(iout)

@@ -331,9 +331,11 @@ void mkref_modup_free(mkref_modup *m) {
  * y_i = MRed(x_i, (Q/q_i)^-1 R), v = (u64)(sum_i float64(y_i)/float64(q_i)) in
  * ascending i (reconstructRNS, basis_extension.go:537-579), then
  * out_j = sum_i y_i (Q/q_i) + v (-Q) mod p_j (multSum, :582-646).
+ * With raw != 0 the value multSum returns is kept as is: hi - hi(lo*pinv*p) + p
+ * + vt[v], congruent to out_j but possibly >= p_j (the reference's "[0, 2q-1]").
  */
-static void modup_coeff(const u64 *x, int stride, const u64 *Q, int nq, const u64 *P, int np,
-                        const mkref_modup *m, u64 *out, int ostride) {
+static void modup_coeff_impl(const u64 *x, int stride, const u64 *Q, int nq, const u64 *P, int np,
+                             const mkref_modup *m, u64 *out, int ostride, int raw) {
     u64 y[64];
     double vi = 0.0;
     for (int i = 0; i < nq; i++) {
@@ -350,13 +352,23 @@ static void modup_coeff(const u64 *x, int stride, const u64 *Q, int nq, const u6
         u64 hi = (u64)(acc >> 64), lo = (u64)acc;
         u64 mq = (u64)(((u128)(lo * m->pinv_out[j]) * pj) >> 64);
         u64 r = hi - mq + pj + m->vtimesqmodp[(size_t)j * (nq + 1) + v];
-        out[(size_t)j * ostride] = r % pj;
+        out[(size_t)j * ostride] = raw ? r : r % pj;
     }
+}
+
+static void modup_coeff(const u64 *x, int stride, const u64 *Q, int nq, const u64 *P, int np,
+                        const mkref_modup *m, u64 *out, int ostride) {
+    modup_coeff_impl(x, stride, Q, nq, P, np, m, out, ostride, 0);
 }
 
 void mkref_modup_exact(int N, const u64 *in, const u64 *Q, int nq,
                        u64 *out, const u64 *P, int np, const mkref_modup *m) {
     for (int c = 0; c < N; c++) modup_coeff(in + c, N, Q, nq, P, np, m, out + c, N);
+}
+
+void mkref_modup_exact_raw(int N, const u64 *in, const u64 *Q, int nq,
+                           u64 *out, const u64 *P, int np, const mkref_modup *m) {
+    for (int c = 0; c < N; c++) modup_coeff_impl(in + c, N, Q, nq, P, np, m, out + c, N, 1);
 }
 
 /* ModDownQPtoQ, basis_extension.go:192-232: out = (x_Q - ModUpPtoQ(x_P)) * P^-1 */
@@ -483,8 +495,14 @@ static const u64 *key_row(const mkref_ctx *c, const u64 *key_digit, int levelQ, 
 /* ExternalProductHoisted (keyswitch_hoisted.go:11-45) */
 void mkref_external_product_hoisted(const mkref_ctx *c, int levelQ, const u64 *a_hoisted,
                                     const u64 *key, u64 *out) {
+    mkref_external_product_n(c, levelQ, mkref_beta(c, levelQ), a_hoisted, key, out);
+}
+
+/* the same over ndig digits (the BFV products run over the 2 beta digits of an
+ * R poly, mkbfv/keyswitch_hoisted.go:14-46) */
+void mkref_external_product_n(const mkref_ctx *c, int levelQ, int beta, const u64 *a_hoisted,
+                              const u64 *key, u64 *out) {
     int N = c->N, K = c->K, L = c->L;
-    int beta = mkref_beta(c, levelQ);
     int rows = levelQ + 1 + K;
     size_t dsz = (size_t)rows * N, ksz = (size_t)(L + K) * N;
     u64 *acc = calloc((size_t)rows * N, sizeof(u64));
@@ -524,27 +542,36 @@ static int union_ids(const int *a, int ka, const int *b, int kb, int *out) {
     return n;
 }
 
-/* x += MRed(key, D) over beta digits of QP rows (MulCoeffsMontgomeryAndAddLvl) */
-static void mac_keyed(const mkref_ctx *c, int levelQ, const u64 *key, const u64 *D, u64 *x) {
-    int N = c->N, rows = levelQ + 1 + c->K, beta = mkref_beta(c, levelQ);
+/* x += MRed(key, D) over ndig digits of QP rows (MulCoeffsMontgomeryAndAddLvl).
+ * x is a switching key like the reference's pools (keyswitch_hoisted.go:60-70):
+ * full-level layout [ndig][L+K][N], rows addressed by ring index. */
+void mkref_mac_keyed_n(const mkref_ctx *c, int levelQ, int ndig, const u64 *key, const u64 *D, u64 *x) {
+    int N = c->N, rows = levelQ + 1 + c->K;
     size_t dsz = (size_t)rows * N, ksz = (size_t)(c->L + c->K) * N;
-    for (int b = 0; b < beta; b++)
+    for (int b = 0; b < ndig; b++)
         for (int r = 0; r < rows; r++) {
             int ri = qp_ring_index(c, levelQ, r);
             u64 q = c->ringQP.q[ri], qinv = c->ringQP.qinv[ri];
             const u64 *kr = key_row(c, key + b * ksz, levelQ, r);
             const u64 *dr = D + b * dsz + (size_t)r * N;
-            u64 *xr = x + b * dsz + (size_t)r * N;
+            u64 *xr = x + b * ksz + (size_t)ri * N;
             for (int i = 0; i < N; i++) xr[i] = addmod(xr[i], mkref_mred(kr[i], dr[i], q, qinv), q);
         }
 }
 
-static void mform_qp(const mkref_ctx *c, int levelQ, u64 *x, int ndigits) {
+static void mac_keyed(const mkref_ctx *c, int levelQ, const u64 *key, const u64 *D, u64 *x) {
+    mkref_mac_keyed_n(c, levelQ, mkref_beta(c, levelQ), key, D, x);
+}
+
+/* MFormLvl over the used rows of a full-layout switching key */
+void mkref_mform_qp(const mkref_ctx *c, int levelQ, u64 *x, int ndigits) {
     int N = c->N, rows = levelQ + 1 + c->K;
+    size_t ksz = (size_t)(c->L + c->K) * N;
     for (int b = 0; b < ndigits; b++)
         for (int r = 0; r < rows; r++) {
-            u64 q = c->ringQP.q[qp_ring_index(c, levelQ, r)];
-            u64 *xr = x + ((size_t)b * rows + r) * N;
+            int ri = qp_ring_index(c, levelQ, r);
+            u64 q = c->ringQP.q[ri];
+            u64 *xr = x + b * ksz + (size_t)ri * N;
             for (int i = 0; i < N; i++) xr[i] = mkref_mform(xr[i], q);
         }
 }
@@ -568,7 +595,8 @@ int mkref_kklss_mulrelin(const mkref_ctx *c, int level,
 
     u64 *D0 = malloc(sizeof(u64) * hsz * (k0 ? k0 : 1));
     u64 *D1 = malloc(sizeof(u64) * hsz * (k1 ? k1 : 1));
-    u64 *x = calloc(hsz, sizeof(u64)), *y = calloc(hsz, sizeof(u64));
+    size_t xsz = (size_t)beta * (c->L + K) * N;   /* full-level switching keys */
+    u64 *x = calloc(xsz, sizeof(u64)), *y = calloc(xsz, sizeof(u64));
     u64 *pool0 = malloc(sizeof(u64) * psz), *pool1 = malloc(sizeof(u64) * psz);
     u64 *pool2 = malloc(sizeof(u64) * psz), *t = malloc(sizeof(u64) * psz);
     u64 *Dt = malloc(sizeof(u64) * hsz);
@@ -579,9 +607,9 @@ int mkref_kklss_mulrelin(const mkref_ctx *c, int level,
 
     /* x = sum d_i (.) D(op0_i), y = sum b_j (.) D(op1_j), then MForm (:84-123) */
     for (int i = 0; i < k0; i++) mac_keyed(c, level, rlk_d[ids0[i]], D0 + i * hsz, x);
-    mform_qp(c, level, x, beta);
+    mkref_mform_qp(c, level, x, beta);
     for (int j = 0; j < k1; j++) mac_keyed(c, level, rlk_b[ids1[j]], D1 + j * hsz, y);
-    mform_qp(c, level, y, beta);
+    mkref_mform_qp(c, level, y, beta);
 
     /* tensor (:126-153) */
     memset(out, 0, sizeof(u64) * psz * (1 + ku));

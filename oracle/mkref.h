@@ -73,6 +73,9 @@ void mkref_modup_free(mkref_modup *m);
 /* exact basis extension of N coefficients from basis Q (in[nq][N]) to P (out[np][N]) */
 void mkref_modup_exact(int N, const u64 *in, const u64 *Q, int nq,
                        u64 *out, const u64 *P, int np, const mkref_modup *m);
+/* same, keeping multSum's unreduced outputs (congruent, possibly >= p_j) */
+void mkref_modup_exact_raw(int N, const u64 *in, const u64 *Q, int nq,
+                           u64 *out, const u64 *P, int np, const mkref_modup *m);
 /* ModDown: out_Q = round-ish((x_QP - ModUp(x_P)) / P) mod Q, coefficient domain */
 void mkref_moddown(int N, const u64 *xq, const u64 *Q, int nq,
                    const u64 *xp, const u64 *P, int np, u64 *outq);
@@ -106,6 +109,13 @@ void mkref_decompose(const mkref_ctx *c, int levelQ, const u64 *a, u64 *out);
  * laid out with stride (L+K) limbs (the key's full level), a_hoisted with (levelQ+1+K). */
 void mkref_external_product_hoisted(const mkref_ctx *c, int levelQ, const u64 *a_hoisted,
                                     const u64 *key, u64 *out);
+/* the same over an explicit digit count */
+void mkref_external_product_n(const mkref_ctx *c, int levelQ, int ndig, const u64 *a_hoisted,
+                              const u64 *key, u64 *out);
+/* x += MRed(key, D) over ndig digits; x is a full-level switching key [ndig][L+K][N] */
+void mkref_mac_keyed_n(const mkref_ctx *c, int levelQ, int ndig, const u64 *key, const u64 *D, u64 *x);
+/* MForm of the rows 0..levelQ and P of a full-level switching key */
+void mkref_mform_qp(const mkref_ctx *c, int levelQ, u64 *x, int ndigits);
 
 /*
  * KKLSS MK-CKKS MulRelinNew(op0, op1, rlkSet) (mkckks/evaluator.go:419) without the
@@ -139,8 +149,42 @@ void mkref_mkf_ckks_mulrelin(const mkref_ctx *c, int level, int k,
                              u64 *out0, u64 *out1);
 void mkref_div_round_last_ntt(const mkref_ctx *c, int level, int n, u64 *poly);
 
-/* ---- BFV ---- */
-/* KKLSS-BFV and MKF-BFV restatements live in mkref_bfv.c */
+/* ---- KKLSS MK-BFV (software/kklss/mkbfv), restated in mkref_bfv.c ---- */
+/* R = Q u QMul (mkbfv/params.go:36-81): the QP context (gamma 2) plus the QMul
+ * chain and the plaintext modulus t.  The decomposition of R polys is restated
+ * for alpha = 1 (K = 2 or 3 with gamma 2, the reference's parameter sets). */
+typedef struct {
+    mkref_ctx qp;          /* Q[0..L) u P[0..K), gamma 2                 */
+    mkref_ring ringR;      /* Q[0..L) then QMul[0..L)                   */
+    u64 qmul[32];
+    u64 t;
+} mkref_bfv;
+
+int  mkref_bfv_init(mkref_bfv *b, int logN, const u64 *Q, int L, const u64 *QMul, const u64 *P, int K,
+                    u64 t, int root_rule);
+void mkref_bfv_free(mkref_bfv *b);
+/* FastBasisExtender.ModUpQtoR (mkbfv/basis_extension.go:50-66): in [L][N] -> out [2L][N] */
+void mkref_bfv_modup_q_to_r(const mkref_bfv *b, const u64 *in, u64 *out);
+/* FastBasisExtender.Rescale (:88-104): floor(QMul x / Q) in R, in [L][N] -> out [2L][N] */
+void mkref_bfv_rescale_q_to_r(const mkref_bfv *b, const u64 *in, u64 *out);
+/* FastBasisExtender.Quantize (:69-85): NTT-domain R poly -> ModDown_QMul(t x) [L][N] */
+void mkref_bfv_quantize(const mkref_bfv *b, const u64 *in_ntt, u64 *out);
+/* KeySwitcher.DecomposeBFV (mkbfv/keyswitch.go:58-83): 2L digits [2L][L+K][N] */
+void mkref_bfv_decompose(const mkref_bfv *b, const u64 *aR, u64 *out);
+/* MulRelinNew (mkbfv/evaluator.go:85-152 + keyswitch_hoisted.go:50-252).
+ * op0/op1: [1+k][L][N] coefficient domain.  Party keys: bkey/dkey [2L][L+K][N]
+ * (branch 0's digits then branch 1's), vkey [L][L+K][N] (branch 0's v).
+ * out: [1+ku][L][N] with the union of ids ascending in ids_out; returns ku. */
+int mkref_kklss_bfv_mulrelin(const mkref_bfv *b,
+                             const u64 *op0, const int *ids0, int k0,
+                             const u64 *op1, const int *ids1, int k1,
+                             const u64 *const *bkey, const u64 *const *dkey,
+                             const u64 *const *vkey, const u64 *crs_u,
+                             u64 *out, int *ids_out);
+mkref_bfv *mkref_bfv_new(int logN, const u64 *Q, int L, const u64 *QMul, const u64 *P, int K, u64 t,
+                         int root_rule);
+void mkref_bfv_delete(mkref_bfv *b);
+void mkref_bfv_ntt_r(const mkref_bfv *b, int limb, u64 *a, int inverse);
 
 /* ---- build-side deterministic PRNG (xoshiro256** seeded via splitmix64) ---- */
 typedef struct { u64 s[4]; } mkref_prng;

@@ -265,6 +265,13 @@ def centered_crt(x, moduli):
 def negacyclic_mul(a, b):
     """Schoolbook product in Z[X]/(X^N+1) of Python-int lists."""
     N = len(a)
+    ma = max((abs(int(x)) for x in a), default=0)
+    mb = max((abs(int(x)) for x in b), default=0)
+    if ma * mb * N < (1 << 62):  # exact in int64: numpy convolution
+        c = np.convolve(np.asarray(a, np.int64), np.asarray(b, np.int64))
+        r = c[:N].copy()
+        r[:N - 1] -= c[N:]
+        return [int(x) for x in r]
     out = [0] * N
     for i, ai in enumerate(a):
         if ai == 0:
@@ -276,3 +283,88 @@ def negacyclic_mul(a, b):
             else:
                 out[k - N] -= ai * bj
     return out
+
+
+class BFVKeys(KKLSSKeys):
+    """KKLSS MK-BFV keys (software/kklss/mkbfv/keygen.go:24-162) over an
+    ``oracle.BfvCtx``.  CRS a1 = CRS[0], a2 = CRS[-3], u = CRS[-1]; per party
+      b_br = MForm(e - a_br s)                       (:46-63)
+      d_br = MForm(G_br,i s + e - a_br r)             (:65-73, GenBFVSwitchingKey :91-162)
+      v    = MForm(-(e + g_i P r) - u s)              (:75-85)
+    with the BFV gadget G_1,i = floor((QQMul/q_i) t [(QQMul/q_i)^-1]_{q_i} P / QMul)
+    over the Q primes and G_2,i likewise over the QMul primes.  The tensors the
+    evaluator consumes: bkey = [b_1; b_2], dkey = [d_1; d_2] (2L digits), vkey = v."""
+
+    def __init__(self, bctx, nparties, seed=0):
+        self.bctx = bctx
+        ctx = bctx.qp
+        self.ctx = ctx
+        g = O.Prng(seed)
+        N, QP, L = ctx.N, ctx.QP, ctx.L
+        self.beta = L
+        self.a = np.stack([mform(g.uniform(QP, N), QP) for _ in range(L)])    # CRS[0]
+        self.a2 = np.stack([mform(g.uniform(QP, N), QP) for _ in range(L)])   # CRS[-3]
+        self.u = np.stack([mform(g.uniform(QP, N), QP) for _ in range(L)])    # CRS[-1]
+        Qprod = 1
+        for q in bctx.Q:
+            Qprod *= q
+        QMprod = 1
+        for q in bctx.QMul:
+            QMprod *= q
+        Pprod = 1
+        for p in bctx.P:
+            Pprod *= p
+        QQ = Qprod * QMprod
+        self.G1 = [(QQ // qi) * bctx.t * pow(QQ // qi, -1, qi) * Pprod // QMprod for qi in bctx.Q]
+        self.G2 = [(QQ // qi) * bctx.t * pow(QQ // qi, -1, qi) * Pprod // QMprod for qi in bctx.QMul]
+        self.sk, self.sk_small, self.pk = [], [], []
+        self.bkey, self.dkey, self.vkey = [], [], []
+        for _ in range(nparties):
+            s_small, r_small = g.ternary(N), g.ternary(N)
+            s, r = self._secret(s_small), self._secret(r_small)
+            self.sk.append(s)
+            self.sk_small.append(s_small)
+            b1 = self._gen_b_crs(g, s, self.a)
+            b2 = self._gen_b_crs(g, s, self.a2)
+            d1 = self._gen_d_bfv(g, s, r, self.a, self.G1)
+            d2 = self._gen_d_bfv(g, s, r, self.a2, self.G2)
+            v = self._gen_v(g, s, r)
+            self.bkey.append(np.concatenate([b1, b2]))
+            self.dkey.append(np.concatenate([d1, d2]))
+            self.vkey.append(v)
+            self.pk.append(self._gen_pk(g, s))
+
+    def _gen_b_crs(self, g, s, crs):
+        QP = self.ctx.QP
+        return np.stack([mform(submod(self._err_ntt(g), mulmod(self._plain(crs[i]), self._plain(s), QP), QP), QP)
+                         for i in range(self.beta)])
+
+    def _gen_d_bfv(self, g, s, r, crs, G):
+        QP = self.ctx.QP
+        splain = self._plain(s)
+        out = []
+        for i in range(self.beta):
+            gs = np.stack([np.array((splain[j].astype(object) * (G[i] % q)) % q, np.uint64)
+                           for j, q in enumerate(QP)])
+            x = addmod(gs, self._err_ntt(g), QP)
+            out.append(mform(submod(x, mulmod(self._plain(crs[i]), self._plain(r), QP), QP), QP))
+        return np.stack(out)
+
+    def encode(self, m):
+        """Plaintext Delta m, Delta = floor(Q / t) (coefficient encoding of m mod t)."""
+        Qprod = 1
+        for q in self.bctx.Q:
+            Qprod *= q
+        delta = Qprod // self.bctx.t
+        vals = [(int(x) % self.bctx.t) * delta for x in m]
+        return np.stack([np.array([v % q for v in vals], np.uint64) for q in self.bctx.Q])
+
+    def decode(self, ct, ids):
+        """round(t (c0 + sum c_i s_i) / Q) mod t (bfv Decryptor + DecodeInt, coefficient form)."""
+        Q = self.bctx.Q
+        Qprod = 1
+        for q in Q:
+            Qprod *= q
+        x = centered_crt(self.decrypt(ct, ids), Q)
+        t = self.bctx.t
+        return [((t * v + Qprod // 2) // Qprod) % t for v in x]

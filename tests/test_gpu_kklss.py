@@ -127,6 +127,24 @@ def test_kklss_mulrelin_parity(setup, ids0, ids1):
     assert np.array_equal(out, exp)
 
 
+def test_kklss_mulrelin_below_top_level(setup):
+    """Operands at level L-2 and L-1 (dropped to L-2): x / y stay full-level
+    switching keys, so every row lands where the external product reads it."""
+    name, ctx, ref = setup
+    keys, u = _keys(ctx, ref, [0, 1, 2, 3], seed=23)
+    g = O.Prng(24)
+    lv = ref.L - 2
+    op0 = np.stack([g.uniform(ref.Q[:lv + 1], ref.N) for _ in range(3)])
+    op1 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
+    scale = float(ref.Q[lv])
+    out, ids, level, _ = ctx.kklss_ckks_mulrelin(op0, [0, 1], op1, [1, 2], scale, scale, scale)
+    nres = lv - level
+    assert nres >= 1
+    exp, eids = _oracle_mulrelin(ref, op0, [0, 1], np.ascontiguousarray(op1[:, :lv + 1]), [1, 2], keys, u, nres)
+    assert ids == eids
+    assert np.array_equal(out, exp)
+
+
 def test_batched_plan_matches_single(setup):
     name, ctx, ref = setup
     mk = _mk()

@@ -359,3 +359,121 @@ def test_mkf_switch_is_a_key_switch():
     si = ctx.ntt_poly(O.small_to_rns(keys.small[1], Q))
     want = G.centered_crt(ctx.ntt_poly(G.mulmod(c, si, Q), inverse=True), Q)
     assert max(abs(a - b) for a, b in zip(got, want)) < 2 ** 30
+
+
+# ---------------------------------------------------------------- KKLSS MK-BFV
+
+def _bfv_small(logN=10, L=3, seed=0, parties=3):
+    Q = O.gen_primes(50, 15, L)
+    QM = O.gen_primes(50, 15, L, exclude=Q)
+    P = O.gen_primes(60, 15, 2)
+    bctx = O.BfvCtx(logN, Q, QM, P, 65537)
+    return bctx, G.BFVKeys(bctx, parties, seed=seed)
+
+
+def test_bfv_modup_and_rescale_closed_forms():
+    """ModUpQtoR keeps [x]_Q in R; Rescale gives floor(QMul [x]_Q / Q) in R
+    (mkbfv/basis_extension.go:50-66, 88-104); raw limbs are congruent."""
+    bctx, _ = _bfv_small(logN=4, parties=0)
+    g = O.Prng(9)
+    x = g.uniform(bctx.Q, bctx.N)
+    X = G.centered_crt(x, bctx.Q)
+    Qp = math.prod(bctx.Q)
+    QMp = math.prod(bctx.QMul)
+    up = bctx.modup_q_to_r(x)
+    rs = bctx.rescale_q_to_r(x)
+    for c in range(bctx.N):
+        xv = X[c] % Qp
+        for l, q in enumerate(bctx.R):
+            assert int(up[l][c]) % q == xv % q
+            assert int(rs[l][c]) % q == (QMp * xv // Qp) % q
+            assert int(up[l][c]) < 3 * q and int(rs[l][c]) < 3 * q
+
+
+def test_bfv_quantize_is_scaled_floor():
+    """Quantize(x) = floor(t [x]_{Q QMul} / QMul) mod Q up to the float64 estimate's +-1
+    (exact here: |error| of the basis extension is 0 or QMul)."""
+    bctx, _ = _bfv_small(logN=4, parties=0)
+    g = O.Prng(10)
+    x = g.uniform(bctx.R, bctx.N)                  # coefficient domain over R
+    xn = bctx.ntt_r(x)
+    got = bctx.quantize(xn)
+    Rp = math.prod(bctx.R)
+    QMp = math.prod(bctx.QMul)
+    Qp = math.prod(bctx.Q)
+    Xv = G.centered_crt(x, bctx.R)
+    for c in range(bctx.N):
+        xv = Xv[c] % Rp
+        tx = (bctx.t * xv) % Rp
+        exp = tx // QMp
+        v = G.centered_crt(got[:, c:c + 1], bctx.Q)[0] % Qp
+        assert v == exp % Qp
+
+
+def test_bfv_decompose_copies_raw_limbs():
+    """Digit i of an R poly is R limb i (its integer value) in every QP row, NTT'd."""
+    bctx, _ = _bfv_small(logN=4, parties=0)
+    g = O.Prng(11)
+    aR = bctx.modup_q_to_r(g.uniform(bctx.Q, bctx.N))
+    D = bctx.decompose(aR)
+    for d in range(2 * bctx.L):
+        for r, q in enumerate(bctx.QP):
+            row = np.array([int(v) % q for v in aR[d]], np.uint64)
+            assert np.array_equal(D[d][r], bctx.qp.ntt(row, r)), (d, r)
+
+
+def test_kklss_bfv_mulrelin_decrypts_exactly():
+    """Dec(MulRelin(ct0, ct1)) = m0 m1 mod t exactly, as mkbfv_test.go:407-413
+    (coefficient encoding: the negacyclic product mod t)."""
+    bctx, keys = _bfv_small()
+    g = O.Prng(12)
+    N, t = bctx.N, bctx.t
+    m = [[int(v) for v in g.uniform([t], N)[0]] for _ in range(4)]
+    p0 = [keys.encrypt(p, keys.encode(x), g) for p, x in zip([0, 1], m[:2])]
+    p1 = [keys.encrypt(p, keys.encode(x), g) for p, x in zip([1, 2], m[2:])]
+    Q = bctx.Q
+    op0 = np.stack([G.addmod(p0[0][0], p0[1][0], Q), p0[0][1], p0[1][1]])
+    op1 = np.stack([G.addmod(p1[0][0], p1[1][0], Q), p1[0][1], p1[1][1]])
+    out, ids = bctx.mulrelin(op0, [0, 1], op1, [1, 2], keys.bkey, keys.dkey, keys.vkey, keys.u)
+    assert ids == [0, 1, 2]
+    a = [(x + y) % t for x, y in zip(m[0], m[1])]
+    b = [(x + y) % t for x, y in zip(m[2], m[3])]
+    exp = [v % t for v in G.negacyclic_mul(a, b)]
+    assert keys.decode(out, ids) == exp
+
+
+def test_bfv_planted_branch_swap_is_caught():
+    """Swapping the two gadget branches of d (a transposed operand) breaks decryption."""
+    bctx, keys = _bfv_small(logN=9, L=3, seed=2, parties=2)
+    g = O.Prng(13)
+    N, t, L = bctx.N, bctx.t, bctx.L
+    m0 = [int(v) for v in g.uniform([t], N)[0]]
+    m1 = [int(v) for v in g.uniform([t], N)[0]]
+    c0 = keys.encrypt(0, keys.encode(m0), g)
+    c1 = keys.encrypt(1, keys.encode(m1), g)
+    swapped = [np.concatenate([d[L:], d[:L]]) for d in keys.dkey]
+    out, ids = bctx.mulrelin(np.stack(c0), [0], np.stack(c1), [1], keys.bkey, swapped, keys.vkey, keys.u)
+    exp = [v % t for v in G.negacyclic_mul(m0, m1)]
+    assert keys.decode(out, ids) != exp
+
+
+def test_kklss_mulrelin_below_top_level_decrypts():
+    """At level L-2 the x / y accumulators stay full-level switching keys (the
+    reference's swkPool layout, keyswitch_hoisted.go:60-70): decryption still
+    gives m0 m1 + small noise (a row-layout mix-up fails by ~Q)."""
+    logN = 9
+    Q, P = O.gen_primes(50, 15, 4), O.gen_primes(60, 15, 2)
+    ctx = O.Ctx(logN, Q, P, gamma=2)
+    keys = G.KKLSSKeys(ctx, 2, seed=21)
+    g = O.Prng(22)
+    N, D, level = ctx.N, 1 << 30, 2
+    x = (np.array([int(v) for v in g.uniform([15], N)[0]]) - 7).astype(np.int64)
+    y = (np.array([int(v) for v in g.uniform([15], N)[0]]) - 7).astype(np.int64)
+    c0 = keys.encrypt(0, O.small_to_rns(x * D, Q), g)
+    c1 = keys.encrypt(1, O.small_to_rns(y * D, Q), g)
+    op0 = np.stack(c0)[:, :level + 1]
+    op1 = np.stack(c1)[:, :level + 1]
+    out, ids = ctx.kklss_mulrelin(op0, [0], op1, [1], keys.b, keys.d, keys.v, keys.u, level=level)
+    got = G.centered_crt(keys.decrypt(out, ids), Q[:level + 1])
+    exp = G.negacyclic_mul([int(v) * D for v in x], [int(v) * D for v in y])
+    assert max(abs(a - b) for a, b in zip(got, exp)) < 2 ** 60
