@@ -162,6 +162,20 @@ typedef struct {
 int mkhip_plan_mkf_ckks(mkhip_ctx *ctx, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out);
 
 /*
+ * MKF MK-BFV (software/mkf/mkf_cpu/bfv.go:212-252, BFVCase.Run) on a BFV context
+ * with gamma = 1 (Lattigo rlwe decomposition; QMul = the KKLSS chain the MKF code
+ * reuses, mkf_cpu/params.go:69-73): fold both sides, lift the left operand to R
+ * and rescale the right one into R, Karatsuba tensor, quantize, relinearize,
+ * k-fold materialization of Gamma.  Keys: an MKF group as above.
+ * Host buffers left/right [1+k][L][N] coefficient domain (C0 then the parties'
+ * components); out [1+k][L][N] coefficient domain: c0 then k copies of Gamma.
+ */
+int mkhip_mkf_bfv_mulrelin(mkhip_ctx *ctx, int group, const uint64_t *left, const uint64_t *right,
+                           uint64_t *out);
+/* batched over device operands: every op has level = L-1 and nres = 0 */
+int mkhip_plan_mkf_bfv(mkhip_ctx *ctx, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out);
+
+/*
  * KKLSS MK-BFV (software/kklss/mkbfv).  A party's key is mkbfv.RelinearizationKey
  * (mkbfv/keys.go:6-9): two mkrlwe relinearization keys, one per gadget branch;
  * the evaluator reads b and d of both branches and v of branch 0

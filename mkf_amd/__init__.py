@@ -35,7 +35,8 @@ EXPORTS = [
     "mkhip_plan_stage_times", "mkhip_plan_stage_count", "mkhip_plan_stage_name",
     "mkhip_register_mkf_group", "mkhip_unregister_mkf_group", "mkhip_mkf_ckks_mulrelin", "mkhip_plan_mkf_ckks",
     "mkhip_register_bfv_party", "mkhip_kklss_bfv_mulrelin", "mkhip_plan_kklss_bfv", "mkhip_bfv_modup_q_to_r",
-    "mkhip_bfv_rescale_q_to_r", "mkhip_bfv_quantize", "mkhip_bfv_decompose",
+    "mkhip_bfv_rescale_q_to_r", "mkhip_bfv_quantize", "mkhip_bfv_decompose", "mkhip_mkf_bfv_mulrelin",
+    "mkhip_plan_mkf_bfv",
 ]
 
 # kernel run by each named plan stage
@@ -47,9 +48,11 @@ STAGE_KERNELS = {
     "decompose_mu2": "k_decomp", "mac_relin": "k_mac2", "moddown_relin": "k_moddown_ntt",
     "intt_last": "k_ntt_batch<inv>", "rescale": "k_rescale_ntt",
     "lift": "k_bfv_lift", "quantize": "k_quantize", "moddown_final": "k_moddown<true>",
+    "decompose_relin": "k_decomp", "materialize": "k_rescale_ntt",
 }
-# the BFV plan's tensor stage runs the R-basis kernel
+# the BFV plans' tensor stage runs the R-basis kernel; MKF-BFV's ModDowns are coefficient-domain
 BFV_STAGE_KERNELS = dict(STAGE_KERNELS, tensor="k_tensor_r")
+MKF_BFV_STAGE_KERNELS = dict(BFV_STAGE_KERNELS, moddown_fold="k_moddown<false>", moddown_relin="k_moddown<false>")
 
 
 class MkhipError(RuntimeError):
@@ -131,6 +134,8 @@ def _declare(L):
     L.mkhip_plan_kklss_bfv.argtypes = [V, I, ctypes.POINTER(KKLSSOp), I, ctypes.POINTER(V)]
     for f in ("mkhip_bfv_modup_q_to_r", "mkhip_bfv_rescale_q_to_r", "mkhip_bfv_quantize", "mkhip_bfv_decompose"):
         getattr(L, f).argtypes = [V, u64p, u64p]
+    L.mkhip_mkf_bfv_mulrelin.argtypes = [V, I, u64p, u64p, u64p]
+    L.mkhip_plan_mkf_bfv.argtypes = [V, I, ctypes.POINTER(MKFOp), I, ctypes.POINTER(V)]
 
 
 def _p(a):
@@ -304,6 +309,14 @@ class Context:
         nlo = lo.value + 1
         return out[:(1 + k) * nlo * self.N].reshape(1 + k, nlo, self.N), lo.value, so.value
 
+    def mkf_bfv_mulrelin(self, group, left, right):
+        """MKF BFVCase.Run: left/right (1+k, L, N) coefficient domain -> out (1+k, L, N)."""
+        left, right = _u64(left), _u64(right)
+        out = np.zeros_like(left)
+        check(lib().mkhip_mkf_bfv_mulrelin(self._h, group, _p(left), _p(right), _p(out)),
+              "mkhip_mkf_bfv_mulrelin")
+        return out
+
     # ---- primitives ----
     def ntt(self, rows, ring_idx, inverse=False):
         rows = _u64(rows)
@@ -361,7 +374,8 @@ class Plan:
 
     scheme "kklss": ops are dicts {op0, op1, out, ids0, ids1, level, nres};
     scheme "bfv":   the same dicts, level = L-1 and nres = 0 (KKLSS MK-BFV);
-    scheme "mkf":   ops are dicts {left, right, out, group, level, nres}."""
+    scheme "mkf":   ops are dicts {left, right, out, group, level, nres};
+    scheme "mkf_bfv": the same dicts, level = L-1 and nres = 0 (MKF MK-BFV)."""
 
     def __init__(self, ctx, ops, use_graph=True, scheme="kklss"):
         h = ctypes.c_void_p()
@@ -377,14 +391,14 @@ class Plan:
                 a.k0, a.k1, a.level, a.nres = len(o["ids0"]), len(o["ids1"]), o["level"], o["nres"]
             fn = "mkhip_plan_kklss_ckks" if scheme == "kklss" else "mkhip_plan_kklss_bfv"
             check(getattr(lib(), fn)(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)), fn)
-        elif scheme == "mkf":
+        elif scheme in ("mkf", "mkf_bfv"):
             arr = (MKFOp * len(ops))()
             for i, o in enumerate(ops):
                 a = arr[i]
                 a.left, a.right, a.out = o["left"], o["right"], o["out"]
                 a.group, a.level, a.nres = o["group"], o["level"], o["nres"]
-            check(lib().mkhip_plan_mkf_ckks(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)),
-                  "mkhip_plan_mkf_ckks")
+            fn = "mkhip_plan_mkf_ckks" if scheme == "mkf" else "mkhip_plan_mkf_bfv"
+            check(getattr(lib(), fn)(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)), fn)
         else:
             raise ValueError(scheme)
         self._h = h

@@ -154,9 +154,10 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     const int alpha = K / gamma;
     if (K > 0 && alpha == 0) return MKHIP_ERR_GADGET;
     if (alpha > kMaxAlpha) return MKHIP_ERR_PARAMS;
-    // KKLSS MK-BFV (mkbfv/params.go:36-81): gamma 2 over Q u P, R = Q u QMul,
-    // plaintext modulus t; the R-poly decomposition is implemented for alpha = 1.
-    if (bfv && (gamma != 2 || alpha != 1 || t < 2)) return MKHIP_ERR_PARAMS;
+    // MK-BFV (mkbfv/params.go:36-81, mkf_cpu/bfv.go:105-137): R = Q u QMul and a
+    // plaintext modulus t.  KKLSS-BFV evaluations additionally need gamma 2 with
+    // alpha = 1 (checked where they are planned); MKF-BFV needs gamma 1.
+    if (bfv && (t < 2 || (gamma != 1 && gamma != 2))) return MKHIP_ERR_PARAMS;
 
     mkhip_ctx *c = new mkhip_ctx();
     c->logN = logN;
@@ -413,7 +414,7 @@ static int upload_pair(mkhip_ctx *c, const uint64_t *x0, const uint64_t *x1, u64
 extern "C" int mkhip_register_bfv_party(mkhip_ctx *c, int party, const uint64_t *b0, const uint64_t *d0,
                                         const uint64_t *v0, const uint64_t *b1, const uint64_t *d1) {
     if (!c || !b0 || !d0 || !v0 || !b1 || !d1 || party < 0) return MKHIP_ERR_ARG;
-    if (!c->bfv) return MKHIP_ERR_SCHEME;
+    if (!c->bfv || c->gamma != 2 || c->alpha != 1) return MKHIP_ERR_SCHEME;
     if (c->parties.count(party)) mkhip_unregister_party(c, party);
     PartyKeys k;
     k.bfv = true;

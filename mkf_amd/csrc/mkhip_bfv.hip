@@ -41,7 +41,7 @@ extern "C" int mkhip_plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op
                                     mkhip_plan **out) {
     if (!c || !ops || nops <= 0 || !out) return MKHIP_ERR_ARG;
     *out = nullptr;
-    if (!c->bfv) return MKHIP_ERR_SCHEME;
+    if (!c->bfv || c->gamma != 2 || c->alpha != 1) return MKHIP_ERR_SCHEME;
     if (!c->d_crs_u) return MKHIP_ERR_NO_KEY;
     const int L = c->L, K = c->K, N = c->N;
     const int level = L - 1;  // BFV ciphertexts carry every Q limb
@@ -288,7 +288,7 @@ extern "C" int mkhip_kklss_bfv_mulrelin(mkhip_ctx *c, const uint64_t *op0, const
                                         int *ku_out) {
     if (!c || !op0 || !op1 || !out || (k0 && !ids0) || (k1 && !ids1) || k0 < 0 || k1 < 0) return MKHIP_ERR_ARG;
     if (k0 > MKHIP_MAX_PARTIES || k1 > MKHIP_MAX_PARTIES) return MKHIP_ERR_TOO_MANY;
-    if (!c->bfv) return MKHIP_ERR_SCHEME;
+    if (!c->bfv || c->gamma != 2 || c->alpha != 1) return MKHIP_ERR_SCHEME;
     const int N = c->N, nl = c->L, level = c->L - 1;
     int ids[2 * MKHIP_MAX_PARTIES], ku = 0;
     for (int i = 0; i < k0; ++i)
@@ -416,7 +416,7 @@ extern "C" int mkhip_bfv_quantize(mkhip_ctx *c, const uint64_t *in_ntt, uint64_t
 
 extern "C" int mkhip_bfv_decompose(mkhip_ctx *c, const uint64_t *aR, uint64_t *out) {
     if (!c || !aR || !out) return MKHIP_ERR_ARG;
-    if (!c->bfv) return MKHIP_ERR_SCHEME;
+    if (!c->bfv || c->gamma != 2 || c->alpha != 1) return MKHIP_ERR_SCHEME;
     const int N = c->N, L = c->L, rows = L + c->K;
     const size_t inw = (size_t)2 * L * N, outw = (size_t)2 * L * rows * N;
     Scratch s;

@@ -68,6 +68,7 @@ extern "C" int mkhip_plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *o
     if (!c || !ops || nops <= 0 || !out) return MKHIP_ERR_ARG;
     *out = nullptr;
     if (c->alpha == 0) return MKHIP_ERR_GADGET;
+    if (c->gamma != 1) return MKHIP_ERR_SCHEME;  // Lattigo rlwe decomposition: alpha = K
     const int level = ops[0].level, nres = ops[0].nres;
     const int N = c->N, K = c->K;
     for (int o = 0; o < nops; ++o) {
@@ -293,5 +294,240 @@ extern "C" int mkhip_mkf_ckks_mulrelin(mkhip_ctx *c, int group, const uint64_t *
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (level_out) *level_out = level - nres;
     if (scale_out) *scale_out = ns;
+    return MKHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// MKF MK-BFV (software/mkf/mkf_cpu/bfv.go:212-252, BFVCase.Run) on a BFV context
+// with gamma = 1.  Ciphertexts stay in the coefficient domain (bfv.go:69-85), so
+// the fold switches return to it (common.go:97-103: InvNTT of every row, then
+// ModDownQPtoQ).  Launches:
+//   decompose       gadget digits of every component of both sides
+//   mac_fold        KeyswitchHoistedNoModDown, every row back to coefficients
+//   moddown_fold    ModDown of every switch summed with C0 (fold, common.go:109-126)
+//   lift            modUpLeft / rescaleRight into R = Q u QMul (bfv.go:154-175)
+//   tensor_ntt      NTT over R of the four lifted polys (in place)
+//   tensor          mu0 = L0 R0, mu1 = L0 R1 + L1 R0, mu2 = L1 R1, then t INTT
+//                   (tensorKaratsuba :179-192; mu1 equals the Karatsuba middle term
+//                   residue for residue)
+//   quantize        ModDown QMul -> Q of the three products (:194-210)
+//   decompose_relin, mac_relin, moddown_relin  switch(p2) added to p0, p1 (:233-237)
+//   materialize     Gamma copied to the k components (common.go:128-139)
+
+extern "C" int mkhip_plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph,
+                                  mkhip_plan **out) {
+    if (!c || !ops || nops <= 0 || !out) return MKHIP_ERR_ARG;
+    *out = nullptr;
+    if (!c->bfv || c->gamma != 1) return MKHIP_ERR_SCHEME;
+    if (c->alpha == 0) return MKHIP_ERR_GADGET;
+    const int L = c->L, K = c->K, N = c->N, level = L - 1;
+    for (int o = 0; o < nops; ++o) {
+        const mkhip_mkf_op &op = ops[o];
+        if (!op.left || !op.right || !op.out) return MKHIP_ERR_ARG;
+        if (op.level != level) return MKHIP_ERR_LEVEL;
+        if (op.nres != 0) return MKHIP_ERR_SCHEME;
+        if (!c->mkf_groups.count(op.group)) return MKHIP_ERR_NO_KEY;
+    }
+    const int nl = L, rows = nl + K, nr = 2 * L;
+    const int beta = (nl + c->alpha - 1) / c->alpha;
+    const size_t limb = (size_t)N * 8, poly = (size_t)nl * limb, rpoly = (size_t)nr * limb;
+    const size_t qp = (size_t)rows * limb, hoist = (size_t)beta * qp;
+
+    struct OpWs {
+        size_t D, facc, fold, lift, tq, p, Dr, racc;
+        int k;
+    };
+    std::vector<OpWs> ws(nops);
+    Carve cv;
+    int kmax = 1;
+    for (int o = 0; o < nops; ++o) {
+        OpWs &w = ws[o];
+        const int k = c->mkf_groups[ops[o].group].k;
+        w.k = k;
+        kmax = std::max(kmax, k);
+        w.D = cv.take(2 * k * hoist);
+        w.facc = cv.take(2 * k * 2 * qp);
+        w.fold = cv.take(4 * poly);
+        w.lift = cv.take(4 * rpoly);
+        w.tq = cv.take(3 * rpoly);
+        w.p = cv.take(3 * poly);
+        w.Dr = cv.take(hoist);
+        w.racc = cv.take(2 * qp);
+    }
+    mkhip_plan *pl = nullptr;
+    int st = plan_create(c, cv.off, &pl);
+    if (st != MKHIP_OK) return st;
+    char *B = (char *)pl->d_mem;
+    auto P = [&](size_t off) { return (u64 *)(B + off); };
+
+    std::vector<DecompDesc> dec, dec_relin;
+    std::vector<Mac2Desc> mac_fold, mac_relin;
+    std::vector<ModDownDesc> md_fold, md_relin;
+    std::vector<BfvLiftDesc> lift;
+    std::vector<NttJob> ntt;
+    std::vector<TensorDesc> ten;
+    std::vector<BfvQuantDesc> quant;
+    std::vector<RescaleNttDesc> mat;
+    for (int o = 0; o < nops; ++o) {
+        const mkhip_mkf_op &op = ops[o];
+        const OpWs &w = ws[o];
+        const MkfGroup &g = c->mkf_groups[op.group];
+        const int k = w.k;
+        const u64 *side[2] = {op.left, op.right};
+        auto facc = [&](int s, int p, int oo) { return P(w.facc) + (((size_t)s * k + p) * 2 + oo) * rows * N; };
+        auto fold = [&](int s, int oo) { return P(w.fold) + ((size_t)s * 2 + oo) * nl * N; };
+        auto lifted = [&](int s, int oo) { return P(w.lift) + ((size_t)s * 2 + oo) * nr * N; };
+        auto prod = [&](int i) { return P(w.p) + (size_t)i * nl * N; };
+        for (int s = 0; s < 2; ++s) {
+            DecompDesc dd = {};
+            for (int p = 0; p < k; ++p) {
+                dd.comp[p] = side[s] + (size_t)(1 + p) * nl * N;
+                Mac2Desc md = {};
+                md.D = P(w.D) + ((size_t)s * k + p) * beta * rows * N;
+                md.swk = g.fold[p];
+                md.acc[0] = facc(s, p, 0);
+                md.acc[1] = facc(s, p, 1);
+                md.level = level;
+                md.intt_all = 1;
+                mac_fold.push_back(md);
+            }
+            dd.D = P(w.D) + (size_t)s * k * beta * rows * N;
+            dd.k = k;
+            dd.level = level;
+            dec.push_back(dd);
+            for (int oo = 0; oo < 2; ++oo) {
+                ModDownDesc m = {};
+                for (int p = 0; p < k; ++p) m.src[p] = facc(s, p, oo);
+                m.nsrc = k;
+                m.base = oo == 0 ? side[s] : nullptr;  // fold: out[0] starts from C0
+                m.dst = fold(s, oo);
+                m.level = level;
+                md_fold.push_back(m);
+                BfvLiftDesc ld = {};
+                ld.src = fold(s, oo);
+                ld.dst = lifted(s, oo);
+                ld.mode = s;  // left: ModUpQtoR, right: Rescale
+                lift.push_back(ld);
+                for (int l = 0; l < nr; ++l) {
+                    NttJob j;
+                    j.src = j.dst = lifted(s, oo) + (size_t)l * N;
+                    j.mod = r_index(l, L, K);
+                    j.reduce = 1;
+                    ntt.push_back(j);
+                }
+            }
+        }
+        for (int i = 0; i < 3; ++i) {
+            TensorDesc td = {};
+            td.out = P(w.tq) + (size_t)i * nr * N;
+            if (i == 1) {  // L0 R1 + L1 R0
+                td.a0 = lifted(0, 0);
+                td.b0 = lifted(1, 0);
+                td.au = lifted(0, 1);
+                td.bu = lifted(1, 1);
+            } else {
+                td.a0 = lifted(0, i / 2);
+                td.b0 = lifted(1, i / 2);
+                td.is_c0 = 1;
+            }
+            ten.push_back(td);
+            BfvQuantDesc qd = {};
+            qd.src = td.out;
+            qd.dst = prod(i);
+            quant.push_back(qd);
+        }
+        DecompDesc d2 = {};
+        d2.comp[0] = prod(2);
+        d2.D = P(w.Dr);
+        d2.k = 1;
+        d2.level = level;
+        dec_relin.push_back(d2);
+        Mac2Desc mr = {};
+        mr.D = P(w.Dr);
+        mr.swk = g.relin;
+        mr.acc[0] = P(w.racc);
+        mr.acc[1] = P(w.racc) + (size_t)rows * N;
+        mr.level = level;
+        mr.intt_all = 1;
+        mac_relin.push_back(mr);
+        for (int oo = 0; oo < 2; ++oo) {
+            ModDownDesc m = {};
+            m.src[0] = mr.acc[oo];
+            m.nsrc = 1;
+            m.base = prod(oo);
+            m.dst = op.out + (size_t)oo * nl * N;  // c0, then the first copy of Gamma
+            m.level = level;
+            md_relin.push_back(m);
+        }
+        if (k > 1) {
+            RescaleNttDesc rd = {};
+            rd.x = op.out + (size_t)nl * N;
+            rd.level = level;
+            rd.nres = 0;
+            for (int p = 1; p < k; ++p) rd.out[p - 1] = op.out + (size_t)(1 + p) * nl * N;
+            rd.nout = k - 1;
+            mat.push_back(rd);
+        }
+    }
+    DescBlob blob;
+    auto &S = pl->stages;
+    S.push_back(make_stage(KDECOMP, blob.add(dec), (int)dec.size(), beta, rows, kmax, "decompose"));
+    S.push_back(make_stage(KMAC2, blob.add(mac_fold), (int)mac_fold.size(), beta, rows, 1, "mac_fold"));
+    S.push_back(make_stage(KMODDOWN, blob.add(md_fold), (int)md_fold.size(), 0, 0, 1, "moddown_fold"));
+    S.push_back(make_stage(KBFV_LIFT, blob.add(lift), (int)lift.size(), 0, 0, 1, "lift"));
+    S.push_back(make_stage(KNTT_FWD, blob.add(ntt), (int)ntt.size(), 0, 0, 1, "tensor_ntt"));
+    S.push_back(make_stage(KBFV_TENSOR, blob.add(ten), (int)ten.size(), nr, 0, 1, "tensor"));
+    S.push_back(make_stage(KBFV_QUANTIZE, blob.add(quant), (int)quant.size(), 0, 0, 1, "quantize"));
+    S.push_back(make_stage(KDECOMP, blob.add(dec_relin), (int)dec_relin.size(), beta, rows, 1, "decompose_relin"));
+    S.push_back(make_stage(KMAC2, blob.add(mac_relin), (int)mac_relin.size(), beta, rows, 1, "mac_relin"));
+    S.push_back(make_stage(KMODDOWN, blob.add(md_relin), (int)md_relin.size(), 0, 0, 1, "moddown_relin"));
+    if (!mat.empty())
+        S.push_back(make_stage(KRESCALE_NTT, blob.add(mat), (int)mat.size(), nl, 0, 1, "materialize"));
+    st = plan_finish(pl, blob, use_graph);
+    if (st != MKHIP_OK) return st;
+    *out = pl;
+    return MKHIP_OK;
+}
+
+// MKF BFVCase.Run on host buffers: left/right [1+k][L][N] coefficient domain;
+// out [1+k][L][N]: c0 then k copies of Gamma.
+extern "C" int mkhip_mkf_bfv_mulrelin(mkhip_ctx *c, int group, const uint64_t *left, const uint64_t *right,
+                                      uint64_t *out) {
+    if (!c || !left || !right || !out) return MKHIP_ERR_ARG;
+    if (!c->bfv || c->gamma != 1) return MKHIP_ERR_SCHEME;
+    auto it = c->mkf_groups.find(group);
+    if (it == c->mkf_groups.end()) return MKHIP_ERR_NO_KEY;
+    const int k = it->second.k, N = c->N, nl = c->L;
+    const size_t in = (size_t)(1 + k) * nl * N, words = 3 * in;
+    std::vector<int> sig = {3, group, k};
+    if (!c->single || c->single->sig != sig || c->single_io_words < words) {
+        mkhip_plan_destroy(c->single);
+        c->single = nullptr;
+        if (c->single_io_words < words) {
+            if (c->single_io) (void)hipFree(c->single_io);
+            c->single_io = nullptr;
+            c->single_io_words = 0;
+            HIP_TRY(hipMalloc(&c->single_io, words * 8));
+            c->single_io_words = words;
+        }
+        mkhip_mkf_op op = {};
+        op.left = c->single_io;
+        op.right = c->single_io + in;
+        op.out = c->single_io + 2 * in;
+        op.group = group;
+        op.level = nl - 1;
+        op.nres = 0;
+        mkhip_plan *pl = nullptr;
+        int s = mkhip_plan_mkf_bfv(c, 1, &op, 0, &pl);
+        if (s != MKHIP_OK) return s;
+        pl->sig = sig;
+        c->single = pl;
+    }
+    HIP_TRY(hipMemcpyAsync(c->single_io, left, in * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->single_io + in, right, in * 8, hipMemcpyHostToDevice, c->stream));
+    int s = mkhip_plan_launch(c->single);
+    if (s != MKHIP_OK) return s;
+    HIP_TRY(hipMemcpyAsync(out, c->single_io + 2 * in, in * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return MKHIP_OK;
 }

@@ -91,7 +91,8 @@ def _declare(L):
                                           ctypes.POINTER(u64p), u64p, u64p, u64p]
     V = ctypes.c_void_p
     L.mkref_bfv_new.restype = V
-    L.mkref_bfv_new.argtypes = [I, u64p, I, u64p, u64p, I, U, I]
+    L.mkref_bfv_new.argtypes = [I, u64p, I, u64p, u64p, I, U, I, I]
+    L.mkref_mkf_bfv_mulrelin.argtypes = [V, I, u64p, u64p, ctypes.POINTER(u64p), u64p, u64p, u64p]
     L.mkref_bfv_delete.argtypes = [V]
     L.mkref_bfv_modup_q_to_r.argtypes = [V, u64p, u64p]
     L.mkref_bfv_rescale_q_to_r.argtypes = [V, u64p, u64p]
@@ -344,20 +345,20 @@ class Ctx:
 
 
 class BfvCtx:
-    """KKLSS MK-BFV context: Q u P (gamma 2, alpha must be 1), QMul (len L), t.
-    ``qp`` is the matching Q u P context for the shared primitives."""
+    """MK-BFV context: Q u P, QMul (len L), t.  gamma 2: KKLSS (alpha = K/2 must
+    be 1); gamma 1: MKF (alpha = K).  ``qp`` is the matching Q u P context."""
 
-    def __init__(self, logN, Q, QMul, P, t, root_rule=0):
+    def __init__(self, logN, Q, QMul, P, t, gamma=2, root_rule=0):
         self.logN, self.N = logN, 1 << logN
         self.Q, self.QMul, self.P = [int(q) for q in Q], [int(q) for q in QMul], [int(p) for p in P]
-        self.L, self.K, self.t = len(Q), len(P), int(t)
+        self.L, self.K, self.t, self.gamma = len(Q), len(P), int(t), gamma
         if len(QMul) != len(Q):
             raise ValueError("len(QMul) must equal len(Q)")
-        if self.K // 2 != 1:
-            raise ValueError("the BFV restatement covers alpha = K/2 = 1")
-        self.qp = Ctx(logN, Q, P, gamma=2, root_rule=root_rule)
+        if gamma == 2 and self.K // 2 != 1:
+            raise ValueError("the KKLSS BFV restatement covers alpha = K/2 = 1")
+        self.qp = Ctx(logN, Q, P, gamma=gamma, root_rule=root_rule)
         q, qm, p = _u64(self.Q), _u64(self.QMul), _u64(self.P)
-        self._h = lib().mkref_bfv_new(logN, _p(q), self.L, _p(qm), _p(p), self.K, self.t, root_rule)
+        self._h = lib().mkref_bfv_new(logN, _p(q), self.L, _p(qm), _p(p), self.K, self.t, gamma, root_rule)
         if not self._h:
             raise ValueError("mkref_bfv_new failed")
 
@@ -424,3 +425,14 @@ class BfvCtx:
                                             b, d, v, _p(_u64(crs_u)), _p(out), iout)
         assert nc == ku + 1
         return out, list(iout)
+
+    def mkf_mulrelin(self, left, right, fold_keys, relin_key):
+        """MKF BFVCase.Run: left/right (1+k, L, N) coefficient domain -> (c0, Gamma)."""
+        k = left.shape[0] - 1
+        keys = [_u64(f) for f in fold_keys]
+        arr = (u64p * k)(*[_p(f) for f in keys])
+        o0 = np.zeros((self.L, self.N), np.uint64)
+        o1 = np.zeros((self.L, self.N), np.uint64)
+        lib().mkref_mkf_bfv_mulrelin(self._h, k, _p(_u64(left)), _p(_u64(right)), arr, _p(_u64(relin_key)),
+                                     _p(o0), _p(o1))
+        return o0, o1

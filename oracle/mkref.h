@@ -149,19 +149,20 @@ void mkref_mkf_ckks_mulrelin(const mkref_ctx *c, int level, int k,
                              u64 *out0, u64 *out1);
 void mkref_div_round_last_ntt(const mkref_ctx *c, int level, int n, u64 *poly);
 
-/* ---- KKLSS MK-BFV (software/kklss/mkbfv), restated in mkref_bfv.c ---- */
-/* R = Q u QMul (mkbfv/params.go:36-81): the QP context (gamma 2) plus the QMul
- * chain and the plaintext modulus t.  The decomposition of R polys is restated
- * for alpha = 1 (K = 2 or 3 with gamma 2, the reference's parameter sets). */
+/* ---- MK-BFV (software/kklss/mkbfv, software/mkf/mkf_cpu/bfv.go), in mkref_bfv.c ---- */
+/* R = Q u QMul (mkbfv/params.go:36-81): the QP context plus the QMul chain and
+ * the plaintext modulus t.  gamma 2 for KKLSS (the decomposition of R polys is
+ * restated for alpha = 1: K = 2 or 3, the reference's parameter sets), gamma 1
+ * for MKF (Lattigo rlwe decomposition, alpha = K). */
 typedef struct {
-    mkref_ctx qp;          /* Q[0..L) u P[0..K), gamma 2                 */
+    mkref_ctx qp;          /* Q[0..L) u P[0..K)                           */
     mkref_ring ringR;      /* Q[0..L) then QMul[0..L)                   */
     u64 qmul[32];
     u64 t;
 } mkref_bfv;
 
 int  mkref_bfv_init(mkref_bfv *b, int logN, const u64 *Q, int L, const u64 *QMul, const u64 *P, int K,
-                    u64 t, int root_rule);
+                    u64 t, int gamma, int root_rule);
 void mkref_bfv_free(mkref_bfv *b);
 /* FastBasisExtender.ModUpQtoR (mkbfv/basis_extension.go:50-66): in [L][N] -> out [2L][N] */
 void mkref_bfv_modup_q_to_r(const mkref_bfv *b, const u64 *in, u64 *out);
@@ -181,8 +182,13 @@ int mkref_kklss_bfv_mulrelin(const mkref_bfv *b,
                              const u64 *const *bkey, const u64 *const *dkey,
                              const u64 *const *vkey, const u64 *crs_u,
                              u64 *out, int *ids_out);
+/* MKF MK-BFV BFVCase.Run (mkf_cpu/bfv.go:212-240) on a gamma = 1 context:
+ * left/right [1+k][L][N] coefficient domain, fold keys / relin key Lattigo
+ * switching keys [beta][2][L+K][N]; out0 = c0, out1 = Gamma ([L][N], coefficient) */
+void mkref_mkf_bfv_mulrelin(const mkref_bfv *b, int k, const u64 *left, const u64 *right,
+                            const u64 *const *fold_keys, const u64 *relin_key, u64 *out0, u64 *out1);
 mkref_bfv *mkref_bfv_new(int logN, const u64 *Q, int L, const u64 *QMul, const u64 *P, int K, u64 t,
-                         int root_rule);
+                         int gamma, int root_rule);
 void mkref_bfv_delete(mkref_bfv *b);
 void mkref_bfv_ntt_r(const mkref_bfv *b, int limb, u64 *a, int inverse);
 

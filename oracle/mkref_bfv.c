@@ -46,10 +46,12 @@ static int b_union(const int *a, int ka, const int *b, int kb, int *out) {
 }
 
 int mkref_bfv_init(mkref_bfv *b, int logN, const u64 *Q, int L, const u64 *QMul, const u64 *P, int K,
-                   u64 t, int root_rule) {
+                   u64 t, int gamma, int root_rule) {
     memset(b, 0, sizeof(*b));
-    if (L <= 0 || L > 32 || K / 2 != 1) return -1;  /* alpha = K / gamma = 1 */
-    if (mkref_ctx_init(&b->qp, logN, Q, L, P, K, 2, root_rule)) return -1;
+    if (L <= 0 || L > 32 || K <= 0) return -1;
+    if (gamma != 1 && gamma != 2) return -1;
+    if (gamma == 2 && K / 2 != 1) return -1;  /* KKLSS: alpha = K / gamma = 1 */
+    if (mkref_ctx_init(&b->qp, logN, Q, L, P, K, gamma, root_rule)) return -1;
     u64 r[64];
     for (int i = 0; i < L; i++) { r[i] = Q[i]; r[L + i] = QMul[i]; b->qmul[i] = QMul[i]; }
     b->t = t;
@@ -136,6 +138,7 @@ void mkref_bfv_quantize(const mkref_bfv *b, const u64 *in_ntt, u64 *out) {
  */
 void mkref_bfv_decompose(const mkref_bfv *b, const u64 *aR, u64 *out) {
     const mkref_ctx *c = &b->qp;
+    if (c->gamma != 2) return;
     int N = c->N, L = c->L, K = c->K, rows = L + K;
     for (int d = 0; d < 2 * L; d++) {
         const u64 *src = aR + (size_t)d * N;
@@ -170,6 +173,7 @@ int mkref_kklss_bfv_mulrelin(const mkref_bfv *b,
                              u64 *out, int *ids_out) {
     const mkref_ctx *c = &b->qp;
     const mkref_ring *R = &b->ringR;
+    if (c->gamma != 2) return -1;
     int N = c->N, L = c->L, K = c->K, level = L - 1;
     int rows = L + K, beta = L;  /* alpha = 1 */
     size_t psz = (size_t)L * N, rsz = 2 * psz, dsz = (size_t)rows * N;
@@ -263,11 +267,86 @@ int mkref_kklss_bfv_mulrelin(const mkref_bfv *b,
     return ku + 1;
 }
 
+/*
+ * MKF MK-BFV, BFVCase.Run (software/mkf/mkf_cpu/bfv.go:212-240) over a gamma = 1
+ * context (Lattigo rlwe decomposition, alpha = K):
+ *   fold both sides with one coefficient-domain key switch per party
+ *     (common.go:79-126, nttOutput = false: InvNTT of every row + ModDownQPtoQ);
+ *   modUpLeft / rescaleRight into R and NTT (bfv.go:154-175; Lattigo
+ *     ring.FastBasisExtender, the algorithm restated in mkref_bfv_modup_q_to_r /
+ *     mkref_bfv_rescale_q_to_r);
+ *   tensorKaratsuba over R (:179-192); quantize each product (:194-210);
+ *   relinearize mu2: out0 = p0 + switch(p2).0, Gamma = p1 + switch(p2).1 (:233-237).
+ * left / right: [1+k][L][N] coefficient domain ([C0, components...]).
+ * The k-fold materialization of Gamma (:238) is the caller's copy.
+ */
+void mkref_mkf_bfv_mulrelin(const mkref_bfv *b, int k, const u64 *left, const u64 *right,
+                            const u64 *const *fold_keys, const u64 *relin_key, u64 *out0, u64 *out1) {
+    const mkref_ctx *c = &b->qp;
+    const mkref_ring *R = &b->ringR;
+    int N = c->N, L = c->L, level = L - 1;
+    size_t psz = (size_t)L * N, rsz = 2 * psz;
+    u64 *f[2][2], *s0 = malloc(sizeof(u64) * psz), *s1 = malloc(sizeof(u64) * psz);
+    const u64 *side[2] = {left, right};
+    for (int s = 0; s < 2; s++) {
+        f[s][0] = malloc(sizeof(u64) * psz);
+        f[s][1] = calloc(psz, sizeof(u64));
+        memcpy(f[s][0], side[s], sizeof(u64) * psz);
+        for (int i = 0; i < k; i++) {
+            mkref_mkf_switch_one(c, level, side[s] + (size_t)(1 + i) * psz, 0, fold_keys[i], 0, s0, s1);
+            for (size_t e = 0; e < psz; e++) {
+                u64 q = c->ringQP.q[e / N];
+                f[s][0][e] = b_addmod(f[s][0][e], s0[e], q);
+                f[s][1][e] = b_addmod(f[s][1][e], s1[e], q);
+            }
+        }
+    }
+    u64 *in[2][2];
+    for (int j = 0; j < 2; j++) {
+        in[0][j] = malloc(sizeof(u64) * rsz);
+        in[1][j] = malloc(sizeof(u64) * rsz);
+        mkref_bfv_modup_q_to_r(b, f[0][j], in[0][j]);
+        mkref_bfv_rescale_q_to_r(b, f[1][j], in[1][j]);
+        ntt_r(b, in[0][j]);
+        ntt_r(b, in[1][j]);
+    }
+    u64 *mu[3];
+    for (int i = 0; i < 3; i++) mu[i] = malloc(sizeof(u64) * rsz);
+    for (size_t e = 0; e < rsz; e++) {
+        int l = (int)(e / N);
+        u64 q = R->q[l], qinv = R->qinv[l];
+        u64 m0 = mkref_mred(mkref_mform(in[0][0][e], q), in[1][0][e], q, qinv);
+        u64 m2 = mkref_mred(mkref_mform(in[0][1][e], q), in[1][1][e], q, qinv);
+        u64 ls = mkref_mform(b_addmod(in[0][0][e], in[0][1][e], q), q);
+        u64 rs = b_addmod(in[1][0][e], in[1][1][e], q);
+        u64 m1 = mkref_mred(ls, rs, q, qinv);
+        m1 = (m1 + 2 * q - m0 - m2) % q;
+        mu[0][e] = m0;
+        mu[1][e] = m1;
+        mu[2][e] = m2;
+    }
+    u64 *p[3];
+    for (int i = 0; i < 3; i++) {
+        p[i] = malloc(sizeof(u64) * psz);
+        mkref_bfv_quantize(b, mu[i], p[i]);
+    }
+    mkref_mkf_switch_one(c, level, p[2], 0, relin_key, 0, s0, s1);
+    for (size_t e = 0; e < psz; e++) {
+        u64 q = c->ringQP.q[e / N];
+        out0[e] = b_addmod(p[0][e], s0[e], q);
+        out1[e] = b_addmod(p[1][e], s1[e], q);
+    }
+    for (int s = 0; s < 2; s++) { free(f[s][0]); free(f[s][1]); free(in[s][0]); free(in[s][1]); }
+    for (int i = 0; i < 3; i++) { free(mu[i]); free(p[i]); }
+    free(s0);
+    free(s1);
+}
+
 mkref_bfv *mkref_bfv_new(int logN, const u64 *Q, int L, const u64 *QMul, const u64 *P, int K, u64 t,
-                         int root_rule) {
+                         int gamma, int root_rule) {
     mkref_bfv *b = malloc(sizeof(*b));
     if (!b) return NULL;
-    if (mkref_bfv_init(b, logN, Q, L, QMul, P, K, t, root_rule)) {
+    if (mkref_bfv_init(b, logN, Q, L, QMul, P, K, t, gamma, root_rule)) {
         free(b);
         return NULL;
     }

@@ -128,3 +128,76 @@ def test_bfv_gpu_mulrelin_decrypts_exactly():
     assert ids == [0, 1]
     exp = [v % t for v in G.negacyclic_mul(m0, m1)]
     assert keys.decode(out, ids) == exp
+
+
+# ---------------------------------------------------------------- MKF MK-BFV
+
+@pytest.fixture(scope="module", params=["bsmall", "b3"])
+def mkf_setup(request):
+    import mkf_amd
+
+    logN, Q, QM, P, t = bfv_config(request.param)
+    ctx = mkf_amd.Context(logN, Q, P, gamma=1, t=t, qmul=QM)  # MKF: Lattigo rlwe, alpha = K
+    ref = O.BfvCtx(logN, Q, QM, P, t, gamma=1)
+    yield request.param, ctx, ref
+    ctx.close()
+
+
+def _mkf_keys(ref, k, seed):
+    g = O.Prng(seed)
+    beta = ref.qp.beta()
+
+    def swk():
+        return np.stack([np.stack([g.uniform(ref.QP, ref.N), g.uniform(ref.QP, ref.N)]) for _ in range(beta)])
+
+    return [swk() for _ in range(k)], swk()
+
+
+def test_mkf_bfv_mulrelin_parity(mkf_setup):
+    name, ctx, ref = mkf_setup
+    k = 2 if name == "bsmall" else 4
+    fold, relin = _mkf_keys(ref, k, seed=51)
+    ctx.register_mkf_group(5, fold, relin)
+    g = O.Prng(52)
+    left = np.stack([g.uniform(ref.Q, ref.N) for _ in range(1 + k)])
+    right = np.stack([g.uniform(ref.Q, ref.N) for _ in range(1 + k)])
+    out = ctx.mkf_bfv_mulrelin(5, left, right)
+    c0, gamma = ref.mkf_mulrelin(left, right, fold, relin)
+    assert np.array_equal(out[0], c0)
+    for p in range(k):
+        assert np.array_equal(out[1 + p], gamma), p
+    ctx.unregister_mkf_group(5)
+
+
+def test_mkf_bfv_batched_plan_matches_oracle(mkf_setup):
+    import mkf_amd
+
+    name, ctx, ref = mkf_setup
+    k = 3
+    fold, relin = _mkf_keys(ref, k, seed=53)
+    ctx.register_mkf_group(6, fold, relin)
+    g = O.Prng(54)
+    L, N, B = ref.L, ref.N, 2
+    ops, bufs, exp = [], [], []
+    for _ in range(B):
+        left = np.stack([g.uniform(ref.Q, N) for _ in range(1 + k)])
+        right = np.stack([g.uniform(ref.Q, N) for _ in range(1 + k)])
+        dl, dr, do = ctx.alloc(left.nbytes), ctx.alloc(right.nbytes), ctx.alloc(left.nbytes)
+        ctx.h2d(dl, left)
+        ctx.h2d(dr, right)
+        bufs += [dl, dr, do]
+        ops.append(dict(left=dl, right=dr, out=do, group=6, level=L - 1, nres=0))
+        c0, gm = ref.mkf_mulrelin(left, right, fold, relin)
+        exp.append(np.stack([c0] + [gm] * k))
+    plan = mkf_amd.Plan(ctx, ops, use_graph=True, scheme="mkf_bfv")
+    for rep in range(3):
+        plan.launch()
+        ctx.sync()
+        for b in range(B):
+            got = ctx.d2h(np.zeros((1 + k, L, N), np.uint64), ops[b]["out"])
+            assert np.array_equal(got, exp[b]), (rep, b)
+        ctx.h2d(ops[0]["out"], np.zeros((1 + k, L, N), np.uint64))  # eager work between replays
+    plan.close()
+    for p in bufs:
+        ctx.free(p)
+    ctx.unregister_mkf_group(6)

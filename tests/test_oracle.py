@@ -477,3 +477,42 @@ def test_kklss_mulrelin_below_top_level_decrypts():
     got = G.centered_crt(keys.decrypt(out, ids), Q[:level + 1])
     exp = G.negacyclic_mul([int(v) * D for v in x], [int(v) * D for v in y])
     assert max(abs(a - b) for a, b in zip(got, exp)) < 2 ** 60
+
+
+# ---------------------------------------------------------------- MKF MK-BFV
+
+def test_mkf_bfv_mulrelin_decrypts_exactly():
+    """MKF BFVCase.Run restated (mkf_cpu/bfv.go:212-252): fold, lift / rescale into
+    R, Karatsuba, quantize, relinearize; Dec_s(c0, Gamma) = (sum m_left)(sum m_right)
+    mod t exactly, as the reference's validation (:242-250)."""
+    logN, L, k = 10, 3, 3
+    Q = O.gen_primes(50, 15, L)
+    QM = O.gen_primes(50, 15, L, exclude=Q)
+    P = O.gen_primes(60, 15, 2)
+    bctx = O.BfvCtx(logN, Q, QM, P, 65537, gamma=1)
+    ctx = bctx.qp
+    keys = G.MKFKeys(ctx, k, seed=41)
+    g = O.Prng(42)
+    N, t = ctx.N, bctx.t
+    Qp = math.prod(Q)
+    delta = Qp // t
+
+    def enc(ms):
+        c0 = np.zeros((L, N), np.uint64)
+        comps = []
+        for p, m in enumerate(ms):
+            pt = np.stack([np.array([(int(x) * delta) % q for x in m], np.uint64) for q in Q])
+            a, b = keys.encrypt(p, pt, g)
+            c0 = G.addmod(c0, ctx.ntt_poly(a, inverse=True), Q)
+            comps.append(ctx.ntt_poly(b, inverse=True))
+        return np.stack([c0] + comps)
+
+    ml = [[int(x) for x in g.uniform([t], N)[0]] for _ in range(k)]
+    mr = [[int(x) for x in g.uniform([t], N)[0]] for _ in range(k)]
+    left, right = enc(ml), enc(mr)
+    c0, gamma = bctx.mkf_mulrelin(left, right, keys.fold, keys.relin)
+    x = G.centered_crt(keys.decrypt(ctx.ntt_poly(c0), ctx.ntt_poly(gamma)), Q)
+    got = [((t * v + Qp // 2) // Qp) % t for v in x]
+    a = [sum(col) % t for col in zip(*ml)]
+    b = [sum(col) % t for col in zip(*mr)]
+    assert got == [v % t for v in G.negacyclic_mul(a, b)]
