@@ -1,18 +1,23 @@
 """Benchmark: KKLSS MK-CKKS mul+relin per second at N=2^14, k=4 (batched vs plain).
 
-Workload (the headline configuration): N = 2^14, 8 Q primes (~50-bit) + 2 P
-primes (~60-bit, gadget digits over the Q limbs, alpha = 1, beta = 8), k = 4
-parties with evaluation keys resident in HBM, a batch of 64 independent
-ciphertext pairs (uniform residues, seeded) per step, one hipGraph launch per
-step.  A step is one mul+relin+rescale of the whole batch; `value` is
-operations per second over the timed steps (inputs already resident in HBM).
+Headline workload: N = 2^14, 8 Q primes (~50-bit) + 2 P primes (~60-bit, gadget
+digits over the Q limbs, alpha = 1, beta = 8), k = 4 parties with evaluation
+keys resident in HBM, a batch of 64 independent ciphertext pairs (uniform
+residues, seeded) per step, one hipGraph launch per step.  A step is one
+mul+relin+rescale of the whole batch; `value` is operations per second over the
+timed steps (inputs already resident in HBM).
 
-Also reported: the plain per-op GPU path (one op per launch, synchronised) with
-p50/p99 latency, the per-stage device times (HIP events on the library's stream),
-the roofline of the dominant kernel, and the CPU oracle timed on a bounded
-sample of the same workload on this host.
+Also reported, each with its batched and plain (one op per launch) rates and
+per-stage device times:
+  mkf_ckks   MKF MK-CKKS at the same shape (gamma 1: alpha = 2, beta = 4), B = 64
+  kklss_bfv  KKLSS MK-BFV, N = 2^14, k = 4, t = 65537, 6 Q + 6 QMul + 2 P
+             (the reference's PN14QP439 primes), B = 32
+  mkf_bfv    MKF MK-BFV at the MK-BFV shape (gamma 1), B = 32
+plus the roofline of the headline's dominant kernel (HIP events on the library's
+stream) and the CPU oracle timed on a bounded sample of the headline workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+                       [--schemes kklss_ckks,mkf_ckks,kklss_bfv,mkf_bfv]
 """
 import argparse
 import json
@@ -27,6 +32,16 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MK-CKKS mul+relin/s at N=2^14 k=4 (batched vs plain); p50/p99 latency"
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+ALL_SCHEMES = ["kklss_ckks", "mkf_ckks", "kklss_bfv", "mkf_bfv"]
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic_kklss_ckks_n14_l8_k4_b64.json")
+
+# The reference's MK-BFV parameter literal PN14QP439 (software/kklss/mkbfv/mkbfv_test.go:76-107)
+BFV_Q = [0x1fffffffe30001, 0x1fffffffd10001, 0x1fffffffbf0001, 0x1fffffffb60001, 0x1fffffff920001,
+         0x3fffffffd60001]
+BFV_QMUL = [0x1fffffffd80001, 0x1fffffffc50001, 0x1fffffffb90001, 0x1fffffffa50001, 0x1fffffff900001,
+            0x3fffffffca0001]
+BFV_P = [0xffffffffffc0001, 0xfffffffff840001]
+BFV_T = 65537
 
 
 def ntt_primes(bits, count):
@@ -71,7 +86,7 @@ def pct(xs, p):
 
 
 def stage_alg_bytes(B, k, L, K, beta, N, W=8):
-    """Algorithmic bytes per launch of each plan stage (each datum counted once).
+    """Algorithmic bytes per launch of each KKLSS MK-CKKS plan stage (each datum once).
 
     Per-unit figures: a limb transform (NTT / INTT) moves 2 N W bytes (read +
     write); a key-switch MAC moves (digit words + key words + output words) W;
@@ -79,17 +94,158 @@ def stage_alg_bytes(B, k, L, K, beta, N, W=8):
     count once per launch."""
     rows = L + K
     limb = N * W
-    return [
-        B * 2 * k * beta * rows * 2 * limb,                                     # decompose: limb NTTs
-        (B * 2 * k * beta * rows + 2 * k * beta * rows + B * 2 * beta * rows) * limb,  # x/y MAC
-        B * 2 * (1 + k) * L * 2 * limb,                                         # tensor input NTTs
-        B * (1 + k) * L * 2 * limb,                                             # tensor + INTT
-        B * 2 * k * (2 * beta * rows + rows) * limb,                            # Ext(x|y): D + x|y + out
-        B * 2 * k * (rows + L) * limb,                                          # ModDown
-        B * k * beta * rows * 2 * limb,                                         # decompose t
-        (B * k * (beta * rows + 2 * rows) + (k + 1) * beta * rows) * limb,      # Ext(v,u): D + out, keys once
-        B * (2 * k * (rows + L) + (1 + k) * (L + L - 1)) * limb,                 # final ModDown + rescale
-    ]
+    return {
+        "decompose": B * 2 * k * beta * rows * 2 * limb,                        # limb NTTs
+        "mac_xy": (B * 2 * k * beta * rows + 2 * k * beta * rows + B * 2 * beta * rows) * limb,
+        "decompose_mac": (B * 2 * k * beta * rows * 2 + 2 * k * beta * rows + B * 2 * beta * rows) * limb,
+        "tensor_ntt": B * 2 * (1 + k) * L * 2 * limb,                           # tensor input NTTs
+        "tensor": B * (1 + k) * L * 2 * limb,                                   # tensor + INTT
+        "ext_xy": B * 2 * k * (2 * beta * rows + rows) * limb,                  # D + x|y + out
+        "moddown_xy": B * 2 * k * (rows + L) * limb,
+        "decompose_t": B * k * beta * rows * 2 * limb,
+        "ext_vu": (B * k * (beta * rows + 2 * rows) + (k + 1) * beta * rows) * limb,  # keys once
+        "moddown_rescale": B * (2 * k * (rows + L) + (1 + k) * (L + L - 1)) * limb,
+    }
+
+
+def op_alg_bytes(scheme, B, k, L, K, beta, N, W=8):
+    """Algorithmic bytes per op: inputs + outputs + keys shared by the batch."""
+    b_in = 2 * (k + 1) * L * N * W
+    if scheme == "kklss_ckks":
+        return b_in + (k + 1) * (L - 1) * N * W + (3 * k + 1) * beta * (L + K) * N * W / B
+    if scheme == "mkf_ckks":
+        return b_in + (k + 1) * (L - 1) * N * W + (k + 1) * beta * 2 * (L + K) * N * W / B
+    if scheme == "kklss_bfv":
+        return b_in + (k + 1) * L * N * W + (5 * k + 1) * beta * (L + K) * N * W / B
+    return b_in + (k + 1) * L * N * W + (k + 1) * beta * 2 * (L + K) * N * W / B
+
+
+class Workload:
+    """One scheme at one shape: context, resident keys, batched + plain plans."""
+
+    def __init__(self, scheme, rng, batch):
+        import mkf_amd
+
+        self.scheme, self.B = scheme, batch
+        self.k = 4
+        if scheme.endswith("ckks"):
+            self.logN, self.L, self.K = 14, 8, 2
+            self.Q, self.P, self.QM, self.t = ntt_primes(50, 8), ntt_primes(60, 2), None, 0
+        else:
+            self.logN, self.L, self.K = 14, len(BFV_Q), len(BFV_P)
+            self.Q, self.P, self.QM, self.t = BFV_Q, BFV_P, BFV_QMUL, BFV_T
+        self.N = 1 << self.logN
+        gamma = 2 if scheme.startswith("kklss") else 1
+        self.ctx = mkf_amd.Context(self.logN, self.Q, self.P, gamma=gamma, t=self.t, qmul=self.QM)
+        self.beta = self.ctx.beta
+        QP, N, k = self.Q + self.P, self.N, self.k
+        self.keys, self.u = [], None
+        if scheme == "kklss_ckks":
+            for pid in range(k):
+                kk = [uniform_stack(rng, QP, N, self.beta) for _ in range(3)]
+                self.ctx.register_party(pid, *kk)
+                self.keys.append(kk)
+        elif scheme == "kklss_bfv":
+            for pid in range(k):
+                b = uniform_stack(rng, QP, N, 2 * self.beta)
+                d = uniform_stack(rng, QP, N, 2 * self.beta)
+                v = uniform_stack(rng, QP, N, self.beta)
+                self.ctx.register_bfv_party(pid, b, d, v)
+        else:
+            fold = [np.stack([np.stack([uniform(rng, QP, N), uniform(rng, QP, N)]) for _ in range(self.beta)])
+                    for _ in range(k)]
+            relin = np.stack([np.stack([uniform(rng, QP, N), uniform(rng, QP, N)]) for _ in range(self.beta)])
+            self.ctx.register_mkf_group(0, fold, relin)
+        if scheme.startswith("kklss"):
+            self.u = uniform_stack(rng, QP, N, self.beta)
+            self.ctx.set_crs_u(self.u)
+        self.nres = 0
+        if scheme.endswith("ckks"):
+            scale = float(self.Q[-1])
+            self.nres, _ = self.ctx.rescale_count(self.L - 1, scale * scale, scale)
+        self.bufs = []
+
+    def make_ops(self, n, rng):
+        L, N, k = self.L, self.N, self.k
+        ids = list(range(k))
+        in_words = (1 + k) * L * N
+        out_words = (1 + k) * (L - self.nres) * N
+        ops = []
+        for _ in range(n):
+            d0, d1, do = self.ctx.alloc(in_words * 8), self.ctx.alloc(in_words * 8), self.ctx.alloc(out_words * 8)
+            self.ctx.h2d(d0, np.concatenate([uniform(rng, self.Q, N) for _ in range(1 + k)]))
+            self.ctx.h2d(d1, np.concatenate([uniform(rng, self.Q, N) for _ in range(1 + k)]))
+            self.bufs += [d0, d1, do]
+            if self.scheme.startswith("kklss"):
+                ops.append(dict(op0=d0, op1=d1, out=do, ids0=ids, ids1=ids, level=L - 1, nres=self.nres))
+            else:
+                ops.append(dict(left=d0, right=d1, out=do, group=0, level=L - 1, nres=self.nres))
+        return ops
+
+    def plan(self, ops):
+        import mkf_amd
+
+        sch = {"kklss_ckks": "kklss", "kklss_bfv": "bfv", "mkf_ckks": "mkf", "mkf_bfv": "mkf_bfv"}[self.scheme]
+        return mkf_amd.Plan(self.ctx, ops, use_graph=True, scheme=sch)
+
+    def close(self):
+        for p in self.bufs:
+            self.ctx.free(p)
+        self.ctx.close()
+
+
+def uniform_stack(rng, moduli, N, n):
+    return np.stack([uniform(rng, moduli, N) for _ in range(n)])
+
+
+def time_plan(ctx, plan, steps, warmup):
+    for _ in range(warmup):
+        plan.launch()
+    ctx.sync()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = time.perf_counter()
+        plan.launch()
+        ctx.sync()
+        lat.append(time.perf_counter() - s)
+    return time.perf_counter() - t0, lat
+
+
+def run_scheme(scheme, args, rng):
+    B = args.batch if scheme.endswith("ckks") else args.bfv_batch
+    w = Workload(scheme, rng, B)
+    ops = w.make_ops(B, rng)
+    sch = {"kklss_ckks": "kklss", "kklss_bfv": "bfv", "mkf_ckks": "mkf", "mkf_bfv": "mkf_bfv"}[scheme]
+    tune_ms = w.ctx.autotune(ops, scheme=sch, iters=2) if args.autotune else []
+    plan = w.plan(ops)
+    elapsed, lat = time_plan(w.ctx, plan, args.steps, args.warmup)
+    ops_s = B * args.steps / elapsed
+    stage_ms = plan.stage_times(args.stage_iters)
+    names = plan.stages
+    res = {
+        "ops_per_s": round(ops_s, 2), "ms_per_step": round(elapsed / args.steps * 1e3, 3), "batch": B,
+        "p50_ms": round(pct(lat, 50) * 1e3, 3), "p99_ms": round(pct(lat, 99) * 1e3, 3),
+        "stages_ms": {names[s]: round(stage_ms[s], 4) for s in range(len(names))},
+        "workspace_gb": round(plan.workspace_bytes / 1e9, 3),
+        "autotune_ms": [round(t, 3) for t in tune_ms],
+        "tuning": {"fused_decomp": w.ctx.get_tuning(1), "xcd_remap": w.ctx.get_tuning(2)},
+        "shape": {"logN": w.logN, "L": w.L, "K": w.K, "k": w.k, "alpha": w.ctx.alpha, "beta": w.beta,
+                  "nres": w.nres, "t": w.t},
+    }
+    alg = op_alg_bytes(scheme, B, w.k, w.L, w.K, w.beta, w.N)
+    res["alg_bytes_per_op"] = int(alg)
+    res["op_gbs"] = round(alg * ops_s / 1e9, 1)
+    if args.plain_ops > 0:
+        pops = w.make_ops(1, rng)
+        pplan = w.plan(pops)
+        pel, plat = time_plan(w.ctx, pplan, args.plain_ops, 3)
+        res["plain"] = {"ops_per_s": round(args.plain_ops / pel, 2), "p50_ms": round(pct(plat, 50) * 1e3, 3),
+                        "p99_ms": round(pct(plat, 99) * 1e3, 3)}
+        res["batched_over_plain"] = round(ops_s / res["plain"]["ops_per_s"], 3)
+        pplan.close()
+    extra = (w, plan, stage_ms, names)
+    return res, extra
 
 
 def cpu_baseline(logN, Q, P, k, keys, u, budget_s):
@@ -111,9 +267,34 @@ def cpu_baseline(logN, Q, P, k, keys, u, budget_s):
         el = time.perf_counter() - t0
         if el >= budget_s or n >= 64:
             break
-    return {"value": n / el, "unit": "ops/s", "cores": 1, "kind": "port",
+    return {"value": round(n / el, 3), "unit": "ops/s", "cores": 1, "kind": "port",
             "sample": "%d KKLSS MK-CKKS mul+relin+rescale ops (N=2^14, L=8, K=2, k=4) with the C oracle, "
                       "single thread, %.1f s" % (n, el)}
+
+
+def copy_calibration(mb):
+    """A plain device-to-device copy of known size, for scaling PMC byte counters."""
+    import torch
+
+    n = int(mb * (1 << 20)) // 8
+    src = torch.ones(n, dtype=torch.int64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if any."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel:
+        return None
+    return t.get("bytes_per_launch")
 
 
 def main():
@@ -122,135 +303,74 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--bfv-batch", type=int, default=32)
     ap.add_argument("--plain-ops", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-iters", type=int, default=3)
+    ap.add_argument("--schemes", default=",".join(ALL_SCHEMES))
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the library's autotuner (plans use the default knobs)")
+    ap.add_argument("--copy-calib-mb", type=float, default=0.0,
+                    help="run a plain D2D copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (ops shard across GPUs "
               "with zero per-op traffic; see the design notes). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    schemes = [s for s in args.schemes.split(",") if s]
+    for s in schemes:
+        if s not in ALL_SCHEMES:
+            ap.error("unknown scheme %s" % s)
 
-    import mkf_amd
-
-    logN, L, K, k, B = 14, 8, 2, 4, args.batch
-    N = 1 << logN
-    Q = ntt_primes(50, L)
-    P = ntt_primes(60, K)
-    QP = Q + P
-    ctx = mkf_amd.Context(logN, Q, P, gamma=2)
-    beta = ctx.beta
+    calib = copy_calibration(args.copy_calib_mb) if args.copy_calib_mb > 0 else None
     rng = np.random.default_rng(0)
-    keys = []
-    for pid in range(k):
-        kk = [np.stack([uniform(rng, QP, N) for _ in range(beta)]) for _ in range(3)]
-        ctx.register_party(pid, *kk)
-        keys.append(kk)
-    u = np.stack([uniform(rng, QP, N) for _ in range(beta)])
-    ctx.set_crs_u(u)
-    ids = list(range(k))
-    scale = float(Q[-1])
-    nres, _ = ctx.rescale_count(L - 1, scale * scale, scale)
+    results, head = {}, None
+    for s in schemes:
+        res, extra = run_scheme(s, args, rng)
+        results[s] = res
+        if s == "kklss_ckks":
+            head = (res, extra)
+        else:
+            extra[1].close()
+            extra[0].close()
+        print("# %s: %.1f ops/s batched" % (s, res["ops_per_s"]), file=sys.stderr, flush=True)
 
-    in_words = (1 + k) * L * N
-    out_words = (1 + k) * (L - nres) * N
+    line = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic"}
+    if head is not None:
+        import mkf_amd
 
-    def make_ops(n):
-        ops, bufs = [], []
-        for _ in range(n):
-            d0, d1 = ctx.alloc(in_words * 8), ctx.alloc(in_words * 8)
-            do = ctx.alloc(out_words * 8)
-            ctx.h2d(d0, np.concatenate([uniform(rng, Q, N) for _ in range(1 + k)]))
-            ctx.h2d(d1, np.concatenate([uniform(rng, Q, N) for _ in range(1 + k)]))
-            bufs += [d0, d1, do]
-            ops.append(dict(op0=d0, op1=d1, out=do, ids0=ids, ids1=ids, level=L - 1, nres=nres))
-        return ops, bufs
-
-    # ---- batched: one graph launch per step --------------------------------
-    ops, bufs = make_ops(B)
-    plan = mkf_amd.Plan(ctx, ops, use_graph=True)
-    for _ in range(args.warmup):
-        plan.launch()
-    ctx.sync()
-    lat = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s = time.perf_counter()
-        plan.launch()
-        ctx.sync()
-        lat.append(time.perf_counter() - s)
-    elapsed = time.perf_counter() - t0
-    batched_ops = B * args.steps / elapsed
-    # per-stage device times (HIP events on the stream the kernels run on)
-    stage_ms = plan.stage_times(args.stage_iters)
-    stage_names = plan.stages
-    ws_bytes = plan.workspace_bytes
-
-    # ---- plain per-op path: one op per launch ------------------------------
-    pbufs, pplan, plat, plain_ops = [], None, [float("nan")], float("nan")
-    if args.plain_ops > 0:
-        pops, pbufs = make_ops(1)
-        pplan = mkf_amd.Plan(ctx, pops, use_graph=True)
-        for _ in range(3):
-            pplan.launch()
-        ctx.sync()
-        plat = []
-        t1 = time.perf_counter()
-        for _ in range(args.plain_ops):
-            s = time.perf_counter()
-            pplan.launch()
-            ctx.sync()
-            plat.append(time.perf_counter() - s)
-        plain_ops = args.plain_ops / (time.perf_counter() - t1)
-
-    # ---- roofline of the dominant kernel ------------------------------------
-    rows = L + K
-    alg = stage_alg_bytes(B, k, L, K, beta, N)
-    dom = max(range(len(stage_ms)), key=lambda s: stage_ms[s])
-    alg_bytes = alg[dom]
-    achieved = alg_bytes / (stage_ms[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": mkf_amd.STAGE_KERNELS[stage_names[dom]], "stage": stage_names[dom],
-                "kernel_ms": round(stage_ms[dom], 4), "alg_bytes_per_launch": alg_bytes}
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(logN, Q, P, k, keys, u, args.cpu_seconds)
-
-    line = {
-        "metric": METRIC,
-        "value": round(batched_ops, 2),
-        "unit": "ops/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64",
-        "data": "synthetic",
-        "config": {"workload": "kklss-mkckks-mulrelin-rescale batched", "logN": logN, "L": L, "K": K,
-                   "k": k, "batch": B, "alpha": ctx.alpha, "beta": beta, "nres": nres, "graph": True},
-        "batched": {"ops_per_s": round(batched_ops, 2), "p50_ms": round(pct(lat, 50) * 1e3, 3),
-                    "p99_ms": round(pct(lat, 99) * 1e3, 3), "ops_per_launch": B},
-        "plain": {"ops_per_s": round(plain_ops, 2), "p50_ms": round(pct(plat, 50) * 1e3, 3),
-                  "p99_ms": round(pct(plat, 99) * 1e3, 3)},
-        "batched_over_plain": round(batched_ops / plain_ops, 3),
-        "stages_ms": {stage_names[s]: round(stage_ms[s], 4) for s in range(len(stage_ms))},
-        "workspace_gb": round(ws_bytes / 1e9, 3),
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
+        res, (w, plan, stage_ms, names) = head
+        B, k, L, K, beta, N = w.B, w.k, w.L, w.K, w.beta, w.N
+        alg = stage_alg_bytes(B, k, L, K, beta, N)
+        dom = max(range(len(stage_ms)), key=lambda s: stage_ms[s])
+        dom_bytes = alg[names[dom]]
+        achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
+        kern = mkf_amd.STAGE_KERNELS[names[dom]]
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kern),
+                    "kernel": kern, "stage": names[dom], "kernel_ms": round(stage_ms[dom], 4),
+                    "alg_bytes_per_launch": dom_bytes}
+        cpu = None
+        if not args.no_cpu:
+            cpu = cpu_baseline(w.logN, w.Q, w.P, k, w.keys, w.u, args.cpu_seconds)
+        line.update({
+            "value": res["ops_per_s"], "unit": "ops/s", "ms_per_step": res["ms_per_step"],
+            "config": {"workload": "kklss-mkckks-mulrelin-rescale batched", "logN": w.logN, "L": L, "K": K,
+                       "k": k, "batch": B, "alpha": w.ctx.alpha, "beta": beta, "nres": w.nres, "graph": True},
+            "batched": {"ops_per_s": res["ops_per_s"], "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"],
+                        "ops_per_launch": B},
+            "plain": res.get("plain"), "batched_over_plain": res.get("batched_over_plain"),
+            "stages_ms": res["stages_ms"], "workspace_gb": res["workspace_gb"],
+            "roofline": roofline, "cpu_baseline": cpu,
+        })
+        plan.close()
+        w.close()
+    line["schemes"] = {s: r for s, r in results.items() if s != "kklss_ckks"}
+    if calib:
+        line["copy_calibration"] = calib
     print(json.dumps(line), flush=True)
-    plan.close()
-    if pplan is not None:
-        pplan.close()
-    for p in bufs + pbufs:
-        ctx.free(p)
-    ctx.close()
 
 
 if __name__ == "__main__":

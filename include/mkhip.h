@@ -203,6 +203,27 @@ int mkhip_bfv_rescale_q_to_r(mkhip_ctx *ctx, const uint64_t *in, uint64_t *out);
 int mkhip_bfv_quantize(mkhip_ctx *ctx, const uint64_t *in_ntt, uint64_t *out);
 int mkhip_bfv_decompose(mkhip_ctx *ctx, const uint64_t *aR, uint64_t *out);
 
+/*
+ * Tuning knobs (no counterpart in the reference: they choose between equivalent
+ * kernel schedules; every setting gives the same bits).
+ *   MKHIP_TUNE_FUSED_DECOMP  alpha = 1 KKLSS plans: the gadget decomposition fused
+ *                            with the x / y key MAC (1) or as two launches (0)
+ *   MKHIP_TUNE_XCD_REMAP     workgroups ordered XCD-contiguously (1) or as issued (0)
+ * A knob applies to plans built after it is set.  mkhip_autotune plans and times
+ * every applicable combination on the caller's batch (ops: mkhip_kklss_op[] for the
+ * KKLSS schemes, mkhip_mkf_op[] for MKF; their outputs are overwritten with the
+ * results), keeps the fastest, and writes up to 8 candidate times (ms per launch).
+ */
+#define MKHIP_TUNE_FUSED_DECOMP 1
+#define MKHIP_TUNE_XCD_REMAP 2
+#define MKHIP_SCHEME_KKLSS_CKKS 0
+#define MKHIP_SCHEME_KKLSS_BFV 1
+#define MKHIP_SCHEME_MKF_CKKS 2
+#define MKHIP_SCHEME_MKF_BFV 3
+int mkhip_set_tuning(mkhip_ctx *ctx, int knob, int value);
+int mkhip_get_tuning(const mkhip_ctx *ctx, int knob, int *value);
+int mkhip_autotune(mkhip_ctx *ctx, int scheme, int nops, const void *ops, int iters, int *ncand, double *ms);
+
 /* reference scale loop (mkckks/evaluator.go:383-388): number of rescales */
 int mkhip_rescale_count(const mkhip_ctx *ctx, int level, double scale, double min_scale, int *nres,
                         double *new_scale);

@@ -36,8 +36,10 @@ EXPORTS = [
     "mkhip_register_mkf_group", "mkhip_unregister_mkf_group", "mkhip_mkf_ckks_mulrelin", "mkhip_plan_mkf_ckks",
     "mkhip_register_bfv_party", "mkhip_kklss_bfv_mulrelin", "mkhip_plan_kklss_bfv", "mkhip_bfv_modup_q_to_r",
     "mkhip_bfv_rescale_q_to_r", "mkhip_bfv_quantize", "mkhip_bfv_decompose", "mkhip_mkf_bfv_mulrelin",
-    "mkhip_plan_mkf_bfv",
+    "mkhip_plan_mkf_bfv", "mkhip_set_tuning", "mkhip_get_tuning", "mkhip_autotune",
 ]
+TUNE_FUSED_DECOMP, TUNE_XCD_REMAP = 1, 2
+SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3}
 
 # kernel run by each named plan stage
 STAGE_KERNELS = {
@@ -48,7 +50,7 @@ STAGE_KERNELS = {
     "decompose_mu2": "k_decomp", "mac_relin": "k_mac2", "moddown_relin": "k_moddown_ntt",
     "intt_last": "k_ntt_batch<inv>", "rescale": "k_rescale_ntt",
     "lift": "k_bfv_lift", "quantize": "k_quantize", "moddown_final": "k_moddown<true>",
-    "decompose_relin": "k_decomp", "materialize": "k_rescale_ntt",
+    "decompose_relin": "k_decomp", "materialize": "k_rescale_ntt", "decompose_mac": "k_decomp_mac",
 }
 # the BFV plans' tensor stage runs the R-basis kernel; MKF-BFV's ModDowns are coefficient-domain
 BFV_STAGE_KERNELS = dict(STAGE_KERNELS, tensor="k_tensor_r")
@@ -135,6 +137,9 @@ def _declare(L):
     for f in ("mkhip_bfv_modup_q_to_r", "mkhip_bfv_rescale_q_to_r", "mkhip_bfv_quantize", "mkhip_bfv_decompose"):
         getattr(L, f).argtypes = [V, u64p, u64p]
     L.mkhip_mkf_bfv_mulrelin.argtypes = [V, I, u64p, u64p, u64p]
+    L.mkhip_set_tuning.argtypes = [V, I, I]
+    L.mkhip_get_tuning.argtypes = [V, I, intp]
+    L.mkhip_autotune.argtypes = [V, I, I, V, I, intp, dblp]
     L.mkhip_plan_mkf_bfv.argtypes = [V, I, ctypes.POINTER(MKFOp), I, ctypes.POINTER(V)]
 
 
@@ -317,6 +322,25 @@ class Context:
               "mkhip_mkf_bfv_mulrelin")
         return out
 
+    # ---- tuning ----
+    def set_tuning(self, knob, value):
+        check(lib().mkhip_set_tuning(self._h, knob, int(value)), "mkhip_set_tuning")
+
+    def get_tuning(self, knob):
+        v = ctypes.c_int()
+        check(lib().mkhip_get_tuning(self._h, knob, ctypes.byref(v)), "mkhip_get_tuning")
+        return v.value
+
+    def autotune(self, ops, scheme="kklss", iters=3):
+        """Time every applicable knob combination on this batch and keep the fastest;
+        returns the candidate times (ms per launch)."""
+        arr = _marshal_ops(scheme, ops)
+        n = ctypes.c_int()
+        ms = (ctypes.c_double * 8)()
+        check(lib().mkhip_autotune(self._h, SCHEME_IDS[scheme], len(ops), ctypes.cast(arr, ctypes.c_void_p),
+                                   iters, ctypes.byref(n), ms), "mkhip_autotune")
+        return list(ms)[:n.value]
+
     # ---- primitives ----
     def ntt(self, rows, ring_idx, inverse=False):
         rows = _u64(rows)
@@ -369,6 +393,29 @@ class Context:
         check(lib().mkhip_sync(self._h), "mkhip_sync")
 
 
+def _marshal_ops(scheme, ops):
+    """dict ops -> the C array the plan builders take (mkhip_kklss_op[] or mkhip_mkf_op[])."""
+    if scheme in ("kklss", "bfv"):
+        arr = (KKLSSOp * len(ops))()
+        for i, o in enumerate(ops):
+            a = arr[i]
+            a.op0, a.op1, a.out = o["op0"], o["op1"], o["out"]
+            for j, v in enumerate(o["ids0"]):
+                a.ids0[j] = v
+            for j, v in enumerate(o["ids1"]):
+                a.ids1[j] = v
+            a.k0, a.k1, a.level, a.nres = len(o["ids0"]), len(o["ids1"]), o["level"], o["nres"]
+        return arr
+    if scheme in ("mkf", "mkf_bfv"):
+        arr = (MKFOp * len(ops))()
+        for i, o in enumerate(ops):
+            a = arr[i]
+            a.left, a.right, a.out = o["left"], o["right"], o["out"]
+            a.group, a.level, a.nres = o["group"], o["level"], o["nres"]
+        return arr
+    raise ValueError(scheme)
+
+
 class Plan:
     """A batched mul+relin over device-resident operands (one hipGraph per launch).
 
@@ -379,28 +426,10 @@ class Plan:
 
     def __init__(self, ctx, ops, use_graph=True, scheme="kklss"):
         h = ctypes.c_void_p()
-        if scheme in ("kklss", "bfv"):
-            arr = (KKLSSOp * len(ops))()
-            for i, o in enumerate(ops):
-                a = arr[i]
-                a.op0, a.op1, a.out = o["op0"], o["op1"], o["out"]
-                for j, v in enumerate(o["ids0"]):
-                    a.ids0[j] = v
-                for j, v in enumerate(o["ids1"]):
-                    a.ids1[j] = v
-                a.k0, a.k1, a.level, a.nres = len(o["ids0"]), len(o["ids1"]), o["level"], o["nres"]
-            fn = "mkhip_plan_kklss_ckks" if scheme == "kklss" else "mkhip_plan_kklss_bfv"
-            check(getattr(lib(), fn)(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)), fn)
-        elif scheme in ("mkf", "mkf_bfv"):
-            arr = (MKFOp * len(ops))()
-            for i, o in enumerate(ops):
-                a = arr[i]
-                a.left, a.right, a.out = o["left"], o["right"], o["out"]
-                a.group, a.level, a.nres = o["group"], o["level"], o["nres"]
-            fn = "mkhip_plan_mkf_ckks" if scheme == "mkf" else "mkhip_plan_mkf_bfv"
-            check(getattr(lib(), fn)(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)), fn)
-        else:
-            raise ValueError(scheme)
+        fn = {"kklss": "mkhip_plan_kklss_ckks", "bfv": "mkhip_plan_kklss_bfv", "mkf": "mkhip_plan_mkf_ckks",
+              "mkf_bfv": "mkhip_plan_mkf_bfv"}[scheme]
+        arr = _marshal_ops(scheme, ops)
+        check(getattr(lib(), fn)(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)), fn)
         self._h = h
         self.ctx = ctx
         self.scheme = scheme

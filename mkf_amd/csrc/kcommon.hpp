@@ -8,18 +8,52 @@ namespace mkhip {
 
 extern __shared__ u64 g_lds[];
 
+// Pointers read from descriptor tables are generic to the compiler; every HBM
+// access goes through these global-address-space views so that it compiles to
+// global (not flat) memory instructions with a scalar base.
+typedef __attribute__((address_space(1))) u64 gu64;
+__device__ __forceinline__ const gu64 *G(const u64 *p) { return (const gu64 *)p; }
+__device__ __forceinline__ gu64 *G(u64 *p) { return (gu64 *)p; }
+
+// Scalar base of a row: p must be the same on every lane of the wave (all call
+// sites pass a per-workgroup row pointer); the per-lane offset is then a single
+// 32-bit register shared by all rows.
+__device__ __forceinline__ const gu64 *uniform_ptr(const u64 *p) {
+    const unsigned long long a = (unsigned long long)p;
+    const u32 lo = __builtin_amdgcn_readfirstlane((u32)a), hi = __builtin_amdgcn_readfirstlane((u32)(a >> 32));
+    return (const gu64 *)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ gu64 *uniform_ptr(u64 *p) { return (gu64 *)uniform_ptr((const u64 *)p); }
+
 template <int LOGN>
 __device__ __forceinline__ void load_row_cached(u64 (&v)[16], const u64 *__restrict__ src, int tid) {
     constexpr int T = (1 << LOGN) / 16;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = src[tid + i * T];
+    for (int i = 0; i < 16; ++i) v[i] = uniform_ptr(src + i * T)[tid];
 }
 
 template <int LOGN>
 __device__ __forceinline__ void store_row(const u64 (&v)[16], u64 *__restrict__ dst, int tid) {
     constexpr int T = (1 << LOGN) / 16;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) dst[tid + i * T] = v[i];
+    for (int i = 0; i < 16; ++i) uniform_ptr(dst + i * T)[tid] = v[i];
+}
+
+// A wave-uniform pointer made opaque to the optimizer: table loads that use it
+// stay where they are written instead of being hoisted out of an enclosing loop
+// and held in registers across its iterations.
+template <class P>
+__device__ __forceinline__ P *opaque(P *p) {
+    const unsigned long long a = (unsigned long long)p;
+    u32 lo = __builtin_amdgcn_readfirstlane((u32)a), hi = __builtin_amdgcn_readfirstlane((u32)(a >> 32));
+    asm volatile("" : "+s"(lo), "+s"(hi));
+    return (P *)(((unsigned long long)hi << 32) | lo);
+}
+// likewise for the lane index, so that the LDS exchange addresses derived from it
+// are recomputed per transform rather than kept live across a loop
+__device__ __forceinline__ int opaque_lane(int t) {
+    asm volatile("" : "+v"(t));
+    return t;
 }
 
 // Row r of a (level+1+K)-row QP polynomial is ring modulus r (Q) or L + r - level - 1 (P).
@@ -27,16 +61,62 @@ __device__ __forceinline__ int ring_index(int r, int level, int L) { return r <=
 
 // 128-bit accumulate of a * b.
 __device__ __forceinline__ void mac128(u64 &hi, u64 &lo, u64 a, u64 b) {
-    u64 l = a * b;
-    u64 h = mulhi64(a, b);
-    lo += l;
-    hi += h + (lo < l ? 1 : 0);
+    u64 ph, pl;
+    mul_wide(a, b, ph, pl);
+    lo += pl;
+    hi += ph + (lo < pl ? 1 : 0);
 }
 
 // Lazy Montgomery reduction of a 128-bit accumulator as in multSum
 // (basis_extension.go:623-645): hi - hi(lo * pinv * p) + p.
 __device__ __forceinline__ u64 mont_reduce128(u64 hi, u64 lo, u64 p, u64 pinv) {
     return hi - mulhi64(lo * pinv, p) + p;
+}
+
+// T * 2^-64 mod q, canonical, for T = sum of at most 32 products of canonical
+// residues (q < 2^61: T < 32 q^2, so hi < 4q is brought below q first).  Equal to
+// the sum of the products' MRed values mod q.
+__device__ __forceinline__ u64 redc_sum(u64 hi, u64 lo, const ModConst &m) {
+    const u64 q2 = m.q << 1;
+    hi = hi >= q2 ? hi - q2 : hi;
+    hi = hi >= m.q ? hi - m.q : hi;
+    const u64 r = mont_reduce128(hi, lo, m.q, m.qinv);
+    return r >= m.q ? r - m.q : r;
+}
+
+// External-product rows: for the 16 values of this lane,
+//   a0[e] = sum_i MRed(k0_i[e], d_i[e]) mod q      (i < ndig, canonical)
+// and with NK = 2 the same with k1, written straight to park1 (the caller's output
+// row, where it waits while a0 is transformed).  d_i = d + i dstride, k_i = k + i
+// kstride.  Digits are walked in the outer loop (all 16 values of a row per
+// digit, four loads in flight per chunk).
+template <int LOGN, int NK>
+__device__ __forceinline__ void ext_rows(u64 (&a0)[16], u64 *park1, const u64 *d, size_t dstride, const u64 *k0,
+                                         const u64 *k1, size_t kstride, int ndig, int tid, const ModConst &m) {
+    constexpr int T = (1 << LOGN) / 16;
+    u64 a1[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) a0[e] = a1[e] = 0;
+    for (int i = 0; i < ndig; ++i) {
+        const u64 *dr = d + i * dstride, *k0r = k0 + i * kstride, *k1r = NK > 1 ? k1 + i * kstride : nullptr;
+#pragma unroll
+        for (int c = 0; c < 16; c += 4) {
+            u64 x[4], y0[4], y1[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                x[e] = uniform_ptr(dr + (c + e) * T)[tid];
+                y0[e] = uniform_ptr(k0r + (c + e) * T)[tid];
+                if (NK > 1) y1[e] = uniform_ptr(k1r + (c + e) * T)[tid];
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                a0[c + e] = add_mod(a0[c + e], mred(y0[e], x[e], m.q, m.qinv), m.q);
+                if (NK > 1) a1[c + e] = add_mod(a1[c + e], mred(y1[e], x[e], m.q, m.qinv), m.q);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if constexpr (NK > 1) store_row<LOGN>(a1, park1, tid);
 }
 
 // a * b mod q for two canonical operands: MRed(MForm(a), b).
@@ -63,7 +143,7 @@ __device__ __forceinline__ PUp modup_p_prepare(const DevTables &T, const u64 *__
         u.y[k] = 0;
         if (k < T.K) {
             const ModConst mp = T.mod[T.L + k];
-            u.y[k] = mred(src[(size_t)(nl + k) * T.N + c], T.md_y[k], mp.q, mp.qinv);
+            u.y[k] = mred(G(src)[(size_t)(nl + k) * T.N + c], T.md_y[k], mp.q, mp.qinv);
             vi += (double)u.y[k] / (double)mp.q;
         }
     }

@@ -57,7 +57,7 @@ __device__ __forceinline__ u64 digit_modup(const DevTables &T, const u64 *__rest
     double vi = 0.0;
     for (int l = 0; l < nd; ++l) {
         const ModConst ml = T.mod[start + l];
-        const u64 x = src[(size_t)(start + l) * N + c];
+        const u64 x = G(src)[(size_t)(start + l) * N + c];
         const u64 y = mred(x, yc[l], ml.q, ml.qinv);
         vi += (double)y / (double)ml.q;
         mac128(hi, lo, y, qs[l]);
@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
     constexpr int N = 1 << LOGN;
     constexpr int TH = N / 16;
     const int tid = threadIdx.x;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int bid = block_index(T);
     const int r = bid % rows;
     const int i = (bid / rows) % beta;
     const int p = (bid / (rows * beta)) % kmax;
@@ -93,10 +93,15 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
     const u64 *src = Dd.comp[p];
     u64 v[16];
     if (!CRT || nd == 1 || own_row) {
-        // copy path / the digit's own rows: the residue itself (reduced mod m)
-        load_row_cached<LOGN>(v, src + (size_t)(own_row ? r : start) * N, tid);
+        // copy path / the digit's own rows: the residue itself, reduced mod m only
+        // when it may exceed the transform's input bound 8m (source values are
+        // below 3 q_src: canonical, or multSum's raw outputs in a BFV lift)
+        const int sl = own_row ? r : start;
+        load_row_cached<LOGN>(v, src + (size_t)sl * N, tid);
+        if (T.mod[r_index(sl, T.L, T.K)].q > 2 * m.q) {
 #pragma unroll
-        for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
+            for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
+        }
     } else {
         const int ebase = i * (kMaxAlpha + 1) + nd;
         for (int e = 0; e < 16; ++e) v[e] = digit_modup(T, src, tid + e * TH, start, nd, ebase, ri, m);
@@ -123,10 +128,10 @@ __global__ void __launch_bounds__(256)
         const size_t koff = (size_t)i * kstride + (size_t)ri * N + c;
         const size_t doff = ((size_t)i * rows + r) * N + c;
         const size_t dstride = (size_t)beta * rows * N;
-        u64 acc = 0;
-        for (int p = 0; p < Md.k; ++p)
-            acc = add_mod(acc, mred(Md.key[p][koff], Md.D[p * dstride + doff], m.q, m.qinv), m.q);
-        Md.xy[koff] = mred(acc, m.r2, m.q, m.qinv);  // full-level key layout, like the keys
+        u64 hi = 0, lo = 0;
+        for (int p = 0; p < Md.k; ++p) mac128(hi, lo, G(Md.key[p])[koff], G(Md.D)[p * dstride + doff]);
+        // MForm(sum_p MRed(key_p, D_p)); full-level key layout, like the keys
+        G(Md.xy)[koff] = mred(redc_sum(hi, lo, m), m.r2, m.q, m.qinv);
     }
 }
 
@@ -139,7 +144,7 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_tensor(DevTables T, const T
     constexpr int N = 1 << LOGN;
     constexpr int TH = N / 16;
     const int tid = threadIdx.x;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int bid = block_index(T);
     const int l = bid % nl;
     const TensorDesc &Dd = descs[bid / nl];
     const ModConst m = T.mod[l];
@@ -150,11 +155,11 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_tensor(DevTables T, const T
         const size_t c = off + tid + e * TH;
         u64 x;
         if (Dd.is_c0) {
-            x = mulmod_m(Dd.a0[c], Dd.b0[c], m);
+            x = mulmod_m(G(Dd.a0)[c], G(Dd.b0)[c], m);
         } else {
             x = 0;
-            if (Dd.au) x = mulmod_m(Dd.b0[c], Dd.au[c], m);
-            if (Dd.bu) x = add_mod(x, mulmod_m(Dd.a0[c], Dd.bu[c], m), m.q);
+            if (Dd.au) x = mulmod_m(G(Dd.b0)[c], G(Dd.au)[c], m);
+            if (Dd.bu) x = add_mod(x, mulmod_m(G(Dd.a0)[c], G(Dd.bu)[c], m), m.q);
         }
         v[e] = x;
     }
@@ -163,7 +168,8 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_tensor(DevTables T, const T
 }
 
 // ---------------------------------------------------------------------------
-// external product MAC + INTT ; blockIdx.x = desc * rows + row
+// external product MAC + INTT ; logical block = row * ndesc + desc (descriptors
+// that share key rows are adjacent, so their workgroups share an XCD's L2)
 
 // NKEY keys share one hoisted decomposition D (the nested products with v_i and
 // u, keyswitch_hoisted.go:177-193): one pass over D feeds both accumulators; the
@@ -174,35 +180,26 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
     constexpr int N = 1 << LOGN;
     constexpr int TH = N / 16;
     const int tid = threadIdx.x;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int r = bid % rows;
-    const ExtDesc &Dd = descs[bid / rows];
+    const int bid = block_index(T);
+    const int ndesc = gridDim.x / rows;
+    const int r = bid / ndesc;
+    const ExtDesc &Dd = descs[bid % ndesc];
     const int ri = ring_index(r, Dd.level, T.L);
     const ModConst m = T.mod[ri];
     const size_t kstride = (size_t)(T.L + T.K) * N;
     const ulonglong2 *itw = T.itw + (size_t)ri * N;
-    u64 a0[16], a1[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) a0[e] = a1[e] = 0;
-    for (int i = 0; i < beta; ++i) {
-        const u64 *dr = Dd.D + ((size_t)i * rows + r) * N;
-        const u64 *k0 = Dd.key[0] + (size_t)i * kstride + (size_t)ri * N;
-        const u64 *k1 = NKEY > 1 ? Dd.key[1] + (size_t)i * kstride + (size_t)ri * N : nullptr;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const u64 dv = dr[tid + e * TH];
-            a0[e] = add_mod(a0[e], mred(k0[tid + e * TH], dv, m.q, m.qinv), m.q);
-            if (NKEY > 1) a1[e] = add_mod(a1[e], mred(k1[tid + e * TH], dv, m.q, m.qinv), m.q);
-        }
-    }
+    u64 a0[16];
     u64 *o1 = NKEY > 1 ? Dd.acc[1] + (size_t)r * N : nullptr;
-    if (NKEY > 1) store_row<LOGN>(a1, o1, tid);
-    ntt_inv<LOGN>(a0, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
+    ext_rows<LOGN, NKEY>(a0, o1, Dd.D + (size_t)r * N, (size_t)rows * N, Dd.key[0] + (size_t)ri * N,
+                         NKEY > 1 ? Dd.key[1] + (size_t)ri * N : nullptr, kstride, beta, tid, m);
+    // opaque(): the second transform reloads its twiddles instead of keeping the
+    // first one's live across it
+    ntt_inv<LOGN>(a0, g_lds, opaque_lane(tid), opaque(itw), m.q, m.ninv, m.ninvp);
     store_row<LOGN>(a0, Dd.acc[0] + (size_t)r * N, tid);
     if (NKEY > 1) {
-        load_row_cached<LOGN>(a1, o1, tid);
-        ntt_inv<LOGN>(a1, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
-        store_row<LOGN>(a1, o1, tid);
+        load_row_cached<LOGN>(a0, o1, tid);
+        ntt_inv<LOGN>(a0, g_lds, opaque_lane(tid), opaque(itw), m.q, m.ninv, m.ninvp);
+        store_row<LOGN>(a0, o1, tid);
     }
 }
 
@@ -219,7 +216,7 @@ __global__ void __launch_bounds__(256) k_moddown(DevTables T, const ModDownDesc 
     const int nl = Dd.level + 1;
     u64 res[kMaxQ];
 #pragma unroll
-    for (int j = 0; j < kMaxQ; ++j) res[j] = (Dd.accumulate && j < nl) ? Dd.dst[(size_t)j * N + c] : 0;
+    for (int j = 0; j < kMaxQ; ++j) res[j] = (Dd.accumulate && j < nl) ? G(Dd.dst)[(size_t)j * N + c] : 0;
     for (int s = 0; s < Dd.nsrc; ++s) {
         const u64 *src = Dd.src[s];
         const PUp up = modup_p_prepare(T, src, nl, c);
@@ -228,7 +225,7 @@ __global__ void __launch_bounds__(256) k_moddown(DevTables T, const ModDownDesc 
             if (j < nl) {
                 const ModConst mq = T.mod[j];
                 const u64 z = modup_p_to_row(T, up, j, mq);
-                const u64 x = src[(size_t)j * N + c];
+                const u64 x = G(src)[(size_t)j * N + c];
                 const u64 md = mred(z + 2 * mq.q - x, T.md_pinv[j], mq.q, mq.qinv);
                 res[j] = add_mod(res[j], md, mq.q);
             }
@@ -237,12 +234,12 @@ __global__ void __launch_bounds__(256) k_moddown(DevTables T, const ModDownDesc 
     if (Dd.base) {
 #pragma unroll
         for (int j = 0; j < kMaxQ; ++j)
-            if (j < nl) res[j] = add_mod(res[j], Dd.base[(size_t)j * N + c], T.mod[j].q);
+            if (j < nl) res[j] = add_mod(res[j], G(Dd.base)[(size_t)j * N + c], T.mod[j].q);
     }
     if (!FINAL) {
 #pragma unroll
         for (int j = 0; j < kMaxQ; ++j)
-            if (j < nl) Dd.dst[(size_t)j * N + c] = res[j];
+            if (j < nl) G(Dd.dst)[(size_t)j * N + c] = res[j];
         return;
     }
     // DivRoundByLastModulus nres times: out_j = (x_j + h - [x_l + h]_{q_l}) q_l^-1
@@ -268,7 +265,7 @@ __global__ void __launch_bounds__(256) k_moddown(DevTables T, const ModDownDesc 
     }
 #pragma unroll
     for (int j = 0; j < kMaxQ; ++j)
-        if (j < top) Dd.out[(size_t)j * N + c] = res[j];
+        if (j < top) G(Dd.out)[(size_t)j * N + c] = res[j];
 }
 
 // ---------------------------------------------------------------------------

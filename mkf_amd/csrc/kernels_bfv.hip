@@ -62,9 +62,9 @@ __global__ void __launch_bounds__(256) k_bfv_lift(DevTables T, const BfvLiftDesc
     for (int i = 0; i < kMaxL; ++i) {
         x[i] = 0;
         if (i < L) {
-            const u64 xi = D.src[(size_t)i * N + c];
+            const u64 xi = G(D.src)[(size_t)i * N + c];
             if (D.mode == 0) {
-                D.dst[(size_t)i * N + c] = xi;
+                G(D.dst)[(size_t)i * N + c] = xi;
                 x[i] = xi;
             } else {
                 const ModConst m = T.mod[i];
@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256) k_bfv_lift(DevTables T, const BfvLiftDesc
         for (int j = 0; j < kMaxL; ++j) {
             if (j < L) {
                 const ModConst m = T.mod[qm0 + j];
-                D.dst[(size_t)(L + j) * N + c] = up_raw(u, L, T.bq_up_qstar + (size_t)j * L,
+                G(D.dst)[(size_t)(L + j) * N + c] = up_raw(u, L, T.bq_up_qstar + (size_t)j * L,
                                                         T.bq_up_vt + (size_t)j * (L + 1), m);
             }
         }
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(256) k_bfv_lift(DevTables T, const BfvLiftDesc
             const ModConst m = T.mod[qm0 + j];
             const u64 z = up_raw(u, L, T.bq_up_qstar + (size_t)j * L, T.bq_up_vt + (size_t)j * (L + 1), m);
             mq[j] = mred(z + 2 * m.q, T.bq_rsc[j], m.q, m.qinv);
-            D.dst[(size_t)(L + j) * N + c] = mq[j];
+            G(D.dst)[(size_t)(L + j) * N + c] = mq[j];
         }
     }
     const Up w = up_prepare(T, mq, L, qm0, T.bq_dn_y);
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(256) k_bfv_lift(DevTables T, const BfvLiftDesc
     for (int i = 0; i < kMaxL; ++i) {
         if (i < L) {
             const ModConst m = T.mod[i];
-            D.dst[(size_t)i * N + c] = up_raw(w, L, T.bq_dn_qstar + (size_t)i * L, T.bq_dn_vt + (size_t)i * (L + 1), m);
+            G(D.dst)[(size_t)i * N + c] = up_raw(w, L, T.bq_dn_qstar + (size_t)i * L, T.bq_dn_vt + (size_t)i * (L + 1), m);
         }
     }
 }
@@ -114,17 +114,17 @@ __global__ void __launch_bounds__(256) k_quantize(DevTables T, const BfvQuantDes
     const BfvQuantDesc &D = descs[blockIdx.y];
     u64 x[kMaxL];
 #pragma unroll
-    for (int j = 0; j < kMaxL; ++j) x[j] = j < L ? D.src[(size_t)(L + j) * N + c] : 0;
+    for (int j = 0; j < kMaxL; ++j) x[j] = j < L ? G(D.src)[(size_t)(L + j) * N + c] : 0;
     const Up u = up_prepare(T, x, L, L + K, T.bq_dn_y);
 #pragma unroll
     for (int i = 0; i < kMaxL; ++i) {
         if (i < L) {
             const ModConst m = T.mod[i];
             const u64 y = up_raw(u, L, T.bq_dn_qstar + (size_t)i * L, T.bq_dn_vt + (size_t)i * (L + 1), m);
-            const u64 xq = D.src[(size_t)i * N + c];
+            const u64 xq = G(D.src)[(size_t)i * N + c];
             u64 r = mred(y + 2 * m.q - xq, T.bq_qz[i], m.q, m.qinv);
-            if (D.accumulate) r = add_mod(r, D.dst[(size_t)i * N + c], m.q);
-            D.dst[(size_t)i * N + c] = r;
+            if (D.accumulate) r = add_mod(r, G(D.dst)[(size_t)i * N + c], m.q);
+            G(D.dst)[(size_t)i * N + c] = r;
         }
     }
 }
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
     constexpr int N = 1 << LOGN;
     constexpr int TH = N / 16;
     const int tid = threadIdx.x;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int bid = block_index(T);
     const int l = bid % nl;
     const TensorDesc &Dd = descs[bid / nl];
     const int ri = r_index(l, T.L, T.K);
@@ -149,11 +149,11 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
         const size_t c = off + tid + e * TH;
         u64 x;
         if (Dd.is_c0) {
-            x = mulmod_m(Dd.a0[c], Dd.b0[c], m);
+            x = mulmod_m(G(Dd.a0)[c], G(Dd.b0)[c], m);
         } else {
             x = 0;
-            if (Dd.au) x = mulmod_m(Dd.b0[c], Dd.au[c], m);
-            if (Dd.bu) x = add_mod(x, mulmod_m(Dd.a0[c], Dd.bu[c], m), m.q);
+            if (Dd.au) x = mulmod_m(G(Dd.b0)[c], G(Dd.au)[c], m);
+            if (Dd.bu) x = add_mod(x, mulmod_m(G(Dd.a0)[c], G(Dd.bu)[c], m), m.q);
         }
         v[e] = x;
     }

@@ -1,0 +1,90 @@
+// Fused KKLSS kernel for gadgets with one prime per digit (alpha = 1), where a
+// gadget digit of a coefficient-domain poly is its limb i copied into every row
+// (DecomposeAndSplit's copy path, mkrlwe/basis_extension.go:443-451):
+//
+//   k_decomp_mac    hoisted decomposition of every party's component fused with
+//                   the x / y accumulation (mkrlwe/keyswitch_hoisted.go:84-123;
+//                   for BFV the two-branch form, mkbfv/keyswitch_hoisted.go:100-148):
+//                   each digit row is transformed once, stored for the later
+//                   external products and multiplied into the accumulator while
+//                   still in registers, so the digits are not read back.
+//
+// The grid is ordered so that the workgroups running together on one XCD are
+// consecutive ops of the same (side, digit, row): they read the same key rows,
+// which then come from that XCD's L2.
+#include "kcommon.hpp"
+
+namespace mkhip {
+
+// blockIdx -> logical ((side * beta + i) * rows + r) * nops + o ; descs[2 o + side]
+template <int LOGN>
+__global__ void __launch_bounds__(1 << (LOGN - 4))
+    k_decomp_mac(DevTables T, const DecompDesc *__restrict__ descs, int nops, int beta, int rows) {
+    constexpr int N = 1 << LOGN;
+    constexpr int TH = N / 16;
+    const int tid = threadIdx.x;
+    int bid = block_index(T);
+    const int o = bid % nops;
+    bid /= nops;
+    const int r = bid % rows;
+    bid /= rows;
+    const int i = bid % beta;
+    const int side = bid / beta;
+    const DecompDesc &Dd = descs[2 * o + side];
+    const int ri = ring_index(r, Dd.level, T.L);
+    const ModConst m = T.mod[ri];
+    const ulonglong2 *tw = T.tw + (size_t)ri * N;
+    const size_t krow = ((size_t)i * (T.L + T.K) + ri) * N;
+    const bool reduce = T.mod[r_index(i, T.L, T.K)].q > 2 * m.q;
+    u64 acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0;
+    for (int p = 0; p < Dd.k; ++p) {
+        u64 v[16];
+        load_row_cached<LOGN>(v, Dd.comp[p] + (size_t)i * N, tid);
+        if (reduce) {  // values below 3 q_src may exceed the transform's bound 8m
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        ntt_fwd<LOGN>(v, g_lds, opaque_lane(tid), opaque(tw), m.q);
+        __builtin_amdgcn_sched_barrier(0);
+        store_row<LOGN>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N, tid);
+        const u64 *key = Dd.key[p] + krow;
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+            acc[e] = add_mod(acc[e], mred(uniform_ptr(key + e * TH)[tid], v[e], m.q, m.qinv), m.q);
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = mred(acc[e], m.r2, m.q, m.qinv);
+    store_row<LOGN>(acc, Dd.xy + krow, tid);
+}
+
+template <int LOGN>
+static hipError_t launch_fused_n(int what, const DevTables &T, const void *descs, int n, int a, int b,
+                                 hipStream_t st) {
+    constexpr int N = 1 << LOGN;
+    constexpr size_t lds = (size_t)N * sizeof(u64);
+    static bool init = false;
+    if (!init) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_decomp_mac<LOGN>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        init = true;
+    }
+    if (what != KDECOMP_MAC) return hipErrorInvalidValue;
+    // n ops (2 descriptors each), a = beta, b = rows
+    hipLaunchKernelGGL((k_decomp_mac<LOGN>), dim3(n * 2 * a * b), dim3(N / 16), lds, st, T,
+                       (const DecompDesc *)descs, n, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (T.alpha != 1) return hipErrorInvalidValue;
+    if (T.logN == 13) return launch_fused_n<13>(what, T, descs, n, a, b, st);
+    if (T.logN == 14) return launch_fused_n<14>(what, T, descs, n, a, b, st);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace mkhip

@@ -18,38 +18,26 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
     constexpr int N = 1 << LOGN;
     constexpr int TH = N / 16;
     const int tid = threadIdx.x;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int bid = block_index(T);
     const int r = bid % rows;
     const Mac2Desc &Dd = descs[bid / rows];
     const int ri = ring_index(r, Dd.level, T.L);
     const ModConst m = T.mod[ri];
     const size_t kstride = (size_t)(T.L + T.K) * N;
-    u64 a0[16], a1[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) a0[e] = a1[e] = 0;
-    for (int b = 0; b < beta; ++b) {
-        const u64 *dr = Dd.D + ((size_t)b * rows + r) * N;
-        const u64 *k0 = Dd.swk + (size_t)(2 * b) * kstride + (size_t)ri * N;
-        const u64 *k1 = k0 + kstride;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const u64 dv = dr[tid + e * TH];
-            a0[e] = add_mod(a0[e], mred(k0[tid + e * TH], dv, m.q, m.qinv), m.q);
-            a1[e] = add_mod(a1[e], mred(k1[tid + e * TH], dv, m.q, m.qinv), m.q);
-        }
-    }
+    u64 a0[16];
     u64 *o0 = Dd.acc[0] + (size_t)r * N, *o1 = Dd.acc[1] + (size_t)r * N;
+    // swk digit b: output 0 at (2b) kstride, output 1 at (2b+1) kstride
+    ext_rows<LOGN, 2>(a0, o1, Dd.D + (size_t)r * N, (size_t)rows * N, Dd.swk + (size_t)ri * N,
+                      Dd.swk + kstride + (size_t)ri * N, 2 * kstride, beta, tid, m);
     if (r > Dd.level || Dd.intt_all) {
         const ulonglong2 *itw = T.itw + (size_t)ri * N;
-        store_row<LOGN>(a1, o1, tid);
-        ntt_inv<LOGN>(a0, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
+        ntt_inv<LOGN>(a0, g_lds, opaque_lane(tid), opaque(itw), m.q, m.ninv, m.ninvp);
         store_row<LOGN>(a0, o0, tid);
-        load_row_cached<LOGN>(a1, o1, tid);
-        ntt_inv<LOGN>(a1, g_lds, tid, itw, m.q, m.ninv, m.ninvp);
-        store_row<LOGN>(a1, o1, tid);
+        load_row_cached<LOGN>(a0, o1, tid);
+        ntt_inv<LOGN>(a0, g_lds, opaque_lane(tid), opaque(itw), m.q, m.ninv, m.ninvp);
+        store_row<LOGN>(a0, o1, tid);
     } else {
         store_row<LOGN>(a0, o0, tid);
-        store_row<LOGN>(a1, o1, tid);
     }
 }
 
@@ -59,33 +47,35 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_moddown_ntt(DevTables T, co
     constexpr int N = 1 << LOGN;
     constexpr int TH = N / 16;
     const int tid = threadIdx.x;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int bid = block_index(T);
     const int j = bid % nl;
     const MdNttDesc &Dd = descs[bid / nl];
     const ModConst m = T.mod[j];
     const ulonglong2 *tw = T.tw + (size_t)j * N;
     const u64 pinv = T.md_pinv[j];
-    u64 res[16];
-    if (Dd.base)
-        load_row_cached<LOGN>(res, Dd.base + (size_t)j * N, tid);
-    else {
+    // sum_a (x_a - NTT(ModUp(x_a,P))) P^-1 = (sum_a x_a - NTT(sum_a ModUp(x_a,P))) P^-1
+    // residue for residue (the NTT is linear mod q_j): one transform per row.
+    u64 z[16], xs[16];
 #pragma unroll
-        for (int e = 0; e < 16; ++e) res[e] = 0;
-    }
+    for (int e = 0; e < 16; ++e) z[e] = xs[e] = 0;
     for (int a = 0; a < Dd.nacc; ++a) {
         const u64 *acc = Dd.acc[a];
-        u64 z[16];
+        const u64 *xr = acc + (size_t)j * N;
         for (int e = 0; e < 16; ++e) {
             const PUp up = modup_p_prepare(T, acc, nl, tid + e * TH);
-            z[e] = modup_p_to_row(T, up, j, m);
+            z[e] = add_mod(z[e], modup_p_to_row(T, up, j, m), m.q);
+            xs[e] = add_mod(xs[e], uniform_ptr(xr + e * TH)[tid], m.q);
         }
-        ntt_fwd<LOGN>(z, g_lds, tid, tw, m.q);
-        const u64 *xr = acc + (size_t)j * N;
-#pragma unroll
-        for (int e = 0; e < 16; ++e)
-            res[e] = add_mod(res[e], mred(z[e] + 2 * m.q - xr[tid + e * TH], pinv, m.q, m.qinv), m.q);
     }
-    store_row<LOGN>(res, Dd.dst + (size_t)j * N, tid);
+    ntt_fwd<LOGN>(z, g_lds, tid, tw, m.q);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) z[e] = mred(z[e] + 2 * m.q - xs[e], pinv, m.q, m.qinv);
+    if (Dd.base) {
+        load_row_cached<LOGN>(xs, Dd.base + (size_t)j * N, tid);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) z[e] = add_mod(z[e], xs[e], m.q);
+    }
+    store_row<LOGN>(z, Dd.dst + (size_t)j * N, tid);
 }
 
 // grid (N/256, ndesc * nl), one coefficient per thread
@@ -95,14 +85,14 @@ __global__ void __launch_bounds__(256) k_karatsuba(DevTables T, const KaraDesc *
     const KaraDesc &Dd = descs[blockIdx.y / nl];
     const size_t c = (size_t)l * N + blockIdx.x * 256 + threadIdx.x;
     const ModConst m = T.mod[l];
-    const u64 l0 = Dd.l0[c], l1 = Dd.l1[c], r0 = Dd.r0[c], r1 = Dd.r1[c];
+    const u64 l0 = G(Dd.l0)[c], l1 = G(Dd.l1)[c], r0 = G(Dd.r0)[c], r1 = G(Dd.r1)[c];
     const u64 mu0 = mulmod_m(l0, r0, m);
     const u64 mu2 = mulmod_m(l1, r1, m);
     u64 mu1 = mulmod_m(add_mod(l0, l1, m.q), add_mod(r0, r1, m.q), m);
     mu1 = sub_mod(sub_mod(mu1, mu0, m.q), mu2, m.q);
-    Dd.mu0[c] = mu0;
-    Dd.mu1[c] = mu1;
-    Dd.mu2[c] = mu2;
+    G(Dd.mu0)[c] = mu0;
+    G(Dd.mu1)[c] = mu1;
+    G(Dd.mu2)[c] = mu2;
 }
 
 // blockIdx.x = desc * nout_limbs + row j ; nout_limbs = level + 1 - nres
@@ -112,7 +102,7 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
     constexpr int N = 1 << LOGN;
     constexpr int TH = N / 16;
     const int tid = threadIdx.x;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int bid = block_index(T);
     const int j = bid % nlo;
     const RescaleNttDesc &Dd = descs[bid / nlo];
     const ModConst m = T.mod[j];
@@ -127,7 +117,7 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
         u64 w[16];
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-            const u64 t = add_mod(Dd.xlast[tid + e * TH], h, qL);
+            const u64 t = add_mod(uniform_ptr(Dd.xlast + e * TH)[tid], h, qL);
             w[e] = sub_mod(reduce64(t, m.q, m.mu), hj, m.q);
         }
         ntt_fwd<LOGN>(w, g_lds, tid, T.tw + (size_t)j * N, m.q);

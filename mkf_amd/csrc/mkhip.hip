@@ -314,6 +314,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
                           &T.bq_dn_qstar, &T.bq_dn_vt, &T.bq_rsc, &T.bq_qz};
     for (int i = 0; i < 9; ++i) *bqt[i] = bfv ? (const u64 *)(base + o_bq[i]) : nullptr;
     T.bfv = bfv ? 1 : 0;
+    T.xcd = 1;
     T.logN = logN;
     T.N = N;
     T.L = L;

@@ -110,6 +110,7 @@ extern "C" int mkhip_plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op
     std::vector<TensorDesc> ten;
     std::vector<BfvQuantDesc> quant;
     std::vector<ExtDesc> ext1, ext2;
+    std::vector<std::vector<ExtDesc> > ext2p(MKHIP_MAX_PARTIES);  // by party slot: key rows shared across ops
     std::vector<ModDownDesc> md1, md2;
     int kmax1 = 1, kmax2 = 1;
     for (int o = 0; o < nops; ++o) {
@@ -130,22 +131,22 @@ extern "C" int mkhip_plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op
         MacDesc m0 = {}, m1 = {};
         for (int p = 0; p < op.k0; ++p) {
             s0.comp[p] = lifted(0, 1 + p);
-            m0.key[p] = c->parties[op.ids0[p]].d;
+            m0.key[p] = s0.key[p] = c->parties[op.ids0[p]].d;
         }
         s0.D = P(w.D0);
         m0.D = s0.D;
         s0.k = m0.k = op.k0;
         s0.level = m0.level = level;
-        m0.xy = P(w.x);
+        m0.xy = s0.xy = P(w.x);
         for (int p = 0; p < op.k1; ++p) {
             s1.comp[p] = lifted(1, 1 + p);
-            m1.key[p] = c->parties[op.ids1[p]].b;
+            m1.key[p] = s1.key[p] = c->parties[op.ids1[p]].b;
         }
         s1.D = P(w.D1);
         m1.D = s1.D;
         s1.k = m1.k = op.k1;
         s1.level = m1.level = level;
-        m1.xy = P(w.y);
+        m1.xy = s1.xy = P(w.y);
         dec1.push_back(s0);
         dec1.push_back(s1);
         mac.push_back(m0);
@@ -231,7 +232,7 @@ extern "C" int mkhip_plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op
             ed.acc[1] = P(w.accu) + (size_t)p * rows * N;
             ed.nkey = 2;
             ed.level = level;
-            ext2.push_back(ed);
+            ext2p[p].push_back(ed);
         }
         {
             ModDownDesc md = {};
@@ -261,11 +262,16 @@ extern "C" int mkhip_plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op
             md2.push_back(md);
         }
     }
+    for (auto &v : ext2p) ext2.insert(ext2.end(), v.begin(), v.end());
     DescBlob blob;
     auto &S = pl->stages;
     S.push_back(make_stage(KBFV_LIFT, blob.add(lift), (int)lift.size(), 0, 0, 1, "lift"));
-    S.push_back(make_stage(KDECOMP, blob.add(dec1), (int)dec1.size(), 2 * beta, rows, kmax1, "decompose"));
-    S.push_back(make_stage(KMAC_XY, blob.add(mac), nops, 2 * beta, rows, 1, "mac_xy"));
+    if (c->tune_fused_decomp) {  // decomposition fused with the x / y MAC
+        S.push_back(make_stage(KDECOMP_MAC, blob.add(dec1), nops, 2 * beta, rows, 1, "decompose_mac"));
+    } else {
+        S.push_back(make_stage(KDECOMP, blob.add(dec1), (int)dec1.size(), 2 * beta, rows, kmax1, "decompose"));
+        S.push_back(make_stage(KMAC_XY, blob.add(mac), nops, 2 * beta, rows, 1, "mac_xy"));
+    }
     S.push_back(make_stage(KNTT_FWD, blob.add(ntt), (int)ntt.size(), 0, 0, 1, "tensor_ntt"));
     S.push_back(make_stage(KBFV_TENSOR, blob.add(ten), (int)ten.size(), nr, 0, 1, "tensor"));
     S.push_back(make_stage(KBFV_QUANTIZE, blob.add(quant), (int)quant.size(), 0, 0, 1, "quantize"));

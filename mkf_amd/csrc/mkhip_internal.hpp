@@ -19,6 +19,8 @@ hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n,
 hipError_t launch_mkf(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st);
 // kernels_bfv.hip
 hipError_t launch_bfv(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st);
+// kernels_fused.hip
+hipError_t launch_fused(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st);
 
 inline int hip_status(hipError_t e) {
     return e == hipSuccess ? MKHIP_OK : (e == hipErrorOutOfMemory ? MKHIP_ERR_OOM : MKHIP_ERR_DEVICE);
@@ -69,6 +71,7 @@ struct mkhip_ctx {
     std::map<int, MkfGroup> mkf_groups;
     size_t swk_words = 0;       // beta x (L+K) x N
     size_t mkf_swk_words = 0;   // beta x 2 x (L+K) x N
+    int tune_fused_decomp = 0;     // MKHIP_TUNE_FUSED_DECOMP (alpha = 1 KKLSS plans)
     mkhip_plan *single = nullptr;  // cached plan of the host-pointer entry point
     mkhip::u64 *single_io = nullptr;
     size_t single_io_words = 0;

@@ -14,6 +14,7 @@ hipError_t plan_stage(mkhip_plan *pl, int s) {
     if (S.what >= KMAC2 && S.what <= KRESCALE_NTT) return launch_mkf(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream);
     if (S.what >= KBFV_LIFT && S.what <= KBFV_QUANTIZE)
         return launch_bfv(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream);
+    if (S.what == KDECOMP_MAC) return launch_fused(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream);
     return launch_kernel(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream, S.c);
 }
 
@@ -140,4 +141,94 @@ extern "C" void mkhip_plan_destroy(mkhip_plan *plan) {
     if (plan->d_mem) (void)hipFree(plan->d_mem);
     if (plan->d_desc) (void)hipFree(plan->d_desc);
     delete plan;
+}
+
+// ---------------------------------------------------------------------------
+// tuning knobs and the autotuner
+
+extern "C" int mkhip_set_tuning(mkhip_ctx *c, int knob, int value) {
+    if (!c) return MKHIP_ERR_ARG;
+    switch (knob) {
+    case MKHIP_TUNE_FUSED_DECOMP:
+        c->tune_fused_decomp = value ? 1 : 0;
+        break;
+    case MKHIP_TUNE_XCD_REMAP:
+        c->T.xcd = value ? 1 : 0;
+        break;
+    default:
+        return MKHIP_ERR_ARG;
+    }
+    // the cached single-op plan was built with the old setting
+    mkhip_plan_destroy(c->single);
+    c->single = nullptr;
+    return MKHIP_OK;
+}
+
+extern "C" int mkhip_get_tuning(const mkhip_ctx *c, int knob, int *value) {
+    if (!c || !value) return MKHIP_ERR_ARG;
+    switch (knob) {
+    case MKHIP_TUNE_FUSED_DECOMP:
+        *value = c->tune_fused_decomp;
+        return MKHIP_OK;
+    case MKHIP_TUNE_XCD_REMAP:
+        *value = c->T.xcd;
+        return MKHIP_OK;
+    default:
+        return MKHIP_ERR_ARG;
+    }
+}
+
+static int build_plan(mkhip_ctx *c, int scheme, int nops, const void *ops, mkhip_plan **pl) {
+    switch (scheme) {
+    case MKHIP_SCHEME_KKLSS_CKKS:
+        return mkhip_plan_kklss_ckks(c, nops, (const mkhip_kklss_op *)ops, 1, pl);
+    case MKHIP_SCHEME_KKLSS_BFV:
+        return mkhip_plan_kklss_bfv(c, nops, (const mkhip_kklss_op *)ops, 1, pl);
+    case MKHIP_SCHEME_MKF_CKKS:
+        return mkhip_plan_mkf_ckks(c, nops, (const mkhip_mkf_op *)ops, 1, pl);
+    case MKHIP_SCHEME_MKF_BFV:
+        return mkhip_plan_mkf_bfv(c, nops, (const mkhip_mkf_op *)ops, 1, pl);
+    default:
+        return MKHIP_ERR_ARG;
+    }
+}
+
+// Every combination of the knobs that apply to the scheme is planned and timed
+// (graph launches, HIP events) on the caller's ops; the fastest is kept.
+extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *ops, int iters, int *ncand,
+                              double *ms) {
+    if (!c || !ops || nops <= 0 || iters <= 0) return MKHIP_ERR_ARG;
+    const bool kklss = scheme == MKHIP_SCHEME_KKLSS_CKKS || scheme == MKHIP_SCHEME_KKLSS_BFV;
+    const int nfused = (kklss && c->alpha == 1) ? 2 : 1;
+    const int old_fused = c->tune_fused_decomp, old_xcd = c->T.xcd;
+    int best = -1, n = 0;
+    double best_ms = 0;
+    for (int fused = 0; fused < nfused; ++fused)
+        for (int xcd = 1; xcd >= 0; --xcd) {
+            mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, nfused > 1 ? fused : old_fused);
+            mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, xcd);
+            mkhip_plan *pl = nullptr;
+            int s = build_plan(c, scheme, nops, ops, &pl);
+            double t = 0;
+            if (s == MKHIP_OK) s = mkhip_plan_launch(pl);  // warm-up
+            if (s == MKHIP_OK) s = mkhip_plan_time(pl, iters, &t);
+            mkhip_plan_destroy(pl);
+            if (s != MKHIP_OK) {
+                mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, old_fused);
+                mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, old_xcd);
+                return s;
+            }
+            if (ms && n < 8) ms[n] = t;
+            if (best < 0 || t < best_ms) {
+                best = n;
+                best_ms = t;
+            }
+            ++n;
+        }
+    // candidate index -> knob values (same loop order)
+    const int fused = nfused > 1 ? best / 2 : old_fused, xcd = 1 - best % 2;
+    mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, fused);
+    mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, xcd);
+    if (ncand) *ncand = n;
+    return MKHIP_OK;
 }

@@ -7,7 +7,8 @@
 //   shoup(a, w)  = a * w mod q, lazily in [0, 2q), with the precomputed companion
 //                  w' = floor(w * 2^64 / q); used for NTT twiddles and constants.
 // All functions are exact integer arithmetic.  The NTT kernels keep values in
-// [0, 4q) (Harvey's lazy butterflies), which needs 4q < 2^63.
+// [0, 8q) (Harvey's lazy butterflies with an approximate Shoup quotient), which
+// needs 8q <= 2^64, i.e. q < 2^61.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,10 +29,27 @@ MKHIP_HD u64 mulhi64(u64 a, u64 b) {
 #endif
 }
 
+// Full 128-bit product a * b.  On the device: four 32 x 32 -> 64 multiply-adds
+// (computing the low and high halves separately would take seven).
+MKHIP_HD void mul_wide(u64 a, u64 b, u64 &hi, u64 &lo) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    const u64 p00 = (u64)a0 * b0;
+    const u64 t = (u64)a0 * b1 + (p00 >> 32);
+    const u64 u = (u64)a1 * b0 + (u32)t;
+    lo = (u << 32) | (u32)p00;
+    hi = (u64)a1 * b1 + (t >> 32) + (u >> 32);
+#else
+    const unsigned __int128 p = (unsigned __int128)a * b;
+    lo = (u64)p;
+    hi = (u64)(p >> 64);
+#endif
+}
+
 // a * b * 2^-64 mod q, canonical, for a * b < q * 2^64 (Lattigo MRed).
 MKHIP_HD u64 mred(u64 a, u64 b, u64 q, u64 qinv) {
-    u64 lo = a * b;
-    u64 hi = mulhi64(a, b);
+    u64 hi, lo;
+    mul_wide(a, b, hi, lo);
     u64 m = lo * qinv;
     u64 mqhi = mulhi64(m, q);
     u64 r = hi - mqhi + q;
@@ -42,6 +60,21 @@ MKHIP_HD u64 mred(u64 a, u64 b, u64 q, u64 qinv) {
 MKHIP_HD u64 shoup_lazy(u64 a, u64 w, u64 wp, u64 q) {
     u64 qh = mulhi64(a, wp);
     return a * w - qh * q;
+}
+
+// hi64(a * b) - e with e in {0, 1, 2}: the a_lo * b_lo partial product and the
+// carries out of the low 32 bits of the cross products are dropped, which leaves
+// one 64-bit and two 32-bit multiplies (instead of four 64-bit partials).
+__device__ __forceinline__ u64 mulhi64_approx(u64 a, u64 b) {
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    const u64 cross = (u64)__umulhi(a0, b1) + (u64)__umulhi(a1, b0);
+    return (u64)a1 * b1 + cross;
+}
+
+// a * w mod q in [0, 4q) for any a < 2^64 and w < q, w' = floor(w 2^64 / q):
+// the exact Shoup quotient gives [0, 2q); the approximate one is at most 2 lower.
+__device__ __forceinline__ u64 shoup_lazy4(u64 a, u64 w, u64 wp, u64 q) {
+    return a * w - mulhi64_approx(a, wp) * q;
 }
 
 MKHIP_HD u64 shoup(u64 a, u64 w, u64 wp, u64 q) {

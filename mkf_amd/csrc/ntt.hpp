@@ -12,8 +12,9 @@
 // Implementation: every thread holds 16 coefficients in registers.  Stages are
 // processed in passes of up to 4 (radix-16 in registers); between passes the
 // limb is exchanged through LDS (N * 8 bytes: 128 KiB at N = 2^14).  Butterflies
-// are Harvey's lazy forms with Shoup twiddles: values stay in [0, 4q) forward
-// and [0, 2q) inverse and are reduced to canonical residues on the way out.
+// are Harvey's lazy forms with Shoup twiddles whose quotient is approximated
+// (shoup_lazy4: product in [0, 4q)); values stay in [0, 8q) forward and
+// [0, 4q) inverse (q < 2^61) and are reduced to canonical residues on the way out.
 #pragma once
 
 #include "modarith.hpp"
@@ -22,18 +23,43 @@ namespace mkhip {
 
 // ---- butterflies ---------------------------------------------------------
 
-__device__ __forceinline__ void fwd_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q2, u64 q) {
-    u64 u = x >= q2 ? x - q2 : x;          // [0, 2q)
-    u64 t = shoup_lazy(y, w, wp, q);       // [0, 2q)
-    x = u + t;                             // [0, 4q)
-    y = u + q2 - t;                        // (0, 4q)
+// q4 = 4q.  In: x, y in [0, 8q).  Out: x, y in [0, 8q).
+__device__ __forceinline__ void fwd_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q4, u64 q) {
+    u64 u = x >= q4 ? x - q4 : x;          // [0, 4q)
+    u64 t = shoup_lazy4(y, w, wp, q);      // [0, 4q)
+    x = u + t;                             // [0, 8q)
+    y = u + q4 - t;                        // (0, 8q)
 }
 
-__device__ __forceinline__ void inv_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q2, u64 q) {
-    u64 u = x, v = y;                      // both in [0, 2q)
+// In: x, y in [0, 4q).  Out: x, y in [0, 4q).
+__device__ __forceinline__ void inv_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q4, u64 q) {
+    u64 u = x, v = y;
     u64 s = u + v;
-    x = s >= q2 ? s - q2 : s;              // [0, 2q)
-    y = shoup_lazy(u + q2 - v, w, wp, q);  // [0, 2q)
+    x = s >= q4 ? s - q4 : s;              // [0, 4q)
+    y = shoup_lazy4(u + q4 - v, w, wp, q); // [0, 4q)
+}
+
+// [0, 8q) -> [0, q)
+__device__ __forceinline__ u64 reduce_8q(u64 x, u64 q) {
+    const u64 q4 = q << 2, q2 = q << 1;
+    x = x >= q4 ? x - q4 : x;
+    x = x >= q2 ? x - q2 : x;
+    return x >= q ? x - q : x;
+}
+
+// Twiddle (w, w') at index idx.  When idx is the same on every lane of the wave
+// the pair is moved to scalar registers (readfirstlane), so it occupies no VGPRs
+// while the butterflies of the stage run.
+template <bool UNIFORM>
+__device__ __forceinline__ ulonglong2 twiddle(const ulonglong2 *__restrict__ tw, int idx) {
+    ulonglong2 w = tw[idx];
+    if (UNIFORM) {
+        const u32 a = __builtin_amdgcn_readfirstlane((u32)w.x), b = __builtin_amdgcn_readfirstlane((u32)(w.x >> 32));
+        const u32 c = __builtin_amdgcn_readfirstlane((u32)w.y), d = __builtin_amdgcn_readfirstlane((u32)(w.y >> 32));
+        w.x = ((u64)b << 32) | a;
+        w.y = ((u64)d << 32) | c;
+    }
+    return w;
 }
 
 // ---- pass geometry -------------------------------------------------------
@@ -57,21 +83,24 @@ struct FwdPass {
     }
     __device__ static __forceinline__ void run(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ tw,
                                                u64 q) {
-        const u64 q2 = q << 1;
+        const u64 q4 = q << 2;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const int col = tid + c * T;
-            const int b = col / TL;
+            // b is the same on every lane of a wave when TL is a multiple of 64:
+            // the twiddles then come through scalar loads
+            const int b = (TL % 64 == 0) ? __builtin_amdgcn_readfirstlane(col / TL) : col / TL;
 #pragma unroll
             for (int u = 0; u < NS; ++u) {
+                __builtin_amdgcn_sched_barrier(0);
                 const int half = R >> (u + 1);
 #pragma unroll
                 for (int grp = 0; grp < (1 << u); ++grp) {
-                    const ulonglong2 w = tw[(1 << (S0 + u)) + b * (1 << u) + grp];
+                    const ulonglong2 w = twiddle<(TL % 64 == 0)>(tw, (1 << (S0 + u)) + b * (1 << u) + grp);
 #pragma unroll
                     for (int kk = 0; kk < half; ++kk) {
                         const int k0 = grp * 2 * half + kk;
-                        fwd_bfly(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, q2, q);
+                        fwd_bfly(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, q4, q);
                     }
                 }
             }
@@ -95,23 +124,24 @@ struct InvPass {
     }
     __device__ static __forceinline__ void run(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ itw,
                                                u64 q) {
-        const u64 q2 = q << 1;
+        const u64 q4 = q << 2;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const int col = tid + c * T;
-            const int b = col / TF;
+            const int b = (TF % 64 == 0) ? __builtin_amdgcn_readfirstlane(col / TF) : col / TF;
 #pragma unroll
             for (int u = 0; u < NS; ++u) {
+                __builtin_amdgcn_sched_barrier(0);
                 const int half = 1 << u;
                 const int m = N >> (U0 + u + 1);
                 const int ngrp = R / (2 * half);
 #pragma unroll
                 for (int grp = 0; grp < ngrp; ++grp) {
-                    const ulonglong2 w = itw[m + b * ngrp + grp];
+                    const ulonglong2 w = twiddle<(TF % 64 == 0)>(itw, m + b * ngrp + grp);
 #pragma unroll
                     for (int kk = 0; kk < half; ++kk) {
                         const int k0 = grp * 2 * half + kk;
-                        inv_bfly(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, q2, q);
+                        inv_bfly(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, q4, q);
                     }
                 }
             }
@@ -137,7 +167,7 @@ struct Linear {
 };
 
 // ---- whole transforms ------------------------------------------------------
-// Input: v[i] = coefficient tid + i*T (values < 4q).  Output: v[i] = NTT value at
+// Input: v[i] = coefficient tid + i*T (values < 8q).  Output: v[i] = NTT value at
 // index tid + i*T, canonical.  s: N words of LDS.
 
 template <int LOGN>
@@ -181,17 +211,12 @@ __device__ __forceinline__ void ntt_fwd(u64 (&v)[16], u64 *s, int tid, const ulo
     lds_exchange<typename P::F2, typename P::F3>(v, s, tid);
     P::F3::run(v, tid, tw, q);
     lds_exchange<typename P::F3, Linear<LOGN> >(v, s, tid);
-    const u64 q2 = q << 1;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        u64 x = v[i];
-        x = x >= q2 ? x - q2 : x;
-        v[i] = x >= q ? x - q : x;
-    }
+    for (int i = 0; i < 16; ++i) v[i] = reduce_8q(v[i], q);
 }
 
-// Inverse NTT including the N^-1 scaling; v canonical (or < 2q) in Linear layout;
-// returns canonical values in Linear layout.
+// Inverse NTT including the N^-1 scaling; v < 4q in Linear layout; returns
+// canonical values in Linear layout.
 template <int LOGN>
 __device__ __forceinline__ void ntt_inv(u64 (&v)[16], u64 *s, int tid, const ulonglong2 *__restrict__ itw,
                                         u64 q, u64 ninv, u64 ninvp) {

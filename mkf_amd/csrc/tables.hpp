@@ -41,6 +41,8 @@ struct DevTables {
     const u64 *bq_qz;         // [L]        q_i - MForm(QMul^-1 mod q_i)  (Quantize)
     int logN, N, L, K, alpha, beta_max, nmod;
     int bfv;                  // 1 when the QMul chain is present
+    int xcd;                  // 1: XCD-contiguous workgroup order (xcd_remap), a tuning knob
+    int pad;
 };
 
 // ring index of limb l of an R = Q u QMul polynomial
@@ -55,7 +57,7 @@ __device__ __forceinline__ int dc_base(const DevTables &T, int digit, int nd) {
 enum KernelId {
     KNTT_FWD = 0,        // k_ntt_batch<LOGN, false>
     KNTT_INV = 1,        // k_ntt_batch<LOGN, true>
-    KDECOMP_MAC = 2,     // k_decomp<LOGN, true>   decomposition + x/y accumulation
+    KDECOMP_MAC = 2,     // k_decomp_mac<LOGN>     alpha = 1: decomposition fused with the x/y MAC
     KTENSOR = 3,         // k_tensor<LOGN>
     KEXT1 = 4,           // k_ext<LOGN, 1>
     KMODDOWN = 5,        // k_moddown<false>
@@ -141,6 +143,11 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
     const int full = n - n % kXcd;
     if (b >= full) return b;
     return (b % kXcd) * (full / kXcd) + b / kXcd;
+}
+
+// Logical index of this workgroup: XCD-contiguous when the context's knob is on.
+__device__ __forceinline__ int block_index(const DevTables &T) {
+    return T.xcd ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
 }
 
 // ---- descriptors -----------------------------------------------------------
