@@ -229,7 +229,8 @@ def run_scheme(scheme, args, rng):
         "stages_ms": {names[s]: round(stage_ms[s], 4) for s in range(len(names))},
         "workspace_gb": round(plan.workspace_bytes / 1e9, 3),
         "autotune_ms": [round(t, 3) for t in tune_ms],
-        "tuning": {"fused_decomp": w.ctx.get_tuning(1), "xcd_remap": w.ctx.get_tuning(2)},
+        "tuning": {"fused_decomp": w.ctx.get_tuning(1), "xcd_remap": w.ctx.get_tuning(2),
+                   "decomp_order": w.ctx.get_tuning(3)},
         "shape": {"logN": w.logN, "L": w.L, "K": w.K, "k": w.k, "alpha": w.ctx.alpha, "beta": w.beta,
                   "nres": w.nres, "t": w.t},
     }
@@ -273,16 +274,20 @@ def cpu_baseline(logN, Q, P, k, keys, u, budget_s):
 
 
 def copy_calibration(mb):
-    """A plain device-to-device copy of known size, for scaling PMC byte counters."""
-    import torch
+    """A plain device-to-device copy kernel of known size (k_copy), run three
+    times, for scaling the PMC byte counters of the other kernels."""
+    import mkf_amd
 
-    n = int(mb * (1 << 20)) // 8
-    src = torch.ones(n, dtype=torch.int64, device="cuda")
-    dst = torch.empty_like(src)
+    ctx = mkf_amd.Context(13, ntt_primes(50, 2), ntt_primes(60, 1), gamma=1)
+    nbytes = int(mb * (1 << 20)) // 16 * 16
+    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
     for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    return {"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}
+        ctx.d2d(dst, src, nbytes)
+    ctx.sync()
+    ctx.free(src)
+    ctx.free(dst)
+    ctx.close()
+    return {"kernel": "k_copy", "bytes_read": nbytes, "bytes_written": nbytes, "launches": 3}
 
 
 def load_traffic(kernel):

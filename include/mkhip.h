@@ -104,6 +104,8 @@ int mkhip_dev_alloc(mkhip_ctx *ctx, size_t bytes, void **dptr);
 int mkhip_dev_free(mkhip_ctx *ctx, void *dptr);
 int mkhip_h2d(mkhip_ctx *ctx, void *dst, const void *src, size_t bytes);
 int mkhip_d2h(mkhip_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* device-to-device copy on the context's stream (asynchronous; see mkhip_sync) */
+int mkhip_d2d(mkhip_ctx *ctx, void *dst, const void *src, size_t bytes);
 int mkhip_sync(mkhip_ctx *ctx);
 
 /*
@@ -209,6 +211,8 @@ int mkhip_bfv_decompose(mkhip_ctx *ctx, const uint64_t *aR, uint64_t *out);
  *   MKHIP_TUNE_FUSED_DECOMP  alpha = 1 KKLSS plans: the gadget decomposition fused
  *                            with the x / y key MAC (1) or as two launches (0)
  *   MKHIP_TUNE_XCD_REMAP     workgroups ordered XCD-contiguously (1) or as issued (0)
+ *   MKHIP_TUNE_DECOMP_ORDER  fused decomposition grid order: ops sharing a key row
+ *                            adjacent (0) or rows sharing an input limb adjacent (1)
  * A knob applies to plans built after it is set.  mkhip_autotune plans and times
  * every applicable combination on the caller's batch (ops: mkhip_kklss_op[] for the
  * KKLSS schemes, mkhip_mkf_op[] for MKF; their outputs are overwritten with the
@@ -216,6 +220,7 @@ int mkhip_bfv_decompose(mkhip_ctx *ctx, const uint64_t *aR, uint64_t *out);
  */
 #define MKHIP_TUNE_FUSED_DECOMP 1
 #define MKHIP_TUNE_XCD_REMAP 2
+#define MKHIP_TUNE_DECOMP_ORDER 3
 #define MKHIP_SCHEME_KKLSS_CKKS 0
 #define MKHIP_SCHEME_KKLSS_BFV 1
 #define MKHIP_SCHEME_MKF_CKKS 2

@@ -36,9 +36,9 @@ EXPORTS = [
     "mkhip_register_mkf_group", "mkhip_unregister_mkf_group", "mkhip_mkf_ckks_mulrelin", "mkhip_plan_mkf_ckks",
     "mkhip_register_bfv_party", "mkhip_kklss_bfv_mulrelin", "mkhip_plan_kklss_bfv", "mkhip_bfv_modup_q_to_r",
     "mkhip_bfv_rescale_q_to_r", "mkhip_bfv_quantize", "mkhip_bfv_decompose", "mkhip_mkf_bfv_mulrelin",
-    "mkhip_plan_mkf_bfv", "mkhip_set_tuning", "mkhip_get_tuning", "mkhip_autotune",
+    "mkhip_plan_mkf_bfv", "mkhip_set_tuning", "mkhip_get_tuning", "mkhip_autotune", "mkhip_d2d",
 ]
-TUNE_FUSED_DECOMP, TUNE_XCD_REMAP = 1, 2
+TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER = 1, 2, 3
 SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3}
 
 # kernel run by each named plan stage
@@ -110,6 +110,7 @@ def _declare(L):
     L.mkhip_dev_free.argtypes = [V, V]
     L.mkhip_h2d.argtypes = [V, V, V, S]
     L.mkhip_d2h.argtypes = [V, V, V, S]
+    L.mkhip_d2d.argtypes = [V, V, V, S]
     L.mkhip_sync.argtypes = [V]
     L.mkhip_plan_kklss_ckks.argtypes = [V, I, ctypes.POINTER(KKLSSOp), I, ctypes.POINTER(V)]
     L.mkhip_plan_launch.argtypes = [V]
@@ -388,6 +389,9 @@ class Context:
     def d2h(self, arr, dptr):
         check(lib().mkhip_d2h(self._h, arr.ctypes.data, dptr, arr.nbytes), "mkhip_d2h")
         return arr
+
+    def d2d(self, dst, src, nbytes):
+        check(lib().mkhip_d2d(self._h, dst, src, nbytes), "mkhip_d2d")
 
     def sync(self):
         check(lib().mkhip_sync(self._h), "mkhip_sync")

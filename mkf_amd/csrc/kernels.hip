@@ -15,6 +15,8 @@
 // limb-major rows coalesced, and produces canonical residues.
 #include "kcommon.hpp"
 
+#include <algorithm>
+
 namespace mkhip {
 
 // ---------------------------------------------------------------------------
@@ -323,6 +325,25 @@ static hipError_t launch_all(int what, const DevTables &T, const void *descs, in
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// Plain device-to-device copy (16-byte words, grid-stride).
+__global__ void __launch_bounds__(256) k_copy(const ulonglong2 *__restrict__ src, ulonglong2 *__restrict__ dst,
+                                             size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {
+    const size_t n = bytes / 16;
+    if (n) {
+        const int grid = (int)std::min<size_t>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, (const ulonglong2 *)src, (ulonglong2 *)dst, n);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const size_t tail = bytes - n * 16;
+    if (tail) return hipMemcpyAsync((char *)dst + n * 16, (const char *)src + n * 16, tail, hipMemcpyDeviceToDevice, st);
+    return hipSuccess;
 }
 
 // what: KernelId (tables.hpp).  NTT: n jobs; decomp: n descs x c (kmax) x a (beta) x

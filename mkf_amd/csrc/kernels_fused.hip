@@ -16,18 +16,28 @@
 
 namespace mkhip {
 
-// blockIdx -> logical ((side * beta + i) * rows + r) * nops + o ; descs[2 o + side]
+// blockIdx -> logical ((side * beta + i) * rows + r) * nops + o (order 0: the ops
+// sharing a key row are adjacent) or ((side * beta + i) * nops + o) * rows + r
+// (order 1: the rows sharing an input limb are adjacent); descs[2 o + side]
 template <int LOGN>
 __global__ void __launch_bounds__(1 << (LOGN - 4))
-    k_decomp_mac(DevTables T, const DecompDesc *__restrict__ descs, int nops, int beta, int rows) {
+    k_decomp_mac(DevTables T, const DecompDesc *__restrict__ descs, int nops, int beta, int rows, int order) {
     constexpr int N = 1 << LOGN;
     constexpr int TH = N / 16;
     const int tid = threadIdx.x;
     int bid = block_index(T);
-    const int o = bid % nops;
-    bid /= nops;
-    const int r = bid % rows;
-    bid /= rows;
+    int o, r;
+    if (order == 0) {
+        o = bid % nops;
+        bid /= nops;
+        r = bid % rows;
+        bid /= rows;
+    } else {
+        r = bid % rows;
+        bid /= rows;
+        o = bid % nops;
+        bid /= nops;
+    }
     const int i = bid % beta;
     const int side = bid / beta;
     const DecompDesc &Dd = descs[2 * o + side];
@@ -75,7 +85,7 @@ static hipError_t launch_fused_n(int what, const DevTables &T, const void *descs
     if (what != KDECOMP_MAC) return hipErrorInvalidValue;
     // n ops (2 descriptors each), a = beta, b = rows
     hipLaunchKernelGGL((k_decomp_mac<LOGN>), dim3(n * 2 * a * b), dim3(N / 16), lds, st, T,
-                       (const DecompDesc *)descs, n, a, b);
+                       (const DecompDesc *)descs, n, a, b, T.decomp_order);
     return hipGetLastError();
 }
 

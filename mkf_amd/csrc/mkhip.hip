@@ -471,6 +471,14 @@ extern "C" int mkhip_dev_alloc(mkhip_ctx *c, size_t bytes, void **p) {
     HIP_TRY(hipMalloc(p, bytes ? bytes : 1));
     return MKHIP_OK;
 }
+// Device-to-device copy on the context stream, by a plain copy kernel (16-byte
+// words; callers keeping ciphertexts resident use it to stage operands).
+extern "C" int mkhip_d2d(mkhip_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return MKHIP_ERR_ARG;
+    HIP_TRY(launch_copy(dst, src, bytes, c->stream));
+    return MKHIP_OK;
+}
+
 extern "C" int mkhip_dev_free(mkhip_ctx *c, void *p) {
     if (!c) return MKHIP_ERR_ARG;
     HIP_TRY(hipFree(p));

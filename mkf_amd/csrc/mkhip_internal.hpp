@@ -15,6 +15,7 @@ namespace mkhip {
 // kernels.hip
 hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st,
                          int c = 1);
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 // kernels_mkf.hip
 hipError_t launch_mkf(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st);
 // kernels_bfv.hip
@@ -71,7 +72,7 @@ struct mkhip_ctx {
     std::map<int, MkfGroup> mkf_groups;
     size_t swk_words = 0;       // beta x (L+K) x N
     size_t mkf_swk_words = 0;   // beta x 2 x (L+K) x N
-    int tune_fused_decomp = 0;     // MKHIP_TUNE_FUSED_DECOMP (alpha = 1 KKLSS plans)
+    int tune_fused_decomp = 1;     // MKHIP_TUNE_FUSED_DECOMP (alpha = 1 KKLSS plans)
     mkhip_plan *single = nullptr;  // cached plan of the host-pointer entry point
     mkhip::u64 *single_io = nullptr;
     size_t single_io_words = 0;

@@ -155,6 +155,9 @@ extern "C" int mkhip_set_tuning(mkhip_ctx *c, int knob, int value) {
     case MKHIP_TUNE_XCD_REMAP:
         c->T.xcd = value ? 1 : 0;
         break;
+    case MKHIP_TUNE_DECOMP_ORDER:
+        c->T.decomp_order = value ? 1 : 0;
+        break;
     default:
         return MKHIP_ERR_ARG;
     }
@@ -172,6 +175,9 @@ extern "C" int mkhip_get_tuning(const mkhip_ctx *c, int knob, int *value) {
         return MKHIP_OK;
     case MKHIP_TUNE_XCD_REMAP:
         *value = c->T.xcd;
+        return MKHIP_OK;
+    case MKHIP_TUNE_DECOMP_ORDER:
+        *value = c->T.decomp_order;
         return MKHIP_OK;
     default:
         return MKHIP_ERR_ARG;
@@ -199,36 +205,50 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
                               double *ms) {
     if (!c || !ops || nops <= 0 || iters <= 0) return MKHIP_ERR_ARG;
     const bool kklss = scheme == MKHIP_SCHEME_KKLSS_CKKS || scheme == MKHIP_SCHEME_KKLSS_BFV;
-    const int nfused = (kklss && c->alpha == 1) ? 2 : 1;
-    const int old_fused = c->tune_fused_decomp, old_xcd = c->T.xcd;
+    // candidates: (fused, order) for alpha = 1 KKLSS plans -- split, fused order 0,
+    // fused order 1 -- each with and without the XCD remap
+    struct Cand {
+        int fused, order, xcd;
+    };
+    std::vector<Cand> cands;
+    const int old_fused = c->tune_fused_decomp, old_xcd = c->T.xcd, old_order = c->T.decomp_order;
+    for (int xcd = 1; xcd >= 0; --xcd) {
+        if (kklss && c->alpha == 1) {
+            cands.push_back({0, old_order, xcd});
+            cands.push_back({1, 0, xcd});
+            cands.push_back({1, 1, xcd});
+        } else {
+            cands.push_back({old_fused, old_order, xcd});
+        }
+    }
     int best = -1, n = 0;
     double best_ms = 0;
-    for (int fused = 0; fused < nfused; ++fused)
-        for (int xcd = 1; xcd >= 0; --xcd) {
-            mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, nfused > 1 ? fused : old_fused);
-            mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, xcd);
-            mkhip_plan *pl = nullptr;
-            int s = build_plan(c, scheme, nops, ops, &pl);
-            double t = 0;
-            if (s == MKHIP_OK) s = mkhip_plan_launch(pl);  // warm-up
-            if (s == MKHIP_OK) s = mkhip_plan_time(pl, iters, &t);
-            mkhip_plan_destroy(pl);
-            if (s != MKHIP_OK) {
-                mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, old_fused);
-                mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, old_xcd);
-                return s;
-            }
-            if (ms && n < 8) ms[n] = t;
-            if (best < 0 || t < best_ms) {
-                best = n;
-                best_ms = t;
-            }
-            ++n;
+    for (const Cand &cd : cands) {
+        mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, cd.fused);
+        mkhip_set_tuning(c, MKHIP_TUNE_DECOMP_ORDER, cd.order);
+        mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, cd.xcd);
+        mkhip_plan *pl = nullptr;
+        int s = build_plan(c, scheme, nops, ops, &pl);
+        double t = 0;
+        if (s == MKHIP_OK) s = mkhip_plan_launch(pl);  // warm-up
+        if (s == MKHIP_OK) s = mkhip_plan_time(pl, iters, &t);
+        mkhip_plan_destroy(pl);
+        if (s != MKHIP_OK) {
+            mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, old_fused);
+            mkhip_set_tuning(c, MKHIP_TUNE_DECOMP_ORDER, old_order);
+            mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, old_xcd);
+            return s;
         }
-    // candidate index -> knob values (same loop order)
-    const int fused = nfused > 1 ? best / 2 : old_fused, xcd = 1 - best % 2;
-    mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, fused);
-    mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, xcd);
+        if (ms && n < 8) ms[n] = t;
+        if (best < 0 || t < best_ms) {
+            best = n;
+            best_ms = t;
+        }
+        ++n;
+    }
+    mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, cands[best].fused);
+    mkhip_set_tuning(c, MKHIP_TUNE_DECOMP_ORDER, cands[best].order);
+    mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, cands[best].xcd);
     if (ncand) *ncand = n;
     return MKHIP_OK;
 }

@@ -149,13 +149,18 @@ struct InvPass {
     }
 };
 
+// LDS word of element idx: bits 4..7 are folded into the bank bits 0..3, so the
+// strided patterns of the radix-16 / radix-4 passes (element strides of 4, 32 and
+// 64 words across the lanes) spread over all banks instead of a quarter of them.
+__device__ __forceinline__ int lds_swizzle(int idx) { return idx ^ ((idx >> 4) & 15); }
+
 template <class From, class To>
 __device__ __forceinline__ void lds_exchange(u64 (&v)[16], u64 *s, int tid) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) s[From::elem(tid, i)] = v[i];
+    for (int i = 0; i < 16; ++i) s[lds_swizzle(From::elem(tid, i))] = v[i];
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = s[To::elem(tid, i)];
+    for (int i = 0; i < 16; ++i) v[i] = s[lds_swizzle(To::elem(tid, i))];
     __syncthreads();
 }
 

@@ -42,7 +42,7 @@ struct DevTables {
     int logN, N, L, K, alpha, beta_max, nmod;
     int bfv;                  // 1 when the QMul chain is present
     int xcd;                  // 1: XCD-contiguous workgroup order (xcd_remap), a tuning knob
-    int pad;
+    int decomp_order;         // fused decomposition grid order (a tuning knob)
 };
 
 // ring index of limb l of an R = Q u QMul polynomial
