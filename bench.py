@@ -352,7 +352,7 @@ def main():
         dom = max(range(len(stage_ms)), key=lambda s: stage_ms[s])
         dom_bytes = alg[names[dom]]
         achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
-        kern = mkf_amd.STAGE_KERNELS[names[dom]]
+        kern = mkf_amd.stage_kernel(names[dom])
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kern),
                     "kernel": kern, "stage": names[dom], "kernel_ms": round(stage_ms[dom], 4),

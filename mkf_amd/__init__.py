@@ -57,6 +57,12 @@ BFV_STAGE_KERNELS = dict(STAGE_KERNELS, tensor="k_tensor_r")
 MKF_BFV_STAGE_KERNELS = dict(BFV_STAGE_KERNELS, moddown_fold="k_moddown<false>", moddown_relin="k_moddown<false>")
 
 
+def stage_kernel(name, table=STAGE_KERNELS):
+    """Kernel a plan stage runs; at N = 2^15 a stage that inverse-transforms limbs is
+    followed by '<stage>_finish' (last stage + N^-1 of the split limbs)."""
+    return "k_intt_finish" if name.endswith("_finish") else table[name]
+
+
 class MkhipError(RuntimeError):
     def __init__(self, status, what=""):
         self.status = status

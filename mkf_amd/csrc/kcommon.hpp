@@ -56,6 +56,56 @@ __device__ __forceinline__ int opaque_lane(int t) {
     return t;
 }
 
+// ---- limb geometry -------------------------------------------------------------
+// One workgroup of TH = NS/16 threads transforms NS = min(N, 2^14) elements in
+// registers + LDS.  A limb of 2^15 is split over H = 2 workgroups (halves): the
+// forward transform's first stage is computed by both from the two halves, the
+// inverse leaves its last stage to k_intt_finish.  The elements a workgroup
+// holds are h NS + tid + e TH (e < 16).
+template <int LOGN>
+struct Geo {
+    static constexpr bool split = LOGN == 15;
+    static constexpr int LS = split ? 14 : LOGN;
+    static constexpr int H = split ? 2 : 1;
+    static constexpr int N = 1 << LOGN;
+    static constexpr int NS = 1 << LS;
+    static constexpr int TH = NS / 16;
+};
+
+// Forward transform of this workgroup's part of a limb whose element c is f(c)
+// (values < 8q); v receives the NTT values at h NS + tid + e TH, canonical.
+template <int LOGN, class F>
+__device__ __forceinline__ void fwd_transform(u64 (&v)[16], F f, int tid, int h, const ulonglong2 *tw, u64 q) {
+    typedef Geo<LOGN> Gm;
+    if constexpr (Gm::split) {
+        u64 lo[16], hi[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            lo[e] = f(tid + e * Gm::TH);
+            hi[e] = f(Gm::NS + tid + e * Gm::TH);
+        }
+        fwd_first_stage(v, lo, hi, tw, q, h);
+        ntt_fwd<Gm::LS>(v, g_lds, tid, tw, q, 2 + h);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = f(tid + e * Gm::TH);
+        ntt_fwd<LOGN>(v, g_lds, tid, tw, q, 1);
+    }
+}
+
+// Inverse transform of this workgroup's elements (v[e] = element h NS + tid + e TH,
+// values < 4q) with the N^-1 (or t N^-1) scaling given; for a split limb the last
+// stage and the scaling are left to k_intt_finish.
+template <int LOGN>
+__device__ __forceinline__ void inv_transform(u64 (&v)[16], int tid, int h, const ulonglong2 *itw, u64 q,
+                                              u64 ninv, u64 ninvp) {
+    typedef Geo<LOGN> Gm;
+    if constexpr (Gm::split)
+        ntt_inv<Gm::LS>(v, g_lds, tid, itw, q, ninv, ninvp, 2 + h, false);
+    else
+        ntt_inv<LOGN>(v, g_lds, tid, itw, q, ninv, ninvp, 1, true);
+}
+
 // Row r of a (level+1+K)-row QP polynomial is ring modulus r (Q) or L + r - level - 1 (P).
 __device__ __forceinline__ int ring_index(int r, int level, int L) { return r <= level ? r : L + (r - level - 1); }
 

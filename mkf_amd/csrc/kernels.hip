@@ -20,27 +20,56 @@
 namespace mkhip {
 
 // ---------------------------------------------------------------------------
-// batched NTT
+// batched NTT ; blockIdx.x = job * H + half
 
 template <int LOGN, bool INV>
-__global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_batch(DevTables T, const NttJob *__restrict__ jobs) {
-    constexpr int N = 1 << LOGN;
+__global__ void __launch_bounds__(Geo<LOGN>::TH) k_ntt_batch(DevTables T, const NttJob *__restrict__ jobs) {
+    typedef Geo<LOGN> Gm;
     const int tid = threadIdx.x;
-    const NttJob j = jobs[blockIdx.x];
+    const int h = Gm::split ? (int)(blockIdx.x % 2) : 0;
+    const NttJob j = jobs[blockIdx.x / Gm::H];
     const ModConst m = T.mod[j.mod];
+    const bool red = j.reduce & 1;
     u64 v[16];
-    load_row_cached<LOGN>(v, j.src, tid);
-    if (j.reduce & 1) {
+    if (INV) {
+        load_row_cached<Gm::LS>(v, j.src + (size_t)h * Gm::NS, tid);
+        if (red) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) v[i] = reduce64(v[i], m.q, m.mu);
+            for (int i = 0; i < 16; ++i) v[i] = reduce64(v[i], m.q, m.mu);
+        }
+        const bool ts = j.reduce & 2;
+        inv_transform<LOGN>(v, tid, h, T.itw + (size_t)j.mod * Gm::N, m.q, ts ? m.tninv : m.ninv,
+                            ts ? m.tninvp : m.ninvp);
+    } else {
+        const gu64 *src = G(j.src);
+        const ulonglong2 *tw = T.tw + (size_t)j.mod * Gm::N;
+        if constexpr (Gm::split) {
+            // In place, a half may not be overwritten while the other workgroup still
+            // reads it for the first stage: workgroup 0 then transforms both halves
+            // (the upper one parked in its destination meanwhile) and workgroup 1 exits.
+            if (j.src == j.dst) {
+                if (h) return;
+                u64 hi[16];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const u64 x = src[tid + e * Gm::TH], y = src[Gm::NS + tid + e * Gm::TH];
+                    v[e] = red ? reduce64(x, m.q, m.mu) : x;
+                    hi[e] = red ? reduce64(y, m.q, m.mu) : y;
+                }
+                fwd_first_stage_both(v, hi, tw, m.q);
+                store_row<Gm::LS>(hi, j.dst + Gm::NS, tid);
+                ntt_fwd<Gm::LS>(v, g_lds, tid, tw, m.q, 2);
+                store_row<Gm::LS>(v, j.dst, tid);
+                load_row_cached<Gm::LS>(v, j.dst + Gm::NS, tid);
+                ntt_fwd<Gm::LS>(v, g_lds, opaque_lane(tid), opaque(tw), m.q, 3);
+                store_row<Gm::LS>(v, j.dst + Gm::NS, tid);
+                return;
+            }
+        }
+        fwd_transform<LOGN>(v, [&](int c) { const u64 x = src[c]; return red ? reduce64(x, m.q, m.mu) : x; }, tid,
+                            h, tw, m.q);
     }
-    if (INV && (j.reduce & 2))
-        ntt_inv<LOGN>(v, g_lds, tid, T.itw + (size_t)j.mod * N, m.q, m.tninv, m.tninvp);
-    else if (INV)
-        ntt_inv<LOGN>(v, g_lds, tid, T.itw + (size_t)j.mod * N, m.q, m.ninv, m.ninvp);
-    else
-        ntt_fwd<LOGN>(v, g_lds, tid, T.tw + (size_t)j.mod * N, m.q);
-    store_row<LOGN>(v, j.dst, tid);
+    store_row<Gm::LS>(v, j.dst + (size_t)h * Gm::NS, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -69,16 +98,18 @@ __device__ __forceinline__ u64 digit_modup(const DevTables &T, const u64 *__rest
     return reduce64(r, m.q, m.mu);
 }
 
-// One workgroup per (desc, component p, digit i, row r): logical block
-// ((desc * kmax + p) * beta + i) * rows + r, XCD-remapped so the rows of one
-// (component, digit) -- which all read the same input limbs -- share an L2.
+// One workgroup per (desc, component p, digit i, row r[, half]): logical block
+// (((desc * kmax + p) * beta + i) * rows + r) * H + h, XCD-remapped so the rows of
+// one (component, digit) -- which all read the same input limbs -- share an L2.
 template <int LOGN, bool CRT>
-__global__ void __launch_bounds__(1 << (LOGN - 4))
+__global__ void __launch_bounds__(Geo<LOGN>::TH)
     k_decomp(DevTables T, const DecompDesc *__restrict__ descs, int kmax, int beta, int rows) {
-    constexpr int N = 1 << LOGN;
-    constexpr int TH = N / 16;
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
     const int tid = threadIdx.x;
-    const int bid = block_index(T);
+    int bid = block_index(T);
+    const int h = Gm::split ? bid % 2 : 0;
+    bid /= Gm::H;
     const int r = bid % rows;
     const int i = (bid / rows) % beta;
     const int p = (bid / (rows * beta)) % kmax;
@@ -93,23 +124,23 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
     const int nd = (level > alpha * (i + 1) - 1) ? alpha : (level % alpha) + 1;
     const bool own_row = (r <= level) && (r >= start) && (r < start + nd);
     const u64 *src = Dd.comp[p];
+    const ulonglong2 *tw = T.tw + (size_t)ri * N;
     u64 v[16];
     if (!CRT || nd == 1 || own_row) {
         // copy path / the digit's own rows: the residue itself, reduced mod m only
         // when it may exceed the transform's input bound 8m (source values are
         // below 3 q_src: canonical, or multSum's raw outputs in a BFV lift)
         const int sl = own_row ? r : start;
-        load_row_cached<LOGN>(v, src + (size_t)sl * N, tid);
-        if (T.mod[r_index(sl, T.L, T.K)].q > 2 * m.q) {
-#pragma unroll
-            for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
-        }
+        const bool red = T.mod[r_index(sl, T.L, T.K)].q > 2 * m.q;
+        const gu64 *row = G(src + (size_t)sl * N);
+        fwd_transform<LOGN>(v, [&](int c) { const u64 x = row[c]; return red ? reduce64(x, m.q, m.mu) : x; }, tid,
+                            h, tw, m.q);
     } else {
         const int ebase = i * (kMaxAlpha + 1) + nd;
-        for (int e = 0; e < 16; ++e) v[e] = digit_modup(T, src, tid + e * TH, start, nd, ebase, ri, m);
+        fwd_transform<LOGN>(v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h, tw,
+                            m.q);
     }
-    ntt_fwd<LOGN>(v, g_lds, tid, T.tw + (size_t)ri * N, m.q);
-    store_row<LOGN>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N, tid);
+    store_row<Gm::LS>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N + (size_t)h * Gm::NS, tid);
 }
 
 // Key-stationary x / y accumulation: grid (N/256, beta*rows, 2 sides); the
@@ -138,23 +169,25 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// tensor product + INTT ; blockIdx.x = desc * nl + limb
-
+// tensor product + INTT ; logical block = (desc * nl + limb) * H + half
 
 template <int LOGN>
-__global__ void __launch_bounds__(1 << (LOGN - 4)) k_tensor(DevTables T, const TensorDesc *__restrict__ descs, int nl) {
-    constexpr int N = 1 << LOGN;
-    constexpr int TH = N / 16;
+__global__ void __launch_bounds__(Geo<LOGN>::TH)
+    k_tensor(DevTables T, const TensorDesc *__restrict__ descs, int nl) {
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
     const int tid = threadIdx.x;
-    const int bid = block_index(T);
+    int bid = block_index(T);
+    const int h = Gm::split ? bid % 2 : 0;
+    bid /= Gm::H;
     const int l = bid % nl;
     const TensorDesc &Dd = descs[bid / nl];
     const ModConst m = T.mod[l];
-    const size_t off = (size_t)l * N;
+    const size_t off = (size_t)l * N + (size_t)h * Gm::NS;
     u64 v[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        const size_t c = off + tid + e * TH;
+        const size_t c = off + tid + e * Gm::TH;
         u64 x;
         if (Dd.is_c0) {
             x = mulmod_m(G(Dd.a0)[c], G(Dd.b0)[c], m);
@@ -165,8 +198,8 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_tensor(DevTables T, const T
         }
         v[e] = x;
     }
-    ntt_inv<LOGN>(v, g_lds, tid, T.itw + (size_t)l * N, m.q, m.ninv, m.ninvp);
-    store_row<LOGN>(v, Dd.out + off, tid);
+    inv_transform<LOGN>(v, tid, h, T.itw + (size_t)l * N, m.q, m.ninv, m.ninvp);
+    store_row<Gm::LS>(v, Dd.out + off, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -177,32 +210,53 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_tensor(DevTables T, const T
 // u, keyswitch_hoisted.go:177-193): one pass over D feeds both accumulators; the
 // second is parked in its output row while the first is transformed.
 template <int LOGN, int NKEY>
-__global__ void __launch_bounds__(1 << (LOGN - 4))
+__global__ void __launch_bounds__(Geo<LOGN>::TH)
     k_ext(DevTables T, const ExtDesc *__restrict__ descs, int beta, int rows) {
-    constexpr int N = 1 << LOGN;
-    constexpr int TH = N / 16;
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
     const int tid = threadIdx.x;
-    const int bid = block_index(T);
-    const int ndesc = gridDim.x / rows;
+    int bid = block_index(T);
+    const int h = Gm::split ? bid % 2 : 0;
+    bid /= Gm::H;
+    const int ndesc = gridDim.x / (rows * Gm::H);
     const int r = bid / ndesc;
     const ExtDesc &Dd = descs[bid % ndesc];
     const int ri = ring_index(r, Dd.level, T.L);
     const ModConst m = T.mod[ri];
     const size_t kstride = (size_t)(T.L + T.K) * N;
+    const size_t hoff = (size_t)h * Gm::NS;
     const ulonglong2 *itw = T.itw + (size_t)ri * N;
     u64 a0[16];
-    u64 *o1 = NKEY > 1 ? Dd.acc[1] + (size_t)r * N : nullptr;
-    ext_rows<LOGN, NKEY>(a0, o1, Dd.D + (size_t)r * N, (size_t)rows * N, Dd.key[0] + (size_t)ri * N,
-                         NKEY > 1 ? Dd.key[1] + (size_t)ri * N : nullptr, kstride, beta, tid, m);
+    u64 *o1 = NKEY > 1 ? Dd.acc[1] + (size_t)r * N + hoff : nullptr;
+    ext_rows<Gm::LS, NKEY>(a0, o1, Dd.D + (size_t)r * N + hoff, (size_t)rows * N, Dd.key[0] + (size_t)ri * N + hoff,
+                           NKEY > 1 ? Dd.key[1] + (size_t)ri * N + hoff : nullptr, kstride, beta, tid, m);
     // opaque(): the second transform reloads its twiddles instead of keeping the
     // first one's live across it
-    ntt_inv<LOGN>(a0, g_lds, opaque_lane(tid), opaque(itw), m.q, m.ninv, m.ninvp);
-    store_row<LOGN>(a0, Dd.acc[0] + (size_t)r * N, tid);
+    inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
+    store_row<Gm::LS>(a0, Dd.acc[0] + (size_t)r * N + hoff, tid);
     if (NKEY > 1) {
-        load_row_cached<LOGN>(a0, o1, tid);
-        ntt_inv<LOGN>(a0, g_lds, opaque_lane(tid), opaque(itw), m.q, m.ninv, m.ninvp);
-        store_row<LOGN>(a0, o1, tid);
+        load_row_cached<Gm::LS>(a0, o1, tid);
+        inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
+        store_row<Gm::LS>(a0, o1, tid);
     }
+}
+
+// ---------------------------------------------------------------------------
+// last inverse stage of split limbs ; grid (N/2/256, ndesc), thread = pair (c, c + N/2)
+
+__global__ void __launch_bounds__(256) k_intt_finish(DevTables T, const FinishDesc *__restrict__ descs) {
+    const int half = T.N / 2;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const FinishDesc &Dd = descs[blockIdx.y];
+    const ModConst m = T.mod[Dd.mod];
+    gu64 *row = G(Dd.row);
+    u64 x = row[c], y = row[c + half];
+    if (Dd.tscale)
+        inv_last_stage(x, y, m.twn, m.twnp, m.tninv, m.tninvp, m.q);
+    else
+        inv_last_stage(x, y, m.wn, m.wnp, m.ninv, m.ninvp, m.q);
+    row[c] = x;
+    row[c + half] = y;
 }
 
 // ---------------------------------------------------------------------------
@@ -282,8 +336,9 @@ static hipError_t allow_lds(F kernel, size_t bytes) {
 template <int LOGN>
 static hipError_t launch_all(int what, const DevTables &T, const void *descs, int n, int a, int b, int c,
                              hipStream_t st) {
-    constexpr int N = 1 << LOGN;
-    constexpr size_t lds = (size_t)N * sizeof(u64);
+    typedef Geo<LOGN> Gm;
+    constexpr size_t lds = (size_t)Gm::NS * sizeof(u64);
+    constexpr int H = Gm::H;
     static bool init = false;
     if (!init) {
         hipError_t e;
@@ -296,30 +351,30 @@ static hipError_t launch_all(int what, const DevTables &T, const void *descs, in
         if ((e = allow_lds(k_ext<LOGN, 2>, lds)) != hipSuccess) return e;
         init = true;
     }
-    const dim3 block(N / 16);
+    const dim3 block(Gm::TH);
     switch (what) {
     case KNTT_FWD:
-        hipLaunchKernelGGL((k_ntt_batch<LOGN, false>), dim3(n), block, lds, st, T, (const NttJob *)descs);
+        hipLaunchKernelGGL((k_ntt_batch<LOGN, false>), dim3(n * H), block, lds, st, T, (const NttJob *)descs);
         break;
     case KNTT_INV:
-        hipLaunchKernelGGL((k_ntt_batch<LOGN, true>), dim3(n), block, lds, st, T, (const NttJob *)descs);
+        hipLaunchKernelGGL((k_ntt_batch<LOGN, true>), dim3(n * H), block, lds, st, T, (const NttJob *)descs);
         break;
     case KDECOMP:  // n descs, a = beta, b = rows, c = kmax
         if (T.alpha > 1)
-            hipLaunchKernelGGL((k_decomp<LOGN, true>), dim3(n * c * a * b), block, lds, st, T,
+            hipLaunchKernelGGL((k_decomp<LOGN, true>), dim3(n * c * a * b * H), block, lds, st, T,
                                (const DecompDesc *)descs, c, a, b);
         else
-            hipLaunchKernelGGL((k_decomp<LOGN, false>), dim3(n * c * a * b), block, lds, st, T,
+            hipLaunchKernelGGL((k_decomp<LOGN, false>), dim3(n * c * a * b * H), block, lds, st, T,
                                (const DecompDesc *)descs, c, a, b);
         break;
     case KTENSOR:
-        hipLaunchKernelGGL((k_tensor<LOGN>), dim3(n * a), block, lds, st, T, (const TensorDesc *)descs, a);
+        hipLaunchKernelGGL((k_tensor<LOGN>), dim3(n * a * H), block, lds, st, T, (const TensorDesc *)descs, a);
         break;
     case KEXT1:
-        hipLaunchKernelGGL((k_ext<LOGN, 1>), dim3(n * b), block, lds, st, T, (const ExtDesc *)descs, a, b);
+        hipLaunchKernelGGL((k_ext<LOGN, 1>), dim3(n * b * H), block, lds, st, T, (const ExtDesc *)descs, a, b);
         break;
     case KEXT2:
-        hipLaunchKernelGGL((k_ext<LOGN, 2>), dim3(n * b), block, lds, st, T, (const ExtDesc *)descs, a, b);
+        hipLaunchKernelGGL((k_ext<LOGN, 2>), dim3(n * b * H), block, lds, st, T, (const ExtDesc *)descs, a, b);
         break;
     default:
         return hipErrorInvalidValue;
@@ -357,6 +412,10 @@ hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n,
                            a, b);
         return hipGetLastError();
     }
+    if (what == KINTT_FINISH) {  // n descs
+        hipLaunchKernelGGL(k_intt_finish, dim3(T.N / 512, n), dim3(256), 0, st, T, (const FinishDesc *)descs);
+        return hipGetLastError();
+    }
     if (what == KMODDOWN || what == KMODDOWN_FINAL) {
         if (what == KMODDOWN)
             hipLaunchKernelGGL((k_moddown<false>), dim3(T.N / 256, n), dim3(256), 0, st, T,
@@ -371,6 +430,8 @@ hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n,
         return launch_all<13>(what, T, descs, n, a, b, c, st);
     case 14:
         return launch_all<14>(what, T, descs, n, a, b, c, st);
+    case 15:
+        return launch_all<15>(what, T, descs, n, a, b, c, st);
     default:
         return hipErrorInvalidValue;
     }

@@ -129,24 +129,26 @@ __global__ void __launch_bounds__(256) k_quantize(DevTables T, const BfvQuantDes
     }
 }
 
-// Tensor over R ; blockIdx.x = desc * 2L + limb.  Inputs: NTT-domain R polys;
-// output: t * INTT(product), coefficient domain, R layout.
+// Tensor over R ; logical block = (desc * 2L + limb) * H + half.  Inputs: NTT-domain
+// R polys; output: t * INTT(product), coefficient domain, R layout.
 template <int LOGN>
-__global__ void __launch_bounds__(1 << (LOGN - 4))
+__global__ void __launch_bounds__(Geo<LOGN>::TH)
     k_tensor_r(DevTables T, const TensorDesc *__restrict__ descs, int nl) {
-    constexpr int N = 1 << LOGN;
-    constexpr int TH = N / 16;
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
     const int tid = threadIdx.x;
-    const int bid = block_index(T);
+    int bid = block_index(T);
+    const int h = Gm::split ? bid % 2 : 0;
+    bid /= Gm::H;
     const int l = bid % nl;
     const TensorDesc &Dd = descs[bid / nl];
     const int ri = r_index(l, T.L, T.K);
     const ModConst m = T.mod[ri];
-    const size_t off = (size_t)l * N;
+    const size_t off = (size_t)l * N + (size_t)h * Gm::NS;
     u64 v[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        const size_t c = off + tid + e * TH;
+        const size_t c = off + tid + e * Gm::TH;
         u64 x;
         if (Dd.is_c0) {
             x = mulmod_m(G(Dd.a0)[c], G(Dd.b0)[c], m);
@@ -157,14 +159,14 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
         }
         v[e] = x;
     }
-    ntt_inv<LOGN>(v, g_lds, tid, T.itw + (size_t)ri * N, m.q, m.tninv, m.tninvp);
-    store_row<LOGN>(v, Dd.out + off, tid);
+    inv_transform<LOGN>(v, tid, h, T.itw + (size_t)ri * N, m.q, m.tninv, m.tninvp);
+    store_row<Gm::LS>(v, Dd.out + off, tid);
 }
 
 template <int LOGN>
 static hipError_t launch_tensor_r(const DevTables &T, const void *descs, int n, int nl, hipStream_t st) {
-    constexpr int N = 1 << LOGN;
-    constexpr size_t lds = (size_t)N * sizeof(u64);
+    typedef Geo<LOGN> Gm;
+    constexpr size_t lds = (size_t)Gm::NS * sizeof(u64);
     static bool init = false;
     if (!init) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_tensor_r<LOGN>),
@@ -172,7 +174,8 @@ static hipError_t launch_tensor_r(const DevTables &T, const void *descs, int n, 
         if (e != hipSuccess) return e;
         init = true;
     }
-    hipLaunchKernelGGL((k_tensor_r<LOGN>), dim3(n * nl), dim3(N / 16), lds, st, T, (const TensorDesc *)descs, nl);
+    hipLaunchKernelGGL((k_tensor_r<LOGN>), dim3(n * nl * Gm::H), dim3(Gm::TH), lds, st, T,
+                       (const TensorDesc *)descs, nl);
     return hipGetLastError();
 }
 
@@ -191,6 +194,7 @@ hipError_t launch_bfv(int what, const DevTables &T, const void *descs, int n, in
     case KBFV_TENSOR:
         if (T.logN == 13) return launch_tensor_r<13>(T, descs, n, a, st);
         if (T.logN == 14) return launch_tensor_r<14>(T, descs, n, a, st);
+        if (T.logN == 15) return launch_tensor_r<15>(T, descs, n, a, st);
         return hipErrorInvalidValue;
     default:
         return hipErrorInvalidValue;

@@ -20,12 +20,15 @@ namespace mkhip {
 // sharing a key row are adjacent) or ((side * beta + i) * nops + o) * rows + r
 // (order 1: the rows sharing an input limb are adjacent); descs[2 o + side]
 template <int LOGN>
-__global__ void __launch_bounds__(1 << (LOGN - 4))
+__global__ void __launch_bounds__(Geo<LOGN>::TH)
     k_decomp_mac(DevTables T, const DecompDesc *__restrict__ descs, int nops, int beta, int rows, int order) {
-    constexpr int N = 1 << LOGN;
-    constexpr int TH = N / 16;
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
+    constexpr int TH = Gm::TH;
     const int tid = threadIdx.x;
     int bid = block_index(T);
+    const int h = Gm::split ? bid % 2 : 0;
+    bid /= Gm::H;
     int o, r;
     if (order == 0) {
         o = bid % nops;
@@ -44,22 +47,21 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
     const int ri = ring_index(r, Dd.level, T.L);
     const ModConst m = T.mod[ri];
     const ulonglong2 *tw = T.tw + (size_t)ri * N;
-    const size_t krow = ((size_t)i * (T.L + T.K) + ri) * N;
+    const size_t hoff = (size_t)h * Gm::NS;
+    const size_t krow = ((size_t)i * (T.L + T.K) + ri) * N + hoff;
     const bool reduce = T.mod[r_index(i, T.L, T.K)].q > 2 * m.q;
     u64 acc[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0;
     for (int p = 0; p < Dd.k; ++p) {
         u64 v[16];
-        load_row_cached<LOGN>(v, Dd.comp[p] + (size_t)i * N, tid);
-        if (reduce) {  // values below 3 q_src may exceed the transform's bound 8m
-#pragma unroll
-            for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
-        }
+        const gu64 *row = G(Dd.comp[p] + (size_t)i * N);
         __builtin_amdgcn_sched_barrier(0);
-        ntt_fwd<LOGN>(v, g_lds, opaque_lane(tid), opaque(tw), m.q);
+        // values below 3 q_src may exceed the transform's bound 8m: reduce then
+        fwd_transform<LOGN>(v, [&](int c) { const u64 x = row[c]; return reduce ? reduce64(x, m.q, m.mu) : x; },
+                            opaque_lane(tid), h, opaque(tw), m.q);
         __builtin_amdgcn_sched_barrier(0);
-        store_row<LOGN>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N, tid);
+        store_row<Gm::LS>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N + hoff, tid);
         const u64 *key = Dd.key[p] + krow;
 #pragma unroll
         for (int e = 0; e < 16; ++e)
@@ -67,14 +69,14 @@ __global__ void __launch_bounds__(1 << (LOGN - 4))
     }
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = mred(acc[e], m.r2, m.q, m.qinv);
-    store_row<LOGN>(acc, Dd.xy + krow, tid);
+    store_row<Gm::LS>(acc, Dd.xy + krow, tid);
 }
 
 template <int LOGN>
 static hipError_t launch_fused_n(int what, const DevTables &T, const void *descs, int n, int a, int b,
                                  hipStream_t st) {
-    constexpr int N = 1 << LOGN;
-    constexpr size_t lds = (size_t)N * sizeof(u64);
+    typedef Geo<LOGN> Gm;
+    constexpr size_t lds = (size_t)Gm::NS * sizeof(u64);
     static bool init = false;
     if (!init) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_decomp_mac<LOGN>),
@@ -84,7 +86,7 @@ static hipError_t launch_fused_n(int what, const DevTables &T, const void *descs
     }
     if (what != KDECOMP_MAC) return hipErrorInvalidValue;
     // n ops (2 descriptors each), a = beta, b = rows
-    hipLaunchKernelGGL((k_decomp_mac<LOGN>), dim3(n * 2 * a * b), dim3(N / 16), lds, st, T,
+    hipLaunchKernelGGL((k_decomp_mac<LOGN>), dim3(n * 2 * a * b * Gm::H), dim3(Gm::TH), lds, st, T,
                        (const DecompDesc *)descs, n, a, b, T.decomp_order);
     return hipGetLastError();
 }
@@ -94,6 +96,7 @@ hipError_t launch_fused(int what, const DevTables &T, const void *descs, int n, 
     if (T.alpha != 1) return hipErrorInvalidValue;
     if (T.logN == 13) return launch_fused_n<13>(what, T, descs, n, a, b, st);
     if (T.logN == 14) return launch_fused_n<14>(what, T, descs, n, a, b, st);
+    if (T.logN == 15) return launch_fused_n<15>(what, T, descs, n, a, b, st);
     return hipErrorInvalidValue;
 }
 

@@ -11,71 +11,83 @@
 
 namespace mkhip {
 
-// blockIdx.x = desc * rows + row (XCD-remapped)
+// logical block = (desc * rows + row) * H + half (XCD-remapped)
 template <int LOGN>
-__global__ void __launch_bounds__(1 << (LOGN - 4))
+__global__ void __launch_bounds__(Geo<LOGN>::TH)
     k_mac2(DevTables T, const Mac2Desc *__restrict__ descs, int beta, int rows) {
-    constexpr int N = 1 << LOGN;
-    constexpr int TH = N / 16;
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
     const int tid = threadIdx.x;
-    const int bid = block_index(T);
+    int bid = block_index(T);
+    const int h = Gm::split ? bid % 2 : 0;
+    bid /= Gm::H;
     const int r = bid % rows;
     const Mac2Desc &Dd = descs[bid / rows];
     const int ri = ring_index(r, Dd.level, T.L);
     const ModConst m = T.mod[ri];
     const size_t kstride = (size_t)(T.L + T.K) * N;
+    const size_t hoff = (size_t)h * Gm::NS;
     u64 a0[16];
-    u64 *o0 = Dd.acc[0] + (size_t)r * N, *o1 = Dd.acc[1] + (size_t)r * N;
+    u64 *o0 = Dd.acc[0] + (size_t)r * N + hoff, *o1 = Dd.acc[1] + (size_t)r * N + hoff;
     // swk digit b: output 0 at (2b) kstride, output 1 at (2b+1) kstride
-    ext_rows<LOGN, 2>(a0, o1, Dd.D + (size_t)r * N, (size_t)rows * N, Dd.swk + (size_t)ri * N,
-                      Dd.swk + kstride + (size_t)ri * N, 2 * kstride, beta, tid, m);
+    ext_rows<Gm::LS, 2>(a0, o1, Dd.D + (size_t)r * N + hoff, (size_t)rows * N, Dd.swk + (size_t)ri * N + hoff,
+                        Dd.swk + kstride + (size_t)ri * N + hoff, 2 * kstride, beta, tid, m);
     if (r > Dd.level || Dd.intt_all) {
         const ulonglong2 *itw = T.itw + (size_t)ri * N;
-        ntt_inv<LOGN>(a0, g_lds, opaque_lane(tid), opaque(itw), m.q, m.ninv, m.ninvp);
-        store_row<LOGN>(a0, o0, tid);
-        load_row_cached<LOGN>(a0, o1, tid);
-        ntt_inv<LOGN>(a0, g_lds, opaque_lane(tid), opaque(itw), m.q, m.ninv, m.ninvp);
-        store_row<LOGN>(a0, o1, tid);
+        inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
+        store_row<Gm::LS>(a0, o0, tid);
+        load_row_cached<Gm::LS>(a0, o1, tid);
+        inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
+        store_row<Gm::LS>(a0, o1, tid);
     } else {
-        store_row<LOGN>(a0, o0, tid);
+        store_row<Gm::LS>(a0, o0, tid);
     }
 }
 
-// blockIdx.x = desc * (level+1) + Q row j
+// logical block = (desc * (level+1) + Q row j) * H + half
 template <int LOGN>
-__global__ void __launch_bounds__(1 << (LOGN - 4)) k_moddown_ntt(DevTables T, const MdNttDesc *__restrict__ descs, int nl) {
-    constexpr int N = 1 << LOGN;
-    constexpr int TH = N / 16;
+__global__ void __launch_bounds__(Geo<LOGN>::TH)
+    k_moddown_ntt(DevTables T, const MdNttDesc *__restrict__ descs, int nl) {
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
     const int tid = threadIdx.x;
-    const int bid = block_index(T);
+    int bid = block_index(T);
+    const int h = Gm::split ? bid % 2 : 0;
+    bid /= Gm::H;
     const int j = bid % nl;
     const MdNttDesc &Dd = descs[bid / nl];
     const ModConst m = T.mod[j];
     const ulonglong2 *tw = T.tw + (size_t)j * N;
     const u64 pinv = T.md_pinv[j];
+    const size_t hoff = (size_t)h * Gm::NS;
     // sum_a (x_a - NTT(ModUp(x_a,P))) P^-1 = (sum_a x_a - NTT(sum_a ModUp(x_a,P))) P^-1
     // residue for residue (the NTT is linear mod q_j): one transform per row.
     u64 z[16], xs[16];
+    fwd_transform<LOGN>(z,
+                        [&](int c) {
+                            u64 acc = 0;
+                            for (int a = 0; a < Dd.nacc; ++a) {
+                                const PUp up = modup_p_prepare(T, Dd.acc[a], nl, c);
+                                acc = add_mod(acc, modup_p_to_row(T, up, j, m), m.q);
+                            }
+                            return acc;
+                        },
+                        tid, h, tw, m.q);
 #pragma unroll
-    for (int e = 0; e < 16; ++e) z[e] = xs[e] = 0;
+    for (int e = 0; e < 16; ++e) xs[e] = 0;
     for (int a = 0; a < Dd.nacc; ++a) {
-        const u64 *acc = Dd.acc[a];
-        const u64 *xr = acc + (size_t)j * N;
-        for (int e = 0; e < 16; ++e) {
-            const PUp up = modup_p_prepare(T, acc, nl, tid + e * TH);
-            z[e] = add_mod(z[e], modup_p_to_row(T, up, j, m), m.q);
-            xs[e] = add_mod(xs[e], uniform_ptr(xr + e * TH)[tid], m.q);
-        }
+        const u64 *xr = Dd.acc[a] + (size_t)j * N + hoff;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) xs[e] = add_mod(xs[e], uniform_ptr(xr + e * Gm::TH)[tid], m.q);
     }
-    ntt_fwd<LOGN>(z, g_lds, tid, tw, m.q);
 #pragma unroll
     for (int e = 0; e < 16; ++e) z[e] = mred(z[e] + 2 * m.q - xs[e], pinv, m.q, m.qinv);
     if (Dd.base) {
-        load_row_cached<LOGN>(xs, Dd.base + (size_t)j * N, tid);
+        load_row_cached<Gm::LS>(xs, Dd.base + (size_t)j * N + hoff, tid);
 #pragma unroll
         for (int e = 0; e < 16; ++e) z[e] = add_mod(z[e], xs[e], m.q);
     }
-    store_row<LOGN>(z, Dd.dst + (size_t)j * N, tid);
+    store_row<Gm::LS>(z, Dd.dst + (size_t)j * N + hoff, tid);
 }
 
 // grid (N/256, ndesc * nl), one coefficient per thread
@@ -95,37 +107,41 @@ __global__ void __launch_bounds__(256) k_karatsuba(DevTables T, const KaraDesc *
     G(Dd.mu2)[c] = mu2;
 }
 
-// blockIdx.x = desc * nout_limbs + row j ; nout_limbs = level + 1 - nres
+// logical block = (desc * nout_limbs + row j) * H + half ; nout_limbs = level + 1 - nres
 template <int LOGN>
-__global__ void __launch_bounds__(1 << (LOGN - 4))
+__global__ void __launch_bounds__(Geo<LOGN>::TH)
     k_rescale_ntt(DevTables T, const RescaleNttDesc *__restrict__ descs, int nlo) {
-    constexpr int N = 1 << LOGN;
-    constexpr int TH = N / 16;
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
     const int tid = threadIdx.x;
-    const int bid = block_index(T);
+    int bid = block_index(T);
+    const int h = Gm::split ? bid % 2 : 0;
+    bid /= Gm::H;
     const int j = bid % nlo;
     const RescaleNttDesc &Dd = descs[bid / nlo];
     const ModConst m = T.mod[j];
+    const size_t hoff = (size_t)h * Gm::NS;
     u64 v[16];
-    load_row_cached<LOGN>(v, Dd.x + (size_t)j * N, tid);
+    load_row_cached<Gm::LS>(v, Dd.x + (size_t)j * N + hoff, tid);
     if (Dd.nres == 1) {
         // out_j = (x_j - NTT_j(([x_L + h]_{q_L} - h) mod q_j)) q_L^-1, h = q_L >> 1
         const int l = Dd.level;
         const u64 qL = T.mod[l].q;
-        const u64 h = qL >> 1;
+        const u64 hq = qL >> 1;
         const u64 hj = T.rs_half[l * T.L + j];
+        const gu64 *xl = G(Dd.xlast);
         u64 w[16];
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const u64 t = add_mod(uniform_ptr(Dd.xlast + e * TH)[tid], h, qL);
-            w[e] = sub_mod(reduce64(t, m.q, m.mu), hj, m.q);
-        }
-        ntt_fwd<LOGN>(w, g_lds, tid, T.tw + (size_t)j * N, m.q);
+        fwd_transform<LOGN>(w,
+                            [&](int c) {
+                                const u64 t = add_mod(xl[c], hq, qL);
+                                return sub_mod(reduce64(t, m.q, m.mu), hj, m.q);
+                            },
+                            tid, h, T.tw + (size_t)j * N, m.q);
         const u64 inv = T.rs_inv[l * T.L + j];
 #pragma unroll
         for (int e = 0; e < 16; ++e) v[e] = mred(sub_mod(v[e], w[e], m.q), inv, m.q, m.qinv);
     }
-    for (int o = 0; o < Dd.nout; ++o) store_row<LOGN>(v, Dd.out[o] + (size_t)j * N, tid);
+    for (int o = 0; o < Dd.nout; ++o) store_row<Gm::LS>(v, Dd.out[o] + (size_t)j * N + hoff, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -139,8 +155,9 @@ static hipError_t allow_lds_mkf(F kernel, size_t bytes) {
 template <int LOGN>
 static hipError_t launch_mkf_n(int what, const DevTables &T, const void *descs, int n, int a, int b,
                                hipStream_t st) {
-    constexpr int N = 1 << LOGN;
-    constexpr size_t lds = (size_t)N * sizeof(u64);
+    typedef Geo<LOGN> Gm;
+    constexpr size_t lds = (size_t)Gm::NS * sizeof(u64);
+    constexpr int H = Gm::H;
     static bool init = false;
     if (!init) {
         hipError_t e;
@@ -149,17 +166,18 @@ static hipError_t launch_mkf_n(int what, const DevTables &T, const void *descs, 
         if ((e = allow_lds_mkf(k_rescale_ntt<LOGN>, lds)) != hipSuccess) return e;
         init = true;
     }
-    const dim3 block(N / 16);
+    const dim3 block(Gm::TH);
     switch (what) {
     case KMAC2:  // n descs, a = beta, b = rows
-        hipLaunchKernelGGL((k_mac2<LOGN>), dim3(n * b), block, lds, st, T, (const Mac2Desc *)descs, a, b);
+        hipLaunchKernelGGL((k_mac2<LOGN>), dim3(n * b * H), block, lds, st, T, (const Mac2Desc *)descs, a, b);
         break;
     case KMD_NTT:  // n descs, a = level + 1
-        hipLaunchKernelGGL((k_moddown_ntt<LOGN>), dim3(n * a), block, lds, st, T, (const MdNttDesc *)descs, a);
+        hipLaunchKernelGGL((k_moddown_ntt<LOGN>), dim3(n * a * H), block, lds, st, T, (const MdNttDesc *)descs,
+                           a);
         break;
     case KRESCALE_NTT:  // n descs, a = output limbs
-        hipLaunchKernelGGL((k_rescale_ntt<LOGN>), dim3(n * a), block, lds, st, T, (const RescaleNttDesc *)descs,
-                           a);
+        hipLaunchKernelGGL((k_rescale_ntt<LOGN>), dim3(n * a * H), block, lds, st, T,
+                           (const RescaleNttDesc *)descs, a);
         break;
     default:
         return hipErrorInvalidValue;
@@ -178,6 +196,8 @@ hipError_t launch_mkf(int what, const DevTables &T, const void *descs, int n, in
         return launch_mkf_n<13>(what, T, descs, n, a, b, st);
     case 14:
         return launch_mkf_n<14>(what, T, descs, n, a, b, st);
+    case 15:
+        return launch_mkf_n<15>(what, T, descs, n, a, b, st);
     default:
         return hipErrorInvalidValue;
     }

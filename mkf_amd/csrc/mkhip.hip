@@ -137,7 +137,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
                                 int gamma, uint64_t t, const uint64_t *qmul, int flags) {
     if (!out || !q || (K > 0 && !p) || L <= 0 || K < 0 || gamma <= 0) return MKHIP_ERR_ARG;
     *out = nullptr;
-    if (logN != 13 && logN != 14) return MKHIP_ERR_PARAMS;
+    if (logN < 13 || logN > 15) return MKHIP_ERR_PARAMS;
     if (L > 16 || K > 4 || L + K > kMaxModuli) return MKHIP_ERR_PARAMS;
     const bool bfv = qmul != nullptr;
     const int N = 1 << logN;
@@ -203,6 +203,12 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
             a = h_mulmod(a, psi, qi);
             b = h_mulmod(b, psiinv, qi);
         }
+        // last inverse stage of a limb split over two workgroups: twiddle itw[1]
+        const u64 w1 = itw[(size_t)i * N + 1].x;
+        m.wn = h_mulmod(w1, m.ninv, qi);
+        m.wnp = h_shoup_companion(m.wn, qi);
+        m.twn = h_mulmod(w1, m.tninv, qi);
+        m.twnp = h_shoup_companion(m.twn, qi);
     }
     // ModDown P -> Q
     std::vector<u64> md_y, md_qstar, md_vt, md_pinv(L);
@@ -518,6 +524,24 @@ struct DevScratch {
 
 }  // namespace
 
+// Split (2^15) inverse transforms: the last stage and the scaling of the nrows
+// rows at base + r N (ring modulus mods[r]); synchronous, nothing below 2^15.
+int mkhip::finish_rows(mkhip_ctx *c, u64 *base, const int *mods, int nrows, int tscale) {
+    if (c->logN != 15 || nrows <= 0) return MKHIP_OK;
+    std::vector<FinishDesc> f(nrows);
+    for (int r = 0; r < nrows; ++r) {
+        f[r].row = base + (size_t)r * c->N;
+        f[r].mod = mods[r];
+        f[r].tscale = tscale;
+    }
+    DevScratch s;
+    HIP_TRY(hipMalloc(&s.p, sizeof(FinishDesc) * nrows));
+    HIP_TRY(hipMemcpyAsync(s.p, f.data(), sizeof(FinishDesc) * nrows, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_kernel(KINTT_FINISH, c->T, s.p, nrows, 0, 0, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MKHIP_OK;
+}
+
 extern "C" int mkhip_ntt_limbs(mkhip_ctx *c, const uint64_t *in, uint64_t *out, const int *ring_idx, int nlimbs,
                                int inverse) {
     if (!c || !in || !out || !ring_idx || nlimbs <= 0) return MKHIP_ERR_ARG;
@@ -539,6 +563,10 @@ extern "C" int mkhip_ntt_limbs(mkhip_ctx *c, const uint64_t *in, uint64_t *out, 
     HIP_TRY(hipMemcpyAsync(din, in, words * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(djobs, jobs.data(), sizeof(NttJob) * nlimbs, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(launch_kernel(inverse ? KNTT_INV : KNTT_FWD, c->T, djobs, nlimbs, 0, 0, c->stream));
+    if (inverse) {
+        const int st = finish_rows(c, dout, ring_idx, nlimbs, 0);
+        if (st != MKHIP_OK) return st;
+    }
     HIP_TRY(hipMemcpyAsync(out, dout, words * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MKHIP_OK;
@@ -624,6 +652,12 @@ extern "C" int mkhip_external_product(mkhip_ctx *c, const uint64_t *a_hoisted, c
     HIP_TRY(hipMemcpyAsync(de, &he, sizeof(he), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(dm, &hm, sizeof(hm), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(launch_kernel(KEXT1, c->T, de, 1, beta, rows, c->stream));
+    {
+        std::vector<int> mods(rows);
+        for (int r = 0; r < rows; ++r) mods[r] = r < nl ? r : c->L + r - nl;
+        const int st = finish_rows(c, dacc, mods.data(), rows, 0);
+        if (st != MKHIP_OK) return st;
+    }
     HIP_TRY(launch_kernel(KMODDOWN, c->T, dm, 1, 0, 0, c->stream));
     HIP_TRY(hipMemcpyAsync(out, dout, outw * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -272,13 +272,24 @@ extern "C" int mkhip_plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op
         S.push_back(make_stage(KDECOMP, blob.add(dec1), (int)dec1.size(), 2 * beta, rows, kmax1, "decompose"));
         S.push_back(make_stage(KMAC_XY, blob.add(mac), nops, 2 * beta, rows, 1, "mac_xy"));
     }
+    Finish fin(c);
     S.push_back(make_stage(KNTT_FWD, blob.add(ntt), (int)ntt.size(), 0, 0, 1, "tensor_ntt"));
     S.push_back(make_stage(KBFV_TENSOR, blob.add(ten), (int)ten.size(), nr, 0, 1, "tensor"));
+    for (const TensorDesc &t : ten)
+        for (int l = 0; l < nr; ++l) fin.row(t.out + (size_t)l * N, l < c->L ? l : K + l, 1);
+    fin.emit(S, blob, "tensor");
     S.push_back(make_stage(KBFV_QUANTIZE, blob.add(quant), (int)quant.size(), 0, 0, 1, "quantize"));
     S.push_back(make_stage(KEXT1, blob.add(ext1), (int)ext1.size(), 2 * beta, rows, 1, "ext_xy"));
+    for (const ExtDesc &e : ext1) fin.qp(e.acc[0], nl, c->L, K, N);
+    fin.emit(S, blob, "ext_xy");
     S.push_back(make_stage(KMODDOWN, blob.add(md1), (int)md1.size(), 0, 0, 1, "moddown_xy"));
     S.push_back(make_stage(KDECOMP, blob.add(dec2), (int)dec2.size(), beta, rows, kmax2, "decompose_t"));
     S.push_back(make_stage(KEXT2, blob.add(ext2), (int)ext2.size(), beta, rows, 1, "ext_vu"));
+    for (const ExtDesc &e : ext2) {
+        fin.qp(e.acc[0], nl, c->L, K, N);
+        fin.qp(e.acc[1], nl, c->L, K, N);
+    }
+    fin.emit(S, blob, "ext_vu");
     S.push_back(make_stage(KMODDOWN_FINAL, blob.add(md2), (int)md2.size(), 0, 0, 1, "moddown_final"));
     st = plan_finish(pl, blob, use_graph);
     if (st != MKHIP_OK) return st;
@@ -414,6 +425,12 @@ extern "C" int mkhip_bfv_quantize(mkhip_ctx *c, const uint64_t *in_ntt, uint64_t
     HIP_TRY(hipMemcpyAsync(djobs, jobs.data(), sizeof(NttJob) * nr, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(dq, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(launch_kernel(KNTT_INV, c->T, djobs, nr, 0, 0, c->stream));
+    {
+        std::vector<int> mods(nr);
+        for (int l = 0; l < nr; ++l) mods[l] = r_index(l, L, K);
+        const int st = finish_rows(c, dx, mods.data(), nr, 1);
+        if (st != MKHIP_OK) return st;
+    }
     HIP_TRY(launch_bfv(KBFV_QUANTIZE, c->T, dq, 1, 0, 0, c->stream));
     HIP_TRY(hipMemcpyAsync(out, dout, ow * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -114,6 +114,8 @@ struct DescBlob {
     }
 };
 
+// mkhip.hip: last stage + scaling of split inverse transforms (see Finish below)
+int finish_rows(mkhip_ctx *c, u64 *base, const int *mods, int nrows, int tscale);
 hipError_t plan_stage(mkhip_plan *pl, int s);
 int plan_launch_raw(mkhip_plan *pl);
 // New plan with its workspace allocated (ws_bytes); *out is null on failure.
@@ -133,5 +135,41 @@ inline PlanStage make_stage(int what, size_t blob_off, int n, int a, int b, int 
     s.name = name;
     return s;
 }
+
+// Limbs of 2^15 are inverse-transformed as two halves that leave the last stage
+// and the N^-1 (or t N^-1) scaling to a KINTT_FINISH launch over the rows the
+// inverse transform wrote.  The collector records those rows (nothing for
+// smaller rings) and emits the stage "<after>_finish" right after its producer.
+struct Finish {
+    bool on = false;
+    std::vector<FinishDesc> d;
+    explicit Finish(const mkhip_ctx *c) : on(c->logN == 15) {}
+    void row(u64 *r, int mod, int tscale = 0) {
+        if (!on) return;
+        FinishDesc f;
+        f.row = r;
+        f.mod = mod;
+        f.tscale = tscale;
+        d.push_back(f);
+    }
+    // rows 0..nl+K-1 of a QP polynomial at level nl-1 (Q rows, then P rows)
+    void qp(u64 *base, int nl, int L, int K, size_t N, bool p_only = false) {
+        for (int r = p_only ? nl : 0; r < nl + K; ++r) row(base + (size_t)r * N, r < nl ? r : L + r - nl);
+    }
+    // outputs of a batched inverse NTT
+    void jobs(const std::vector<NttJob> &v) {
+        for (const NttJob &j : v) row(j.dst, j.mod, (j.reduce & 2) ? 1 : 0);
+    }
+    // the rows k_mac2 inverse-transforms: P rows, or every row with intt_all
+    void mac2(const std::vector<Mac2Desc> &v, int L, int K, size_t N) {
+        for (const Mac2Desc &m : v)
+            for (int o = 0; o < 2; ++o) qp(m.acc[o], m.level + 1, L, K, N, !m.intt_all);
+    }
+    void emit(std::vector<PlanStage> &S, DescBlob &blob, const std::string &after) {
+        if (!on || d.empty()) return;
+        S.push_back(make_stage(KINTT_FINISH, blob.add(d), (int)d.size(), 0, 0, 1, (after + "_finish").c_str()));
+        d.clear();
+    }
+};
 
 }  // namespace mkhip

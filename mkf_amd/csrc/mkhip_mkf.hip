@@ -225,17 +225,29 @@ extern "C" int mkhip_plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *o
     }
     DescBlob blob;
     auto &S = pl->stages;
+    Finish fin(c);
     S.push_back(make_stage(KNTT_INV, blob.add(intt_in), (int)intt_in.size(), 0, 0, 1, "intt_in"));
+    fin.jobs(intt_in);
+    fin.emit(S, blob, "intt_in");
     S.push_back(make_stage(KDECOMP, blob.add(dec), (int)dec.size(), beta, rows, kmax, "decompose"));
     S.push_back(make_stage(KMAC2, blob.add(mac_fold), (int)mac_fold.size(), beta, rows, 1, "mac_fold"));
+    fin.mac2(mac_fold, c->L, c->K, N);
+    fin.emit(S, blob, "mac_fold");
     S.push_back(make_stage(KMD_NTT, blob.add(md_fold), (int)md_fold.size(), nl, 0, 1, "moddown_fold"));
     S.push_back(make_stage(KKARA, blob.add(kara), (int)kara.size(), nl, 0, 1, "karatsuba"));
     S.push_back(make_stage(KNTT_INV, blob.add(intt_mu2), (int)intt_mu2.size(), 0, 0, 1, "intt_mu2"));
+    fin.jobs(intt_mu2);
+    fin.emit(S, blob, "intt_mu2");
     S.push_back(make_stage(KDECOMP, blob.add(dec_mu2), (int)dec_mu2.size(), beta, rows, 1, "decompose_mu2"));
     S.push_back(make_stage(KMAC2, blob.add(mac_relin), (int)mac_relin.size(), beta, rows, 1, "mac_relin"));
+    fin.mac2(mac_relin, c->L, c->K, N);
+    fin.emit(S, blob, "mac_relin");
     S.push_back(make_stage(KMD_NTT, blob.add(md_relin), (int)md_relin.size(), nl, 0, 1, "moddown_relin"));
-    if (nres == 1)
+    if (nres == 1) {
         S.push_back(make_stage(KNTT_INV, blob.add(intt_last), (int)intt_last.size(), 0, 0, 1, "intt_last"));
+        fin.jobs(intt_last);
+        fin.emit(S, blob, "intt_last");
+    }
     S.push_back(make_stage(KRESCALE_NTT, blob.add(resc), (int)resc.size(), nl - nres, 0, 1, "rescale"));
     st = plan_finish(pl, blob, use_graph);
     if (st != MKHIP_OK) return st;
@@ -471,15 +483,23 @@ extern "C" int mkhip_plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *op
     }
     DescBlob blob;
     auto &S = pl->stages;
+    Finish fin(c);
     S.push_back(make_stage(KDECOMP, blob.add(dec), (int)dec.size(), beta, rows, kmax, "decompose"));
     S.push_back(make_stage(KMAC2, blob.add(mac_fold), (int)mac_fold.size(), beta, rows, 1, "mac_fold"));
+    fin.mac2(mac_fold, L, K, N);
+    fin.emit(S, blob, "mac_fold");
     S.push_back(make_stage(KMODDOWN, blob.add(md_fold), (int)md_fold.size(), 0, 0, 1, "moddown_fold"));
     S.push_back(make_stage(KBFV_LIFT, blob.add(lift), (int)lift.size(), 0, 0, 1, "lift"));
     S.push_back(make_stage(KNTT_FWD, blob.add(ntt), (int)ntt.size(), 0, 0, 1, "tensor_ntt"));
     S.push_back(make_stage(KBFV_TENSOR, blob.add(ten), (int)ten.size(), nr, 0, 1, "tensor"));
+    for (const TensorDesc &t : ten)
+        for (int l = 0; l < nr; ++l) fin.row(t.out + (size_t)l * N, r_index(l, L, K), 1);
+    fin.emit(S, blob, "tensor");
     S.push_back(make_stage(KBFV_QUANTIZE, blob.add(quant), (int)quant.size(), 0, 0, 1, "quantize"));
     S.push_back(make_stage(KDECOMP, blob.add(dec_relin), (int)dec_relin.size(), beta, rows, 1, "decompose_relin"));
     S.push_back(make_stage(KMAC2, blob.add(mac_relin), (int)mac_relin.size(), beta, rows, 1, "mac_relin"));
+    fin.mac2(mac_relin, L, K, N);
+    fin.emit(S, blob, "mac_relin");
     S.push_back(make_stage(KMODDOWN, blob.add(md_relin), (int)md_relin.size(), 0, 0, 1, "moddown_relin"));
     if (!mat.empty())
         S.push_back(make_stage(KRESCALE_NTT, blob.add(mat), (int)mat.size(), nl, 0, 1, "materialize"));

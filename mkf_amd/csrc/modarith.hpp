@@ -106,6 +106,8 @@ struct ModConst {
     u64 ninvp;  // Shoup companion of ninv
     u64 tninv;  // t * N^-1 mod q (BFV: Quantize's MulScalar(t) folded into the INTT)
     u64 tninvp; // Shoup companion of tninv
+    u64 wn, wnp;    // psiinv[1] * N^-1 (last inverse stage of a split limb) + companion
+    u64 twn, twnp;  // t * psiinv[1] * N^-1 + companion
 };
 
 // host helpers (exact, 128-bit); host-only functions, parsed in both compilation passes

@@ -81,8 +81,10 @@ struct FwdPass {
         const int b = col / TL, r = col % TL;
         return b * R * TL + r + k * TL;
     }
-    __device__ static __forceinline__ void run(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ tw,
-                                               u64 q) {
+    // hh: twiddle multiplier -- 1 for a whole transform; 2 + h for half h of a
+    // transform of twice this size (its stage s twiddles start at 2^(s+1) + h 2^s)
+    __device__ static __forceinline__ void run(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ tw, u64 q,
+                                               int hh) {
         const u64 q4 = q << 2;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
@@ -96,7 +98,7 @@ struct FwdPass {
                 const int half = R >> (u + 1);
 #pragma unroll
                 for (int grp = 0; grp < (1 << u); ++grp) {
-                    const ulonglong2 w = twiddle<(TL % 64 == 0)>(tw, (1 << (S0 + u)) + b * (1 << u) + grp);
+                    const ulonglong2 w = twiddle<(TL % 64 == 0)>(tw, (1 << (S0 + u)) * hh + b * (1 << u) + grp);
 #pragma unroll
                     for (int kk = 0; kk < half; ++kk) {
                         const int k0 = grp * 2 * half + kk;
@@ -122,8 +124,8 @@ struct InvPass {
         const int b = col / TF, r = col % TF;
         return b * R * TF + r + k * TF;
     }
-    __device__ static __forceinline__ void run(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ itw,
-                                               u64 q) {
+    __device__ static __forceinline__ void run(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ itw, u64 q,
+                                               int hh) {
         const u64 q4 = q << 2;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
@@ -137,7 +139,7 @@ struct InvPass {
                 const int ngrp = R / (2 * half);
 #pragma unroll
                 for (int grp = 0; grp < ngrp; ++grp) {
-                    const ulonglong2 w = twiddle<(TF % 64 == 0)>(itw, m + b * ngrp + grp);
+                    const ulonglong2 w = twiddle<(TF % 64 == 0)>(itw, m * hh + b * ngrp + grp);
 #pragma unroll
                     for (int kk = 0; kk < half; ++kk) {
                         const int k0 = grp * 2 * half + kk;
@@ -202,41 +204,84 @@ struct NttPlan<13> {
     typedef InvPass<13, 9, 4> I3;
 };
 
-// Forward NTT; v in Linear layout in and out; returns canonical values.
+// Forward NTT; v in Linear layout in and out; returns canonical values.  With
+// hh = 2 + h it is the transform of half h of a limb of 2^(LOGN+1) after that
+// limb's first stage (fwd_first_stage).
 template <int LOGN>
 __device__ __forceinline__ void ntt_fwd(u64 (&v)[16], u64 *s, int tid, const ulonglong2 *__restrict__ tw,
-                                        u64 q) {
+                                        u64 q, int hh = 1) {
     typedef NttPlan<LOGN> P;
     static_assert(P::F0::TL == Linear<LOGN>::T, "first pass must be the linear layout");
-    P::F0::run(v, tid, tw, q);
+    P::F0::run(v, tid, tw, q, hh);
     lds_exchange<typename P::F0, typename P::F1>(v, s, tid);
-    P::F1::run(v, tid, tw, q);
+    P::F1::run(v, tid, tw, q, hh);
     lds_exchange<typename P::F1, typename P::F2>(v, s, tid);
-    P::F2::run(v, tid, tw, q);
+    P::F2::run(v, tid, tw, q, hh);
     lds_exchange<typename P::F2, typename P::F3>(v, s, tid);
-    P::F3::run(v, tid, tw, q);
+    P::F3::run(v, tid, tw, q, hh);
     lds_exchange<typename P::F3, Linear<LOGN> >(v, s, tid);
 #pragma unroll
     for (int i = 0; i < 16; ++i) v[i] = reduce_8q(v[i], q);
 }
 
 // Inverse NTT including the N^-1 scaling; v < 4q in Linear layout; returns
-// canonical values in Linear layout.
+// canonical values in Linear layout.  With hh = 2 + h and scale = false it runs
+// every stage of a limb of 2^(LOGN+1) that stays inside half h and leaves the
+// values in [0, 4q); inv_last_stage finishes the limb.
 template <int LOGN>
 __device__ __forceinline__ void ntt_inv(u64 (&v)[16], u64 *s, int tid, const ulonglong2 *__restrict__ itw,
-                                        u64 q, u64 ninv, u64 ninvp) {
+                                        u64 q, u64 ninv, u64 ninvp, int hh = 1, bool scale = true) {
     typedef NttPlan<LOGN> P;
     static_assert(P::I3::TF * 16 == (1 << LOGN), "last inverse pass must be the linear layout");
     lds_exchange<Linear<LOGN>, typename P::I0>(v, s, tid);
-    P::I0::run(v, tid, itw, q);
+    P::I0::run(v, tid, itw, q, hh);
     lds_exchange<typename P::I0, typename P::I1>(v, s, tid);
-    P::I1::run(v, tid, itw, q);
+    P::I1::run(v, tid, itw, q, hh);
     lds_exchange<typename P::I1, typename P::I2>(v, s, tid);
-    P::I2::run(v, tid, itw, q);
+    P::I2::run(v, tid, itw, q, hh);
     lds_exchange<typename P::I2, typename P::I3>(v, s, tid);
-    P::I3::run(v, tid, itw, q);
+    P::I3::run(v, tid, itw, q, hh);
+    if (scale) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = shoup(v[i], ninv, ninvp, q);
+        for (int i = 0; i < 16; ++i) v[i] = shoup(v[i], ninv, ninvp, q);
+    }
+}
+
+// ---- limbs of 2^15: two workgroups per limb, one per half -----------------
+// Forward: the first stage (t = N/2, twiddle tw[1]) pairs element j of the
+// lower half with element j of the upper half; half h keeps x + w y (h = 0) or
+// x - w y (h = 1), after which each half is a transform of N/2 (hh = 2 + h).
+// lo / hi: this lane's elements of the two halves (values < 8q); out in [0, 8q).
+__device__ __forceinline__ void fwd_first_stage(u64 (&v)[16], const u64 (&lo)[16], const u64 (&hi)[16],
+                                                const ulonglong2 *__restrict__ tw, u64 q, int h) {
+    const ulonglong2 w = twiddle<true>(tw, 1);
+    const u64 q4 = q << 2;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        u64 x = lo[i], y = hi[i];
+        fwd_bfly(x, y, w.x, w.y, q4, q);
+        v[i] = h ? y : x;
+    }
+}
+
+// The same stage keeping both results: lo <- x + w y, hi <- x - w y.
+__device__ __forceinline__ void fwd_first_stage_both(u64 (&lo)[16], u64 (&hi)[16], const ulonglong2 *__restrict__ tw,
+                                                     u64 q) {
+    const ulonglong2 w = twiddle<true>(tw, 1);
+    const u64 q4 = q << 2;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) fwd_bfly(lo[i], hi[i], w.x, w.y, q4, q);
+}
+
+// Inverse: after each half ran its N/2 stages (values in [0, 4q)), the last stage
+// (t = N/2, twiddle itw[1]) and the N^-1 scaling: x' = (x + y) N^-1,
+// y' = (x - y) (w N^-1), canonical.  wn = w N^-1 mod q with its Shoup companion.
+__device__ __forceinline__ void inv_last_stage(u64 &x, u64 &y, u64 wn, u64 wnp, u64 ninv, u64 ninvp, u64 q) {
+    const u64 q4 = q << 2;
+    const u64 sum = x + y;
+    const u64 dif = x + q4 - y;
+    x = shoup(sum, ninv, ninvp, q);
+    y = shoup(dif, wn, wnp, q);
 }
 
 }  // namespace mkhip

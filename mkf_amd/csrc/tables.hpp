@@ -72,6 +72,14 @@ enum KernelId {
     KBFV_LIFT = 14,      // k_bfv_lift             ModUpQtoR / Rescale into R
     KBFV_TENSOR = 15,    // k_tensor<LOGN, true>   tensor over R, t x, INTT
     KBFV_QUANTIZE = 16,  // k_quantize             ModDown R -> Q (+ accumulate)
+    KINTT_FINISH = 17,   // k_intt_finish          last inverse stage of split (2^15) limbs
+};
+
+// One limb of 2^15 whose halves ran every inverse stage but the last: apply it
+// in place with the N^-1 (tscale: t N^-1) scaling.
+struct FinishDesc {
+    u64 *row;
+    int mod, tscale;
 };
 
 // ---- BFV descriptors ----------------------------------------------------------

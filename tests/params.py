@@ -26,6 +26,8 @@ def config(name):
         return 14, primes(50, 8), primes(60, 2)
     if name == "small":  # 4 Q + 2 P at N=2^13 (KKLSS-compatible variant of c1)
         return 13, primes(50, 4), primes(60, 2)
+    if name == "n15":  # 3 Q + 2 P at N=2^15 (limbs split over two workgroups)
+        return 15, primes(50, 3), primes(60, 2)
     if name == "P1":
         return 14, [0x7ffffffffe70001, 0xffffffff00001, 0xfffffffe40001, 0xfffffffe20001,
                     0xfffffffbe0001, 0xfffffffa60001], [0xffffffffffc0001, 0xfffffffff840001]
@@ -53,4 +55,7 @@ def bfv_config(name):
     if name == "bsmall":  # 3 Q + 3 QMul + 2 P at N=2^13
         Q = primes(50, 3)
         return 13, Q, primes(50, 3, exclude=Q), primes(60, 2), 65537
+    if name == "b15":  # 2 Q + 2 QMul + 2 P at N=2^15
+        Q = primes(50, 2)
+        return 15, Q, primes(50, 2, exclude=Q), primes(60, 2), 65537
     raise KeyError(name)

@@ -13,7 +13,7 @@ from params import bfv_config
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module", params=["bsmall", "b3"])
+@pytest.fixture(scope="module", params=["bsmall", "b3", "b15"])
 def setup(request):
     import mkf_amd
 
@@ -55,7 +55,7 @@ def test_quantize_and_decompose_match_oracle(setup):
 def test_bfv_mulrelin_parity(setup, shape):
     name, ctx, ref = setup
     ids0, ids1 = shape
-    if name == "b3" and shape[0] != [0, 1, 2, 3]:
+    if name in ("b3", "b15") and shape[0] != [0, 1, 2, 3]:
         pytest.skip("party-set variants run at the small shape")
     b, d, v, u = _keys(ref, 4, seed=33)
     for pid in range(4):
@@ -132,7 +132,7 @@ def test_bfv_gpu_mulrelin_decrypts_exactly():
 
 # ---------------------------------------------------------------- MKF MK-BFV
 
-@pytest.fixture(scope="module", params=["bsmall", "b3"])
+@pytest.fixture(scope="module", params=["bsmall", "b3", "b15"])
 def mkf_setup(request):
     import mkf_amd
 

@@ -20,7 +20,7 @@ def _mk():
     return mkf_amd
 
 
-@pytest.fixture(scope="module", params=["small", "c2"])
+@pytest.fixture(scope="module", params=["small", "c2", "n15"])
 def setup(request):
     mk = _mk()
     logN, Q, P = config(request.param)
@@ -112,7 +112,7 @@ def _oracle_mulrelin(ref, op0, ids0, op1, ids1, keys, u, nres):
 @pytest.mark.parametrize("ids0,ids1", [([0, 1], [0, 1]), ([0, 2], [1, 2]), ([3], [0, 1, 2])])
 def test_kklss_mulrelin_parity(setup, ids0, ids1):
     name, ctx, ref = setup
-    if name == "c2" and ids0 != [0, 1]:
+    if name in ("c2", "n15") and ids0 != [0, 1]:
         pytest.skip("party-set variants are covered at the small size")
     keys, u = _keys(ctx, ref, [0, 1, 2, 3], seed=21)
     g = O.Prng(22)

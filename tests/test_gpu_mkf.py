@@ -11,7 +11,7 @@ from params import config
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module", params=["c1", "c2"])
+@pytest.fixture(scope="module", params=["c1", "c2", "n15"])
 def setup(request):
     import mkf_amd
 
