@@ -249,6 +249,100 @@ def run_scheme(scheme, args, rng):
     return res, extra
 
 
+# uneven batch: (logN, L, K, scheme) per ring degree; K = 1 only admits the MKF
+# construction (KKLSS needs alpha = floor(K/2) >= 1)
+UNEVEN_SHAPES = {13: (4, 1, "mkf"), 14: (8, 2, "kklss"), 15: (16, 3, "kklss")}
+UNEVEN_K = (2, 4, 8, 16)
+
+
+def run_uneven(args, rng):
+    """256 MK-CKKS mul+relin ops, k uniform in {2,4,8,16}, N uniform in
+    {2^13,2^14,2^15} (4/8/16 Q primes, 1/2/3 P primes), one graph launch per step."""
+    import mkf_amd
+
+    nops = args.uneven_ops
+    draws = [(int(rng.choice(list(UNEVEN_SHAPES))), int(rng.choice(UNEVEN_K))) for _ in range(nops)]
+    nparty = max(UNEVEN_K)
+    ctxs, bufs, branches = {}, [], []
+    t_setup = time.perf_counter()
+    for logN, (L, K, scheme) in UNEVEN_SHAPES.items():
+        sel = [k for (n, k) in draws if n == logN]
+        if not sel:
+            continue
+        Q, P, N = ntt_primes(50, L), ntt_primes(60, K), 1 << logN
+        QP = Q + P
+        ctx = mkf_amd.Context(logN, Q, P, gamma=2 if scheme == "kklss" else 1)
+        ctxs[logN] = ctx
+        beta = ctx.beta
+        if scheme == "kklss":
+            for pid in range(nparty):
+                ctx.register_party(pid, *[uniform_stack(rng, QP, N, beta) for _ in range(3)])
+            ctx.set_crs_u(uniform_stack(rng, QP, N, beta))
+        else:
+            def swk():
+                return np.stack([np.stack([uniform(rng, QP, N), uniform(rng, QP, N)]) for _ in range(beta)])
+            for k in UNEVEN_K:
+                ctx.register_mkf_group(k, [swk() for _ in range(k)], swk())
+        # operand pool: 2 (1 + kmax) uniform polys; an op reads 1 + k consecutive ones per side
+        npool = 2 * (1 + nparty)
+        pool = ctx.alloc(npool * L * N * 8)
+        ctx.h2d(pool, np.concatenate([uniform(rng, Q, N) for _ in range(npool)]))
+        bufs.append((ctx, pool))
+        scale = float(Q[-1])
+        nres, _ = ctx.rescale_count(L - 1, scale * scale, scale)
+        ops = []
+        for k in sel:
+            s0, s1 = (int(x) for x in rng.integers(0, npool - k, size=2))
+            out = ctx.alloc((1 + k) * (L - nres) * N * 8)
+            bufs.append((ctx, out))
+            a, b = pool + s0 * L * N * 8, pool + s1 * L * N * 8
+            if scheme == "kklss":
+                ids = sorted(int(x) for x in rng.choice(nparty, size=k, replace=False))
+                ops.append(dict(op0=a, op1=b, out=out, ids0=ids, ids1=ids, level=L - 1, nres=nres))
+            else:
+                ops.append(dict(left=a, right=b, out=out, group=k, level=L - 1, nres=nres))
+        tune = []
+        if args.autotune:
+            tune = ctx.autotune(ops[:args.uneven_tune_ops], scheme=scheme, iters=2)
+        branches.append((logN, scheme, ctx, ops, tune))
+    setup_s = time.perf_counter() - t_setup
+    batch = mkf_amd.Batch()
+    info = {}
+    budget = int(args.uneven_budget_gb * 1e9)
+    for logN, scheme, ctx, ops, tune in branches:
+        nch = batch.add(ctx, ops, scheme, ws_budget=budget)
+        ks = [len(o["ids0"]) if scheme == "kklss" else o["group"] for o in ops]
+        info["N=2^%d" % logN] = {"scheme": scheme + "_ckks", "L": UNEVEN_SHAPES[logN][0],
+                                 "K": UNEVEN_SHAPES[logN][1], "ops": len(ops),
+                                 "k_counts": {str(k): ks.count(k) for k in UNEVEN_K}, "chunks": nch,
+                                 "autotune_ms": [round(t, 3) for t in tune],
+                                 "tuning": {"fused_decomp": ctx.get_tuning(1), "xcd_remap": ctx.get_tuning(2),
+                                            "decomp_order": ctx.get_tuning(3)}}
+    batch.finalize()
+    for _ in range(args.warmup):
+        batch.submit()
+    batch.wait()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.uneven_steps):
+        s = time.perf_counter()
+        batch.submit()
+        batch.wait()
+        lat.append(time.perf_counter() - s)
+    el = time.perf_counter() - t0
+    res = {"ops": nops, "ops_per_s": round(nops * args.uneven_steps / el, 2),
+           "ms_per_launch": round(el / args.uneven_steps * 1e3, 3), "p50_ms": round(pct(lat, 50) * 1e3, 3),
+           "p99_ms": round(pct(lat, 99) * 1e3, 3), "graph_launches_per_step": 1,
+           "workspace_gb": round(batch.workspace_bytes / 1e9, 3), "ws_budget_gb_per_branch": args.uneven_budget_gb,
+           "branches": info, "setup_s": round(setup_s, 1)}
+    batch.close()
+    for ctx, p in bufs:
+        ctx.free(p)
+    for ctx in ctxs.values():
+        ctx.close()
+    return res
+
+
 def cpu_baseline(logN, Q, P, k, keys, u, budget_s):
     """The CPU oracle (C restatement of the reference path, 1 thread) on a bounded sample."""
     import oracle as O
@@ -316,6 +410,13 @@ def main():
     ap.add_argument("--schemes", default=",".join(ALL_SCHEMES))
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the library's autotuner (plans use the default knobs)")
+    ap.add_argument("--uneven-ops", type=int, default=256,
+                    help="ops of the uneven batch (k in {2,4,8,16}, N in {2^13,2^14,2^15}); 0 skips it")
+    ap.add_argument("--uneven-steps", type=int, default=5)
+    ap.add_argument("--uneven-budget-gb", type=float, default=24.0,
+                    help="workspace budget of each ring degree's branch of the uneven batch")
+    ap.add_argument("--uneven-tune-ops", type=int, default=4,
+                    help="ops of each branch the autotuner times")
     ap.add_argument("--copy-calib-mb", type=float, default=0.0,
                     help="run a plain D2D copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -373,6 +474,9 @@ def main():
         plan.close()
         w.close()
     line["schemes"] = {s: r for s, r in results.items() if s != "kklss_ckks"}
+    if args.uneven_ops > 0:
+        line["uneven_batch"] = run_uneven(args, rng)
+        print("# uneven batch: %.1f ops/s" % line["uneven_batch"]["ops_per_s"], file=sys.stderr, flush=True)
     if calib:
         line["copy_calibration"] = calib
     print(json.dumps(line), flush=True)

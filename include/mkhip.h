@@ -229,6 +229,90 @@ int mkhip_set_tuning(mkhip_ctx *ctx, int knob, int value);
 int mkhip_get_tuning(const mkhip_ctx *ctx, int knob, int *value);
 int mkhip_autotune(mkhip_ctx *ctx, int scheme, int nops, const void *ops, int iters, int *ncand, double *ms);
 
+/*
+ * Uneven batches in one hipGraph launch (no counterpart in the reference, which
+ * runs one MulRelinNew at a time; this is the batch_submit / batch_wait surface a
+ * Go caller uses to hand over many ops at once).  Each mkhip_batch_add gives the
+ * ops of one context -- any mix of party counts, device-resident operands, one
+ * scheme (MKHIP_SCHEME_*, ops of the matching struct type) -- as one branch.  Its
+ * ops are cut in order into chunks whose plan workspace fits ws_budget bytes
+ * (0: no limit; a chunk holds at least one op); the chunks of a branch run in
+ * sequence in one shared workspace, the branches run concurrently.
+ * mkhip_batch_finalize runs the batch once eagerly and captures it as one graph;
+ * mkhip_batch_submit launches that graph (asynchronous), mkhip_batch_wait blocks
+ * until it completed.  Contexts used by a batch must outlive it.
+ */
+typedef struct mkhip_batch mkhip_batch;
+int mkhip_batch_create(mkhip_batch **out);
+int mkhip_batch_add(mkhip_batch *batch, mkhip_ctx *ctx, int scheme, int nops, const void *ops, size_t ws_budget,
+                    int *nchunks);
+int mkhip_batch_finalize(mkhip_batch *batch);
+int mkhip_batch_submit(mkhip_batch *batch);
+int mkhip_batch_wait(mkhip_batch *batch);
+/* average device time of `iters` back-to-back graph launches (HIP events), ms */
+int mkhip_batch_time(mkhip_batch *batch, int iters, double *ms_per_launch);
+/* device bytes held: workspace arenas + descriptor tables */
+size_t mkhip_batch_workspace_bytes(const mkhip_batch *batch);
+int mkhip_batch_branch_info(const mkhip_batch *batch, int branch, int *nchunks, size_t *arena_bytes);
+void mkhip_batch_destroy(mkhip_batch *batch);
+
+/*
+ * Serving (no counterpart in the reference: the enqueue / poll surface of a
+ * request loop).  A server owns `streams` HIP streams and a device pool of
+ * mem_bytes; each class is one context (its parties' keys resident in HBM) and
+ * one scheme.  A scheduler thread batches waiting requests continuously: when a
+ * stream is free, the class with the oldest waiting request gets a batch of its
+ * waiting requests (oldest first, at most max_batch) whose device block --
+ * operands, outputs, workspace, descriptors -- fits the largest free extent of
+ * the pool; the request waits otherwise (admission by free device memory).
+ * Batches on different streams overlap on the device.  A request that cannot
+ * fit an empty pool fails with MKHIP_ERR_OOM.
+ *
+ * Requests carry host operands at the top level (L limbs per component) in the
+ * layouts of the single-op entry points; out receives the result ((1 + ku) x
+ * (L - nres) x N for KKLSS, ku = |ids0 u ids1| in ascending id order; (1 + k) x
+ * (L - nres) x N for MKF).  in0, in1 and out must stay valid until the request
+ * completed; pinned buffers (mkhip_host_alloc) make the copies asynchronous.
+ * Contexts attached to a server must not be used by other threads while it runs.
+ */
+typedef struct mkhip_server mkhip_server;
+typedef struct {
+    int streams;      /* batches in flight at once (>= 1)                        */
+    int max_batch;    /* ops per batch at most (>= 1)                            */
+    size_t mem_bytes; /* device pool for operands, outputs, workspace, descriptors */
+} mkhip_server_config;
+typedef struct {
+    int cls;                                         /* from mkhip_server_add_class */
+    const uint64_t *in0, *in1;                       /* host operands               */
+    uint64_t *out;                                   /* host result                 */
+    int ids0[MKHIP_MAX_PARTIES], ids1[MKHIP_MAX_PARTIES];
+    int k0, k1;                                      /* KKLSS party ids per operand */
+    int group;                                       /* MKF party group             */
+    int nres;                                        /* CKKS rescales; 0 for BFV    */
+} mkhip_request;
+typedef struct {
+    uint64_t completed, batches, max_batch;
+    double mean_batch;
+    int max_inflight;
+    size_t pool_bytes, pool_free_largest;
+} mkhip_server_stats_t;
+int mkhip_server_create(const mkhip_server_config *cfg, mkhip_server **out);
+int mkhip_server_add_class(mkhip_server *server, mkhip_ctx *ctx, int scheme, int *cls);
+int mkhip_server_start(mkhip_server *server);
+int mkhip_server_enqueue(mkhip_server *server, const mkhip_request *req, uint64_t *ticket);
+/* done = 1 once the request finished; status, latency (enqueue -> results in
+ * host memory) and queueing time (enqueue -> batch launch) are then reported
+ * and the ticket is released */
+int mkhip_server_poll(mkhip_server *server, uint64_t ticket, int *done, int *status, double *latency_ms,
+                      double *queue_ms);
+int mkhip_server_wait(mkhip_server *server, uint64_t ticket, int *status, double *latency_ms, double *queue_ms);
+int mkhip_server_stats(mkhip_server *server, mkhip_server_stats_t *stats);
+/* stops the scheduler after the batches in flight completed */
+void mkhip_server_destroy(mkhip_server *server);
+/* pinned host memory */
+int mkhip_host_alloc(size_t bytes, void **ptr);
+int mkhip_host_free(void *ptr);
+
 /* reference scale loop (mkckks/evaluator.go:383-388): number of rescales */
 int mkhip_rescale_count(const mkhip_ctx *ctx, int level, double scale, double min_scale, int *nres,
                         double *new_scale);

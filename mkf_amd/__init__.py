@@ -37,6 +37,11 @@ EXPORTS = [
     "mkhip_register_bfv_party", "mkhip_kklss_bfv_mulrelin", "mkhip_plan_kklss_bfv", "mkhip_bfv_modup_q_to_r",
     "mkhip_bfv_rescale_q_to_r", "mkhip_bfv_quantize", "mkhip_bfv_decompose", "mkhip_mkf_bfv_mulrelin",
     "mkhip_plan_mkf_bfv", "mkhip_set_tuning", "mkhip_get_tuning", "mkhip_autotune", "mkhip_d2d",
+    "mkhip_batch_create", "mkhip_batch_add", "mkhip_batch_finalize", "mkhip_batch_submit", "mkhip_batch_wait",
+    "mkhip_batch_time", "mkhip_batch_workspace_bytes", "mkhip_batch_branch_info", "mkhip_batch_destroy",
+    "mkhip_server_create", "mkhip_server_add_class", "mkhip_server_start", "mkhip_server_enqueue",
+    "mkhip_server_poll", "mkhip_server_wait", "mkhip_server_stats", "mkhip_server_destroy", "mkhip_host_alloc",
+    "mkhip_host_free",
 ]
 TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER = 1, 2, 3
 SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3}
@@ -68,6 +73,26 @@ class MkhipError(RuntimeError):
         self.status = status
         msg = lib().mkhip_status_string(status).decode()
         super().__init__("%s: %s (%s)" % (what, STATUS.get(status, status), msg))
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("streams", ctypes.c_int), ("max_batch", ctypes.c_int), ("mem_bytes", ctypes.c_size_t)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("cls", ctypes.c_int), ("in0", ctypes.c_void_p), ("in1", ctypes.c_void_p), ("out", ctypes.c_void_p),
+        ("ids0", ctypes.c_int * MAX_PARTIES), ("ids1", ctypes.c_int * MAX_PARTIES),
+        ("k0", ctypes.c_int), ("k1", ctypes.c_int), ("group", ctypes.c_int), ("nres", ctypes.c_int),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("max_batch", ctypes.c_uint64),
+        ("mean_batch", ctypes.c_double), ("max_inflight", ctypes.c_int), ("pool_bytes", ctypes.c_size_t),
+        ("pool_free_largest", ctypes.c_size_t),
+    ]
 
 
 class KKLSSOp(ctypes.Structure):
@@ -148,6 +173,27 @@ def _declare(L):
     L.mkhip_get_tuning.argtypes = [V, I, intp]
     L.mkhip_autotune.argtypes = [V, I, I, V, I, intp, dblp]
     L.mkhip_plan_mkf_bfv.argtypes = [V, I, ctypes.POINTER(MKFOp), I, ctypes.POINTER(V)]
+    L.mkhip_batch_create.argtypes = [ctypes.POINTER(V)]
+    L.mkhip_batch_add.argtypes = [V, V, I, I, V, S, intp]
+    for f in ("mkhip_batch_finalize", "mkhip_batch_submit", "mkhip_batch_wait"):
+        getattr(L, f).argtypes = [V]
+    L.mkhip_batch_time.argtypes = [V, I, dblp]
+    L.mkhip_batch_workspace_bytes.argtypes = [V]
+    L.mkhip_batch_workspace_bytes.restype = S
+    L.mkhip_batch_branch_info.argtypes = [V, I, intp, ctypes.POINTER(S)]
+    L.mkhip_batch_destroy.argtypes = [V]
+    L.mkhip_batch_destroy.restype = None
+    L.mkhip_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(V)]
+    L.mkhip_server_add_class.argtypes = [V, V, I, intp]
+    L.mkhip_server_start.argtypes = [V]
+    L.mkhip_server_enqueue.argtypes = [V, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
+    L.mkhip_server_poll.argtypes = [V, ctypes.c_uint64, intp, intp, dblp, dblp]
+    L.mkhip_server_wait.argtypes = [V, ctypes.c_uint64, intp, dblp, dblp]
+    L.mkhip_server_stats.argtypes = [V, ctypes.POINTER(ServerStats)]
+    L.mkhip_server_destroy.argtypes = [V]
+    L.mkhip_server_destroy.restype = None
+    L.mkhip_host_alloc.argtypes = [S, ctypes.POINTER(V)]
+    L.mkhip_host_free.argtypes = [V]
 
 
 def _p(a):
@@ -469,6 +515,163 @@ class Plan:
     def close(self):
         if getattr(self, "_h", None):
             lib().mkhip_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """An uneven batch run by one hipGraph launch (mkhip_batch).
+
+    add(ctx, ops, scheme, ws_budget) appends the ops of one context as a branch
+    (any mix of party counts; ops as for Plan); the branch's ops are cut into
+    chunks whose workspace fits ws_budget bytes and run in sequence in one
+    arena, the branches run concurrently.  finalize() captures the graph."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().mkhip_batch_create(ctypes.byref(h)), "mkhip_batch_create")
+        self._h = h
+        self._keep = []  # contexts must outlive the batch
+
+    def add(self, ctx, ops, scheme="kklss", ws_budget=0):
+        arr = _marshal_ops(scheme, ops)
+        n = ctypes.c_int()
+        check(lib().mkhip_batch_add(self._h, ctx.handle, SCHEME_IDS[scheme], len(ops),
+                                    ctypes.cast(arr, ctypes.c_void_p), int(ws_budget), ctypes.byref(n)),
+              "mkhip_batch_add")
+        self._keep.append(ctx)
+        return n.value
+
+    def finalize(self):
+        check(lib().mkhip_batch_finalize(self._h), "mkhip_batch_finalize")
+
+    def submit(self):
+        check(lib().mkhip_batch_submit(self._h), "mkhip_batch_submit")
+
+    def wait(self):
+        check(lib().mkhip_batch_wait(self._h), "mkhip_batch_wait")
+
+    def time(self, iters):
+        ms = ctypes.c_double()
+        check(lib().mkhip_batch_time(self._h, iters, ctypes.byref(ms)), "mkhip_batch_time")
+        return ms.value
+
+    @property
+    def workspace_bytes(self):
+        return lib().mkhip_batch_workspace_bytes(self._h)
+
+    def branch_info(self, i):
+        n, b = ctypes.c_int(), ctypes.c_size_t()
+        check(lib().mkhip_batch_branch_info(self._h, i, ctypes.byref(n), ctypes.byref(b)), "mkhip_batch_branch_info")
+        return n.value, b.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mkhip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """A uint64 numpy array over pinned host memory (mkhip_host_alloc)."""
+
+    def __init__(self, shape):
+        n = int(np.prod(shape))
+        p = ctypes.c_void_p()
+        check(lib().mkhip_host_alloc(max(n, 1) * 8, ctypes.byref(p)), "mkhip_host_alloc")
+        self._p = p
+        self.a = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint64)), shape=(max(n, 1),))[:n]
+        self.a = self.a.reshape(shape)
+
+    @property
+    def ptr(self):
+        return self._p.value
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self.a = None
+            lib().mkhip_host_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching (mkhip_server).
+
+    add_class(ctx, scheme) registers a context (keys resident) as a request class;
+    start() launches the scheduler thread; enqueue() takes host operands (numpy
+    uint64 arrays that must stay alive until the request completed) and returns a
+    ticket; poll()/wait() report (status, latency_ms, queue_ms)."""
+
+    def __init__(self, streams=4, max_batch=64, mem_bytes=8 << 30):
+        cfg = ServerConfig(streams, max_batch, int(mem_bytes))
+        h = ctypes.c_void_p()
+        check(lib().mkhip_server_create(ctypes.byref(cfg), ctypes.byref(h)), "mkhip_server_create")
+        self._h = h
+        self._keep = []
+        self._schemes = []
+
+    def add_class(self, ctx, scheme="kklss"):
+        c = ctypes.c_int()
+        check(lib().mkhip_server_add_class(self._h, ctx.handle, SCHEME_IDS[scheme], ctypes.byref(c)),
+              "mkhip_server_add_class")
+        self._keep.append(ctx)
+        self._schemes.append(scheme)
+        return c.value
+
+    def start(self):
+        check(lib().mkhip_server_start(self._h), "mkhip_server_start")
+
+    def enqueue(self, cls, in0, in1, out, ids0=(), ids1=(), group=0, nres=0):
+        for a in (in0, in1, out):
+            if a.dtype != np.uint64 or not a.flags["C_CONTIGUOUS"]:
+                raise TypeError("expected C-contiguous uint64 arrays")
+        r = Request()
+        r.cls, r.in0, r.in1, r.out = cls, in0.ctypes.data, in1.ctypes.data, out.ctypes.data
+        for j, v in enumerate(ids0):
+            r.ids0[j] = v
+        for j, v in enumerate(ids1):
+            r.ids1[j] = v
+        r.k0, r.k1, r.group, r.nres = len(ids0), len(ids1), group, nres
+        t = ctypes.c_uint64()
+        check(lib().mkhip_server_enqueue(self._h, ctypes.byref(r), ctypes.byref(t)), "mkhip_server_enqueue")
+        return t.value
+
+    def poll(self, ticket):
+        d, st, lat, qd = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        check(lib().mkhip_server_poll(self._h, ticket, ctypes.byref(d), ctypes.byref(st), ctypes.byref(lat),
+                                      ctypes.byref(qd)), "mkhip_server_poll")
+        return (st.value, lat.value, qd.value) if d.value else None
+
+    def wait(self, ticket):
+        st, lat, qd = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        check(lib().mkhip_server_wait(self._h, ticket, ctypes.byref(st), ctypes.byref(lat), ctypes.byref(qd)),
+              "mkhip_server_wait")
+        return st.value, lat.value, qd.value
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().mkhip_server_stats(self._h, ctypes.byref(s)), "mkhip_server_stats")
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mkhip_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -37,8 +37,8 @@ int find_id(const int *ids, int k, int id) {
 
 }  // namespace
 
-extern "C" int mkhip_plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, int use_graph,
-                                    mkhip_plan **out) {
+int mkhip::plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, int use_graph, mkhip_plan **out,
+                       const PlanMem &pm) {
     if (!c || !ops || nops <= 0 || !out) return MKHIP_ERR_ARG;
     *out = nullptr;
     if (!c->bfv || c->gamma != 2 || c->alpha != 1) return MKHIP_ERR_SCHEME;
@@ -98,7 +98,7 @@ extern "C" int mkhip_plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op
         w.accu = cv.take(op.k0 * qp);
     }
     mkhip_plan *pl = nullptr;
-    int st = plan_create(c, cv.off, &pl);
+    int st = plan_create(c, cv.off, &pl, pm);
     if (st != MKHIP_OK) return st;
     char *B = (char *)pl->d_mem;
     auto P = [&](size_t off) { return (u64 *)(B + off); };
@@ -295,6 +295,11 @@ extern "C" int mkhip_plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op
     if (st != MKHIP_OK) return st;
     *out = pl;
     return MKHIP_OK;
+}
+
+extern "C" int mkhip_plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, int use_graph,
+                                     mkhip_plan **out) {
+    return plan_kklss_bfv(c, nops, ops, use_graph, out, PlanMem());
 }
 
 // ---------------------------------------------------------------------------

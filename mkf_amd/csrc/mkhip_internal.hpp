@@ -78,11 +78,31 @@ struct mkhip_ctx {
     size_t single_io_words = 0;
 };
 
+namespace mkhip {
+// Where a plan's device memory comes from.  By default the plan allocates its
+// workspace and descriptor tables.  With ws / desc set it uses the caller's
+// buffers (which it never frees; desc is filled by an asynchronous copy on
+// `stream` from a host copy the plan keeps).  With query set nothing is allocated
+// or built: the builder reports the sizes it needs and returns no plan.
+struct PlanMem {
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    void *desc = nullptr;
+    size_t desc_bytes = 0;
+    hipStream_t stream = nullptr;
+    bool query = false;
+    size_t *ws_need = nullptr, *desc_need = nullptr;
+};
+}  // namespace mkhip
+
 struct mkhip_plan {
     mkhip_ctx *ctx = nullptr;
     void *d_mem = nullptr;   // workspace
     void *d_desc = nullptr;  // descriptor tables
     size_t bytes = 0;        // workspace + descriptor bytes
+    size_t desc_bytes = 0;   // descriptor bytes
+    mkhip::PlanMem mem;      // external buffers / size query
+    std::vector<char> host_desc;  // source of the asynchronous upload into mem.desc
     std::vector<PlanStage> stages;
     bool graph = false;
     hipGraph_t g = nullptr;
@@ -116,14 +136,29 @@ struct DescBlob {
 
 // mkhip.hip: last stage + scaling of split inverse transforms (see Finish below)
 int finish_rows(mkhip_ctx *c, u64 *base, const int *mods, int nrows, int tscale);
-hipError_t plan_stage(mkhip_plan *pl, int s);
-int plan_launch_raw(mkhip_plan *pl);
-// New plan with its workspace allocated (ws_bytes); *out is null on failure.
-int plan_create(mkhip_ctx *c, size_t ws_bytes, mkhip_plan **out);
+// Launch stage s (on `st`, default the context stream) / every stage.
+hipError_t plan_stage(mkhip_plan *pl, int s, hipStream_t st = nullptr);
+int plan_launch_raw(mkhip_plan *pl, hipStream_t st = nullptr);
+// New plan whose workspace (ws_bytes) is allocated or taken from pm; *out is null
+// on failure.
+int plan_create(mkhip_ctx *c, size_t ws_bytes, mkhip_plan **out, const PlanMem &pm = PlanMem());
 // Upload the descriptor tables, turn each stage's blob offset (stage.descs holds
 // the offset as an integer on entry) into a device pointer, and optionally
-// capture the whole launch sequence into a hipGraph.  Destroys the plan on failure.
-int plan_finish(mkhip_plan *pl, const DescBlob &blob, int use_graph);
+// capture the whole launch sequence into a hipGraph.  Destroys the plan on
+// failure, and after reporting the sizes of a query (pl is then null).
+int plan_finish(mkhip_plan *&pl, const DescBlob &blob, int use_graph);
+// Plan builders (the C entry points call them with a default PlanMem).
+int plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, int use_graph, mkhip_plan **out,
+                    const PlanMem &pm);
+int plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, int use_graph, mkhip_plan **out,
+                   const PlanMem &pm);
+int plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out,
+                  const PlanMem &pm);
+int plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out,
+                 const PlanMem &pm);
+// Builder of a scheme (MKHIP_SCHEME_*) over ops of the matching struct type.
+int plan_build(mkhip_ctx *c, int scheme, int nops, const void *ops, int use_graph, mkhip_plan **out,
+               const PlanMem &pm);
 inline PlanStage make_stage(int what, size_t blob_off, int n, int a, int b, int c, const char *name) {
     PlanStage s;
     s.what = what;

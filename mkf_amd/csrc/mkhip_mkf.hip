@@ -63,8 +63,8 @@ extern "C" int mkhip_unregister_mkf_group(mkhip_ctx *c, int group) {
     return MKHIP_OK;
 }
 
-extern "C" int mkhip_plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph,
-                                   mkhip_plan **out) {
+int mkhip::plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out,
+                       const PlanMem &pm) {
     if (!c || !ops || nops <= 0 || !out) return MKHIP_ERR_ARG;
     *out = nullptr;
     if (c->alpha == 0) return MKHIP_ERR_GADGET;
@@ -106,7 +106,7 @@ extern "C" int mkhip_plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *o
         w.last = cv.take(2 * limb);
     }
     mkhip_plan *pl = nullptr;
-    int st = plan_create(c, cv.off, &pl);
+    int st = plan_create(c, cv.off, &pl, pm);
     if (st != MKHIP_OK) return st;
     char *B = (char *)pl->d_mem;
     auto P = [&](size_t off) { return (u64 *)(B + off); };
@@ -255,6 +255,11 @@ extern "C" int mkhip_plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *o
     return MKHIP_OK;
 }
 
+extern "C" int mkhip_plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph,
+                                     mkhip_plan **out) {
+    return plan_mkf_ckks(c, nops, ops, use_graph, out, PlanMem());
+}
+
 // MKF CKKSCase.Run on host buffers: left/right [1+k][level+1][N] NTT domain;
 // out [1+k][level_out+1][N]: c0 then k copies of Gamma.  The rescale error of
 // Lattigo's Evaluator.Rescale is returned (ckks.go:141-149).
@@ -326,8 +331,8 @@ extern "C" int mkhip_mkf_ckks_mulrelin(mkhip_ctx *c, int group, const uint64_t *
 //   decompose_relin, mac_relin, moddown_relin  switch(p2) added to p0, p1 (:233-237)
 //   materialize     Gamma copied to the k components (common.go:128-139)
 
-extern "C" int mkhip_plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph,
-                                  mkhip_plan **out) {
+int mkhip::plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out,
+                       const PlanMem &pm) {
     if (!c || !ops || nops <= 0 || !out) return MKHIP_ERR_ARG;
     *out = nullptr;
     if (!c->bfv || c->gamma != 1) return MKHIP_ERR_SCHEME;
@@ -367,7 +372,7 @@ extern "C" int mkhip_plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *op
         w.racc = cv.take(2 * qp);
     }
     mkhip_plan *pl = nullptr;
-    int st = plan_create(c, cv.off, &pl);
+    int st = plan_create(c, cv.off, &pl, pm);
     if (st != MKHIP_OK) return st;
     char *B = (char *)pl->d_mem;
     auto P = [&](size_t off) { return (u64 *)(B + off); };
@@ -507,6 +512,11 @@ extern "C" int mkhip_plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *op
     if (st != MKHIP_OK) return st;
     *out = pl;
     return MKHIP_OK;
+}
+
+extern "C" int mkhip_plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph,
+                                     mkhip_plan **out) {
+    return plan_mkf_bfv(c, nops, ops, use_graph, out, PlanMem());
 }
 
 // MKF BFVCase.Run on host buffers: left/right [1+k][L][N] coefficient domain;

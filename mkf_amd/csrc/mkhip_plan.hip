@@ -8,44 +8,78 @@ using namespace mkhip;
 
 namespace mkhip {
 
-hipError_t plan_stage(mkhip_plan *pl, int s) {
+hipError_t plan_stage(mkhip_plan *pl, int s, hipStream_t st) {
     const PlanStage &S = pl->stages[s];
     mkhip_ctx *c = pl->ctx;
-    if (S.what >= KMAC2 && S.what <= KRESCALE_NTT) return launch_mkf(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream);
-    if (S.what >= KBFV_LIFT && S.what <= KBFV_QUANTIZE)
-        return launch_bfv(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream);
-    if (S.what == KDECOMP_MAC) return launch_fused(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream);
-    return launch_kernel(S.what, c->T, S.descs, S.n, S.a, S.b, c->stream, S.c);
+    if (!st) st = c->stream;
+    if (S.what >= KMAC2 && S.what <= KRESCALE_NTT) return launch_mkf(S.what, c->T, S.descs, S.n, S.a, S.b, st);
+    if (S.what >= KBFV_LIFT && S.what <= KBFV_QUANTIZE) return launch_bfv(S.what, c->T, S.descs, S.n, S.a, S.b, st);
+    if (S.what == KDECOMP_MAC) return launch_fused(S.what, c->T, S.descs, S.n, S.a, S.b, st);
+    return launch_kernel(S.what, c->T, S.descs, S.n, S.a, S.b, st, S.c);
 }
 
-int plan_launch_raw(mkhip_plan *pl) {
-    for (int s = 0; s < (int)pl->stages.size(); ++s) HIP_TRY(plan_stage(pl, s));
+int plan_launch_raw(mkhip_plan *pl, hipStream_t st) {
+    for (int s = 0; s < (int)pl->stages.size(); ++s) HIP_TRY(plan_stage(pl, s, st));
     return MKHIP_OK;
 }
 
-int plan_create(mkhip_ctx *c, size_t ws_bytes, mkhip_plan **out) {
+int plan_create(mkhip_ctx *c, size_t ws_bytes, mkhip_plan **out, const PlanMem &pm) {
     *out = nullptr;
     mkhip_plan *pl = new mkhip_plan();
     pl->ctx = c;
-    hipError_t e = hipMalloc(&pl->d_mem, std::max<size_t>(ws_bytes, 256));
-    if (e != hipSuccess) {
-        delete pl;
-        return hip_status(e);
-    }
+    pl->mem = pm;
     pl->bytes = ws_bytes;
+    if (pm.query) {
+        if (pm.ws_need) *pm.ws_need = ws_bytes;
+    } else if (pm.ws) {
+        if (pm.ws_bytes < ws_bytes) {
+            delete pl;
+            return MKHIP_ERR_OOM;
+        }
+        pl->d_mem = pm.ws;
+    } else {
+        hipError_t e = hipMalloc(&pl->d_mem, std::max<size_t>(ws_bytes, 256));
+        if (e != hipSuccess) {
+            delete pl;
+            return hip_status(e);
+        }
+    }
     *out = pl;
     return MKHIP_OK;
 }
 
-int plan_finish(mkhip_plan *pl, const DescBlob &blob, int use_graph) {
+int plan_finish(mkhip_plan *&pl, const DescBlob &blob, int use_graph) {
     mkhip_ctx *c = pl->ctx;
-    hipError_t e = hipMalloc(&pl->d_desc, std::max<size_t>(blob.host.size(), 256));
-    if (e == hipSuccess) e = hipMemcpy(pl->d_desc, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice);
+    const PlanMem &pm = pl->mem;
+    const size_t db = std::max<size_t>(blob.host.size(), 256);
+    if (pm.query) {
+        if (pm.desc_need) *pm.desc_need = db;
+        delete pl;
+        pl = nullptr;
+        return MKHIP_OK;
+    }
+    hipError_t e = hipSuccess;
+    if (pm.desc) {
+        if (pm.desc_bytes < db) {
+            mkhip_plan_destroy(pl);
+            pl = nullptr;
+            return MKHIP_ERR_OOM;
+        }
+        pl->d_desc = pm.desc;
+        pl->host_desc = blob.host;
+        e = hipMemcpyAsync(pl->d_desc, pl->host_desc.data(), pl->host_desc.size(), hipMemcpyHostToDevice,
+                           pm.stream ? pm.stream : c->stream);
+    } else {
+        e = hipMalloc(&pl->d_desc, db);
+        if (e == hipSuccess) e = hipMemcpy(pl->d_desc, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         mkhip_plan_destroy(pl);
+        pl = nullptr;
         return hip_status(e);
     }
     pl->bytes += blob.host.size();
+    pl->desc_bytes = blob.host.size();
     for (PlanStage &s : pl->stages) s.descs = (const char *)pl->d_desc + reinterpret_cast<size_t>(s.descs);
     if (use_graph) {
         // first launch eagerly (sets kernel attributes), then capture the sequence
@@ -60,11 +94,28 @@ int plan_finish(mkhip_plan *pl, const DescBlob &blob, int use_graph) {
         if (st == MKHIP_OK) st = hip_status(hipGraphInstantiate(&pl->ge, pl->g, nullptr, nullptr, 0));
         if (st != MKHIP_OK) {
             mkhip_plan_destroy(pl);
+            pl = nullptr;
             return st;
         }
         pl->graph = true;
     }
     return MKHIP_OK;
+}
+
+int plan_build(mkhip_ctx *c, int scheme, int nops, const void *ops, int use_graph, mkhip_plan **out,
+               const PlanMem &pm) {
+    switch (scheme) {
+    case MKHIP_SCHEME_KKLSS_CKKS:
+        return plan_kklss_ckks(c, nops, (const mkhip_kklss_op *)ops, use_graph, out, pm);
+    case MKHIP_SCHEME_KKLSS_BFV:
+        return plan_kklss_bfv(c, nops, (const mkhip_kklss_op *)ops, use_graph, out, pm);
+    case MKHIP_SCHEME_MKF_CKKS:
+        return plan_mkf_ckks(c, nops, (const mkhip_mkf_op *)ops, use_graph, out, pm);
+    case MKHIP_SCHEME_MKF_BFV:
+        return plan_mkf_bfv(c, nops, (const mkhip_mkf_op *)ops, use_graph, out, pm);
+    default:
+        return MKHIP_ERR_ARG;
+    }
 }
 
 }  // namespace mkhip
@@ -138,8 +189,8 @@ extern "C" void mkhip_plan_destroy(mkhip_plan *plan) {
     if (!plan) return;
     if (plan->ge) (void)hipGraphExecDestroy(plan->ge);
     if (plan->g) (void)hipGraphDestroy(plan->g);
-    if (plan->d_mem) (void)hipFree(plan->d_mem);
-    if (plan->d_desc) (void)hipFree(plan->d_desc);
+    if (plan->d_mem && !plan->mem.ws) (void)hipFree(plan->d_mem);
+    if (plan->d_desc && !plan->mem.desc) (void)hipFree(plan->d_desc);
     delete plan;
 }
 
@@ -184,21 +235,6 @@ extern "C" int mkhip_get_tuning(const mkhip_ctx *c, int knob, int *value) {
     }
 }
 
-static int build_plan(mkhip_ctx *c, int scheme, int nops, const void *ops, mkhip_plan **pl) {
-    switch (scheme) {
-    case MKHIP_SCHEME_KKLSS_CKKS:
-        return mkhip_plan_kklss_ckks(c, nops, (const mkhip_kklss_op *)ops, 1, pl);
-    case MKHIP_SCHEME_KKLSS_BFV:
-        return mkhip_plan_kklss_bfv(c, nops, (const mkhip_kklss_op *)ops, 1, pl);
-    case MKHIP_SCHEME_MKF_CKKS:
-        return mkhip_plan_mkf_ckks(c, nops, (const mkhip_mkf_op *)ops, 1, pl);
-    case MKHIP_SCHEME_MKF_BFV:
-        return mkhip_plan_mkf_bfv(c, nops, (const mkhip_mkf_op *)ops, 1, pl);
-    default:
-        return MKHIP_ERR_ARG;
-    }
-}
-
 // Every combination of the knobs that apply to the scheme is planned and timed
 // (graph launches, HIP events) on the caller's ops; the fastest is kept.
 extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *ops, int iters, int *ncand,
@@ -228,7 +264,7 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
         mkhip_set_tuning(c, MKHIP_TUNE_DECOMP_ORDER, cd.order);
         mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, cd.xcd);
         mkhip_plan *pl = nullptr;
-        int s = build_plan(c, scheme, nops, ops, &pl);
+        int s = plan_build(c, scheme, nops, ops, 1, &pl, PlanMem());
         double t = 0;
         if (s == MKHIP_OK) s = mkhip_plan_launch(pl);  // warm-up
         if (s == MKHIP_OK) s = mkhip_plan_time(pl, iters, &t);
