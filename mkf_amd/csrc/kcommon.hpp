@@ -72,10 +72,24 @@ struct Geo {
     static constexpr int TH = NS / 16;
 };
 
+// MODE: -1 picks the unreduced butterflies at run time when the modulus allows
+// them; 1 / 0 force the unreduced / reduced form (the caller checked fwd_lazy_ok).
+template <int LOGN, int MODE = -1>
+__device__ __forceinline__ void fwd_ntt_mode(u64 (&v)[16], int tid, const ulonglong2 *tw, const ModConst &m,
+                                             int hh) {
+    if constexpr (MODE == 1)
+        ntt_fwd<LOGN, true>(v, g_lds, tid, tw, m.q, hh, m.mu);
+    else if constexpr (MODE == 0)
+        ntt_fwd<LOGN, false>(v, g_lds, tid, tw, m.q, hh);
+    else
+        ntt_fwd_any<LOGN>(v, g_lds, tid, tw, m.q, m.mu, hh);
+}
+
 // Forward transform of this workgroup's part of a limb whose element c is f(c)
 // (values < 8q); v receives the NTT values at h NS + tid + e TH, canonical.
-template <int LOGN, class F>
-__device__ __forceinline__ void fwd_transform(u64 (&v)[16], F f, int tid, int h, const ulonglong2 *tw, u64 q) {
+template <int LOGN, int MODE = -1, class F>
+__device__ __forceinline__ void fwd_transform(u64 (&v)[16], F f, int tid, int h, const ulonglong2 *tw,
+                                              const ModConst &m) {
     typedef Geo<LOGN> Gm;
     if constexpr (Gm::split) {
         u64 lo[16], hi[16];
@@ -84,12 +98,42 @@ __device__ __forceinline__ void fwd_transform(u64 (&v)[16], F f, int tid, int h,
             lo[e] = f(tid + e * Gm::TH);
             hi[e] = f(Gm::NS + tid + e * Gm::TH);
         }
-        fwd_first_stage(v, lo, hi, tw, q, h);
-        ntt_fwd<Gm::LS>(v, g_lds, tid, tw, q, 2 + h);
+        fwd_first_stage(v, lo, hi, tw, m.q, h);
+        fwd_ntt_mode<Gm::LS, MODE>(v, tid, tw, m, 2 + h);
     } else {
 #pragma unroll
         for (int e = 0; e < 16; ++e) v[e] = f(tid + e * Gm::TH);
-        ntt_fwd<LOGN>(v, g_lds, tid, tw, q, 1);
+        fwd_ntt_mode<LOGN, MODE>(v, tid, tw, m, 1);
+    }
+}
+
+// fwd_transform of a stored limb (row: its N words, values < 3q when red, else
+// < 8q).  The reduction is a wave-uniform branch over all values, not a
+// per-value select, and the loads use the row's scalar base.
+template <int LOGN, int MODE = -1>
+__device__ __forceinline__ void fwd_transform_row(u64 (&v)[16], const u64 *row, bool red, int tid, int h,
+                                                  const ulonglong2 *tw, const ModConst &m) {
+    typedef Geo<LOGN> Gm;
+    if constexpr (Gm::split) {
+        u64 lo[16], hi[16];
+        load_row_cached<Gm::LS>(lo, row, tid);
+        load_row_cached<Gm::LS>(hi, row + Gm::NS, tid);
+        if (red) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                lo[e] = reduce64(lo[e], m.q, m.mu);
+                hi[e] = reduce64(hi[e], m.q, m.mu);
+            }
+        }
+        fwd_first_stage(v, lo, hi, tw, m.q, h);
+        fwd_ntt_mode<Gm::LS, MODE>(v, tid, tw, m, 2 + h);
+    } else {
+        load_row_cached<LOGN>(v, row, tid);
+        if (red) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
+        }
+        fwd_ntt_mode<LOGN, MODE>(v, tid, tw, m, 1);
     }
 }
 

@@ -41,7 +41,6 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH) k_ntt_batch(DevTables T, const 
         inv_transform<LOGN>(v, tid, h, T.itw + (size_t)j.mod * Gm::N, m.q, ts ? m.tninv : m.ninv,
                             ts ? m.tninvp : m.ninvp);
     } else {
-        const gu64 *src = G(j.src);
         const ulonglong2 *tw = T.tw + (size_t)j.mod * Gm::N;
         if constexpr (Gm::split) {
             // In place, a half may not be overwritten while the other workgroup still
@@ -50,24 +49,29 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH) k_ntt_batch(DevTables T, const 
             if (j.src == j.dst) {
                 if (h) return;
                 u64 hi[16];
+                load_row_cached<Gm::LS>(v, j.src, tid);
+                load_row_cached<Gm::LS>(hi, j.src + Gm::NS, tid);
+                if (red) {
 #pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const u64 x = src[tid + e * Gm::TH], y = src[Gm::NS + tid + e * Gm::TH];
-                    v[e] = red ? reduce64(x, m.q, m.mu) : x;
-                    hi[e] = red ? reduce64(y, m.q, m.mu) : y;
+                    for (int e = 0; e < 16; ++e) {
+                        v[e] = reduce64(v[e], m.q, m.mu);
+                        hi[e] = reduce64(hi[e], m.q, m.mu);
+                    }
                 }
                 fwd_first_stage_both(v, hi, tw, m.q);
                 store_row<Gm::LS>(hi, j.dst + Gm::NS, tid);
-                ntt_fwd<Gm::LS>(v, g_lds, tid, tw, m.q, 2);
+                ntt_fwd_any<Gm::LS>(v, g_lds, tid, tw, m.q, m.mu, 2);
                 store_row<Gm::LS>(v, j.dst, tid);
                 load_row_cached<Gm::LS>(v, j.dst + Gm::NS, tid);
-                ntt_fwd<Gm::LS>(v, g_lds, opaque_lane(tid), opaque(tw), m.q, 3);
+                ntt_fwd_any<Gm::LS>(v, g_lds, opaque_lane(tid), opaque(tw), m.q, m.mu, 3);
                 store_row<Gm::LS>(v, j.dst + Gm::NS, tid);
                 return;
             }
         }
-        fwd_transform<LOGN>(v, [&](int c) { const u64 x = src[c]; return red ? reduce64(x, m.q, m.mu) : x; }, tid,
-                            h, tw, m.q);
+        if (fwd_lazy_ok(m.q))
+            fwd_transform_row<LOGN, 1>(v, j.src, red, tid, h, tw, m);
+        else
+            fwd_transform_row<LOGN, 0>(v, j.src, red, tid, h, tw, m);
     }
     store_row<Gm::LS>(v, j.dst + (size_t)h * Gm::NS, tid);
 }
@@ -98,6 +102,51 @@ __device__ __forceinline__ u64 digit_modup(const DevTables &T, const u64 *__rest
     return reduce64(r, m.q, m.mu);
 }
 
+// One digit row: the digit's value (copy path / the digit's own rows: the residue
+// itself, reduced mod m only when it may exceed the transform's input bound 8m --
+// source values are below 3 q_src: canonical, or multSum's raw outputs in a BFV
+// lift; otherwise the exact CRT ModUp), forward-transformed, stored at dst.
+template <int LOGN, bool CRT, int MODE>
+__device__ __forceinline__ void decomp_row(const DevTables &T, const u64 *src, u64 *dst, int start, int nd,
+                                           bool own_row, int i, int r, int ri, int h, int tid,
+                                           const ulonglong2 *tw, const ModConst &m) {
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
+    u64 v[16];
+    if (!CRT || nd == 1 || own_row) {
+        const int sl = own_row ? r : start;
+        const bool red = T.mod[r_index(sl, T.L, T.K)].q > 2 * m.q;
+        fwd_transform_row<LOGN, MODE>(v, src + (size_t)sl * N, red, tid, h, tw, m);
+    } else {
+        const int ebase = i * (kMaxAlpha + 1) + nd;
+        fwd_transform<LOGN, MODE>(v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h,
+                                  tw, m);
+    }
+    store_row<Gm::LS>(v, dst, tid);
+}
+
+// decomp_row for an unsplit limb (N <= 2^14).
+template <int LOGN, bool CRT, int MODE>
+__device__ __forceinline__ void decomp_flat(const DevTables &T, const u64 *src, u64 *dst, int start, int nd,
+                                            bool own_row, int i, int r, int ri, int tid, const ulonglong2 *tw,
+                                            const ModConst &m) {
+    constexpr int N = 1 << LOGN;
+    u64 v[16];
+    if (!CRT || nd == 1 || own_row) {
+        const int sl = own_row ? r : start;
+        load_row_cached<LOGN>(v, src + (size_t)sl * N, tid);
+        if (T.mod[r_index(sl, T.L, T.K)].q > 2 * m.q) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
+        }
+    } else {
+        const int ebase = i * (kMaxAlpha + 1) + nd;
+        for (int e = 0; e < 16; ++e) v[e] = digit_modup(T, src, tid + e * (N / 16), start, nd, ebase, ri, m);
+    }
+    fwd_ntt_mode<LOGN, MODE>(v, tid, tw, m, 1);
+    store_row<LOGN>(v, dst, tid);
+}
+
 // One workgroup per (desc, component p, digit i, row r[, half]): logical block
 // (((desc * kmax + p) * beta + i) * rows + r) * H + h, XCD-remapped so the rows of
 // one (component, digit) -- which all read the same input limbs -- share an L2.
@@ -125,22 +174,22 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     const bool own_row = (r <= level) && (r >= start) && (r < start + nd);
     const u64 *src = Dd.comp[p];
     const ulonglong2 *tw = T.tw + (size_t)ri * N;
-    u64 v[16];
-    if (!CRT || nd == 1 || own_row) {
-        // copy path / the digit's own rows: the residue itself, reduced mod m only
-        // when it may exceed the transform's input bound 8m (source values are
-        // below 3 q_src: canonical, or multSum's raw outputs in a BFV lift)
-        const int sl = own_row ? r : start;
-        const bool red = T.mod[r_index(sl, T.L, T.K)].q > 2 * m.q;
-        const gu64 *row = G(src + (size_t)sl * N);
-        fwd_transform<LOGN>(v, [&](int c) { const u64 x = row[c]; return red ? reduce64(x, m.q, m.mu) : x; }, tid,
-                            h, tw, m.q);
-    } else {
-        const int ebase = i * (kMaxAlpha + 1) + nd;
-        fwd_transform<LOGN>(v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h, tw,
-                            m.q);
+    u64 *dst = Dd.D + (((size_t)p * beta + i) * rows + r) * N + (size_t)h * Gm::NS;
+    if constexpr (!Gm::split) {
+        const bool crt_row = CRT && !(nd == 1 || own_row);
+        if (!crt_row && fwd_lazy_ok(m.q))
+            decomp_flat<LOGN, CRT, 1>(T, src, dst, start, nd, own_row, i, r, ri, tid, tw, m);
+        else
+            decomp_flat<LOGN, CRT, 0>(T, src, dst, start, nd, own_row, i, r, ri, tid, tw, m);
+        return;
     }
-    store_row<Gm::LS>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N + (size_t)h * Gm::NS, tid);
+    if (CRT && !(nd == 1 || own_row)) {  // CRT digit rows: one ModUp per value, then the transform
+        decomp_row<LOGN, true, 0>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
+    } else if (fwd_lazy_ok(m.q)) {
+        decomp_row<LOGN, false, 1>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
+    } else {
+        decomp_row<LOGN, false, 0>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
+    }
 }
 
 // Key-stationary x / y accumulation: grid (N/256, beta*rows, 2 sides); the

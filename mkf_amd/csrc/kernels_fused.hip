@@ -19,12 +19,39 @@ namespace mkhip {
 // blockIdx -> logical ((side * beta + i) * rows + r) * nops + o (order 0: the ops
 // sharing a key row are adjacent) or ((side * beta + i) * nops + o) * rows + r
 // (order 1: the rows sharing an input limb are adjacent); descs[2 o + side]
+template <int LOGN, int MODE>
+__device__ __forceinline__ void decomp_mac_row(const DecompDesc &Dd, int i, int r, int beta, int rows, int h,
+                                               bool reduce, size_t krow, int tid, const ulonglong2 *tw,
+                                               const ModConst &m) {
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
+    constexpr int TH = Gm::TH;
+    const size_t hoff = (size_t)h * Gm::NS;
+    u64 acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0;
+    for (int p = 0; p < Dd.k; ++p) {
+        u64 v[16];
+        __builtin_amdgcn_sched_barrier(0);
+        // values below 3 q_src may exceed the transform's bound 8m: reduce then
+        fwd_transform_row<LOGN, MODE>(v, Dd.comp[p] + (size_t)i * N, reduce, opaque_lane(tid), h, opaque(tw), m);
+        __builtin_amdgcn_sched_barrier(0);
+        store_row<Gm::LS>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N + hoff, tid);
+        const u64 *key = Dd.key[p] + krow;
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+            acc[e] = add_mod(acc[e], mred(uniform_ptr(key + e * TH)[tid], v[e], m.q, m.qinv), m.q);
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = mred(acc[e], m.r2, m.q, m.qinv);
+    store_row<Gm::LS>(acc, Dd.xy + krow, tid);
+}
+
 template <int LOGN>
 __global__ void __launch_bounds__(Geo<LOGN>::TH)
     k_decomp_mac(DevTables T, const DecompDesc *__restrict__ descs, int nops, int beta, int rows, int order) {
     typedef Geo<LOGN> Gm;
     constexpr int N = Gm::N;
-    constexpr int TH = Gm::TH;
     const int tid = threadIdx.x;
     int bid = block_index(T);
     const int h = Gm::split ? bid % 2 : 0;
@@ -50,26 +77,12 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     const size_t hoff = (size_t)h * Gm::NS;
     const size_t krow = ((size_t)i * (T.L + T.K) + ri) * N + hoff;
     const bool reduce = T.mod[r_index(i, T.L, T.K)].q > 2 * m.q;
-    u64 acc[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) acc[e] = 0;
-    for (int p = 0; p < Dd.k; ++p) {
-        u64 v[16];
-        const gu64 *row = G(Dd.comp[p] + (size_t)i * N);
-        __builtin_amdgcn_sched_barrier(0);
-        // values below 3 q_src may exceed the transform's bound 8m: reduce then
-        fwd_transform<LOGN>(v, [&](int c) { const u64 x = row[c]; return reduce ? reduce64(x, m.q, m.mu) : x; },
-                            opaque_lane(tid), h, opaque(tw), m.q);
-        __builtin_amdgcn_sched_barrier(0);
-        store_row<Gm::LS>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N + hoff, tid);
-        const u64 *key = Dd.key[p] + krow;
-#pragma unroll
-        for (int e = 0; e < 16; ++e)
-            acc[e] = add_mod(acc[e], mred(uniform_ptr(key + e * TH)[tid], v[e], m.q, m.qinv), m.q);
-    }
-#pragma unroll
-    for (int e = 0; e < 16; ++e) acc[e] = mred(acc[e], m.r2, m.q, m.qinv);
-    store_row<Gm::LS>(acc, Dd.xy + krow, tid);
+    // one instance of the loop per butterfly form (a wave-uniform branch), so that
+    // neither form's registers are allocated around the other's
+    if (fwd_lazy_ok(m.q))
+        decomp_mac_row<LOGN, 1>(Dd, i, r, beta, rows, h, reduce, krow, tid, tw, m);
+    else
+        decomp_mac_row<LOGN, 0>(Dd, i, r, beta, rows, h, reduce, krow, tid, tw, m);
 }
 
 template <int LOGN>

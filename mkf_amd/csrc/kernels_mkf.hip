@@ -63,16 +63,18 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     // sum_a (x_a - NTT(ModUp(x_a,P))) P^-1 = (sum_a x_a - NTT(sum_a ModUp(x_a,P))) P^-1
     // residue for residue (the NTT is linear mod q_j): one transform per row.
     u64 z[16], xs[16];
-    fwd_transform<LOGN>(z,
-                        [&](int c) {
-                            u64 acc = 0;
-                            for (int a = 0; a < Dd.nacc; ++a) {
-                                const PUp up = modup_p_prepare(T, Dd.acc[a], nl, c);
-                                acc = add_mod(acc, modup_p_to_row(T, up, j, m), m.q);
-                            }
-                            return acc;
-                        },
-                        tid, h, tw, m.q);
+    auto f = [&](int c) {
+        u64 acc = 0;
+        for (int a = 0; a < Dd.nacc; ++a) {
+            const PUp up = modup_p_prepare(T, Dd.acc[a], nl, c);
+            acc = add_mod(acc, modup_p_to_row(T, up, j, m), m.q);
+        }
+        return acc;
+    };
+    if (fwd_lazy_ok(m.q))
+        fwd_transform<LOGN, 1>(z, f, tid, h, tw, m);
+    else
+        fwd_transform<LOGN, 0>(z, f, tid, h, tw, m);
 #pragma unroll
     for (int e = 0; e < 16; ++e) xs[e] = 0;
     for (int a = 0; a < Dd.nacc; ++a) {
@@ -122,7 +124,6 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     const ModConst m = T.mod[j];
     const size_t hoff = (size_t)h * Gm::NS;
     u64 v[16];
-    load_row_cached<Gm::LS>(v, Dd.x + (size_t)j * N + hoff, tid);
     if (Dd.nres == 1) {
         // out_j = (x_j - NTT_j(([x_L + h]_{q_L} - h) mod q_j)) q_L^-1, h = q_L >> 1
         const int l = Dd.level;
@@ -131,15 +132,21 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
         const u64 hj = T.rs_half[l * T.L + j];
         const gu64 *xl = G(Dd.xlast);
         u64 w[16];
-        fwd_transform<LOGN>(w,
-                            [&](int c) {
-                                const u64 t = add_mod(xl[c], hq, qL);
-                                return sub_mod(reduce64(t, m.q, m.mu), hj, m.q);
-                            },
-                            tid, h, T.tw + (size_t)j * N, m.q);
+        auto f = [&](int c) {
+            const u64 t = add_mod(xl[c], hq, qL);
+            return sub_mod(reduce64(t, m.q, m.mu), hj, m.q);
+        };
+        if (fwd_lazy_ok(m.q))
+            fwd_transform<LOGN, 1>(w, f, tid, h, T.tw + (size_t)j * N, m);
+        else
+            fwd_transform<LOGN, 0>(w, f, tid, h, T.tw + (size_t)j * N, m);
+        // x_j is loaded after the transform: it is not held across it
+        load_row_cached<Gm::LS>(v, Dd.x + (size_t)j * N + hoff, tid);
         const u64 inv = T.rs_inv[l * T.L + j];
 #pragma unroll
         for (int e = 0; e < 16; ++e) v[e] = mred(sub_mod(v[e], w[e], m.q), inv, m.q, m.qinv);
+    } else {
+        load_row_cached<Gm::LS>(v, Dd.x + (size_t)j * N + hoff, tid);
     }
     for (int o = 0; o < Dd.nout; ++o) store_row<Gm::LS>(v, Dd.out[o] + (size_t)j * N + hoff, tid);
 }

@@ -31,6 +31,19 @@ __device__ __forceinline__ void fwd_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q4, 
     y = u + q4 - t;                        // (0, 8q)
 }
 
+// Unreduced form for moduli with headroom (q < 2^57): no conditional subtraction;
+// each stage adds at most 4q to the bound (x + t, x + 4q - t with t < 4q), so 15
+// stages from inputs < 8q stay below 68q < 2^64.
+__device__ __forceinline__ void fwd_bfly_lazy(u64 &x, u64 &y, u64 w, u64 wp, u64 q4, u64 q) {
+    const u64 u = x;
+    const u64 t = shoup_lazy4(y, w, wp, q);  // [0, 4q)
+    x = u + t;
+    y = u + q4 - t;
+}
+
+// The moduli whose forward transforms run unreduced.
+__device__ __forceinline__ bool fwd_lazy_ok(u64 q) { return q < (1ull << 57); }
+
 // In: x, y in [0, 4q).  Out: x, y in [0, 4q).
 __device__ __forceinline__ void inv_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q4, u64 q) {
     u64 u = x, v = y;
@@ -83,6 +96,7 @@ struct FwdPass {
     }
     // hh: twiddle multiplier -- 1 for a whole transform; 2 + h for half h of a
     // transform of twice this size (its stage s twiddles start at 2^(s+1) + h 2^s)
+    template <bool LAZY = false>
     __device__ static __forceinline__ void run(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ tw, u64 q,
                                                int hh) {
         const u64 q4 = q << 2;
@@ -102,7 +116,10 @@ struct FwdPass {
 #pragma unroll
                     for (int kk = 0; kk < half; ++kk) {
                         const int k0 = grp * 2 * half + kk;
-                        fwd_bfly(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, q4, q);
+                        if (LAZY)
+                            fwd_bfly_lazy(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, q4, q);
+                        else
+                            fwd_bfly(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, q4, q);
                     }
                 }
             }
@@ -207,21 +224,42 @@ struct NttPlan<13> {
 // Forward NTT; v in Linear layout in and out; returns canonical values.  With
 // hh = 2 + h it is the transform of half h of a limb of 2^(LOGN+1) after that
 // limb's first stage (fwd_first_stage).
-template <int LOGN>
+// LAZY (q < 2^57, mu = floor(2^64 / q)): unreduced butterflies, one Barrett
+// reduction per value at the end.
+template <int LOGN, bool LAZY = false>
 __device__ __forceinline__ void ntt_fwd(u64 (&v)[16], u64 *s, int tid, const ulonglong2 *__restrict__ tw,
-                                        u64 q, int hh = 1) {
+                                        u64 q, int hh = 1, u64 mu = 0) {
     typedef NttPlan<LOGN> P;
     static_assert(P::F0::TL == Linear<LOGN>::T, "first pass must be the linear layout");
-    P::F0::run(v, tid, tw, q, hh);
+    P::F0::template run<LAZY>(v, tid, tw, q, hh);
     lds_exchange<typename P::F0, typename P::F1>(v, s, tid);
-    P::F1::run(v, tid, tw, q, hh);
+    P::F1::template run<LAZY>(v, tid, tw, q, hh);
     lds_exchange<typename P::F1, typename P::F2>(v, s, tid);
-    P::F2::run(v, tid, tw, q, hh);
+    P::F2::template run<LAZY>(v, tid, tw, q, hh);
     lds_exchange<typename P::F2, typename P::F3>(v, s, tid);
-    P::F3::run(v, tid, tw, q, hh);
+    P::F3::template run<LAZY>(v, tid, tw, q, hh);
     lds_exchange<typename P::F3, Linear<LOGN> >(v, s, tid);
+    if (LAZY) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = reduce_8q(v[i], q);
+        for (int i = 0; i < 16; ++i) {
+            v[i] = reduce64(v[i], q, mu);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = reduce_8q(v[i], q);
+    }
+}
+
+// Forward NTT choosing the unreduced form when the modulus allows it (a
+// wave-uniform branch: one limb has one modulus).
+template <int LOGN>
+__device__ __forceinline__ void ntt_fwd_any(u64 (&v)[16], u64 *s, int tid, const ulonglong2 *__restrict__ tw,
+                                            u64 q, u64 mu, int hh = 1) {
+    if (fwd_lazy_ok(q))
+        ntt_fwd<LOGN, true>(v, s, tid, tw, q, hh, mu);
+    else
+        ntt_fwd<LOGN, false>(v, s, tid, tw, q, hh);
 }
 
 // Inverse NTT including the N^-1 scaling; v < 4q in Linear layout; returns
