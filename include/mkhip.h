@@ -64,7 +64,7 @@ const char *mkhip_status_string(int status);
  * mkrlwe.NewParameters (mkrlwe/params.go:16-61) and mkckks.NewParameters
  * (mkckks/params.go:17-25): N = 2^logN, Q[0..L), P[0..K), gamma = 2 for the KKLSS
  * evaluators (alpha = K/gamma) and 1 for MKF (Lattigo rlwe, alpha = K).
- * logN in {13, 14}.  t and qmul are for BFV contexts (0 / NULL otherwise): a
+ * logN in {13, 14, 15}.  t and qmul are for BFV contexts (0 / NULL otherwise): a
  * non-null qmul (L primes, the QMul chain of mkbfv.ParametersLiteral,
  * mkbfv/params.go:12-81) makes a KKLSS MK-BFV context over R = Q u QMul with
  * plaintext modulus t; it needs gamma = 2 and alpha = K/2 = 1 (K = 2 or 3).

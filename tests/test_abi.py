@@ -55,3 +55,17 @@ def test_status_strings():
 
     for code in mkf_amd.STATUS:
         assert mkf_amd.lib().mkhip_status_string(code)
+
+
+def test_cpp_host_mirror_compiles():
+    """The C++ mirror of the reference's evaluator API is self-contained C++17 over the C ABI."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    hdr = os.path.join(ROOT, "mkf_amd", "host", "mkhip.hpp")
+    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-x", "c++", hdr],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
