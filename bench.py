@@ -343,6 +343,131 @@ def run_uneven(args, rng):
     return res
 
 
+# serving classes: (scheme, logN, L, K); BFV adds L QMul primes and t = 65537
+SERVE_CLASSES = [("kklss", 14, 8, 2), ("kklss", 15, 16, 3), ("bfv", 14, 6, 2), ("bfv", 15, 8, 2)]
+
+
+def _serve_setup(args, rng, specs, nparty, kmax):
+    import mkf_amd
+
+    classes = []
+    for scheme, logN, L, K in specs:
+        N = 1 << logN
+        if scheme == "bfv" and logN == 14:
+            Q, QM, P, t = BFV_Q, BFV_QMUL, BFV_P, BFV_T
+        elif scheme == "bfv":
+            allq = ntt_primes(55, 2 * L)
+            Q, QM, P, t = allq[:L], allq[L:], ntt_primes(61, K), BFV_T
+        else:
+            Q, QM, P, t = ntt_primes(50, L), None, ntt_primes(60, K), 0
+        ctx = mkf_amd.Context(logN, Q, P, gamma=2, t=t, qmul=QM)
+        QP, beta = Q + P, ctx.beta
+        for pid in range(nparty):
+            if scheme == "kklss":
+                ctx.register_party(pid, *[uniform_stack(rng, QP, N, beta) for _ in range(3)])
+            else:
+                ctx.register_bfv_party(pid, uniform_stack(rng, QP, N, 2 * beta), uniform_stack(rng, QP, N, 2 * beta),
+                                       uniform_stack(rng, QP, N, beta))
+        ctx.set_crs_u(uniform_stack(rng, QP, N, beta))
+        nres = 0
+        if scheme == "kklss":
+            scale = float(Q[-1])
+            nres, _ = ctx.rescale_count(L - 1, scale * scale, scale)
+        # operand pool in pinned host memory: a request reads 1 + k consecutive polys per side
+        npool = 2 * (1 + kmax)
+        pool = mkf_amd.PinnedArray((npool, L, N))
+        for i in range(npool):
+            pool.a[i] = uniform(rng, Q, N)
+        # results land in a ring of pinned output slots (the bench does not read them)
+        outs = [mkf_amd.PinnedArray((1 + kmax, L - nres, N)) for _ in range(8)]
+        classes.append(dict(scheme=scheme, logN=logN, L=L, K=K, ctx=ctx, nres=nres, pool=pool, npool=npool,
+                            outs=outs))
+    return classes
+
+
+def _serve_run(classes, args, rng, rate, duration, streams, nparty, kmin, kmax):
+    """Poisson arrivals at `rate` req/s for `duration` s, k uniform in kmin..kmax;
+    returns latency stats."""
+    import mkf_amd
+
+    srv = mkf_amd.Server(streams=streams, max_batch=args.serve_max_batch, mem_bytes=int(args.serve_mem_gb * 1e9))
+    ids = [srv.add_class(c["ctx"], c["scheme"]) for c in classes]
+    srv.start()
+    # draw the whole arrival schedule first, then replay it against the wall clock
+    n = max(1, int(rng.poisson(rate * duration)))
+    gaps = rng.exponential(1.0 / rate, size=n)
+    reqs = []
+    for i in range(n):
+        ci = int(rng.integers(0, len(classes)))
+        c = classes[ci]
+        k = int(rng.integers(kmin, kmax + 1))
+        pids = sorted(int(x) for x in rng.choice(nparty, size=k, replace=False))
+        s0, s1 = (int(x) for x in rng.integers(0, c["npool"] - k, size=2))
+        out = c["outs"][i % len(c["outs"])].a[:1 + k]
+        reqs.append((ids[ci], c, pids, s0, s1, out, k))
+    tickets, t0 = [], time.perf_counter()
+    due = t0
+    for (cl, c, pids, s0, s1, out, k), gap in zip(reqs, gaps):
+        due += gap
+        while True:
+            now = time.perf_counter()
+            if now >= due:
+                break
+            time.sleep(min(due - now, 0.0005))
+        pa = c["pool"].a
+        tickets.append(srv.enqueue(cl, pa[s0:s0 + 1 + k], pa[s1:s1 + 1 + k], out, ids0=pids, ids1=pids,
+                                   nres=c["nres"]))
+    lat, qd, bad = [], [], 0
+    for t in tickets:
+        st, l, q = srv.wait(t)
+        bad += st != 0
+        lat.append(l)
+        qd.append(q)
+    el = time.perf_counter() - t0
+    stats = srv.stats()
+    srv.close()
+    return {"offered_rps": rate, "requests": n, "errors": bad, "achieved_ops_per_s": round(n / el, 2),
+            "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3),
+            "queue_p50_ms": round(pct(qd, 50), 3), "mean_batch": round(stats["mean_batch"], 2),
+            "max_batch": stats["max_batch"], "max_inflight": stats["max_inflight"]}
+
+
+def _serve_sweep(args, rng, specs, nparty, kmin, kmax, loads):
+    t_setup = time.perf_counter()
+    classes = _serve_setup(args, rng, specs, nparty, kmax)
+    setup_s = time.perf_counter() - t_setup
+    run = lambda rate, dur: _serve_run(classes, args, rng, rate, dur, args.serve_streams, nparty, kmin, kmax)
+    # capacity: a saturating burst (every request at once)
+    sat = run(1e6, args.serve_burst / 1e6)
+    sweep = []
+    for f in loads:
+        sweep.append(run(sat["achieved_ops_per_s"] * f, args.serve_seconds))
+        sweep[-1]["load"] = f
+    res = {"classes": [{"scheme": c["scheme"] + ("_ckks" if c["scheme"] == "kklss" else ""), "logN": c["logN"],
+                        "L": c["L"], "K": c["K"]} for c in classes],
+           "parties": nparty, "k": [kmin, kmax], "streams": args.serve_streams, "max_batch": args.serve_max_batch,
+           "mem_gb": args.serve_mem_gb, "saturated": sat, "sweep": sweep, "setup_s": round(setup_s, 1)}
+    for c in classes:
+        for o in c["outs"]:
+            o.close()
+        c["pool"].close()
+        c["ctx"].close()
+    return res
+
+
+def run_serving(args, rng):
+    """Serving loop: Poisson arrivals of KKLSS MK-CKKS / MK-BFV mul+relin requests,
+    k uniform in 2..16, N in {2^14, 2^15}, operands from registered parties whose
+    keys stay resident; continuous batching, admission by device memory."""
+    return _serve_sweep(args, rng, SERVE_CLASSES, args.serve_parties, 2, args.serve_kmax, args.serve_load)
+
+
+def run_serving_headline(args, rng):
+    """The serving loop at the headline shape (KKLSS MK-CKKS, N=2^14, L=8, K=2,
+    k=4): Poisson arrivals at fractions of the saturated rate."""
+    return _serve_sweep(args, rng, [("kklss", 14, 8, 2)], 4, 4, 4, args.serve_headline_load)
+
+
 def cpu_baseline(logN, Q, P, k, keys, u, budget_s):
     """The CPU oracle (C restatement of the reference path, 1 thread) on a bounded sample."""
     import oracle as O
@@ -417,6 +542,18 @@ def main():
                     help="workspace budget of each ring degree's branch of the uneven batch")
     ap.add_argument("--uneven-tune-ops", type=int, default=4,
                     help="ops of each branch the autotuner times")
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="length of each Poisson arrival run of the serving loop; 0 skips serving")
+    ap.add_argument("--serve-load", type=lambda x: [float(v) for v in x.split(",")], default=[0.3, 0.6, 0.9],
+                    help="offered load as fractions of the measured saturated rate")
+    ap.add_argument("--serve-burst", type=int, default=96, help="requests of the saturating burst")
+    ap.add_argument("--serve-headline-load", type=lambda x: [float(v) for v in x.split(",")],
+                    default=[0.5, 0.7, 0.85, 0.95])
+    ap.add_argument("--serve-parties", type=int, default=32)
+    ap.add_argument("--serve-kmax", type=int, default=16)
+    ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--serve-max-batch", type=int, default=32)
+    ap.add_argument("--serve-mem-gb", type=float, default=64.0)
     ap.add_argument("--copy-calib-mb", type=float, default=0.0,
                     help="run a plain D2D copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -474,6 +611,20 @@ def main():
         plan.close()
         w.close()
     line["schemes"] = {s: r for s, r in results.items() if s != "kklss_ckks"}
+    if args.serve_seconds > 0:
+        line["serving_headline"] = run_serving_headline(args, rng)
+        sh = line["serving_headline"]
+        plain = line.get("plain") or {}
+        ok = [r for r in sh["sweep"] if r["p99_ms"] < 10.0 and r["errors"] == 0]
+        best = max(ok, key=lambda r: r["achieved_ops_per_s"]) if ok else None
+        sh["best_under_p99_10ms"] = best
+        if best and plain.get("ops_per_s"):
+            sh["served_over_plain"] = round(best["achieved_ops_per_s"] / plain["ops_per_s"], 3)
+        print("# serving (headline shape): saturated %.1f ops/s" % sh["saturated"]["achieved_ops_per_s"],
+              file=sys.stderr, flush=True)
+        line["serving"] = run_serving(args, rng)
+        print("# serving: saturated %.1f ops/s" % line["serving"]["saturated"]["achieved_ops_per_s"],
+              file=sys.stderr, flush=True)
     if args.uneven_ops > 0:
         line["uneven_batch"] = run_uneven(args, rng)
         print("# uneven batch: %.1f ops/s" % line["uneven_batch"]["ops_per_s"], file=sys.stderr, flush=True)
