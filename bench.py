@@ -98,7 +98,7 @@ def stage_alg_bytes(B, k, L, K, beta, N, W=8):
         "decompose": B * 2 * k * beta * rows * 2 * limb,                        # limb NTTs
         "mac_xy": (B * 2 * k * beta * rows + 2 * k * beta * rows + B * 2 * beta * rows) * limb,
         "decompose_mac": (B * 2 * k * beta * rows * 2 + 2 * k * beta * rows + B * 2 * beta * rows) * limb,
-        "tensor_ntt": B * 2 * (1 + k) * L * 2 * limb,                           # tensor input NTTs
+        "tensor_ntt": B * 2 * L * 2 * limb,                 # NTT of the c0s (the other inputs are digit rows)
         "tensor": B * (1 + k) * L * 2 * limb,                                   # tensor + INTT
         "ext_xy": B * 2 * k * (2 * beta * rows + rows) * limb,                  # D + x|y + out
         "moddown_xy": B * 2 * k * (rows + L) * limb,

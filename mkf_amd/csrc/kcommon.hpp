@@ -39,6 +39,38 @@ __device__ __forceinline__ void store_row(const u64 (&v)[16], u64 *__restrict__ 
     for (int i = 0; i < 16; ++i) uniform_ptr(dst + i * T)[tid] = v[i];
 }
 
+// 16-byte global accesses (two consecutive words)
+typedef unsigned long long u64v2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) u64v2 gv2;
+
+// Rows in the Quad layout of the transforms (Quad<LOGN> in ntt.hpp): v[i] is
+// element Quad::elem(tid, i); the R consecutive words of each column move as
+// 16-byte accesses.
+template <int LOGN>
+__device__ __forceinline__ void load_row_quad(u64 (&v)[16], const u64 *__restrict__ src, int tid) {
+    typedef Quad<LOGN> Qd;
+    const gu64 *s = uniform_ptr(src);
+#pragma unroll
+    for (int i = 0; i < 16; i += 2) {
+        const u64v2 x = *(const gv2 *)(s + Qd::elem(tid, i));
+        v[i] = x.x;
+        v[i + 1] = x.y;
+    }
+}
+
+template <int LOGN>
+__device__ __forceinline__ void store_row_quad(const u64 (&v)[16], u64 *__restrict__ dst, int tid) {
+    typedef Quad<LOGN> Qd;
+    gu64 *d = uniform_ptr(dst);
+#pragma unroll
+    for (int i = 0; i < 16; i += 2) {
+        u64v2 x;
+        x.x = v[i];
+        x.y = v[i + 1];
+        *(gv2 *)(d + Qd::elem(tid, i)) = x;
+    }
+}
+
 // A wave-uniform pointer made opaque to the optimizer: table loads that use it
 // stay where they are written instead of being hoisted out of an enclosing loop
 // and held in registers across its iterations.
@@ -178,7 +210,8 @@ __device__ __forceinline__ u64 redc_sum(u64 hi, u64 lo, const ModConst &m) {
     return r >= m.q ? r - m.q : r;
 }
 
-// External-product rows: for the 16 values of this lane,
+// External-product rows (Quad layout: value e is element Quad::elem(tid, e)): for
+// the 16 values of this lane,
 //   a0[e] = sum_i MRed(k0_i[e], d_i[e]) mod q      (i < ndig, canonical)
 // and with NK = 2 the same with k1, written straight to park1 (the caller's output
 // row, where it waits while a0 is transformed).  d_i = d + i dstride, k_i = k + i
@@ -187,7 +220,7 @@ __device__ __forceinline__ u64 redc_sum(u64 hi, u64 lo, const ModConst &m) {
 template <int LOGN, int NK>
 __device__ __forceinline__ void ext_rows(u64 (&a0)[16], u64 *park1, const u64 *d, size_t dstride, const u64 *k0,
                                          const u64 *k1, size_t kstride, int ndig, int tid, const ModConst &m) {
-    constexpr int T = (1 << LOGN) / 16;
+    typedef Quad<LOGN> Qd;
     u64 a1[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) a0[e] = a1[e] = 0;
@@ -197,10 +230,18 @@ __device__ __forceinline__ void ext_rows(u64 (&a0)[16], u64 *park1, const u64 *d
         for (int c = 0; c < 16; c += 4) {
             u64 x[4], y0[4], y1[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                x[e] = uniform_ptr(dr + (c + e) * T)[tid];
-                y0[e] = uniform_ptr(k0r + (c + e) * T)[tid];
-                if (NK > 1) y1[e] = uniform_ptr(k1r + (c + e) * T)[tid];
+            for (int e = 0; e < 4; e += 2) {
+                const int off = Qd::elem(tid, c + e);
+                const u64v2 xd = *(const gv2 *)(uniform_ptr(dr) + off), yd = *(const gv2 *)(uniform_ptr(k0r) + off);
+                x[e] = xd.x;
+                x[e + 1] = xd.y;
+                y0[e] = yd.x;
+                y0[e + 1] = yd.y;
+                if (NK > 1) {
+                    const u64v2 zd = *(const gv2 *)(uniform_ptr(k1r) + off);
+                    y1[e] = zd.x;
+                    y1[e + 1] = zd.y;
+                }
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -210,7 +251,7 @@ __device__ __forceinline__ void ext_rows(u64 (&a0)[16], u64 *park1, const u64 *d
             __builtin_amdgcn_sched_barrier(0);
         }
     }
-    if constexpr (NK > 1) store_row<LOGN>(a1, park1, tid);
+    if constexpr (NK > 1) store_row_quad<LOGN>(a1, park1, tid);
 }
 
 // a * b mod q for two canonical operands: MRed(MForm(a), b).

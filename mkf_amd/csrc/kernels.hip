@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH) k_ntt_batch(DevTables T, const 
     const bool red = j.reduce & 1;
     u64 v[16];
     if (INV) {
-        load_row_cached<Gm::LS>(v, j.src + (size_t)h * Gm::NS, tid);
+        load_row_quad<Gm::LS>(v, j.src + (size_t)h * Gm::NS, tid);
         if (red) {
 #pragma unroll
             for (int i = 0; i < 16; ++i) v[i] = reduce64(v[i], m.q, m.mu);
@@ -61,10 +61,10 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH) k_ntt_batch(DevTables T, const 
                 fwd_first_stage_both(v, hi, tw, m.q);
                 store_row<Gm::LS>(hi, j.dst + Gm::NS, tid);
                 ntt_fwd_any<Gm::LS>(v, g_lds, tid, tw, m.q, m.mu, 2);
-                store_row<Gm::LS>(v, j.dst, tid);
+                store_row_quad<Gm::LS>(v, j.dst, tid);
                 load_row_cached<Gm::LS>(v, j.dst + Gm::NS, tid);
                 ntt_fwd_any<Gm::LS>(v, g_lds, opaque_lane(tid), opaque(tw), m.q, m.mu, 3);
-                store_row<Gm::LS>(v, j.dst + Gm::NS, tid);
+                store_row_quad<Gm::LS>(v, j.dst + Gm::NS, tid);
                 return;
             }
         }
@@ -73,7 +73,11 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH) k_ntt_batch(DevTables T, const 
         else
             fwd_transform_row<LOGN, 0>(v, j.src, red, tid, h, tw, m);
     }
-    store_row<Gm::LS>(v, j.dst + (size_t)h * Gm::NS, tid);
+    // inverse output: natural layout; forward output: Quad layout
+    if (INV)
+        store_row<Gm::LS>(v, j.dst + (size_t)h * Gm::NS, tid);
+    else
+        store_row_quad<Gm::LS>(v, j.dst + (size_t)h * Gm::NS, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -122,7 +126,7 @@ __device__ __forceinline__ void decomp_row(const DevTables &T, const u64 *src, u
         fwd_transform<LOGN, MODE>(v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h,
                                   tw, m);
     }
-    store_row<Gm::LS>(v, dst, tid);
+    store_row_quad<Gm::LS>(v, dst, tid);
 }
 
 // decomp_row for an unsplit limb (N <= 2^14).
@@ -144,7 +148,7 @@ __device__ __forceinline__ void decomp_flat(const DevTables &T, const u64 *src, 
         for (int e = 0; e < 16; ++e) v[e] = digit_modup(T, src, tid + e * (N / 16), start, nd, ebase, ri, m);
     }
     fwd_ntt_mode<LOGN, MODE>(v, tid, tw, m, 1);
-    store_row<LOGN>(v, dst, tid);
+    store_row_quad<LOGN>(v, dst, tid);
 }
 
 // One workgroup per (desc, component p, digit i, row r[, half]): logical block
@@ -233,17 +237,19 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     const TensorDesc &Dd = descs[bid / nl];
     const ModConst m = T.mod[l];
     const size_t off = (size_t)l * N + (size_t)h * Gm::NS;
+    const size_t uoff = (size_t)l * (Dd.ustride > 1 ? Dd.ustride : 1) * N + (size_t)h * Gm::NS;
     u64 v[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        const size_t c = off + tid + e * Gm::TH;
+        const int qe = Quad<Gm::LS>::elem(tid, e);  // the inverse transform's input layout
+        const size_t c = off + qe, cu = uoff + qe;
         u64 x;
         if (Dd.is_c0) {
             x = mulmod_m(G(Dd.a0)[c], G(Dd.b0)[c], m);
         } else {
             x = 0;
-            if (Dd.au) x = mulmod_m(G(Dd.b0)[c], G(Dd.au)[c], m);
-            if (Dd.bu) x = add_mod(x, mulmod_m(G(Dd.a0)[c], G(Dd.bu)[c], m), m.q);
+            if (Dd.au) x = mulmod_m(G(Dd.b0)[c], G(Dd.au)[cu], m);
+            if (Dd.bu) x = add_mod(x, mulmod_m(G(Dd.a0)[c], G(Dd.bu)[cu], m), m.q);
         }
         v[e] = x;
     }
@@ -284,7 +290,7 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
     store_row<Gm::LS>(a0, Dd.acc[0] + (size_t)r * N + hoff, tid);
     if (NKEY > 1) {
-        load_row_cached<Gm::LS>(a0, o1, tid);
+        load_row_quad<Gm::LS>(a0, o1, tid);
         inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
         store_row<Gm::LS>(a0, o1, tid);
     }

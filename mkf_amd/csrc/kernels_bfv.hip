@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     u64 v[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        const size_t c = off + tid + e * Gm::TH;
+        const size_t c = off + Quad<Gm::LS>::elem(tid, e);  // the inverse transform's input layout
         u64 x;
         if (Dd.is_c0) {
             x = mulmod_m(G(Dd.a0)[c], G(Dd.b0)[c], m);

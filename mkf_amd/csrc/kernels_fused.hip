@@ -36,15 +36,18 @@ __device__ __forceinline__ void decomp_mac_row(const DecompDesc &Dd, int i, int 
         // values below 3 q_src may exceed the transform's bound 8m: reduce then
         fwd_transform_row<LOGN, MODE>(v, Dd.comp[p] + (size_t)i * N, reduce, opaque_lane(tid), h, opaque(tw), m);
         __builtin_amdgcn_sched_barrier(0);
-        store_row<Gm::LS>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N + hoff, tid);
-        const u64 *key = Dd.key[p] + krow;
+        store_row_quad<Gm::LS>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N + hoff, tid);
+        const gu64 *key = uniform_ptr(Dd.key[p] + krow);
 #pragma unroll
-        for (int e = 0; e < 16; ++e)
-            acc[e] = add_mod(acc[e], mred(uniform_ptr(key + e * TH)[tid], v[e], m.q, m.qinv), m.q);
+        for (int e = 0; e < 16; e += 2) {  // key words in the transform's Quad layout
+            const u64v2 kw = *(const gv2 *)(key + Quad<Gm::LS>::elem(tid, e));
+            acc[e] = add_mod(acc[e], mred(kw.x, v[e], m.q, m.qinv), m.q);
+            acc[e + 1] = add_mod(acc[e + 1], mred(kw.y, v[e + 1], m.q, m.qinv), m.q);
+        }
     }
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = mred(acc[e], m.r2, m.q, m.qinv);
-    store_row<Gm::LS>(acc, Dd.xy + krow, tid);
+    store_row_quad<Gm::LS>(acc, Dd.xy + krow, tid);
 }
 
 template <int LOGN>

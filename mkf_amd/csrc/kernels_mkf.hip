@@ -36,11 +36,11 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
         const ulonglong2 *itw = T.itw + (size_t)ri * N;
         inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
         store_row<Gm::LS>(a0, o0, tid);
-        load_row_cached<Gm::LS>(a0, o1, tid);
+        load_row_quad<Gm::LS>(a0, o1, tid);
         inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
         store_row<Gm::LS>(a0, o1, tid);
-    } else {
-        store_row<Gm::LS>(a0, o0, tid);
+    } else {  // NTT-domain rows stay in the MAC's Quad layout
+        store_row_quad<Gm::LS>(a0, o0, tid);
     }
 }
 
@@ -78,18 +78,19 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
 #pragma unroll
     for (int e = 0; e < 16; ++e) xs[e] = 0;
     for (int a = 0; a < Dd.nacc; ++a) {
-        const u64 *xr = Dd.acc[a] + (size_t)j * N + hoff;
+        u64 xa[16];
+        load_row_quad<Gm::LS>(xa, Dd.acc[a] + (size_t)j * N + hoff, tid);
 #pragma unroll
-        for (int e = 0; e < 16; ++e) xs[e] = add_mod(xs[e], uniform_ptr(xr + e * Gm::TH)[tid], m.q);
+        for (int e = 0; e < 16; ++e) xs[e] = add_mod(xs[e], xa[e], m.q);
     }
 #pragma unroll
     for (int e = 0; e < 16; ++e) z[e] = mred(z[e] + 2 * m.q - xs[e], pinv, m.q, m.qinv);
     if (Dd.base) {
-        load_row_cached<Gm::LS>(xs, Dd.base + (size_t)j * N + hoff, tid);
+        load_row_quad<Gm::LS>(xs, Dd.base + (size_t)j * N + hoff, tid);
 #pragma unroll
         for (int e = 0; e < 16; ++e) z[e] = add_mod(z[e], xs[e], m.q);
     }
-    store_row<Gm::LS>(z, Dd.dst + (size_t)j * N + hoff, tid);
+    store_row_quad<Gm::LS>(z, Dd.dst + (size_t)j * N + hoff, tid);
 }
 
 // grid (N/256, ndesc * nl), one coefficient per thread
@@ -141,14 +142,14 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
         else
             fwd_transform<LOGN, 0>(w, f, tid, h, T.tw + (size_t)j * N, m);
         // x_j is loaded after the transform: it is not held across it
-        load_row_cached<Gm::LS>(v, Dd.x + (size_t)j * N + hoff, tid);
+        load_row_quad<Gm::LS>(v, Dd.x + (size_t)j * N + hoff, tid);
         const u64 inv = T.rs_inv[l * T.L + j];
 #pragma unroll
         for (int e = 0; e < 16; ++e) v[e] = mred(sub_mod(v[e], w[e], m.q), inv, m.q, m.qinv);
     } else {
-        load_row_cached<Gm::LS>(v, Dd.x + (size_t)j * N + hoff, tid);
+        load_row_quad<Gm::LS>(v, Dd.x + (size_t)j * N + hoff, tid);
     }
-    for (int o = 0; o < Dd.nout; ++o) store_row<Gm::LS>(v, Dd.out[o] + (size_t)j * N + hoff, tid);
+    for (int o = 0; o < Dd.nout; ++o) store_row_quad<Gm::LS>(v, Dd.out[o] + (size_t)j * N + hoff, tid);
 }
 
 // ---------------------------------------------------------------------------

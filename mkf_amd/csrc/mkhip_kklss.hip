@@ -80,7 +80,7 @@ int mkhip::plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, in
         w.D1 = cv.take(op.k1 * hoist);
         w.x = cv.take(keyw);  // full-level switching keys (k_ext reads them like keys)
         w.y = cv.take(keyw);
-        w.tnt = cv.take((2 + op.k0 + op.k1) * poly);
+        w.tnt = cv.take((c->alpha == 1 ? 2 : 2 + op.k0 + op.k1) * poly);  // alpha 1: the c0s only
         w.ct = cv.take((1 + ku) * poly);
         w.accx = cv.take(op.k1 * qp);
         w.accy = cv.take(op.k0 * qp);
@@ -133,21 +133,27 @@ int mkhip::plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, in
         mac.push_back(m1);
         kmax1 = std::max(kmax1, std::max(op.k0, op.k1));
         kmax2 = std::max(kmax2, op.k0);
-        // NTT of every input component: op0 c0, op0 comps, op1 c0, op1 comps
+        // NTT of every input component: op0 c0, op0 comps, op1 c0, op1 comps.  With
+        // one prime per digit, limb l of a decomposed component transformed mod q_l
+        // is row l of its digit l (keyswitch.go:22-32: the digit is the limb itself),
+        // so only the c0s are transformed here and the tensor reads the other
+        // components from the decomposition's diagonal rows.
         u64 *tnt = P(w.tnt);
         const int nc0 = 1 + op.k0, nc1 = 1 + op.k1;
+        const bool diag = c->alpha == 1;
         for (int comp = 0; comp < nc0 + nc1; ++comp) {
+            if (diag && comp != 0 && comp != nc0) continue;
             const u64 *src = comp < nc0 ? op.op0 + (size_t)comp * nl * N : op.op1 + (size_t)(comp - nc0) * nl * N;
             for (int l = 0; l < nl; ++l) {
                 NttJob j;
                 j.src = src + (size_t)l * N;
-                j.dst = tnt + ((size_t)comp * nl + l) * N;
+                j.dst = tnt + ((size_t)(diag ? (comp == 0 ? 0 : 1) : comp) * nl + l) * N;
                 j.mod = l;
                 j.reduce = 0;
                 ntt.push_back(j);
             }
         }
-        const u64 *a0 = tnt, *b0 = tnt + (size_t)nc0 * nl * N;
+        const u64 *a0 = tnt, *b0 = tnt + (size_t)(diag ? 1 : nc0) * nl * N;
         // tensor into ct (coefficient domain)
         u64 *ct = P(w.ct);
         {
@@ -164,8 +170,14 @@ int mkhip::plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, in
             TensorDesc td = {};
             td.a0 = a0;
             td.b0 = b0;
-            td.au = p0 >= 0 ? tnt + (size_t)(1 + p0) * nl * N : nullptr;
-            td.bu = p1 >= 0 ? tnt + (size_t)(nc0 + 1 + p1) * nl * N : nullptr;
+            if (diag) {  // limb l at D[p][digit l][row l]
+                td.au = p0 >= 0 ? P(w.D0) + (size_t)p0 * beta * rows * N : nullptr;
+                td.bu = p1 >= 0 ? P(w.D1) + (size_t)p1 * beta * rows * N : nullptr;
+                td.ustride = rows + 1;
+            } else {
+                td.au = p0 >= 0 ? tnt + (size_t)(1 + p0) * nl * N : nullptr;
+                td.bu = p1 >= 0 ? tnt + (size_t)(nc0 + 1 + p1) * nl * N : nullptr;
+            }
             td.out = ct + (size_t)(1 + u) * nl * N;
             td.is_c0 = 0;
             ten.push_back(td);

@@ -221,9 +221,26 @@ struct NttPlan<13> {
     typedef InvPass<13, 9, 4> I3;
 };
 
-// Forward NTT; v in Linear layout in and out; returns canonical values.  With
-// hh = 2 + h it is the transform of half h of a limb of 2^(LOGN+1) after that
-// limb's first stage (fwd_first_stage).
+// "Quad" layout: the layout of the last forward pass, which is also the layout
+// of the first inverse pass.  Lane tid holds, for c < 16/R, the R consecutive
+// elements (tid + c N/16) R + k (R = 4 at N = 2^14, 2 at N = 2^13), so a wave
+// reads or writes 64 R consecutive words per c.  Forward transforms leave their
+// output in this layout and inverse transforms take their input in it: the
+// exchange through LDS to the natural layout is skipped on both sides, and the
+// kernels read and write their rows at these element positions instead.
+template <int LOGN>
+struct Quad {
+    typedef typename NttPlan<LOGN>::F3 F;
+    typedef typename NttPlan<LOGN>::I0 I;
+    static constexpr int R = F::R;
+    static constexpr int T = (1 << LOGN) / 16;
+    static_assert(F::TL == 1 && I::TF == 1 && I::R == R, "last forward / first inverse pass must match");
+    __device__ static __forceinline__ int elem(int tid, int i) { return (tid + (i / R) * T) * R + (i % R); }
+};
+
+// Forward NTT; v in Linear layout in, Quad layout out; returns canonical values.
+// With hh = 2 + h it is the transform of half h of a limb of 2^(LOGN+1) after
+// that limb's first stage (fwd_first_stage).
 // LAZY (q < 2^57, mu = floor(2^64 / q)): unreduced butterflies, one Barrett
 // reduction per value at the end.
 template <int LOGN, bool LAZY = false>
@@ -238,7 +255,6 @@ __device__ __forceinline__ void ntt_fwd(u64 (&v)[16], u64 *s, int tid, const ulo
     P::F2::template run<LAZY>(v, tid, tw, q, hh);
     lds_exchange<typename P::F2, typename P::F3>(v, s, tid);
     P::F3::template run<LAZY>(v, tid, tw, q, hh);
-    lds_exchange<typename P::F3, Linear<LOGN> >(v, s, tid);
     if (LAZY) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -262,7 +278,7 @@ __device__ __forceinline__ void ntt_fwd_any(u64 (&v)[16], u64 *s, int tid, const
         ntt_fwd<LOGN, false>(v, s, tid, tw, q, hh);
 }
 
-// Inverse NTT including the N^-1 scaling; v < 4q in Linear layout; returns
+// Inverse NTT including the N^-1 scaling; v < 4q in Quad layout; returns
 // canonical values in Linear layout.  With hh = 2 + h and scale = false it runs
 // every stage of a limb of 2^(LOGN+1) that stays inside half h and leaves the
 // values in [0, 4q); inv_last_stage finishes the limb.
@@ -271,7 +287,6 @@ __device__ __forceinline__ void ntt_inv(u64 (&v)[16], u64 *s, int tid, const ulo
                                         u64 q, u64 ninv, u64 ninvp, int hh = 1, bool scale = true) {
     typedef NttPlan<LOGN> P;
     static_assert(P::I3::TF * 16 == (1 << LOGN), "last inverse pass must be the linear layout");
-    lds_exchange<Linear<LOGN>, typename P::I0>(v, s, tid);
     P::I0::run(v, tid, itw, q, hh);
     lds_exchange<typename P::I0, typename P::I1>(v, s, tid);
     P::I1::run(v, tid, itw, q, hh);

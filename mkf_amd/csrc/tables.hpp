@@ -187,7 +187,7 @@ struct TensorDesc {
     const u64 *a0, *b0, *au, *bu;  // au / bu null when the party is absent on that side
     u64 *out;
     int is_c0;
-    int pad;
+    int ustride;  // limb l of au / bu at l * ustride * N (0 or 1: contiguous)
 };
 
 // Key-stationary x / y accumulation over a batch: for one side of every op,
