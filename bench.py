@@ -509,16 +509,15 @@ def copy_calibration(mb):
     return {"kernel": "k_copy", "bytes_read": nbytes, "bytes_written": nbytes, "launches": 3}
 
 
-def load_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if any."""
+def load_pmc_summary(kernel):
+    """The committed rocprofv3 PMC summary of `kernel` (HBM bytes per launch from
+    FETCH_SIZE / WRITE_SIZE, VALUBusy), if it covers that kernel."""
     try:
         with open(TRAFFIC_JSON) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
-    if t.get("kernel") != kernel:
-        return None
-    return t.get("bytes_per_launch")
+        return {}
+    return t if t.get("kernel") == kernel else {}
 
 
 def main():
@@ -591,8 +590,10 @@ def main():
         dom_bytes = alg[names[dom]]
         achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
         kern = mkf_amd.stage_kernel(names[dom])
+        pmc = load_pmc_summary(kern)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kern),
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc.get("bytes_per_launch"),
+                    "valu_busy_pct": pmc.get("valu_busy_pct"),
                     "kernel": kern, "stage": names[dom], "kernel_ms": round(stage_ms[dom], 4),
                     "alg_bytes_per_launch": dom_bytes}
         cpu = None
