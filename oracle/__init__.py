@@ -102,6 +102,20 @@ def _declare(L):
     L.mkref_kklss_bfv_mulrelin.restype = I
     L.mkref_kklss_bfv_mulrelin.argtypes = [V, u64p, intp, I, u64p, intp, I, ctypes.POINTER(u64p),
                                            ctypes.POINTER(u64p), ctypes.POINTER(u64p), u64p, u64p, intp]
+    L.mkref_galois_element.restype = U
+    L.mkref_galois_element.argtypes = [I, ctypes.c_longlong]
+    L.mkref_galois_conjugate.restype = U
+    L.mkref_galois_conjugate.argtypes = [I]
+    L.mkref_galois_inverse.restype = U
+    L.mkref_galois_inverse.argtypes = [I, U]
+    L.mkref_permute.argtypes = [V, I, u64p, U, u64p]
+    L.mkref_permute_ntt_index.argtypes = [I, U, u64p]
+    L.mkref_automorphism_ntt.argtypes = [V, I, u64p, U, u64p]
+    L.mkref_external_product_as_written.argtypes = [V, I, u64p, u64p, u64p]
+    L.mkref_kklss_rotate_hoisted.argtypes = [V, I, I, u64p, u64p, ctypes.POINTER(u64p), u64p, U, u64p]
+    L.mkref_kklss_rotate.argtypes = [V, I, I, u64p, ctypes.POINTER(u64p), u64p, U, u64p]
+    L.mkref_kklss_conjugate.argtypes = [V, I, I, u64p, ctypes.POINTER(u64p), u64p, u64p]
+    L.mkref_mkf_ckks_rotate.argtypes = [V, I, I, u64p, ctypes.POINTER(u64p), U, u64p, u64p]
     L.mkref_prng_seed.argtypes = [ctypes.c_void_p, U]
     L.mkref_prng_next.restype = U
     L.mkref_prng_next.argtypes = [ctypes.c_void_p]
@@ -342,6 +356,112 @@ class Ctx:
         lib().mkref_mkf_ckks_mulrelin(self._h, level, k, _p(_u64(left)), _p(_u64(right)), arr,
                                       _p(_u64(relin_key)), _p(o0), _p(o1))
         return o0, o1
+
+
+    # ---- Galois key switches (mkref_rot.c) ----
+
+    def galois_element(self, k):
+        return galois_element(self.logN, k)
+
+    def permute(self, a, gal):
+        a = _u64(a)
+        out = np.zeros_like(a)
+        lib().mkref_permute(self._h, a.shape[0] - 1, _p(a), gal, _p(out))
+        return out
+
+    def automorphism_ntt(self, a, gal):
+        a = _u64(a)
+        out = np.zeros_like(a)
+        lib().mkref_automorphism_ntt(self._h, a.shape[0] - 1, _p(a), gal, _p(out))
+        return out
+
+    def external_product_as_written(self, a, key):
+        a = _u64(a)
+        out = np.zeros_like(a)
+        lib().mkref_external_product_as_written(self._h, a.shape[0] - 1, _p(a), _p(_u64(key)), _p(out))
+        return out
+
+    @staticmethod
+    def _keyarr(keys):
+        keys = [_u64(k) for k in keys]
+        return keys, (u64p * max(len(keys), 1))(*[_p(k) for k in keys])
+
+    def kklss_rotate_hoisted(self, ct, D, rks, a, gal):
+        ct = _u64(ct)
+        keep, arr = self._keyarr(rks)
+        out = np.zeros_like(ct)
+        D = _u64(D) if len(rks) else np.zeros(1, np.uint64)
+        lib().mkref_kklss_rotate_hoisted(self._h, ct.shape[1] - 1, ct.shape[0] - 1, _p(ct), _p(D), arr,
+                                         _p(_u64(a)), gal, _p(out))
+        return out
+
+    def kklss_rotate(self, ct, rks, a, gal):
+        ct = _u64(ct)
+        keep, arr = self._keyarr(rks)
+        out = np.zeros_like(ct)
+        lib().mkref_kklss_rotate(self._h, ct.shape[1] - 1, ct.shape[0] - 1, _p(ct), arr, _p(_u64(a)), gal, _p(out))
+        return out
+
+    def kklss_conjugate(self, ct, cks, a):
+        ct = _u64(ct)
+        keep, arr = self._keyarr(cks)
+        out = np.zeros_like(ct)
+        lib().mkref_kklss_conjugate(self._h, ct.shape[1] - 1, ct.shape[0] - 1, _p(ct), arr, _p(_u64(a)), _p(out))
+        return out
+
+    def kklss_rotate_new(self, ct, ids, rotidx, rot_keys, crs, hoisted=False):
+        """mkckks/mkbfv Evaluator.RotateNew / RotateHoistedNew orchestration
+        (mkckks/evaluator.go:496-529, :598-624; mkbfv/evaluator.go:164-194):
+        rotidx normalised into [0, N/2); 0 copies; a rotation with a CRS entry runs
+        once; otherwise (RotateNew only) the binary expansion of rotidx is applied
+        as rotations by 1, 2, 4, ...  rot_keys[(party, r)], crs[r].  Raises
+        KeyError where the reference panics."""
+        half = self.N // 2
+        r = rotidx % half
+        if r == 0:
+            return _u64(ct).copy()
+        steps = [r] if r in crs else None
+        if steps is None:
+            if hoisted:
+                raise KeyError("hoisted rotation needs a precomputed rotation key")
+            steps, k, rr = [], 1, r
+            while rr > 0:
+                if rr % 2:
+                    steps.append(k)
+                rr //= 2
+                k *= 2
+        cur = _u64(ct)
+        for st in steps:
+            rks = [rot_keys[(pid, st)] for pid in ids]
+            cur = self.kklss_rotate(cur, rks, crs[st], galois_element(self.logN, st))
+        return cur
+
+    def mkf_ckks_rotate(self, ct, keys, gal):
+        ct = _u64(ct)
+        keep, arr = self._keyarr(keys)
+        o0 = np.zeros_like(ct[0])
+        o1 = np.zeros_like(ct[0])
+        lib().mkref_mkf_ckks_rotate(self._h, ct.shape[1] - 1, ct.shape[0] - 1, _p(ct), arr, gal, _p(o0), _p(o1))
+        return o0, o1
+
+
+def galois_element(logN, k):
+    """GaloisElementForColumnRotationBy: 5^(k mod 2N) mod 2N."""
+    return int(lib().mkref_galois_element(logN, k))
+
+
+def galois_conjugate(logN):
+    return int(lib().mkref_galois_conjugate(logN))
+
+
+def galois_inverse(logN, g):
+    return int(lib().mkref_galois_inverse(logN, g))
+
+
+def permute_ntt_index(logN, gal):
+    out = np.zeros(1 << logN, np.uint64)
+    lib().mkref_permute_ntt_index(logN, gal, _p(out))
+    return out
 
 
 class BfvCtx:

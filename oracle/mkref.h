@@ -149,6 +149,29 @@ void mkref_mkf_ckks_mulrelin(const mkref_ctx *c, int level, int k,
                              u64 *out0, u64 *out1);
 void mkref_div_round_last_ntt(const mkref_ctx *c, int level, int n, u64 *poly);
 
+/* ---- Galois key switches (mkrlwe Rotate/Conjugate, MKF RotateNew), in mkref_rot.c ---- */
+u64  mkref_galois_element(int logN, long long k);   /* 5^(k mod 2N) mod 2N          */
+u64  mkref_galois_conjugate(int logN);             /* 2N - 1                        */
+u64  mkref_galois_inverse(int logN, u64 g);        /* g^(2N-1) mod 2N               */
+/* coefficient-domain automorphism of limbs 0..level (negation written q - x) */
+void mkref_permute(const mkref_ctx *c, int level, const u64 *in, u64 gal, u64 *out);
+/* NTT-slot permutation: NTT(Permute(a))[i] = NTT(a)[index[i]] */
+void mkref_permute_ntt_index(int logN, u64 gal, u64 *index);
+/* InvNTT, Permute, NTT of limbs 0..level */
+void mkref_automorphism_ntt(const mkref_ctx *c, int level, const u64 *in, u64 gal, u64 *out);
+/* ExternalProduct as the reference writes it (one decomposition buffer reused) */
+void mkref_external_product_as_written(const mkref_ctx *c, int levelQ, const u64 *a, const u64 *key, u64 *out);
+/* KKLSS RotateHoisted / Rotate / Conjugate: in and out [1+k][level+1][N] coefficient domain */
+void mkref_kklss_rotate_hoisted(const mkref_ctx *c, int level, int k, const u64 *in, const u64 *D,
+                                const u64 *const *rk, const u64 *a, u64 gal, u64 *out);
+void mkref_kklss_rotate(const mkref_ctx *c, int level, int k, const u64 *in, const u64 *const *rk, const u64 *a,
+                        u64 gal, u64 *out);
+void mkref_kklss_conjugate(const mkref_ctx *c, int level, int k, const u64 *in, const u64 *const *ck,
+                           const u64 *a, u64 *out);
+/* MKF RotateNew: in [1+k][level+1][N] NTT; outputs sigma(C0) + sum u0_i and sum u1_i (NTT) */
+void mkref_mkf_ckks_rotate(const mkref_ctx *c, int level, int k, const u64 *in, const u64 *const *keys, u64 gal,
+                           u64 *out0, u64 *out1);
+
 /* ---- MK-BFV (software/kklss/mkbfv, software/mkf/mkf_cpu/bfv.go), in mkref_bfv.c ---- */
 /* R = Q u QMul (mkbfv/params.go:36-81): the QP context plus the QMul chain and
  * the plaintext modulus t.  gamma 2 for KKLSS (the decomposition of R polys is

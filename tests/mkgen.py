@@ -172,6 +172,59 @@ class KKLSSKeys:
         return acc
 
 
+def galois_small(x, gal):
+    """sigma_gal(x) for an integer polynomial x (length N) in Z[X]/(X^N+1):
+    X^i -> X^(i gal), folded with X^N = -1."""
+    N = len(x)
+    out = [0] * N
+    for i, v in enumerate(x):
+        e = (i * gal) % (2 * N)
+        if e < N:
+            out[e] += int(v)
+        else:
+            out[e - N] -= int(v)
+    return np.array(out, np.int64)
+
+
+def kklss_galois_keys(keys, gal_of, g, rotations=(), conj_crs=None, crs=None):
+    """Rotation and conjugation keys for a KKLSSKeys/BFVKeys instance
+    (mkrlwe/keygen.go:190-268), returned as dicts.
+
+    rotation key of party i for rotidx r (GenRotationKey :190-230):
+        rk_j = MForm(-a_j sigma_{g^-1}(s) + e_j + g_j P s),   a = CRS[r], g = galEl(r)
+    conjugation key (GenConjugationKey :240-268):
+        ck_j = MForm(-a_j s + e_j + g_j P sigma_{2N-1}(s)),   a = CRS[-2]
+    crs maps r -> MForm uniform CRS (generated when absent)."""
+    ctx = keys.ctx
+    QP, N = ctx.QP, ctx.N
+    crs = {} if crs is None else crs
+    rot = {}
+    for r in rotations:
+        if r not in crs:
+            crs[r] = np.stack([mform(g.uniform(QP, N), QP) for _ in range(keys.beta)])
+        ginv = pow(gal_of(r), -1, 2 * N)
+        for pid, s_small in enumerate(keys.sk_small):
+            s_rot = ctx.ntt_poly(O.small_to_rns(galois_small(s_small, ginv), QP))
+            out = []
+            for j in range(keys.beta):
+                x = keys._gadget(g, keys.sk[pid], j)
+                ar = mulmod(keys._plain(crs[r][j]), s_rot, QP)
+                out.append(mform(submod(x, ar, QP), QP))
+            rot[(pid, r)] = np.stack(out)
+    conj = {}
+    if conj_crs is not None:
+        for pid, s_small in enumerate(keys.sk_small):
+            s = keys._plain(keys.sk[pid])
+            s_c = mform(ctx.ntt_poly(O.small_to_rns(galois_small(s_small, 2 * N - 1), QP)), QP)
+            out = []
+            for j in range(keys.beta):
+                x = keys._gadget(g, s_c, j)
+                as_ = mulmod(keys._plain(conj_crs[j]), s, QP)
+                out.append(mform(submod(x, as_, QP), QP))
+            conj[pid] = np.stack(out)
+    return rot, conj, crs
+
+
 class MKFKeys:
     """MKF aggregate-secret keys (software/mkf/mkf_cpu/common.go:32-51) over a
     gamma = 1 context: secrets s_i, s = sum s_i, fold keys s_i -> s and the
@@ -214,6 +267,10 @@ class MKFKeys:
                 v0[idx] = np.array((v0[idx].astype(object) + s_in[idx].astype(object) * (Pprod % q)) % q, np.uint64)
             out.append(np.stack([mform(v0, QP), mform(a, QP)]))
         return np.stack(out)
+
+    def rotation_keys(self, gal, g):
+        """Party-specific Galois keys sigma_gal(s_i) -> s (true_mkf_ckks.go:379-390)."""
+        return [self._swk(g, self._ntt_small(galois_small(x, gal)), self.s) for x in self.small]
 
     def _pk(self, g, s):
         ctx = self.ctx
