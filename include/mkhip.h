@@ -76,7 +76,11 @@ void mkhip_ctx_destroy(mkhip_ctx *ctx);
 int mkhip_ctx_info(const mkhip_ctx *ctx, int *alpha, int *beta, size_t *swk_words);
 
 /*
- * Resident evaluation keys (HBM).  CRS index -1 is the KKLSS `u` (params.go:38).
+ * Resident evaluation keys (HBM).  CRS index -1 is the KKLSS `u` (params.go:38);
+ * the rotation / conjugation CRS indices are described with the Galois key switches.
+ * Plans refer to the resident keys: destroy the plans that use a key before
+ * removing or replacing it (the host-pointer entry points do this for their own
+ * cached plan).
  * A party's relinearization key (b, d, v) replaces mkrlwe.RelinearizationKey
  * (keys.go:133-136) and RelinearizationKeySet.AddRelinearizationKey (keys.go:277).
  */
@@ -206,6 +210,79 @@ int mkhip_bfv_quantize(mkhip_ctx *ctx, const uint64_t *in_ntt, uint64_t *out);
 int mkhip_bfv_decompose(mkhip_ctx *ctx, const uint64_t *aR, uint64_t *out);
 
 /*
+ * Galois key switches (rotations and conjugation).
+ *
+ * KKLSS (mkrlwe, both the MK-CKKS and the MK-BFV evaluators).  Keys are mkrlwe
+ * switching keys [beta][L+K][N] (MForm, NTT) as the relinearization keys:
+ *   mkhip_set_crs(ctx, r, a)            CRS[r] of rotation r (1 <= r < N/2) or of
+ *                                       conjugation (r = -2) (params.go:35-58, AddCRS)
+ *   mkhip_add_rotation_key(ctx, id, r)  RotationKeySet.AddRotationKey (keys.go:133-143),
+ *                                       a key made by GenRotationKey (keygen.go:190-230)
+ *   mkhip_del_rotation_key              RotationKeySet.DelRotationKey (keys.go:146-148)
+ *   mkhip_add_conjugation_key           ConjugationKeySet.AddConjugationKey (keys.go:208-211),
+ *                                       GenConjugationKey (keygen.go:240-268)
+ *   mkhip_del_conjugation_key           ConjugationKeySet.DelConjugationKey (keys.go:213-216)
+ *
+ * mkhip_kklss_rotate: Evaluator.RotateNew (hoisted = 0; mkckks/evaluator.go:488-529,
+ * mkbfv/evaluator.go:156-194) or RotateHoistedNew (hoisted = 1; mkckks :590-624).
+ * rotidx is normalised into [0, N/2); 0 copies the input; a rotation with a CRS
+ * entry is one key switch (keyswitch.go:243-307); otherwise RotateNew applies the
+ * rotations 1, 2, 4, ... of rotidx's binary expansion one after the other and
+ * RotateHoistedNew fails with MKHIP_ERR_NO_KEY (the reference panics).
+ * mkhip_kklss_conjugate: Evaluator.ConjugateNew (keyswitch.go:309-344).
+ * in / out: [1+k][level+1][N] coefficient domain, parties ids[k]; out keeps the
+ * party order.  Every key switch uses the hoisted external product
+ * (keyswitch_hoisted.go:11-45); the reference's non-hoisted ExternalProduct
+ * (keyswitch.go:82-130) multiplies every key digit by the last digit's
+ * decomposition, so for more than one digit its Rotate / Conjugate outputs differ
+ * from these (and from its own RotateHoisted).
+ */
+#define MKHIP_MAX_ROTATIONS 16
+#define MKHIP_CONJUGATE (-2) /* rot[] entry of a conjugation (its CRS index) */
+int mkhip_add_rotation_key(mkhip_ctx *ctx, int party, int rotidx, const uint64_t *swk);
+int mkhip_del_rotation_key(mkhip_ctx *ctx, int party, int rotidx);
+int mkhip_add_conjugation_key(mkhip_ctx *ctx, int party, const uint64_t *swk);
+int mkhip_del_conjugation_key(mkhip_ctx *ctx, int party);
+int mkhip_kklss_rotate(mkhip_ctx *ctx, const uint64_t *in, int level, const int *ids, int k, int rotidx,
+                       int hoisted, uint64_t *out);
+int mkhip_kklss_conjugate(mkhip_ctx *ctx, const uint64_t *in, int level, const int *ids, int k, uint64_t *out);
+
+/* Batched over device operands.  One decomposition of the input serves every
+ * rotation of the op (the hoisting of RotateHoisted); a conjugation decomposes
+ * the conjugated input.  rot[r] is a normalised rotidx with a CRS entry, or
+ * MKHIP_CONJUGATE; out holds nrot ciphertexts back to back, each
+ * [1+k][level+1][N].  Every op of a plan has the same level. */
+typedef struct {
+    const uint64_t *in; /* device, [1+k][level+1][N] coefficient domain */
+    uint64_t *out;      /* device, nrot x [1+k][level+1][N]             */
+    int ids[MKHIP_MAX_PARTIES];
+    int k, level, nrot;
+    int rot[MKHIP_MAX_ROTATIONS];
+} mkhip_rotate_op;
+int mkhip_plan_kklss_rotate(mkhip_ctx *ctx, int nops, const mkhip_rotate_op *ops, int use_graph,
+                            mkhip_plan **out);
+
+/*
+ * MKF party-specific rotation (paperMKFCKKSEvaluator.RotateNew,
+ * software/mkf/mkf_cpu/true_mkf_ckks.go:323-353) on a gamma = 1 CKKS context.
+ * mkhip_register_mkf_rotation stores, for the Galois element of `rotation`
+ * (5^rotation mod 2N), the group's k party keys sigma(s_i) -> s_aggregate
+ * (Lattigo switching keys [beta][2][L+K][N], true_mkf_ckks.go:379-390).
+ * mkhip_mkf_ckks_rotate: in [1+k][level+1][N] NTT domain; out [1+k][level+1][N]
+ * NTT domain: sigma(C0) + sum_i u0_i, then k copies of sum_i u1_i.  A missing key
+ * set is MKHIP_ERR_NO_KEY (the reference's error at :328-330).
+ */
+int mkhip_register_mkf_rotation(mkhip_ctx *ctx, int group, int rotation, const uint64_t *const *keys);
+int mkhip_mkf_ckks_rotate(mkhip_ctx *ctx, int group, const uint64_t *in, int level, int rotation, uint64_t *out);
+typedef struct {
+    const uint64_t *in; /* device, [1+k][level+1][N] NTT domain */
+    uint64_t *out;      /* device, [1+k][level+1][N] NTT domain */
+    int group, level, rotation;
+} mkhip_mkf_rotate_op;
+int mkhip_plan_mkf_rotate(mkhip_ctx *ctx, int nops, const mkhip_mkf_rotate_op *ops, int use_graph,
+                          mkhip_plan **out);
+
+/*
  * Tuning knobs (no counterpart in the reference: they choose between equivalent
  * kernel schedules; every setting gives the same bits).
  *   MKHIP_TUNE_FUSED_DECOMP  alpha = 1 KKLSS plans: the gadget decomposition fused
@@ -225,6 +302,8 @@ int mkhip_bfv_decompose(mkhip_ctx *ctx, const uint64_t *aR, uint64_t *out);
 #define MKHIP_SCHEME_KKLSS_BFV 1
 #define MKHIP_SCHEME_MKF_CKKS 2
 #define MKHIP_SCHEME_MKF_BFV 3
+#define MKHIP_SCHEME_KKLSS_ROTATE 4 /* ops: mkhip_rotate_op[]     */
+#define MKHIP_SCHEME_MKF_ROTATE 5   /* ops: mkhip_mkf_rotate_op[] */
 int mkhip_set_tuning(mkhip_ctx *ctx, int knob, int value);
 int mkhip_get_tuning(const mkhip_ctx *ctx, int knob, int *value);
 int mkhip_autotune(mkhip_ctx *ctx, int scheme, int nops, const void *ops, int iters, int *ncand, double *ms);

@@ -41,10 +41,14 @@ EXPORTS = [
     "mkhip_batch_time", "mkhip_batch_workspace_bytes", "mkhip_batch_branch_info", "mkhip_batch_destroy",
     "mkhip_server_create", "mkhip_server_add_class", "mkhip_server_start", "mkhip_server_enqueue",
     "mkhip_server_poll", "mkhip_server_wait", "mkhip_server_stats", "mkhip_server_destroy", "mkhip_host_alloc",
-    "mkhip_host_free",
+    "mkhip_host_free", "mkhip_add_rotation_key", "mkhip_del_rotation_key", "mkhip_add_conjugation_key",
+    "mkhip_del_conjugation_key", "mkhip_kklss_rotate", "mkhip_kklss_conjugate", "mkhip_plan_kklss_rotate",
+    "mkhip_register_mkf_rotation", "mkhip_mkf_ckks_rotate", "mkhip_plan_mkf_rotate",
 ]
 TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER = 1, 2, 3
-SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3}
+SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3, "rotate": 4, "mkf_rotate": 5}
+MAX_ROTATIONS = 16
+CONJUGATE = -2
 
 # kernel run by each named plan stage
 STAGE_KERNELS = {
@@ -56,6 +60,7 @@ STAGE_KERNELS = {
     "intt_last": "k_ntt_batch<inv>", "rescale": "k_rescale_ntt",
     "lift": "k_bfv_lift", "quantize": "k_quantize", "moddown_final": "k_moddown<true>",
     "decompose_relin": "k_decomp", "materialize": "k_rescale_ntt", "decompose_mac": "k_decomp_mac",
+    "conj_permute": "k_automorph", "permute": "k_automorph", "ext": "k_ext<2>", "moddown": "k_moddown<false>",
 }
 # the BFV plans' tensor stage runs the R-basis kernel; MKF-BFV's ModDowns are coefficient-domain
 BFV_STAGE_KERNELS = dict(STAGE_KERNELS, tensor="k_tensor_r")
@@ -107,6 +112,20 @@ class MKFOp(ctypes.Structure):
     _fields_ = [
         ("left", ctypes.c_void_p), ("right", ctypes.c_void_p), ("out", ctypes.c_void_p),
         ("group", ctypes.c_int), ("level", ctypes.c_int), ("nres", ctypes.c_int),
+    ]
+
+
+class RotateOp(ctypes.Structure):
+    _fields_ = [
+        ("in_", ctypes.c_void_p), ("out", ctypes.c_void_p), ("ids", ctypes.c_int * MAX_PARTIES),
+        ("k", ctypes.c_int), ("level", ctypes.c_int), ("nrot", ctypes.c_int), ("rot", ctypes.c_int * 16),
+    ]
+
+
+class MKFRotateOp(ctypes.Structure):
+    _fields_ = [
+        ("in_", ctypes.c_void_p), ("out", ctypes.c_void_p),
+        ("group", ctypes.c_int), ("level", ctypes.c_int), ("rotation", ctypes.c_int),
     ]
 
 
@@ -163,6 +182,16 @@ def _declare(L):
     L.mkhip_unregister_mkf_group.argtypes = [V, I]
     L.mkhip_mkf_ckks_mulrelin.argtypes = [V, I, u64p, u64p, I, D, D, D, u64p, intp, dblp]
     L.mkhip_plan_mkf_ckks.argtypes = [V, I, ctypes.POINTER(MKFOp), I, ctypes.POINTER(V)]
+    L.mkhip_add_rotation_key.argtypes = [V, I, I, u64p]
+    L.mkhip_del_rotation_key.argtypes = [V, I, I]
+    L.mkhip_add_conjugation_key.argtypes = [V, I, u64p]
+    L.mkhip_del_conjugation_key.argtypes = [V, I]
+    L.mkhip_kklss_rotate.argtypes = [V, u64p, I, intp, I, I, I, u64p]
+    L.mkhip_kklss_conjugate.argtypes = [V, u64p, I, intp, I, u64p]
+    L.mkhip_plan_kklss_rotate.argtypes = [V, I, ctypes.POINTER(RotateOp), I, ctypes.POINTER(V)]
+    L.mkhip_register_mkf_rotation.argtypes = [V, I, I, ctypes.POINTER(u64p)]
+    L.mkhip_mkf_ckks_rotate.argtypes = [V, I, u64p, I, I, u64p]
+    L.mkhip_plan_mkf_rotate.argtypes = [V, I, ctypes.POINTER(MKFRotateOp), I, ctypes.POINTER(V)]
     L.mkhip_register_bfv_party.argtypes = [V, I, u64p, u64p, u64p, u64p, u64p]
     L.mkhip_kklss_bfv_mulrelin.argtypes = [V, u64p, intp, I, u64p, intp, I, u64p, intp, intp]
     L.mkhip_plan_kklss_bfv.argtypes = [V, I, ctypes.POINTER(KKLSSOp), I, ctypes.POINTER(V)]
@@ -265,6 +294,48 @@ class Context:
     def unregister_party(self, pid):
         check(lib().mkhip_unregister_party(self._h, pid), "mkhip_unregister_party")
 
+    def set_crs(self, index, a):
+        """CRS of a rotation (index = rotidx in [1, N/2)) or of conjugation (CONJUGATE)."""
+        a = _u64(a)
+        assert a.size == self.swk_words
+        check(lib().mkhip_set_crs(self._h, index, _p(a)), "mkhip_set_crs")
+
+    def add_rotation_key(self, pid, rotidx, key):
+        key = _u64(key)
+        assert key.size == self.swk_words
+        check(lib().mkhip_add_rotation_key(self._h, pid, rotidx, _p(key)), "mkhip_add_rotation_key")
+
+    def del_rotation_key(self, pid, rotidx):
+        check(lib().mkhip_del_rotation_key(self._h, pid, rotidx), "mkhip_del_rotation_key")
+
+    def add_conjugation_key(self, pid, key):
+        key = _u64(key)
+        assert key.size == self.swk_words
+        check(lib().mkhip_add_conjugation_key(self._h, pid, _p(key)), "mkhip_add_conjugation_key")
+
+    def del_conjugation_key(self, pid):
+        check(lib().mkhip_del_conjugation_key(self._h, pid), "mkhip_del_conjugation_key")
+
+    def kklss_rotate(self, ct, ids, rotidx, hoisted=False):
+        """KKLSS Evaluator.RotateNew / RotateHoistedNew; ct (1+k, level+1, N) coefficient domain."""
+        ct = _u64(ct)
+        k = len(ids)
+        assert ct.shape[0] == 1 + k
+        out = np.zeros_like(ct)
+        i = (ctypes.c_int * max(k, 1))(*ids)
+        check(lib().mkhip_kklss_rotate(self._h, _p(ct), ct.shape[1] - 1, i, k, rotidx, int(hoisted), _p(out)),
+              "mkhip_kklss_rotate")
+        return out
+
+    def kklss_conjugate(self, ct, ids):
+        ct = _u64(ct)
+        k = len(ids)
+        assert ct.shape[0] == 1 + k
+        out = np.zeros_like(ct)
+        i = (ctypes.c_int * max(k, 1))(*ids)
+        check(lib().mkhip_kklss_conjugate(self._h, _p(ct), ct.shape[1] - 1, i, k, _p(out)), "mkhip_kklss_conjugate")
+        return out
+
     # ---- evaluator ----
     def rescale_count(self, level, scale, min_scale):
         n, s = ctypes.c_int(), ctypes.c_double()
@@ -350,6 +421,19 @@ class Context:
         arr = (u64p * len(keys))(*[_p(f) for f in keys])
         check(lib().mkhip_register_mkf_group(self._h, group, len(keys), arr, _p(relin)),
               "mkhip_register_mkf_group")
+
+    def register_mkf_rotation(self, group, rotation, keys):
+        keys = [_u64(x) for x in keys]
+        arr = (u64p * len(keys))(*[_p(x) for x in keys])
+        check(lib().mkhip_register_mkf_rotation(self._h, group, rotation, arr), "mkhip_register_mkf_rotation")
+
+    def mkf_ckks_rotate(self, group, ct, rotation):
+        """MKF RotateNew; ct (1+k, level+1, N) NTT domain -> (1+k, level+1, N)."""
+        ct = _u64(ct)
+        out = np.zeros_like(ct)
+        check(lib().mkhip_mkf_ckks_rotate(self._h, group, _p(ct), ct.shape[1] - 1, rotation, _p(out)),
+              "mkhip_mkf_ckks_rotate")
+        return out
 
     def unregister_mkf_group(self, group):
         check(lib().mkhip_unregister_mkf_group(self._h, group), "mkhip_unregister_mkf_group")
@@ -462,6 +546,23 @@ def _marshal_ops(scheme, ops):
                 a.ids1[j] = v
             a.k0, a.k1, a.level, a.nres = len(o["ids0"]), len(o["ids1"]), o["level"], o["nres"]
         return arr
+    if scheme == "rotate":
+        arr = (RotateOp * len(ops))()
+        for i, o in enumerate(ops):
+            a = arr[i]
+            a.in_, a.out = o["in"], o["out"]
+            for j, v in enumerate(o["ids"]):
+                a.ids[j] = v
+            a.k, a.level, a.nrot = len(o["ids"]), o["level"], len(o["rot"])
+            for j, v in enumerate(o["rot"]):
+                a.rot[j] = v
+        return arr
+    if scheme == "mkf_rotate":
+        arr = (MKFRotateOp * len(ops))()
+        for i, o in enumerate(ops):
+            a = arr[i]
+            a.in_, a.out, a.group, a.level, a.rotation = o["in"], o["out"], o["group"], o["level"], o["rotation"]
+        return arr
     if scheme in ("mkf", "mkf_bfv"):
         arr = (MKFOp * len(ops))()
         for i, o in enumerate(ops):
@@ -478,12 +579,16 @@ class Plan:
     scheme "kklss": ops are dicts {op0, op1, out, ids0, ids1, level, nres};
     scheme "bfv":   the same dicts, level = L-1 and nres = 0 (KKLSS MK-BFV);
     scheme "mkf":   ops are dicts {left, right, out, group, level, nres};
-    scheme "mkf_bfv": the same dicts, level = L-1 and nres = 0 (MKF MK-BFV)."""
+    scheme "mkf_bfv": the same dicts, level = L-1 and nres = 0 (MKF MK-BFV);
+    scheme "rotate": Galois key switches, dicts {in, out, ids, level, rot} with rot a
+                     list of normalised rotidx (CRS present) or CONJUGATE;
+    scheme "mkf_rotate": dicts {in, out, group, level, rotation}."""
 
     def __init__(self, ctx, ops, use_graph=True, scheme="kklss"):
         h = ctypes.c_void_p()
         fn = {"kklss": "mkhip_plan_kklss_ckks", "bfv": "mkhip_plan_kklss_bfv", "mkf": "mkhip_plan_mkf_ckks",
-              "mkf_bfv": "mkhip_plan_mkf_bfv"}[scheme]
+              "mkf_bfv": "mkhip_plan_mkf_bfv", "rotate": "mkhip_plan_kklss_rotate",
+              "mkf_rotate": "mkhip_plan_mkf_rotate"}[scheme]
         arr = _marshal_ops(scheme, ops)
         check(getattr(lib(), fn)(ctx.handle, len(ops), arr, int(use_graph), ctypes.byref(h)), fn)
         self._h = h

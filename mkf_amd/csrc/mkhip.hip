@@ -347,6 +347,12 @@ extern "C" void mkhip_ctx_destroy(mkhip_ctx *c) {
         hipFree(kv.second.relin);
     }
     if (c->d_crs_u) hipFree(c->d_crs_u);
+    for (auto &kv : c->crs) hipFree(kv.second);
+    for (auto &kv : c->rot_keys) hipFree(kv.second);
+    for (auto &kv : c->conj_keys) hipFree(kv.second);
+    for (auto &kv : c->mkf_groups)
+        for (auto &r : kv.second.rot)
+            for (u64 *p : r.second) hipFree(p);
     if (c->d_tables) hipFree(c->d_tables);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -360,7 +366,14 @@ extern "C" int mkhip_ctx_info(const mkhip_ctx *c, int *alpha, int *beta, size_t 
     return MKHIP_OK;
 }
 
-static int upload_new(mkhip_ctx *c, const uint64_t *src, u64 **dst) {
+void mkhip::drop_cached_plan(mkhip_ctx *c) {
+    if (!c->single) return;
+    (void)hipStreamSynchronize(c->stream);
+    mkhip_plan_destroy(c->single);
+    c->single = nullptr;
+}
+
+int mkhip::upload_key(mkhip_ctx *c, const uint64_t *src, u64 **dst) {
     void *p = nullptr;
     HIP_TRY(hipMalloc(&p, c->swk_words * 8));
     hipError_t e = hipMemcpy(p, src, c->swk_words * 8, hipMemcpyHostToDevice);
@@ -374,12 +387,28 @@ static int upload_new(mkhip_ctx *c, const uint64_t *src, u64 **dst) {
 
 extern "C" int mkhip_set_crs(mkhip_ctx *c, int index, const uint64_t *swk) {
     if (!c || !swk) return MKHIP_ERR_ARG;
-    if (index != -1) return MKHIP_ERR_ARG;
-    if (c->d_crs_u) {
-        hipFree(c->d_crs_u);
-        c->d_crs_u = nullptr;
+    if (index == -1) {
+        if (c->d_crs_u) {
+            drop_cached_plan(c);
+            hipStreamSynchronize(c->stream);
+            hipFree(c->d_crs_u);
+            c->d_crs_u = nullptr;
+        }
+        return upload_key(c, swk, &c->d_crs_u);
     }
-    return upload_new(c, swk, &c->d_crs_u);
+    // rotation CRS (index = rotidx in [1, N/2)) or conjugation CRS (-2)
+    if (index != -2 && (index < 1 || index >= c->N / 2)) return MKHIP_ERR_ARG;
+    u64 *p = nullptr;
+    int s = upload_key(c, swk, &p);
+    if (s != MKHIP_OK) return s;
+    auto it = c->crs.find(index);
+    if (it != c->crs.end()) {
+        drop_cached_plan(c);
+        hipStreamSynchronize(c->stream);
+        hipFree(it->second);
+    }
+    c->crs[index] = p;
+    return MKHIP_OK;
 }
 
 extern "C" int mkhip_register_party(mkhip_ctx *c, int party, const uint64_t *b, const uint64_t *d, const uint64_t *v) {
@@ -389,12 +418,12 @@ extern "C" int mkhip_register_party(mkhip_ctx *c, int party, const uint64_t *b, 
     if (c->parties.count(party)) mkhip_unregister_party(c, party);
     PartyKeys k;
     int s;
-    if ((s = upload_new(c, b, &k.b)) != MKHIP_OK) return s;
-    if ((s = upload_new(c, d, &k.d)) != MKHIP_OK) {
+    if ((s = upload_key(c, b, &k.b)) != MKHIP_OK) return s;
+    if ((s = upload_key(c, d, &k.d)) != MKHIP_OK) {
         hipFree(k.b);
         return s;
     }
-    if ((s = upload_new(c, v, &k.v)) != MKHIP_OK) {
+    if ((s = upload_key(c, v, &k.v)) != MKHIP_OK) {
         hipFree(k.b);
         hipFree(k.d);
         return s;
@@ -431,7 +460,7 @@ extern "C" int mkhip_register_bfv_party(mkhip_ctx *c, int party, const uint64_t 
         hipFree(k.b);
         return s;
     }
-    if ((s = upload_new(c, v0, &k.v)) != MKHIP_OK) {
+    if ((s = upload_key(c, v0, &k.v)) != MKHIP_OK) {
         hipFree(k.b);
         hipFree(k.d);
         return s;
@@ -444,6 +473,7 @@ extern "C" int mkhip_unregister_party(mkhip_ctx *c, int party) {
     if (!c) return MKHIP_ERR_ARG;
     auto it = c->parties.find(party);
     if (it == c->parties.end()) return MKHIP_ERR_UNKNOWN_PARTY;
+    drop_cached_plan(c);
     hipStreamSynchronize(c->stream);
     hipFree(it->second.b);
     hipFree(it->second.d);

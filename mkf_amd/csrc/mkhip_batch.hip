@@ -37,8 +37,17 @@ struct mkhip_batch {
 namespace {
 
 size_t op_size(int scheme) {
-    return (scheme == MKHIP_SCHEME_KKLSS_CKKS || scheme == MKHIP_SCHEME_KKLSS_BFV) ? sizeof(mkhip_kklss_op)
-                                                                                  : sizeof(mkhip_mkf_op);
+    switch (scheme) {
+    case MKHIP_SCHEME_KKLSS_CKKS:
+    case MKHIP_SCHEME_KKLSS_BFV:
+        return sizeof(mkhip_kklss_op);
+    case MKHIP_SCHEME_KKLSS_ROTATE:
+        return sizeof(mkhip_rotate_op);
+    case MKHIP_SCHEME_MKF_ROTATE:
+        return sizeof(mkhip_mkf_rotate_op);
+    default:
+        return sizeof(mkhip_mkf_op);
+    }
 }
 
 // workspace bytes of a plan over ops [0, n)
@@ -81,7 +90,7 @@ extern "C" int mkhip_batch_add(mkhip_batch *b, mkhip_ctx *c, int scheme, int nop
                                size_t ws_budget, int *nchunks) {
     if (!b || !c || !ops || nops <= 0) return MKHIP_ERR_ARG;
     if (b->ge) return MKHIP_ERR_ARG;  // already finalized
-    if (scheme < MKHIP_SCHEME_KKLSS_CKKS || scheme > MKHIP_SCHEME_MKF_BFV) return MKHIP_ERR_ARG;
+    if (scheme < MKHIP_SCHEME_KKLSS_CKKS || scheme > MKHIP_SCHEME_MKF_ROTATE) return MKHIP_ERR_ARG;
     const char *o = (const char *)ops;
     const size_t osz = op_size(scheme);
     // chunks: the longest prefix of the remaining ops whose workspace fits the

@@ -22,6 +22,8 @@ hipError_t launch_mkf(int what, const DevTables &T, const void *descs, int n, in
 hipError_t launch_bfv(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st);
 // kernels_fused.hip
 hipError_t launch_fused(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st);
+// kernels_galois.hip
+hipError_t launch_galois(int what, const DevTables &T, const void *descs, int n, hipStream_t st);
 
 inline int hip_status(hipError_t e) {
     return e == hipSuccess ? MKHIP_OK : (e == hipErrorOutOfMemory ? MKHIP_ERR_OOM : MKHIP_ERR_DEVICE);
@@ -48,6 +50,9 @@ struct MkfGroup {
     int k = 0;
     std::vector<mkhip::u64 *> fold;
     mkhip::u64 *relin = nullptr;
+    // party-specific Galois keys sigma(s_i) -> s_aggregate by Galois element
+    // (true_mkf_ckks.go:323-333), k keys each
+    std::map<mkhip::u64, std::vector<mkhip::u64 *> > rot;
 };
 
 // One launch of a plan: kernel id + descriptor table + launch geometry.
@@ -68,6 +73,9 @@ struct mkhip_ctx {
     mkhip::DevTables T;
     hipStream_t stream = nullptr;
     mkhip::u64 *d_crs_u = nullptr;
+    std::map<int, mkhip::u64 *> crs;             // CRS of rotations (index = rotidx) and conjugation (-2)
+    std::map<std::pair<int, int>, mkhip::u64 *> rot_keys;  // (party, rotidx) -> rotation key
+    std::map<int, mkhip::u64 *> conj_keys;       // party -> conjugation key
     std::map<int, PartyKeys> parties;
     std::map<int, MkfGroup> mkf_groups;
     size_t swk_words = 0;       // beta x (L+K) x N
@@ -156,6 +164,15 @@ int plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph
                   const PlanMem &pm);
 int plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out,
                  const PlanMem &pm);
+// Upload one switching key (swk_words words) into a new device allocation.
+int upload_key(mkhip_ctx *c, const uint64_t *src, u64 **dst);
+// Drop the plan cached by the host-pointer entry points: it holds key pointers,
+// so every key removal or replacement calls this before freeing the old key.
+void drop_cached_plan(mkhip_ctx *c);
+int plan_kklss_rotate(mkhip_ctx *c, int nops, const mkhip_rotate_op *ops, int use_graph, mkhip_plan **out,
+                      const PlanMem &pm);
+int plan_mkf_rotate(mkhip_ctx *c, int nops, const mkhip_mkf_rotate_op *ops, int use_graph, mkhip_plan **out,
+                    const PlanMem &pm);
 // Builder of a scheme (MKHIP_SCHEME_*) over ops of the matching struct type.
 int plan_build(mkhip_ctx *c, int scheme, int nops, const void *ops, int use_graph, mkhip_plan **out,
                const PlanMem &pm);

@@ -56,9 +56,12 @@ extern "C" int mkhip_unregister_mkf_group(mkhip_ctx *c, int group) {
     if (!c) return MKHIP_ERR_ARG;
     auto it = c->mkf_groups.find(group);
     if (it == c->mkf_groups.end()) return MKHIP_ERR_NO_KEY;
+    drop_cached_plan(c);
     (void)hipStreamSynchronize(c->stream);
     for (u64 *p : it->second.fold) (void)hipFree(p);
     (void)hipFree(it->second.relin);
+    for (auto &r : it->second.rot)
+        for (u64 *p : r.second) (void)hipFree(p);
     c->mkf_groups.erase(it);
     return MKHIP_OK;
 }

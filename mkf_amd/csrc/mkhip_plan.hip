@@ -15,6 +15,7 @@ hipError_t plan_stage(mkhip_plan *pl, int s, hipStream_t st) {
     if (S.what >= KMAC2 && S.what <= KRESCALE_NTT) return launch_mkf(S.what, c->T, S.descs, S.n, S.a, S.b, st);
     if (S.what >= KBFV_LIFT && S.what <= KBFV_QUANTIZE) return launch_bfv(S.what, c->T, S.descs, S.n, S.a, S.b, st);
     if (S.what == KDECOMP_MAC) return launch_fused(S.what, c->T, S.descs, S.n, S.a, S.b, st);
+    if (S.what == KAUTO) return launch_galois(S.what, c->T, S.descs, S.n, st);
     return launch_kernel(S.what, c->T, S.descs, S.n, S.a, S.b, st, S.c);
 }
 
@@ -113,6 +114,10 @@ int plan_build(mkhip_ctx *c, int scheme, int nops, const void *ops, int use_grap
         return plan_mkf_ckks(c, nops, (const mkhip_mkf_op *)ops, use_graph, out, pm);
     case MKHIP_SCHEME_MKF_BFV:
         return plan_mkf_bfv(c, nops, (const mkhip_mkf_op *)ops, use_graph, out, pm);
+    case MKHIP_SCHEME_KKLSS_ROTATE:
+        return plan_kklss_rotate(c, nops, (const mkhip_rotate_op *)ops, use_graph, out, pm);
+    case MKHIP_SCHEME_MKF_ROTATE:
+        return plan_mkf_rotate(c, nops, (const mkhip_mkf_rotate_op *)ops, use_graph, out, pm);
     default:
         return MKHIP_ERR_ARG;
     }

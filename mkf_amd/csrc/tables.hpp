@@ -73,6 +73,18 @@ enum KernelId {
     KBFV_TENSOR = 15,    // k_tensor<LOGN, true>   tensor over R, t x, INTT
     KBFV_QUANTIZE = 16,  // k_quantize             ModDown R -> Q (+ accumulate)
     KINTT_FINISH = 17,   // k_intt_finish          last inverse stage of split (2^15) limbs
+    KAUTO = 18,          // k_automorph            Galois automorphism (coefficient or NTT domain)
+};
+
+// Galois automorphism X -> X^gal of nl limbs, src -> dst (no aliasing).
+// Coefficient domain: Lattigo ring.Permute (restated inline at
+// mkrlwe/keyswitch.go:283-303), negation written q - x.  NTT domain: the slot
+// permutation of ring.PermuteNTTIndex.
+struct AutoDesc {
+    const u64 *src;
+    u64 *dst;
+    u64 gal, gal_inv;  // Galois element (odd, < 2N) and its inverse mod 2N
+    int nl, ntt;
 };
 
 // One limb of 2^15 whose halves ran every inverse stage but the last: apply it

@@ -179,3 +179,21 @@ def test_batched_plan_matches_single(setup):
     plan.close()
     for p in bufs:
         ctx.free(p)
+
+
+def test_unregistered_party_is_not_served_from_the_cached_plan(setup):
+    """The host entry point caches its plan; removing a party's keys drops it, so a
+    repeat call reports the missing key instead of reading freed memory."""
+    name, ctx, ref = setup
+    if name != "small":
+        pytest.skip("host-side cache logic, one size")
+    mk = _mk()
+    _keys(ctx, ref, [0, 1], seed=41)
+    g = O.Prng(42)
+    op0 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
+    scale = float(ref.Q[-1])
+    ctx.kklss_ckks_mulrelin(op0, [0, 1], op0, [0, 1], scale, scale, scale)
+    ctx.unregister_party(1)
+    with pytest.raises(mk.MkhipError) as e:
+        ctx.kklss_ckks_mulrelin(op0, [0, 1], op0, [0, 1], scale, scale, scale)
+    assert e.value.status == -4
