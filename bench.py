@@ -14,7 +14,9 @@ per-stage device times:
              (the reference's PN14QP439 primes), B = 32
   mkf_bfv    MKF MK-BFV at the MK-BFV shape (gamma 1), B = 32
 plus the roofline of the headline's dominant kernel (HIP events on the library's
-stream) and the CPU oracle timed on a bounded sample of the headline workload.
+stream), the CPU oracle timed on a bounded sample of the headline workload, and
+the Galois key switches (KKLSS hoisted rotations, conjugation, MKF rotation) at
+the headline shape.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
                        [--schemes kklss_ckks,mkf_ckks,kklss_bfv,mkf_bfv]
@@ -247,6 +249,90 @@ def run_scheme(scheme, args, rng):
         pplan.close()
     extra = (w, plan, stage_ms, names)
     return res, extra
+
+
+ROT_STEPS = (1, 2, 4, 8)
+
+
+def run_rotations(args, rng):
+    """Galois key switches at the headline shape (N = 2^14, 8 Q + 2 P, k = 4):
+    KKLSS RotateHoisted of every op by 1, 2, 4 and 8 from one decomposition,
+    KKLSS rotation by 1 alone, conjugation, and the MKF party-specific rotation
+    (gamma 1); B ops per launch, one hipGraph per step."""
+    import mkf_amd
+
+    logN, L, K, k, B = 14, 8, 2, 4, args.rot_batch
+    N = 1 << logN
+    Q, P = ntt_primes(50, L), ntt_primes(60, K)
+    QP = Q + P
+    out = {"shape": {"logN": logN, "L": L, "K": K, "k": k, "batch": B}}
+    ctx = mkf_amd.Context(logN, Q, P, gamma=2)
+    beta = ctx.beta
+    for r in ROT_STEPS:
+        ctx.set_crs(r, uniform_stack(rng, QP, N, beta))
+        for pid in range(k):
+            ctx.add_rotation_key(pid, r, uniform_stack(rng, QP, N, beta))
+    ctx.set_crs(mkf_amd.CONJUGATE, uniform_stack(rng, QP, N, beta))
+    for pid in range(k):
+        ctx.add_conjugation_key(pid, uniform_stack(rng, QP, N, beta))
+    words = (1 + k) * L * N
+    bufs = []
+
+    def leg(rots):
+        ops = []
+        for _ in range(B):
+            din, dout = ctx.alloc(words * 8), ctx.alloc(len(rots) * words * 8)
+            ctx.h2d(din, np.concatenate([uniform(rng, Q, N) for _ in range(1 + k)]))
+            bufs.extend([din, dout])
+            ops.append({"in": din, "out": dout, "ids": list(range(k)), "level": L - 1, "rot": list(rots)})
+        plan = mkf_amd.Plan(ctx, ops, use_graph=True, scheme="rotate")
+        el, lat = time_plan(ctx, plan, args.rot_steps, 2)
+        st = plan.stage_times(args.stage_iters)
+        names = plan.stages
+        n = B * len(rots) * args.rot_steps
+        # keys read once per launch: 2 (rotation key + CRS) per party per rotation
+        key_bytes = len(rots) * (k + 1) * beta * (L + K) * N * 8
+        io_bytes = B * (1 + len(rots)) * words * 8
+        res = {"rotations_per_s": round(n / el, 1), "ms_per_step": round(el / args.rot_steps * 1e3, 3),
+               "p99_ms": round(pct(lat, 99) * 1e3, 3), "rotations_per_launch": B * len(rots),
+               "alg_gbs": round((key_bytes + io_bytes) * args.rot_steps / el / 1e9, 1),
+               "stages_ms": {names[i]: round(st[i], 4) for i in range(len(names))}}
+        plan.close()
+        return res
+
+    out["kklss_hoisted_1_2_4_8"] = leg(ROT_STEPS)
+    out["kklss_rotate_1"] = leg((1,))
+    out["kklss_conjugate"] = leg((mkf_amd.CONJUGATE,))
+    for p_ in bufs:
+        ctx.free(p_)
+    ctx.close()
+    # MKF party-specific rotation
+    ctx = mkf_amd.Context(logN, Q, P, gamma=1)
+    beta = ctx.beta
+
+    def swk():
+        return np.stack([np.stack([uniform(rng, QP, N), uniform(rng, QP, N)]) for _ in range(beta)])
+
+    ctx.register_mkf_group(0, [swk() for _ in range(k)], swk())
+    ctx.register_mkf_rotation(0, 1, [swk() for _ in range(k)])
+    ops, bufs = [], []
+    for _ in range(B):
+        din, dout = ctx.alloc(words * 8), ctx.alloc(words * 8)
+        ctx.h2d(din, np.concatenate([uniform(rng, Q, N) for _ in range(1 + k)]))
+        bufs.extend([din, dout])
+        ops.append({"in": din, "out": dout, "group": 0, "level": L - 1, "rotation": 1})
+    plan = mkf_amd.Plan(ctx, ops, use_graph=True, scheme="mkf_rotate")
+    el, lat = time_plan(ctx, plan, args.rot_steps, 2)
+    st = plan.stage_times(args.stage_iters)
+    out["mkf_rotate_1"] = {"rotations_per_s": round(B * args.rot_steps / el, 1),
+                           "ms_per_step": round(el / args.rot_steps * 1e3, 3),
+                           "p99_ms": round(pct(lat, 99) * 1e3, 3), "rotations_per_launch": B,
+                           "stages_ms": {plan.stages[i]: round(st[i], 4) for i in range(len(st))}}
+    plan.close()
+    for p_ in bufs:
+        ctx.free(p_)
+    ctx.close()
+    return out
 
 
 # uneven batch: (logN, L, K, scheme) per ring degree; K = 1 only admits the MKF
@@ -553,6 +639,8 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--serve-max-batch", type=int, default=32)
     ap.add_argument("--serve-mem-gb", type=float, default=64.0)
+    ap.add_argument("--rot-batch", type=int, default=16, help="ops per launch of the rotation legs (0: skip)")
+    ap.add_argument("--rot-steps", type=int, default=5)
     ap.add_argument("--copy-calib-mb", type=float, default=0.0,
                     help="run a plain D2D copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -629,6 +717,10 @@ def main():
     if args.uneven_ops > 0:
         line["uneven_batch"] = run_uneven(args, rng)
         print("# uneven batch: %.1f ops/s" % line["uneven_batch"]["ops_per_s"], file=sys.stderr, flush=True)
+    if args.rot_batch > 0:
+        line["rotations"] = run_rotations(args, rng)
+        print("# rotations: %.1f hoisted rotations/s" % line["rotations"]["kklss_hoisted_1_2_4_8"]["rotations_per_s"],
+              file=sys.stderr, flush=True)
     if calib:
         line["copy_calibration"] = calib
     print(json.dumps(line), flush=True)

@@ -11,7 +11,12 @@
 //   mkckks::Evaluator::MulRelinNew(op0, op1, rlkSet)   mkckks/evaluator.go:419
 //   mkbfv::RelinearizationKey{Value [2], ID}           mkbfv/keys.go:6-9
 //   mkbfv::Evaluator::MulRelinNew(op0, op1, rlkSet)    mkbfv/evaluator.go:85
+//   mkrlwe::RotationKeySet / ConjugationKeySet         mkrlwe/keys.go:39-67, 125-160, 201-225
+//   mkckks / mkbfv Evaluator::RotateNew, RotateHoistedNew, ConjugateNew
+//                                                      mkckks/evaluator.go:488-545, 590-624;
+//                                                      mkbfv/evaluator.go:156-210
 //   mkf::CKKSEvaluator / mkf::BFVEvaluator             mkf_cpu/ckks.go:102-166, bfv.go:212-252
+//   mkf::CKKSEvaluator::RotateNew                      mkf_cpu/true_mkf_ckks.go:323-353
 // The KKLSS evaluators panic on bad input (keyswitch_hoisted.go:53-59, keys.go:
 // 190-198): here they throw mkhip_host::Panic with the reference's message.  The
 // MKF calls return an error value: here a non-empty std::string.
@@ -124,6 +129,48 @@ struct RelinearizationKeySet {
     }
 };
 
+// RotationKey / RotationKeySet (keys.go:39-44, 60-62, 125-160)
+struct RotationKey {
+    SwitchingKey Value;
+    std::string ID;
+    unsigned RotIdx = 0;
+};
+
+struct RotationKeySet {
+    std::map<std::string, std::map<unsigned, RotationKey>> Value;
+    void AddRotationKey(const RotationKey &rk) { Value[rk.ID][rk.RotIdx] = rk; }
+    void DelRotationKey(const std::string &id, unsigned rotidx) {
+        auto it = Value.find(id);
+        if (it != Value.end()) it->second.erase(rotidx);
+    }
+    const RotationKey &GetRotationKey(const std::string &id, unsigned rotidx) const {
+        auto it = Value.find(id);
+        if (it != Value.end()) {
+            auto jt = it->second.find(rotidx);
+            if (jt != it->second.end()) return jt->second;
+        }
+        throw Panic("cannot GetRotationKeys: there is no rotation key with given id", MKHIP_ERR_NO_KEY);
+    }
+};
+
+// ConjugationKey / ConjugationKeySet (keys.go:46-50, 64-67, 201-225)
+struct ConjugationKey {
+    SwitchingKey Value;
+    std::string ID;
+};
+
+struct ConjugationKeySet {
+    std::map<std::string, ConjugationKey> Value;
+    void AddConjugationKey(const ConjugationKey &ck) { Value[ck.ID] = ck; }
+    void DelConjugationKey(const std::string &id) { Value.erase(id); }
+    const ConjugationKey &GetConjugationKey(const std::string &id) const {
+        auto it = Value.find(id);
+        if (it == Value.end())
+            throw Panic("cannot GetConjugationKey: there is no conjugation key with given id", MKHIP_ERR_NO_KEY);
+        return it->second;
+    }
+};
+
 struct Ciphertext {
     std::map<std::string, Poly> Value;  // "0" is c0, the other keys are party ids
     int Level() const { return Value.empty() ? -1 : Value.begin()->second.Level(); }
@@ -157,6 +204,81 @@ inline void flatten_ct(const mkrlwe::Ciphertext &ct, const std::vector<std::stri
     ct.Value.at("0").append_to(out, nlimbs);
     for (const std::string &id : ids) ct.Value.at(id).append_to(out, nlimbs);
 }
+
+// Rotation / conjugation keys of a KKLSS evaluator: the CRS of every rotation
+// and of conjugation is uploaded with the evaluator; party keys on first use.
+struct GaloisKeys {
+    std::set<std::pair<std::string, unsigned>> rot;
+    std::set<std::string> conj;
+
+    static void upload_crs(mkhip_ctx *c, const mkrlwe::Parameters &params) {
+        for (const auto &kv : params.CRS) {
+            const int idx = kv.first;
+            if (idx != -2 && (idx < 1 || idx >= params.N() / 2)) continue;
+            std::vector<uint64_t> a;
+            flatten_qp(kv.second.Value, a);
+            const int st = mkhip_set_crs(c, idx, a.data());
+            if (st != MKHIP_OK) throw Panic("cannot NewEvaluator: " + status_text(st), st);
+        }
+    }
+
+    // Evaluator.rotate / rotateHoisted / conjugate (mkckks/evaluator.go:496-545,
+    // 598-624): the keys each step needs are fetched (panicking like the reference
+    // when one is missing) and uploaded, then the device runs the steps.
+    mkrlwe::Ciphertext run(mkhip_ctx *c, PartyTable &parties, const mkrlwe::Parameters &params,
+                           const mkrlwe::Ciphertext &ct, int rotidx, bool hoisted, bool conjugate,
+                           const mkrlwe::RotationKeySet *rks, const mkrlwe::ConjugationKeySet *cks) {
+        const std::vector<std::string> ids = ct.IDSet();
+        std::vector<int> g;
+        for (const auto &id : ids) g.push_back(parties.id(id));
+        const int N = params.N(), half = N / 2;
+        if (conjugate) {
+            for (size_t i = 0; i < ids.size(); ++i) {
+                if (conj.count(ids[i])) continue;
+                std::vector<uint64_t> k;
+                flatten_qp(cks->GetConjugationKey(ids[i]).Value.Value, k);
+                const int st = mkhip_add_conjugation_key(c, g[i], k.data());
+                if (st != MKHIP_OK) throw Panic("AddConjugationKey: " + status_text(st), st);
+                conj.insert(ids[i]);
+            }
+        } else {
+            const int r = ((rotidx % half) + half) % half;
+            std::vector<unsigned> steps;
+            if (r != 0) {
+                if (params.CRS.count(r)) {
+                    steps.push_back((unsigned)r);
+                } else if (hoisted) {
+                    throw Panic("Hoisted rotation only works for precomputed rotation keys", MKHIP_ERR_NO_KEY);
+                } else {
+                    for (unsigned kk = 1, rr = (unsigned)r; rr > 0; kk *= 2, rr /= 2)
+                        if (rr % 2) steps.push_back(kk);
+                }
+            }
+            for (unsigned st_r : steps)
+                for (size_t i = 0; i < ids.size(); ++i) {
+                    if (rot.count({ids[i], st_r})) continue;
+                    std::vector<uint64_t> k;
+                    flatten_qp(rks->GetRotationKey(ids[i], st_r).Value.Value, k);
+                    const int st = mkhip_add_rotation_key(c, g[i], (int)st_r, k.data());
+                    if (st != MKHIP_OK) throw Panic("AddRotationKey: " + status_text(st), st);
+                    rot.insert({ids[i], st_r});
+                }
+        }
+        const int nl = ct.Level() + 1;
+        std::vector<uint64_t> in, out;
+        flatten_ct(ct, ids, nl, in);
+        out.resize(in.size());
+        const int st = conjugate ? mkhip_kklss_conjugate(c, in.data(), nl - 1, g.data(), (int)g.size(), out.data())
+                                 : mkhip_kklss_rotate(c, in.data(), nl - 1, g.data(), (int)g.size(), rotidx,
+                                                      hoisted ? 1 : 0, out.data());
+        if (st != MKHIP_OK) throw Panic(std::string(conjugate ? "Conjugate: " : "Rotate: ") + status_text(st), st);
+        mkrlwe::Ciphertext res;
+        res.Value["0"] = Poly::from(out.data(), N, nl, false);
+        for (size_t i = 0; i < ids.size(); ++i)
+            res.Value[ids[i]] = Poly::from(out.data() + (1 + i) * (size_t)nl * N, N, nl, false);
+        return res;
+    }
+};
 }  // namespace detail
 
 namespace mkckks {
@@ -184,6 +306,22 @@ class Evaluator {
             st = mkhip_set_crs(ctx_.c, -1, u.data());
             if (st != MKHIP_OK) throw Panic("cannot NewEvaluator: " + status_text(st), st);
         }
+        detail::GaloisKeys::upload_crs(ctx_.c, params);
+    }
+
+    // RotateNew (evaluator.go:488-529): rotidx normalised into [0, N/2); a rotation
+    // without a CRS entry runs as rotations by 1, 2, 4, ...
+    Ciphertext RotateNew(const Ciphertext &ct0, int rotidx, const mkrlwe::RotationKeySet &rkSet) {
+        return wrap(galois_.run(ctx_.c, parties_, params_, ct0, rotidx, false, false, &rkSet, nullptr), ct0);
+    }
+    // RotateHoistedNew (evaluator.go:590-624); the hoisted form is computed on the
+    // device inside the call (HoistedForm :546-556 has no separate host object here).
+    Ciphertext RotateHoistedNew(const Ciphertext &ct0, int rotidx, const mkrlwe::RotationKeySet &rkSet) {
+        return wrap(galois_.run(ctx_.c, parties_, params_, ct0, rotidx, true, false, &rkSet, nullptr), ct0);
+    }
+    // ConjugateNew (evaluator.go:533-545)
+    Ciphertext ConjugateNew(const Ciphertext &ct0, const mkrlwe::ConjugationKeySet &ckSet) {
+        return wrap(galois_.run(ctx_.c, parties_, params_, ct0, 0, false, true, nullptr, &ckSet), ct0);
     }
 
     // MulRelinNew (evaluator.go:419-446): tensor + relinearization + rescale by the
@@ -222,21 +360,30 @@ class Evaluator {
     mkhip_ctx *handle() const { return ctx_.c; }
 
   private:
+    static Ciphertext wrap(mkrlwe::Ciphertext &&c, const Ciphertext &like) {
+        Ciphertext out;
+        out.Value = std::move(c.Value);
+        out.Scale = like.Scale;
+        return out;
+    }
     int ensure(const std::string &id, const mkrlwe::RelinearizationKeySet &rlkSet) {
-        if (parties_.gid.count(id)) return parties_.gid[id];
+        const int g = parties_.id(id);
+        if (relin_.count(id)) return g;
         const mkrlwe::RelinearizationKey &rlk = rlkSet.GetRelinearizationKey(id);
         std::vector<uint64_t> b, d, v;
         detail::flatten_qp(rlk.Value[0].Value, b);
         detail::flatten_qp(rlk.Value[1].Value, d);
         detail::flatten_qp(rlk.Value[2].Value, v);
-        const int g = parties_.id(id);
         const int st = mkhip_register_party(ctx_.c, g, b.data(), d.data(), v.data());
         if (st != MKHIP_OK) throw Panic("AddRelinearizationKey: " + status_text(st), st);
+        relin_.insert(id);
         return g;
     }
     Parameters params_;
     detail::Ctx ctx_;
     detail::PartyTable parties_;
+    std::set<std::string> relin_;
+    detail::GaloisKeys galois_;
 };
 
 }  // namespace mkckks
@@ -284,6 +431,15 @@ class Evaluator {
             st = mkhip_set_crs(ctx_.c, -1, u.data());
             if (st != MKHIP_OK) throw Panic("cannot NewEvaluator: " + status_text(st), st);
         }
+        detail::GaloisKeys::upload_crs(ctx_.c, params);
+    }
+
+    // RotateNew / ConjugateNew (evaluator.go:156-210): the mkrlwe key switches over Q
+    Ciphertext RotateNew(const Ciphertext &ct0, int rotidx, const mkrlwe::RotationKeySet &rkSet) {
+        return wrap(galois_.run(ctx_.c, parties_, params_, ct0, rotidx, false, false, &rkSet, nullptr));
+    }
+    Ciphertext ConjugateNew(const Ciphertext &ct0, const mkrlwe::ConjugationKeySet &ckSet) {
+        return wrap(galois_.run(ctx_.c, parties_, params_, ct0, 0, false, true, nullptr, &ckSet));
     }
 
     // MulRelinNew (evaluator.go:85-91): operands at the top level, coefficient domain.
@@ -316,8 +472,13 @@ class Evaluator {
     mkhip_ctx *handle() const { return ctx_.c; }
 
   private:
+    static Ciphertext wrap(mkrlwe::Ciphertext &&c) {
+        Ciphertext out;
+        out.Value = std::move(c.Value);
+        return out;
+    }
     int ensure(const std::string &id, const RelinearizationKeySet &rlkSet) {
-        if (parties_.gid.count(id)) return parties_.gid[id];
+        if (relin_.count(id)) return parties_.gid[id];
         const RelinearizationKey &rlk = rlkSet.GetRelinearizationKey(id);
         std::vector<uint64_t> b0, d0, v0, b1, d1;
         detail::flatten_qp(rlk.Value[0].Value[0].Value, b0);
@@ -328,11 +489,14 @@ class Evaluator {
         const int g = parties_.id(id);
         const int st = mkhip_register_bfv_party(ctx_.c, g, b0.data(), d0.data(), v0.data(), b1.data(), d1.data());
         if (st != MKHIP_OK) throw Panic("AddRelinearizationKey: " + status_text(st), st);
+        relin_.insert(id);
         return g;
     }
     Parameters params_;
     detail::Ctx ctx_;
     detail::PartyTable parties_;
+    std::set<std::string> relin_;
+    detail::GaloisKeys galois_;
 };
 
 }  // namespace mkbfv
@@ -349,6 +513,9 @@ struct SwitchingKey {
 struct OnlineKeys {
     std::vector<SwitchingKey> Fold;
     SwitchingKey Relin;
+    // party-specific Galois keys sigma(s_i) -> s_aggregate by Galois element
+    // (paperMKFCKKSOnlineKeys.Rotations, true_mkf_ckks.go:30-34, 370-392)
+    std::map<uint64_t, std::vector<SwitchingKey>> Rotations;
 };
 
 // multiCiphertext (common.go:10-15) and outputCiphertext (:19-25).
@@ -413,10 +580,44 @@ class CKKSEvaluator {
         for (const auto &f : fold) fp.push_back(f.data());
         const std::vector<uint64_t> relin = detail2::flatten_swk(keys.Relin);
         mkhip_unregister_mkf_group(ctx_.c, 0);
-        const int st = mkhip_register_mkf_group(ctx_.c, 0, (int)fp.size(), fp.data(), relin.data());
+        int st = mkhip_register_mkf_group(ctx_.c, 0, (int)fp.size(), fp.data(), relin.data());
         if (st != MKHIP_OK) return status_text(st);
         k_ = (int)fp.size();
+        rotations_.clear();
+        for (const auto &kv : keys.Rotations) {
+            if ((int)kv.second.size() != k_) continue;  // RotateNew reports the set as missing
+            std::vector<std::vector<uint64_t>> rk;
+            std::vector<const uint64_t *> rp;
+            for (const auto &f : kv.second) rk.push_back(detail2::flatten_swk(f));
+            for (const auto &f : rk) rp.push_back(f.data());
+            const int rot = rotation_of(kv.first);
+            if (rot < 0) continue;
+            st = mkhip_register_mkf_rotation(ctx_.c, 0, rot, rp.data());
+            if (st != MKHIP_OK) return status_text(st);
+            rotations_.insert(kv.first);
+        }
         return "";
+    }
+    // RotateNew (true_mkf_ckks.go:323-353): NTT-domain input, output C0 and k
+    // copies of the second component at the input's level and scale.
+    std::pair<MultiCiphertext, std::string> RotateNew(const MultiCiphertext &in, int rotation) {
+        if (!err_.empty()) return {MultiCiphertext(), err_};
+        if ((int)in.Components.size() != k_) return {MultiCiphertext(), "component count does not match the keys"};
+        const uint64_t gal = galois_element(rotation);
+        if (!rotations_.count(gal))
+            return {MultiCiphertext(), "party-specific Galois keys missing for rotation " + std::to_string(rotation) +
+                                           " (element " + std::to_string(gal) + ")"};
+        const int N = params_.N(), nl = in.Level + 1;
+        const std::vector<uint64_t> x = detail2::flatten_mc(in, nl);
+        std::vector<uint64_t> out(x.size());
+        const int st = mkhip_mkf_ckks_rotate(ctx_.c, 0, x.data(), in.Level, rotation, out.data());
+        if (st != MKHIP_OK) return {MultiCiphertext(), status_text(st)};
+        MultiCiphertext r;
+        r.C0 = Poly::from(out.data(), N, nl, true);
+        for (int p = 0; p < k_; ++p) r.Components.push_back(Poly::from(out.data() + (size_t)(1 + p) * nl * N, N, nl, true));
+        r.Level = in.Level;
+        r.Scale = in.Scale;
+        return {r, ""};
     }
     std::pair<OutputCiphertext, std::string> MulRelin(const MultiCiphertext &left, const MultiCiphertext &right) {
         if (!err_.empty()) return {OutputCiphertext(), err_};
@@ -435,11 +636,26 @@ class CKKSEvaluator {
     mkhip_ctx *handle() const { return ctx_.c; }
 
   private:
+    // GaloisElementForColumnRotationBy: 5^(k mod 2N) mod 2N
+    uint64_t galois_element(long long k) const {
+        const uint64_t mask = 2 * (uint64_t)params_.N() - 1;
+        uint64_t r = 1, b = 5, e = (uint64_t)k & mask;
+        for (; e; e >>= 1, b = (b * b) & mask)
+            if (e & 1) r = (r * b) & mask;
+        return r;
+    }
+    // a rotation in [0, N/2) with this Galois element, or -1
+    int rotation_of(uint64_t gal) const {
+        for (int r = 0; r < params_.N() / 2; ++r)
+            if (galois_element(r) == gal) return r;
+        return -1;
+    }
     mkrlwe::Parameters params_;
     double scale_;
     mkhip_host::detail::Ctx ctx_;
     std::string err_;
     int k_ = 0;
+    std::set<uint64_t> rotations_;
 };
 
 // BFVCase.Run (bfv.go:212-252): coefficient-domain operands at the top level.

@@ -324,6 +324,158 @@ static void test_mkf(bool bfv) {
     std::printf("ok %s MulRelin (N=2^13, L=%d, K=%d, k=%d)\n", tag, L, K, k);
 }
 
+
+// Rotations and conjugation through the mirror: mkckks Evaluator RotateNew (the
+// power-of-two fallback), RotateHoistedNew's panic, ConjugateNew, and the MKF
+// party-specific RotateNew with its error value (mkckks_test.go:443-458,
+// :493-505; true_mkf_ckks.go:323-353).
+static void test_rotations() {
+    const int logN = 13, N = 1 << logN, L = 4, K = 2;
+    mkckks::Parameters params;
+    params.LogN = logN;
+    params.Q = primes(50, L);
+    params.P = primes(60, K);
+    params.Gamma = 2;
+    params.DefaultScale = (double)params.Q.back();
+    const int beta = params.Beta(L - 1);
+    std::vector<uint64_t> QP = params.Q;
+    QP.insert(QP.end(), params.P.begin(), params.P.end());
+    mkref_prng g;
+    mkref_prng_seed(&g, 505);
+    auto swk = [&]() {
+        std::vector<uint64_t> w;
+        for (int d = 0; d < beta; ++d) {
+            auto r = uniform(g, QP, N);
+            w.insert(w.end(), r.begin(), r.end());
+        }
+        return w;
+    };
+    std::map<int, std::vector<uint64_t>> crs;
+    for (int idx : {1, 2, -2}) {
+        crs[idx] = swk();
+        params.CRS[idx].Value = to_qp(crs[idx], beta, L, K, N);
+    }
+    const std::vector<std::string> names = {"alice", "bob"};
+    mkrlwe::RotationKeySet rks;
+    mkrlwe::ConjugationKeySet cks;
+    std::map<std::pair<int, int>, std::vector<uint64_t>> rk;
+    std::vector<std::vector<uint64_t>> ck;
+    for (int p = 0; p < 2; ++p) {
+        for (unsigned r : {1u, 2u}) {
+            rk[{p, (int)r}] = swk();
+            mkrlwe::RotationKey k;
+            k.ID = names[p];
+            k.RotIdx = r;
+            k.Value.Value = to_qp(rk[{p, (int)r}], beta, L, K, N);
+            rks.AddRotationKey(k);
+        }
+        ck.push_back(swk());
+        mkrlwe::ConjugationKey c;
+        c.ID = names[p];
+        c.Value.Value = to_qp(ck.back(), beta, L, K, N);
+        cks.AddConjugationKey(c);
+    }
+    std::vector<uint64_t> flat;
+    mkckks::Ciphertext ct;
+    ct.Scale = params.DefaultScale;
+    for (int c = 0; c < 3; ++c) {
+        auto x = uniform(g, params.Q, N);
+        flat.insert(flat.end(), x.begin(), x.end());
+        ct.Value[c == 0 ? "0" : names[c - 1]] = Poly::from(x.data(), N, L, false);
+    }
+    mkckks::Evaluator eval(params);
+    // rotation by 3 = rotation by 1 then by 2 (no CRS[3])
+    mkckks::Ciphertext rot = eval.RotateNew(ct, 3, rks);
+    mkref_ctx ref;
+    mkref_ctx_init(&ref, logN, params.Q.data(), L, params.P.data(), K, 2, 0);
+    std::vector<uint64_t> t1(flat.size()), t2(flat.size());
+    const uint64_t *k1[2] = {rk[{0, 1}].data(), rk[{1, 1}].data()};
+    const uint64_t *k2[2] = {rk[{0, 2}].data(), rk[{1, 2}].data()};
+    mkref_kklss_rotate(&ref, L - 1, 2, flat.data(), k1, crs[1].data(), mkref_galois_element(logN, 1), t1.data());
+    mkref_kklss_rotate(&ref, L - 1, 2, t1.data(), k2, crs[2].data(), mkref_galois_element(logN, 2), t2.data());
+    CHECK(rot.Scale == ct.Scale && rot.Value.size() == 3, "rotate shape");
+    CHECK(same(rot.Value["0"], t2.data(), L, N), "rotate c0");
+    for (int p = 0; p < 2; ++p)
+        CHECK(same(rot.Value[names[p]], t2.data() + (size_t)(1 + p) * L * N, L, N), "rotate component");
+    bool panicked = false;
+    try {
+        eval.RotateHoistedNew(ct, 3, rks);
+    } catch (const Panic &e) {
+        panicked = std::string(e.what()).find("Hoisted rotation only works") != std::string::npos;
+    }
+    CHECK(panicked, "hoisted rotation without a precomputed key panics");
+    mkckks::Ciphertext hr = eval.RotateHoistedNew(ct, 1, rks);
+    CHECK(same(hr.Value["0"], t1.data(), L, N), "hoisted rotate c0");
+    // conjugation
+    mkckks::Ciphertext cj = eval.ConjugateNew(ct, cks);
+    std::vector<uint64_t> ec(flat.size());
+    const uint64_t *cp[2] = {ck[0].data(), ck[1].data()};
+    mkref_kklss_conjugate(&ref, L - 1, 2, flat.data(), cp, crs[-2].data(), ec.data());
+    for (int c = 0; c < 3; ++c)
+        CHECK(same(cj.Value[c == 0 ? "0" : names[c - 1]], ec.data() + (size_t)c * L * N, L, N), "conjugate");
+    // a party without a rotation key panics with the reference's message
+    mkckks::Ciphertext ct3 = ct;
+    ct3.Value["carol"] = ct.Value["alice"];
+    panicked = false;
+    try {
+        eval.RotateNew(ct3, 1, rks);
+    } catch (const Panic &e) {
+        panicked = std::string(e.what()).find("there is no rotation key with given id") != std::string::npos;
+    }
+    CHECK(panicked, "rotate missing key panics");
+    mkref_ctx_free(&ref);
+    std::printf("ok kklss RotateNew / RotateHoistedNew / ConjugateNew (N=2^13, L=4, K=2, alice+bob)\n");
+
+    // MKF party-specific rotation (gamma = 1 context)
+    const int Km = 1, betam = L;
+    std::vector<uint64_t> Pm = primes(60, Km);
+    std::vector<uint64_t> QPm = params.Q;
+    QPm.insert(QPm.end(), Pm.begin(), Pm.end());
+    mkrlwe::Parameters pm;
+    pm.LogN = logN;
+    pm.Q = params.Q;
+    pm.P = Pm;
+    pm.Gamma = 1;
+    auto mswk = [&]() {
+        std::vector<uint64_t> w;
+        for (int d = 0; d < 2 * betam; ++d) {
+            auto r = uniform(g, QPm, N);
+            w.insert(w.end(), r.begin(), r.end());
+        }
+        return w;
+    };
+    mkf::OnlineKeys keys;
+    for (int p = 0; p < 2; ++p) keys.Fold.push_back(to_swk(mswk(), betam, L, Km, N));
+    keys.Relin = to_swk(mswk(), betam, L, Km, N);
+    std::vector<std::vector<uint64_t>> rot1 = {mswk(), mswk()};
+    const uint64_t gal1 = mkref_galois_element(logN, 1);
+    for (const auto &w : rot1) keys.Rotations[gal1].push_back(to_swk(w, betam, L, Km, N));
+    mkf::CKKSEvaluator me(pm, (double)params.Q.back());
+    CHECK(me.SetKeys(keys).empty(), "mkf keys with rotations");
+    mkf::MultiCiphertext in;
+    std::vector<uint64_t> fin;
+    in.Level = L - 1;
+    in.Scale = 3.0;
+    for (int c = 0; c < 3; ++c) {
+        auto x = uniform(g, params.Q, N);
+        fin.insert(fin.end(), x.begin(), x.end());
+        (c == 0 ? in.C0 : (in.Components.emplace_back(), in.Components.back())) = Poly::from(x.data(), N, L, true);
+    }
+    auto mr = me.RotateNew(in, 1);
+    CHECK(mr.second.empty(), "mkf rotate");
+    mkref_ctx mref;
+    mkref_ctx_init(&mref, logN, params.Q.data(), L, Pm.data(), Km, 1, 0);
+    std::vector<uint64_t> o0((size_t)L * N), o1((size_t)L * N);
+    const uint64_t *rp[2] = {rot1[0].data(), rot1[1].data()};
+    mkref_mkf_ckks_rotate(&mref, L - 1, 2, fin.data(), rp, gal1, o0.data(), o1.data());
+    mkref_ctx_free(&mref);
+    CHECK(same(mr.first.C0, o0.data(), L, N), "mkf rotate C0");
+    CHECK(mr.first.Components.size() == 2 && mr.first.Scale == 3.0, "mkf rotate shape");
+    for (const Poly &c : mr.first.Components) CHECK(same(c, o1.data(), L, N), "mkf rotate component");
+    CHECK(!me.RotateNew(in, 2).second.empty(), "mkf rotate missing keys is an error value");
+    std::printf("ok mkf RotateNew (N=2^13, L=4, K=1, k=2)\n");
+}
+
 int main() {
     std::setvbuf(stdout, nullptr, _IONBF, 0);  // every line reaches the log as it is printed
     try {
@@ -331,6 +483,7 @@ int main() {
         test_kklss_bfv();
         test_mkf(false);
         test_mkf(true);
+        test_rotations();
     } catch (const std::exception &e) {
         std::printf("FAIL unexpected exception: %s\n", e.what());
         return 1;
