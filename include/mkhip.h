@@ -290,6 +290,8 @@ int mkhip_plan_mkf_rotate(mkhip_ctx *ctx, int nops, const mkhip_mkf_rotate_op *o
  *   MKHIP_TUNE_XCD_REMAP     workgroups ordered XCD-contiguously (1) or as issued (0)
  *   MKHIP_TUNE_DECOMP_ORDER  fused decomposition grid order: ops sharing a key row
  *                            adjacent (0) or rows sharing an input limb adjacent (1)
+ *   MKHIP_TUNE_FP64          transforms of limbs whose modulus is below 2^50 in FP64
+ *                            arithmetic (1, default) or in 64-bit integer arithmetic (0)
  * A knob applies to plans built after it is set.  mkhip_autotune plans and times
  * every applicable combination on the caller's batch (ops: mkhip_kklss_op[] for the
  * KKLSS schemes, mkhip_mkf_op[] for MKF; their outputs are overwritten with the
@@ -298,6 +300,7 @@ int mkhip_plan_mkf_rotate(mkhip_ctx *ctx, int nops, const mkhip_mkf_rotate_op *o
 #define MKHIP_TUNE_FUSED_DECOMP 1
 #define MKHIP_TUNE_XCD_REMAP 2
 #define MKHIP_TUNE_DECOMP_ORDER 3
+#define MKHIP_TUNE_FP64 4
 #define MKHIP_SCHEME_KKLSS_CKKS 0
 #define MKHIP_SCHEME_KKLSS_BFV 1
 #define MKHIP_SCHEME_MKF_CKKS 2

@@ -45,7 +45,7 @@ EXPORTS = [
     "mkhip_del_conjugation_key", "mkhip_kklss_rotate", "mkhip_kklss_conjugate", "mkhip_plan_kklss_rotate",
     "mkhip_register_mkf_rotation", "mkhip_mkf_ckks_rotate", "mkhip_plan_mkf_rotate",
 ]
-TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER = 1, 2, 3
+TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64 = 1, 2, 3, 4
 SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3, "rotate": 4, "mkf_rotate": 5}
 MAX_ROTATIONS = 16
 CONJUGATE = -2

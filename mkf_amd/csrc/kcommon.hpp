@@ -2,6 +2,7 @@
 #pragma once
 
 #include "ntt.hpp"
+#include "nttfp.hpp"
 #include "tables.hpp"
 
 namespace mkhip {
@@ -169,6 +170,79 @@ __device__ __forceinline__ void fwd_transform_row(u64 (&v)[16], const u64 *row, 
     }
 }
 
+// fwd_transform_row in FP64 (modulus q < 2^50): v receives canonical residues as
+// doubles, in the Quad layout.  Inputs below 2q (red: below 2^64, reduced first).
+__device__ __forceinline__ FpMod fp_mod(const ModConst &m) {
+    FpMod f;
+    f.q = m.qd;
+    f.qinv = m.qinvd;
+    return f;
+}
+
+// fwd_transform in FP64: element c of the limb is f(c) (a word below 2q).
+template <int LOGN, class F>
+__device__ __forceinline__ void fwd_transform_fp(double (&v)[16], F f, int tid, int h, const double *twd,
+                                                 const ModConst &m) {
+    typedef Geo<LOGN> Gm;
+    const FpMod fm = fp_mod(m);
+    if constexpr (Gm::split) {
+        const double w = twiddle_fp<true>(twd, 1);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const double r = fp_mulmod(fp_from_u64(f(Gm::NS + tid + e * Gm::TH)), w, fm);
+            const double x = fp_from_u64(f(tid + e * Gm::TH));
+            v[e] = h ? x - r : x + r;
+        }
+        ntt_fwd_fp<Gm::LS>(v, g_lds, tid, twd, fm, 2 + h, 1);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = fp_from_u64(f(tid + e * Gm::TH));
+        ntt_fwd_fp<LOGN>(v, g_lds, tid, twd, fm, 1, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = fp_canon(v[e], fm);
+}
+
+template <int LOGN>
+__device__ __forceinline__ void fwd_transform_row_fp(double (&v)[16], const u64 *row, bool red, int tid, int h,
+                                                     const double *twd, const ModConst &m) {
+    typedef Geo<LOGN> Gm;
+    const FpMod fm = fp_mod(m);
+    if constexpr (Gm::split) {
+        u64 lo[16], hi[16];
+        load_row_cached<Gm::LS>(lo, row, tid);
+        load_row_cached<Gm::LS>(hi, row + Gm::NS, tid);
+        if (red) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                lo[e] = reduce64(lo[e], m.q, m.mu);
+                hi[e] = reduce64(hi[e], m.q, m.mu);
+            }
+        }
+        // first stage (t = N/2, twiddle tw[1]): half h keeps x + w y or x - w y
+        const double w = twiddle_fp<true>(twd, 1);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const double r = fp_mulmod(fp_from_u64(hi[e]), w, fm);
+            const double x = fp_from_u64(lo[e]);
+            v[e] = h ? x - r : x + r;
+        }
+        ntt_fwd_fp<Gm::LS>(v, g_lds, tid, twd, fm, 2 + h, 1);
+    } else {
+        u64 x[16];
+        load_row_cached<LOGN>(x, row, tid);
+        if (red) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) x[e] = reduce64(x[e], m.q, m.mu);
+        }
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = fp_from_u64(x[e]);
+        ntt_fwd_fp<LOGN>(v, g_lds, tid, twd, fm, 1, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = fp_canon(v[e], fm);
+}
+
 // Inverse transform of this workgroup's elements (v[e] = element h NS + tid + e TH,
 // values < 4q) with the N^-1 (or t N^-1) scaling given; for a split limb the last
 // stage and the scaling are left to k_intt_finish.
@@ -180,6 +254,24 @@ __device__ __forceinline__ void inv_transform(u64 (&v)[16], int tid, int h, cons
         ntt_inv<Gm::LS>(v, g_lds, tid, itw, q, ninv, ninvp, 2 + h, false);
     else
         ntt_inv<LOGN>(v, g_lds, tid, itw, q, ninv, ninvp, 1, true);
+}
+
+// inv_transform choosing the FP64 arithmetic for moduli below 2^50 (a wave-uniform
+// branch).  tscale: t N^-1 instead of N^-1.  Same words as the integer form.
+template <int LOGN>
+__device__ __forceinline__ void inv_transform_sel(u64 (&v)[16], int tid, int h, const DevTables &T, int ri,
+                                                  const ModConst &m, bool tscale) {
+    typedef Geo<LOGN> Gm;
+    if (fp_ok(m.q) && T.fp64) {
+        const double *itwd = opaque(T.itwd + (size_t)ri * Gm::N);
+        if constexpr (Gm::split)
+            ntt_inv_fp<Gm::LS>(v, g_lds, tid, itwd, fp_mod(m), 0.0, 2 + h, false);
+        else
+            ntt_inv_fp<LOGN>(v, g_lds, tid, itwd, fp_mod(m), fp_from_u64(tscale ? m.tninv : m.ninv), 1, true);
+    } else {
+        inv_transform<LOGN>(v, tid, h, opaque(T.itw + (size_t)ri * Gm::N), m.q, tscale ? m.tninv : m.ninv,
+                            tscale ? m.tninvp : m.ninvp);
+    }
 }
 
 // Row r of a (level+1+K)-row QP polynomial is ring modulus r (Q) or L + r - level - 1 (P).

@@ -37,9 +37,7 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH) k_ntt_batch(DevTables T, const 
 #pragma unroll
             for (int i = 0; i < 16; ++i) v[i] = reduce64(v[i], m.q, m.mu);
         }
-        const bool ts = j.reduce & 2;
-        inv_transform<LOGN>(v, tid, h, T.itw + (size_t)j.mod * Gm::N, m.q, ts ? m.tninv : m.ninv,
-                            ts ? m.tninvp : m.ninvp);
+        inv_transform_sel<LOGN>(v, tid, h, T, j.mod, m, (j.reduce & 2) != 0);
     } else {
         const ulonglong2 *tw = T.tw + (size_t)j.mod * Gm::N;
         if constexpr (Gm::split) {
@@ -129,6 +127,30 @@ __device__ __forceinline__ void decomp_row(const DevTables &T, const u64 *src, u
     store_row_quad<Gm::LS>(v, dst, tid);
 }
 
+// decomp_row in FP64 for a modulus below 2^50 (the CRT rows' ModUp stays integer;
+// its canonical output is transformed in FP64).
+template <int LOGN, bool CRT>
+__device__ __forceinline__ void decomp_fp(const DevTables &T, const u64 *src, u64 *dst, int start, int nd,
+                                          bool own_row, int i, int r, int ri, int h, int tid, const ModConst &m) {
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
+    const double *twd = T.twd + (size_t)ri * N;
+    double v[16];
+    if (!CRT || nd == 1 || own_row) {
+        const int sl = own_row ? r : start;
+        const bool red = T.mod[r_index(sl, T.L, T.K)].q > 2 * m.q;
+        fwd_transform_row_fp<LOGN>(v, src + (size_t)sl * N, red, tid, h, twd, m);
+    } else {
+        const int ebase = i * (kMaxAlpha + 1) + nd;
+        fwd_transform_fp<LOGN>(v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h,
+                               twd, m);
+    }
+    u64 w[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) w[e] = fp_word(v[e]);
+    store_row_quad<Gm::LS>(w, dst, tid);
+}
+
 // decomp_row for an unsplit limb (N <= 2^14).
 template <int LOGN, bool CRT, int MODE>
 __device__ __forceinline__ void decomp_flat(const DevTables &T, const u64 *src, u64 *dst, int start, int nd,
@@ -179,6 +201,10 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     const u64 *src = Dd.comp[p];
     const ulonglong2 *tw = T.tw + (size_t)ri * N;
     u64 *dst = Dd.D + (((size_t)p * beta + i) * rows + r) * N + (size_t)h * Gm::NS;
+    if (fp_ok(m.q) && T.fp64) {
+        decomp_fp<LOGN, CRT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, m);
+        return;
+    }
     if constexpr (!Gm::split) {
         const bool crt_row = CRT && !(nd == 1 || own_row);
         if (!crt_row && fwd_lazy_ok(m.q))
@@ -253,7 +279,7 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
         }
         v[e] = x;
     }
-    inv_transform<LOGN>(v, tid, h, T.itw + (size_t)l * N, m.q, m.ninv, m.ninvp);
+    inv_transform_sel<LOGN>(v, tid, h, T, l, m, false);
     store_row<Gm::LS>(v, Dd.out + off, tid);
 }
 
@@ -280,18 +306,17 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     const ModConst m = T.mod[ri];
     const size_t kstride = (size_t)(T.L + T.K) * N;
     const size_t hoff = (size_t)h * Gm::NS;
-    const ulonglong2 *itw = T.itw + (size_t)ri * N;
     u64 a0[16];
     u64 *o1 = NKEY > 1 ? Dd.acc[1] + (size_t)r * N + hoff : nullptr;
     ext_rows<Gm::LS, NKEY>(a0, o1, Dd.D + (size_t)r * N + hoff, (size_t)rows * N, Dd.key[0] + (size_t)ri * N + hoff,
                            NKEY > 1 ? Dd.key[1] + (size_t)ri * N + hoff : nullptr, kstride, beta, tid, m);
     // opaque(): the second transform reloads its twiddles instead of keeping the
     // first one's live across it
-    inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
+    inv_transform_sel<LOGN>(a0, opaque_lane(tid), h, T, ri, m, false);
     store_row<Gm::LS>(a0, Dd.acc[0] + (size_t)r * N + hoff, tid);
     if (NKEY > 1) {
         load_row_quad<Gm::LS>(a0, o1, tid);
-        inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
+        inv_transform_sel<LOGN>(a0, opaque_lane(tid), h, T, ri, m, false);
         store_row<Gm::LS>(a0, o1, tid);
     }
 }

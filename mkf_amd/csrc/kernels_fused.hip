@@ -50,6 +50,51 @@ __device__ __forceinline__ void decomp_mac_row(const DecompDesc &Dd, int i, int 
     store_row_quad<Gm::LS>(acc, Dd.xy + krow, tid);
 }
 
+// decomp_mac_row for a modulus below 2^50: FP64 transform and MAC.  The stored
+// digit rows and the accumulator are the same canonical words as the integer
+// form's: x = MForm(sum_p MRed(key_p, D_p)) = sum_p key_p D_p mod q.
+template <int LOGN>
+__device__ __forceinline__ void decomp_mac_row_fp(const DecompDesc &Dd, int i, int r, int beta, int rows, int h,
+                                                  bool reduce, size_t krow, int tid, const double *twd,
+                                                  const ModConst &m) {
+    typedef Geo<LOGN> Gm;
+    constexpr int N = Gm::N;
+    const size_t hoff = (size_t)h * Gm::NS;
+    const FpMod fm = fp_mod(m);
+    double acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0;
+    for (int p = 0; p < Dd.k; ++p) {
+        double v[16];
+        __builtin_amdgcn_sched_barrier(0);
+        fwd_transform_row_fp<LOGN>(v, Dd.comp[p] + (size_t)i * N, reduce, opaque_lane(tid), h, opaque(twd), m);
+        __builtin_amdgcn_sched_barrier(0);
+        {
+            u64 w[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) w[e] = fp_word(v[e]);
+            store_row_quad<Gm::LS>(w, Dd.D + (((size_t)p * beta + i) * rows + r) * N + hoff, tid);
+        }
+        const gu64 *key = uniform_ptr(Dd.key[p] + krow);
+        // products below 0.9q each: the accumulator is reduced every fourth party
+        const bool red = (p & 3) == 3;
+#pragma unroll
+        for (int e = 0; e < 16; e += 2) {  // key words in the transform's Quad layout
+            const u64v2 kw = *(const gv2 *)(key + Quad<Gm::LS>::elem(tid, e));
+            acc[e] += fp_mulmod(fp_from_u64(kw.x), v[e], fm);
+            acc[e + 1] += fp_mulmod(fp_from_u64(kw.y), v[e + 1], fm);
+            if (red) {
+                acc[e] = fp_reduce(acc[e], fm);
+                acc[e + 1] = fp_reduce(acc[e + 1], fm);
+            }
+        }
+    }
+    u64 out[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) out[e] = fp_word(fp_canon(acc[e], fm));
+    store_row_quad<Gm::LS>(out, Dd.xy + krow, tid);
+}
+
 template <int LOGN>
 __global__ void __launch_bounds__(Geo<LOGN>::TH)
     k_decomp_mac(DevTables T, const DecompDesc *__restrict__ descs, int nops, int beta, int rows, int order) {
@@ -82,7 +127,9 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     const bool reduce = T.mod[r_index(i, T.L, T.K)].q > 2 * m.q;
     // one instance of the loop per butterfly form (a wave-uniform branch), so that
     // neither form's registers are allocated around the other's
-    if (fwd_lazy_ok(m.q))
+    if (fp_ok(m.q) && T.fp64)
+        decomp_mac_row_fp<LOGN>(Dd, i, r, beta, rows, h, reduce, krow, tid, T.twd + (size_t)ri * N, m);
+    else if (fwd_lazy_ok(m.q))
         decomp_mac_row<LOGN, 1>(Dd, i, r, beta, rows, h, reduce, krow, tid, tw, m);
     else
         decomp_mac_row<LOGN, 0>(Dd, i, r, beta, rows, h, reduce, krow, tid, tw, m);

@@ -33,11 +33,10 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     ext_rows<Gm::LS, 2>(a0, o1, Dd.D + (size_t)r * N + hoff, (size_t)rows * N, Dd.swk + (size_t)ri * N + hoff,
                         Dd.swk + kstride + (size_t)ri * N + hoff, 2 * kstride, beta, tid, m);
     if (r > Dd.level || Dd.intt_all) {
-        const ulonglong2 *itw = T.itw + (size_t)ri * N;
-        inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
+        inv_transform_sel<LOGN>(a0, opaque_lane(tid), h, T, ri, m, false);
         store_row<Gm::LS>(a0, o0, tid);
         load_row_quad<Gm::LS>(a0, o1, tid);
-        inv_transform<LOGN>(a0, opaque_lane(tid), h, opaque(itw), m.q, m.ninv, m.ninvp);
+        inv_transform_sel<LOGN>(a0, opaque_lane(tid), h, T, ri, m, false);
         store_row<Gm::LS>(a0, o1, tid);
     } else {  // NTT-domain rows stay in the MAC's Quad layout
         store_row_quad<Gm::LS>(a0, o0, tid);

@@ -181,6 +181,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     // ---- host tables ----
     std::vector<ModConst> mods(nmod);
     std::vector<ulonglong2> tw((size_t)nmod * N), itw((size_t)nmod * N);
+    std::vector<double> twd((size_t)nmod * N), itwd((size_t)nmod * N);
     for (int i = 0; i < nmod; ++i) {
         const u64 qi = all[i];
         ModConst &m = mods[i];
@@ -200,6 +201,8 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
             const u64 idx = bitrev((u64)j, logN);
             tw[(size_t)i * N + idx] = make_ulonglong2(a, h_shoup_companion(a, qi));
             itw[(size_t)i * N + idx] = make_ulonglong2(b, h_shoup_companion(b, qi));
+            twd[(size_t)i * N + idx] = (double)a;
+            itwd[(size_t)i * N + idx] = (double)b;
             a = h_mulmod(a, psi, qi);
             b = h_mulmod(b, psiinv, qi);
         }
@@ -209,6 +212,8 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
         m.wnp = h_shoup_companion(m.wn, qi);
         m.twn = h_mulmod(w1, m.tninv, qi);
         m.twnp = h_shoup_companion(m.twn, qi);
+        m.qd = (double)qi;
+        m.qinvd = 1.0 / (double)qi;
     }
     // ModDown P -> Q
     std::vector<u64> md_y, md_qstar, md_vt, md_pinv(L);
@@ -281,6 +286,8 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     size_t o_mod = blob.add(mods.data(), mods.size() * sizeof(ModConst));
     size_t o_tw = blob.add(tw.data(), tw.size() * sizeof(ulonglong2));
     size_t o_itw = blob.add(itw.data(), itw.size() * sizeof(ulonglong2));
+    size_t o_twd = blob.add(twd.data(), twd.size() * sizeof(double));
+    size_t o_itwd = blob.add(itwd.data(), itwd.size() * sizeof(double));
     size_t o_mdy = blob.add(md_y.data(), md_y.size() * 8);
     size_t o_mdq = blob.add(md_qstar.data(), md_qstar.size() * 8);
     size_t o_mdv = blob.add(md_vt.data(), md_vt.size() * 8);
@@ -307,6 +314,8 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     T.mod = (const ModConst *)(base + o_mod);
     T.tw = (const ulonglong2 *)(base + o_tw);
     T.itw = (const ulonglong2 *)(base + o_itw);
+    T.twd = (const double *)(base + o_twd);
+    T.itwd = (const double *)(base + o_itwd);
     T.md_y = (const u64 *)(base + o_mdy);
     T.md_qstar = (const u64 *)(base + o_mdq);
     T.md_vt = (const u64 *)(base + o_mdv);
@@ -321,6 +330,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     for (int i = 0; i < 9; ++i) *bqt[i] = bfv ? (const u64 *)(base + o_bq[i]) : nullptr;
     T.bfv = bfv ? 1 : 0;
     T.xcd = 1;
+    T.fp64 = 1;
     T.logN = logN;
     T.N = N;
     T.L = L;

@@ -214,6 +214,9 @@ extern "C" int mkhip_set_tuning(mkhip_ctx *c, int knob, int value) {
     case MKHIP_TUNE_DECOMP_ORDER:
         c->T.decomp_order = value ? 1 : 0;
         break;
+    case MKHIP_TUNE_FP64:
+        c->T.fp64 = value ? 1 : 0;
+        break;
     default:
         return MKHIP_ERR_ARG;
     }
@@ -234,6 +237,9 @@ extern "C" int mkhip_get_tuning(const mkhip_ctx *c, int knob, int *value) {
         return MKHIP_OK;
     case MKHIP_TUNE_DECOMP_ORDER:
         *value = c->T.decomp_order;
+        return MKHIP_OK;
+    case MKHIP_TUNE_FP64:
+        *value = c->T.fp64;
         return MKHIP_OK;
     default:
         return MKHIP_ERR_ARG;

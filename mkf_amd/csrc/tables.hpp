@@ -15,6 +15,8 @@ struct DevTables {
     const ModConst *mod;      // [nmod] : Q[0..L) then P[0..K) then (BFV) QMul[0..L)
     const ulonglong2 *tw;     // [nmod][N] forward twiddles (w, Shoup w'), Lattigo index order
     const ulonglong2 *itw;    // [nmod][N] inverse twiddles
+    const double *twd;        // [nmod][N] forward twiddles as doubles (exact for q < 2^50)
+    const double *itwd;       // [nmod][N] inverse twiddles as doubles
     // ModDown Q u P -> Q (basis_extension.go:192-232 with paramsPtoQ[K-1]):
     const u64 *md_y;          // [K]       MForm((P/p_i)^-1 mod p_i)
     const u64 *md_qstar;      // [L][K]    MForm((P/p_i) mod q_j)
@@ -43,6 +45,7 @@ struct DevTables {
     int bfv;                  // 1 when the QMul chain is present
     int xcd;                  // 1: XCD-contiguous workgroup order (xcd_remap), a tuning knob
     int decomp_order;         // fused decomposition grid order (a tuning knob)
+    int fp64;                 // FP64 transforms for moduli below 2^50 (a tuning knob)
 };
 
 // ring index of limb l of an R = Q u QMul polynomial
