@@ -40,6 +40,24 @@ __device__ __forceinline__ void store_row(const u64 (&v)[16], u64 *__restrict__ 
     for (int i = 0; i < 16; ++i) uniform_ptr(dst + i * T)[tid] = v[i];
 }
 
+// Asynchronous copy of one row of N = 2^LOGN words (natural order) into LDS words
+// 0..N-1, without registers: global_load_lds_dwordx4 writes lane l's 16 bytes at
+// the wave's LDS base + 16 l.  Block b of 128 words goes to LDS words 128 b.. ;
+// each of the N/1024 waves copies 8 blocks.  Complete after s_waitcnt vmcnt(0) in
+// every wave and a barrier.
+template <int LOGN>
+__device__ __forceinline__ void row_to_lds_async(const u64 *row, int tid) {
+    typedef __attribute__((address_space(1))) void gvoid;
+    typedef __attribute__((address_space(3))) void lvoid;
+    constexpr int NW = (1 << LOGN) / 16 / 64;  // waves of the workgroup
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int b = j * NW + wave;
+        __builtin_amdgcn_global_load_lds((gvoid *)(row + b * 128 + lane * 2), (lvoid *)(g_lds + b * 128), 16, 0, 0);
+    }
+}
+
 // 16-byte global accesses (two consecutive words)
 typedef unsigned long long u64v2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) u64v2 gv2;
@@ -344,6 +362,62 @@ __device__ __forceinline__ void ext_rows(u64 (&a0)[16], u64 *park1, const u64 *d
         }
     }
     if constexpr (NK > 1) store_row_quad<LOGN>(a1, park1, tid);
+}
+
+// ext_rows in FP64 (modulus below 2^50): a0[e] = sum_i MRed(k0_i[e], d_i[e]) as a
+// double congruent residue (|a0| < 2.6q), computed as (sum_i k0_i d_i) 2^-64; with
+// NK = 2 the k1 sum is written to park1 as canonical words.
+template <int LOGN, int NK>
+__device__ __forceinline__ void ext_rows_fp(double (&a0)[16], u64 *park1, const u64 *d, size_t dstride,
+                                            const u64 *k0, const u64 *k1, size_t kstride, int ndig, int tid,
+                                            const ModConst &m) {
+    typedef Quad<LOGN> Qd;
+    const FpMod fm = fp_mod(m);
+    double a1[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) a0[e] = a1[e] = 0.0;
+    for (int i = 0; i < ndig; ++i) {
+        const u64 *dr = d + i * dstride, *k0r = k0 + i * kstride, *k1r = NK > 1 ? k1 + i * kstride : nullptr;
+        // products below 0.6q: reduce every fourth digit
+        const bool red = (i & 3) == 3;
+#pragma unroll
+        for (int c = 0; c < 16; c += 4) {
+            u64 x[4], y0[4], y1[4];
+#pragma unroll
+            for (int e = 0; e < 4; e += 2) {
+                const int off = Qd::elem(tid, c + e);
+                const u64v2 xd = *(const gv2 *)(uniform_ptr(dr) + off), yd = *(const gv2 *)(uniform_ptr(k0r) + off);
+                x[e] = xd.x;
+                x[e + 1] = xd.y;
+                y0[e] = yd.x;
+                y0[e + 1] = yd.y;
+                if (NK > 1) {
+                    const u64v2 zd = *(const gv2 *)(uniform_ptr(k1r) + off);
+                    y1[e] = zd.x;
+                    y1[e + 1] = zd.y;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double xv = fp_from_u64(x[e]);
+                a0[c + e] += fp_mulmod(fp_from_u64(y0[e]), xv, fm);
+                if (NK > 1) a1[c + e] += fp_mulmod(fp_from_u64(y1[e]), xv, fm);
+                if (red) {
+                    a0[c + e] = fp_reduce(a0[c + e], fm);
+                    if (NK > 1) a1[c + e] = fp_reduce(a1[c + e], fm);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) a0[e] = fp_mulmod(fp_reduce(a0[e], fm), m.rinvd, fm);
+    if constexpr (NK > 1) {
+        u64 w[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) w[e] = fp_word(fp_canon(fp_mulmod(fp_reduce(a1[e], fm), m.rinvd, fm), fm));
+        store_row_quad<LOGN>(w, park1, tid);
+    }
 }
 
 // a * b mod q for two canonical operands: MRed(MForm(a), b).

@@ -308,11 +308,25 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     const size_t hoff = (size_t)h * Gm::NS;
     u64 a0[16];
     u64 *o1 = NKEY > 1 ? Dd.acc[1] + (size_t)r * N + hoff : nullptr;
-    ext_rows<Gm::LS, NKEY>(a0, o1, Dd.D + (size_t)r * N + hoff, (size_t)rows * N, Dd.key[0] + (size_t)ri * N + hoff,
-                           NKEY > 1 ? Dd.key[1] + (size_t)ri * N + hoff : nullptr, kstride, beta, tid, m);
-    // opaque(): the second transform reloads its twiddles instead of keeping the
-    // first one's live across it
-    inv_transform_sel<LOGN>(a0, opaque_lane(tid), h, T, ri, m, false);
+    if (fp_ok(m.q) && T.fp64) {
+        double d0[16];
+        ext_rows_fp<Gm::LS, NKEY>(d0, o1, Dd.D + (size_t)r * N + hoff, (size_t)rows * N,
+                                  Dd.key[0] + (size_t)ri * N + hoff,
+                                  NKEY > 1 ? Dd.key[1] + (size_t)ri * N + hoff : nullptr, kstride, beta, tid, m);
+        const double *itwd = opaque(T.itwd + (size_t)ri * N);
+        const int t = opaque_lane(tid);
+        if constexpr (Gm::split)
+            ntt_inv_fp_d<Gm::LS>(d0, a0, g_lds, t, itwd, fp_mod(m), 0.0, 2 + h, false);
+        else
+            ntt_inv_fp_d<LOGN>(d0, a0, g_lds, t, itwd, fp_mod(m), fp_from_u64(m.ninv), 1, true);
+    } else {
+        ext_rows<Gm::LS, NKEY>(a0, o1, Dd.D + (size_t)r * N + hoff, (size_t)rows * N,
+                               Dd.key[0] + (size_t)ri * N + hoff,
+                               NKEY > 1 ? Dd.key[1] + (size_t)ri * N + hoff : nullptr, kstride, beta, tid, m);
+        // opaque(): the second transform reloads its twiddles instead of keeping the
+        // first one's live across it
+        inv_transform_sel<LOGN>(a0, opaque_lane(tid), h, T, ri, m, false);
+    }
     store_row<Gm::LS>(a0, Dd.acc[0] + (size_t)r * N + hoff, tid);
     if (NKEY > 1) {
         load_row_quad<Gm::LS>(a0, o1, tid);

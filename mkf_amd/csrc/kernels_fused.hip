@@ -64,10 +64,33 @@ __device__ __forceinline__ void decomp_mac_row_fp(const DecompDesc &Dd, int i, i
     double acc[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.0;
+    // Unsplit limbs: each party's digit row is copied into LDS asynchronously while
+    // the previous party's row finishes its transform and MAC, then read from LDS.
+    if constexpr (!Gm::split)
+        if (Dd.k > 0) row_to_lds_async<LOGN>(Dd.comp[0] + (size_t)i * N, tid);
     for (int p = 0; p < Dd.k; ++p) {
         double v[16];
         __builtin_amdgcn_sched_barrier(0);
-        fwd_transform_row_fp<LOGN>(v, Dd.comp[p] + (size_t)i * N, reduce, opaque_lane(tid), h, opaque(twd), m);
+        if constexpr (Gm::split) {
+            fwd_transform_row_fp<LOGN>(v, Dd.comp[p] + (size_t)i * N, reduce, opaque_lane(tid), h, opaque(twd), m);
+        } else {
+            const FpMod fm0 = fp_mod(m);
+            const int t = opaque_lane(tid);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                u64 x = g_lds[t + e * Gm::TH];
+                if (reduce) x = reduce64(x, m.q, m.mu);
+                v[e] = fp_from_u64(x);
+            }
+            __syncthreads();
+            ntt_fwd_fp_head<LOGN>(v, g_lds, t, opaque(twd), fm0, 1, 0);
+            if (p + 1 < Dd.k) row_to_lds_async<LOGN>(Dd.comp[p + 1] + (size_t)i * N, t);
+            ntt_fwd_fp_tail<LOGN>(v, t, opaque(twd), fm0, 1, 0);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[e] = fp_canon(v[e], fm0);
+        }
         __builtin_amdgcn_sched_barrier(0);
         {
             u64 w[16];

@@ -214,6 +214,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
         m.twnp = h_shoup_companion(m.twn, qi);
         m.qd = (double)qi;
         m.qinvd = 1.0 / (double)qi;
+        m.rinvd = (double)h_inv((u64)((((u128)1) << 64) % qi), qi);
     }
     // ModDown P -> Q
     std::vector<u64> md_y, md_qstar, md_vt, md_pinv(L);

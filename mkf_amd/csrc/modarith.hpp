@@ -109,6 +109,7 @@ struct ModConst {
     u64 wn, wnp;    // psiinv[1] * N^-1 (last inverse stage of a split limb) + companion
     u64 twn, twnp;  // t * psiinv[1] * N^-1 + companion
     double qd, qinvd;  // q and fl(1 / q) for the FP64 transforms (q < 2^50)
+    double rinvd;      // 2^-64 mod q (exact as a double for q < 2^53): MRed's factor
 };
 
 // host helpers (exact, 128-bit); host-only functions, parsed in both compilation passes

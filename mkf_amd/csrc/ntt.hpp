@@ -63,14 +63,22 @@ __device__ __forceinline__ u64 reduce_8q(u64 x, u64 q) {
 // Twiddle (w, w') at index idx.  When idx is the same on every lane of the wave
 // the pair is moved to scalar registers (readfirstlane), so it occupies no VGPRs
 // while the butterflies of the stage run.
+// A uniform index is read through the constant address space, which compiles to a
+// scalar load (waited on with lgkmcnt, so it is not held up behind the wave's
+// outstanding vector-memory operations).
 template <bool UNIFORM>
 __device__ __forceinline__ ulonglong2 twiddle(const ulonglong2 *__restrict__ tw, int idx) {
-    ulonglong2 w = tw[idx];
+    ulonglong2 w;
     if (UNIFORM) {
-        const u32 a = __builtin_amdgcn_readfirstlane((u32)w.x), b = __builtin_amdgcn_readfirstlane((u32)(w.x >> 32));
-        const u32 c = __builtin_amdgcn_readfirstlane((u32)w.y), d = __builtin_amdgcn_readfirstlane((u32)(w.y >> 32));
-        w.x = ((u64)b << 32) | a;
-        w.y = ((u64)d << 32) | c;
+        typedef __attribute__((address_space(4))) const u64 cu64;
+        const cu64 *p = (const cu64 *)tw + 2 * __builtin_amdgcn_readfirstlane(idx);
+        w.x = p[0];
+        w.y = p[1];
+    } else {
+        typedef __attribute__((address_space(1))) const u64 gu64c;
+        const gu64c *p = (const gu64c *)tw + 2 * idx;
+        w.x = p[0];
+        w.y = p[1];
     }
     return w;
 }

@@ -62,13 +62,12 @@ __device__ __forceinline__ u64 fp_word(double r) {
 
 template <bool UNIFORM>
 __device__ __forceinline__ double twiddle_fp(const double *__restrict__ tw, int idx) {
-    double w = tw[idx];
-    if (UNIFORM) {
-        const u64 b = (u64)__double_as_longlong(w);
-        const u32 lo = __builtin_amdgcn_readfirstlane((u32)b), hi = __builtin_amdgcn_readfirstlane((u32)(b >> 32));
-        w = __longlong_as_double((long long)(((u64)hi << 32) | lo));
+    if (UNIFORM) {  // scalar load, as twiddle<true>
+        typedef __attribute__((address_space(4))) const double cd;
+        return ((cd *)tw)[__builtin_amdgcn_readfirstlane(idx)];
     }
-    return w;
+    typedef __attribute__((address_space(1))) const double gd;
+    return ((gd *)tw)[idx];
 }
 
 template <int LOGN, int S0, int NS>
@@ -121,8 +120,7 @@ __device__ __forceinline__ void lds_exchange_fp(double (&v)[16], u64 *s, int tid
 // ntt_fwd; par = 1 shifts the reduction schedule by one stage (used after a
 // first stage computed outside).
 template <int LOGN>
-__device__ __forceinline__ void ntt_fwd_fp(double (&v)[16], u64 *s, int tid, const double *__restrict__ tw,
-                                           const FpMod &m, int hh = 1, int par = 0) {
+struct FpPlan {
     typedef NttPlan<LOGN> P;
     typedef FwdPassFp<LOGN, 0, 4> F0;
     typedef FwdPassFp<LOGN, 4, 4> F1;
@@ -131,13 +129,33 @@ __device__ __forceinline__ void ntt_fwd_fp(double (&v)[16], u64 *s, int tid, con
     static_assert(F0::TL == P::F0::TL && F1::TL == P::F1::TL && F2::TL == P::F2::TL && F3::TL == P::F3::TL &&
                       F3::R == P::F3::R,
                   "the passes of the integer transform");
-    F0::run(v, tid, tw, m, hh, par);
+};
+
+// The passes that use LDS (all but the last); LDS is free again when this returns.
+template <int LOGN>
+__device__ __forceinline__ void ntt_fwd_fp_head(double (&v)[16], u64 *s, int tid, const double *__restrict__ tw,
+                                                const FpMod &m, int hh, int par) {
+    typedef FpPlan<LOGN> Q;
+    typedef NttPlan<LOGN> P;
+    Q::F0::run(v, tid, tw, m, hh, par);
     lds_exchange_fp<typename P::F0, typename P::F1>(v, s, tid);
-    F1::run(v, tid, tw, m, hh, par);
+    Q::F1::run(v, tid, tw, m, hh, par);
     lds_exchange_fp<typename P::F1, typename P::F2>(v, s, tid);
-    F2::run(v, tid, tw, m, hh, par);
+    Q::F2::run(v, tid, tw, m, hh, par);
     lds_exchange_fp<typename P::F2, typename P::F3>(v, s, tid);
-    F3::run(v, tid, tw, m, hh, par);
+}
+
+template <int LOGN>
+__device__ __forceinline__ void ntt_fwd_fp_tail(double (&v)[16], int tid, const double *__restrict__ tw,
+                                                const FpMod &m, int hh, int par) {
+    FpPlan<LOGN>::F3::run(v, tid, tw, m, hh, par);
+}
+
+template <int LOGN>
+__device__ __forceinline__ void ntt_fwd_fp(double (&v)[16], u64 *s, int tid, const double *__restrict__ tw,
+                                           const FpMod &m, int hh = 1, int par = 0) {
+    ntt_fwd_fp_head<LOGN>(v, s, tid, tw, m, hh, par);
+    ntt_fwd_fp_tail<LOGN>(v, tid, tw, m, hh, par);
 }
 
 // Inverse passes in FP64 (Gentleman-Sande): x' = reduce(x + y) with |x'| <= q/2,
@@ -180,6 +198,27 @@ struct InvPassFp : InvPass<LOGN, U0, NS> {
 // scale the N^-1 (or t N^-1) factor `ninv` is applied and the words returned are
 // canonical; without it (a half of a split limb) they are canonical residues of
 // the values before the last stage.
+// The same on residues already held as doubles (|d| < 2.6q); words out.
+template <int LOGN>
+__device__ __forceinline__ void ntt_inv_fp_d(double (&d)[16], u64 (&v)[16], u64 *s, int tid,
+                                             const double *__restrict__ itw, const FpMod &m, double ninv, int hh,
+                                             bool scale) {
+    typedef NttPlan<LOGN> P;
+    typedef InvPassFp<LOGN, 0, LOGN - 12> I0;
+    typedef InvPassFp<LOGN, LOGN - 12, 4> I1;
+    typedef InvPassFp<LOGN, LOGN - 8, 4> I2;
+    typedef InvPassFp<LOGN, LOGN - 4, 4> I3;
+    I0::run(d, tid, itw, m, hh);
+    lds_exchange_fp<typename P::I0, typename P::I1>(d, s, tid);
+    I1::run(d, tid, itw, m, hh);
+    lds_exchange_fp<typename P::I1, typename P::I2>(d, s, tid);
+    I2::run(d, tid, itw, m, hh);
+    lds_exchange_fp<typename P::I2, typename P::I3>(d, s, tid);
+    I3::run(d, tid, itw, m, hh);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = fp_word(fp_canon(scale ? fp_mulmod(d[i], ninv, m) : d[i], m));
+}
+
 template <int LOGN>
 __device__ __forceinline__ void ntt_inv_fp(u64 (&v)[16], u64 *s, int tid, const double *__restrict__ itw,
                                            const FpMod &m, double ninv, int hh, bool scale) {

@@ -197,3 +197,32 @@ def test_unregistered_party_is_not_served_from_the_cached_plan(setup):
     with pytest.raises(mk.MkhipError) as e:
         ctx.kklss_ckks_mulrelin(op0, [0, 1], op0, [0, 1], scale, scale, scale)
     assert e.value.status == -4
+
+
+def test_fp64_and_integer_arithmetic_agree(setup):
+    """Limbs whose modulus is below 2^50 run the FP64 transforms; extreme inputs
+    (all q-1, alternating q-1 / 0, a single q-1) match the oracle, and a whole
+    mul+relin gives the same words with the integer arithmetic forced."""
+    name, ctx, ref = setup
+    mk = _mk()
+    N, Q = ref.N, ref.Q
+    x = np.stack([np.full(N, q - 1, np.uint64) if i % 3 == 0 else
+                  np.where(np.arange(N) % 2 == 0, q - 1, 0).astype(np.uint64) if i % 3 == 1 else
+                  np.where(np.arange(N) == N - 1, q - 1, 0).astype(np.uint64) for i, q in enumerate(Q)])
+    rows = list(range(len(Q)))
+    fwd = ctx.ntt(x, rows)
+    assert np.array_equal(fwd, np.stack([ref.ntt(x[i], i) for i in rows]))
+    assert np.array_equal(ctx.ntt(x, rows, inverse=True), np.stack([ref.ntt(x[i], i, inverse=True) for i in rows]))
+    assert np.array_equal(ctx.ntt(fwd, rows, inverse=True), x)
+    _keys(ctx, ref, [0, 1], seed=43)
+    g = O.Prng(44)
+    op0 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
+    op1 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
+    scale = float(ref.Q[-1])
+    a = ctx.kklss_ckks_mulrelin(op0, [0, 1], op1, [0, 1], scale, scale, scale)[0]
+    ctx.set_tuning(mk.TUNE_FP64, 0)
+    try:
+        b = ctx.kklss_ckks_mulrelin(op0, [0, 1], op1, [0, 1], scale, scale, scale)[0]
+    finally:
+        ctx.set_tuning(mk.TUNE_FP64, 1)
+    assert np.array_equal(a, b)
