@@ -85,9 +85,11 @@ __device__ __forceinline__ void decomp_mac_row_fp(const DecompDesc &Dd, int i, i
                 v[e] = fp_from_u64(x);
             }
             __syncthreads();
-            ntt_fwd_fp_head<LOGN>(v, g_lds, t, opaque(twd), fm0, 1, 0);
+            // the accumulator is live across the transform: twiddles gathered per
+            // stage only (registers)
+            ntt_fwd_fp_head<LOGN, false>(v, g_lds, t, opaque(twd), fm0, 1, 0);
             if (p + 1 < Dd.k) row_to_lds_async<LOGN>(Dd.comp[p + 1] + (size_t)i * N, t);
-            ntt_fwd_fp_tail<LOGN>(v, t, opaque(twd), fm0, 1, 0);
+            ntt_fwd_fp_tail<LOGN, false>(v, t, opaque(twd), fm0, 1, 0);
 #pragma unroll
             for (int e = 0; e < 16; ++e) v[e] = fp_canon(v[e], fm0);
         }

@@ -73,14 +73,33 @@ __device__ __forceinline__ double twiddle_fp(const double *__restrict__ tw, int 
 template <int LOGN, int S0, int NS>
 struct FwdPassFp : FwdPass<LOGN, S0, NS> {
     typedef FwdPass<LOGN, S0, NS> B;
-    // stage S0 + u reduces u when (S0 + u + par) is odd
+    // stage S0 + u reduces u when (S0 + u + par) is odd.  GATHER: per-lane twiddles
+    // loaded together at the start (more registers, one memory wait per pass).
+    template <bool GATHER = true>
     __device__ static __forceinline__ void run(double (&v)[16], int tid, const double *__restrict__ tw,
                                                const FpMod &m, int hh, int par) {
         constexpr int R = B::R, C = B::C, TL = B::TL, T = B::T;
+        constexpr bool UNI = TL % 64 == 0;
+        constexpr bool PRE = !UNI && GATHER;
+        // per-lane twiddles: all of the pass's are gathered up front, so the pass
+        // waits for memory once instead of once per stage
+        double twl[PRE ? C * (R - 1) : 1];
+        if constexpr (PRE) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int b = (tid + c * T) / TL;
+#pragma unroll
+                for (int u = 0; u < NS; ++u)
+#pragma unroll
+                    for (int grp = 0; grp < (1 << u); ++grp)
+                        twl[c * (R - 1) + (1 << u) - 1 + grp] =
+                            twiddle_fp<false>(tw, (1 << (S0 + u)) * hh + b * (1 << u) + grp);
+            }
+        }
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const int col = tid + c * T;
-            const int b = (TL % 64 == 0) ? __builtin_amdgcn_readfirstlane(col / TL) : col / TL;
+            const int b = UNI ? __builtin_amdgcn_readfirstlane(col / TL) : col / TL;
 #pragma unroll
             for (int u = 0; u < NS; ++u) {
                 __builtin_amdgcn_sched_barrier(0);
@@ -88,7 +107,8 @@ struct FwdPassFp : FwdPass<LOGN, S0, NS> {
                 const bool red = ((S0 + u + par) & 1) != 0;
 #pragma unroll
                 for (int grp = 0; grp < (1 << u); ++grp) {
-                    const double w = twiddle_fp<(TL % 64 == 0)>(tw, (1 << (S0 + u)) * hh + b * (1 << u) + grp);
+                    const double w = PRE ? twl[c * (R - 1) + (1 << u) - 1 + grp]
+                                         : twiddle_fp<UNI>(tw, (1 << (S0 + u)) * hh + b * (1 << u) + grp);
 #pragma unroll
                     for (int kk = 0; kk < half; ++kk) {
                         const int k0 = grp * 2 * half + kk;
@@ -132,30 +152,30 @@ struct FpPlan {
 };
 
 // The passes that use LDS (all but the last); LDS is free again when this returns.
-template <int LOGN>
+template <int LOGN, bool GATHER = true>
 __device__ __forceinline__ void ntt_fwd_fp_head(double (&v)[16], u64 *s, int tid, const double *__restrict__ tw,
                                                 const FpMod &m, int hh, int par) {
     typedef FpPlan<LOGN> Q;
     typedef NttPlan<LOGN> P;
-    Q::F0::run(v, tid, tw, m, hh, par);
+    Q::F0::template run<GATHER>(v, tid, tw, m, hh, par);
     lds_exchange_fp<typename P::F0, typename P::F1>(v, s, tid);
-    Q::F1::run(v, tid, tw, m, hh, par);
+    Q::F1::template run<GATHER>(v, tid, tw, m, hh, par);
     lds_exchange_fp<typename P::F1, typename P::F2>(v, s, tid);
-    Q::F2::run(v, tid, tw, m, hh, par);
+    Q::F2::template run<GATHER>(v, tid, tw, m, hh, par);
     lds_exchange_fp<typename P::F2, typename P::F3>(v, s, tid);
 }
 
-template <int LOGN>
+template <int LOGN, bool GATHER = true>
 __device__ __forceinline__ void ntt_fwd_fp_tail(double (&v)[16], int tid, const double *__restrict__ tw,
                                                 const FpMod &m, int hh, int par) {
-    FpPlan<LOGN>::F3::run(v, tid, tw, m, hh, par);
+    FpPlan<LOGN>::F3::template run<GATHER>(v, tid, tw, m, hh, par);
 }
 
-template <int LOGN>
+template <int LOGN, bool GATHER = true>
 __device__ __forceinline__ void ntt_fwd_fp(double (&v)[16], u64 *s, int tid, const double *__restrict__ tw,
                                            const FpMod &m, int hh = 1, int par = 0) {
-    ntt_fwd_fp_head<LOGN>(v, s, tid, tw, m, hh, par);
-    ntt_fwd_fp_tail<LOGN>(v, tid, tw, m, hh, par);
+    ntt_fwd_fp_head<LOGN, GATHER>(v, s, tid, tw, m, hh, par);
+    ntt_fwd_fp_tail<LOGN, GATHER>(v, tid, tw, m, hh, par);
 }
 
 // Inverse passes in FP64 (Gentleman-Sande): x' = reduce(x + y) with |x'| <= q/2,
@@ -167,10 +187,28 @@ struct InvPassFp : InvPass<LOGN, U0, NS> {
     __device__ static __forceinline__ void run(double (&v)[16], int tid, const double *__restrict__ itw,
                                                const FpMod &m, int hh) {
         constexpr int R = B::R, C = B::C, TF = B::TF, T = B::T, N = B::N;
+        constexpr bool UNI = TF % 64 == 0;
+        // per-lane twiddles gathered up front (one memory wait per pass); slot of
+        // (stage u, group grp): R - (R >> u) + grp
+        double twl[UNI ? 1 : C * (R - 1)];
+        if constexpr (!UNI) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int b = (tid + c * T) / TF;
+#pragma unroll
+                for (int u = 0; u < NS; ++u) {
+                    const int ngrp = R >> (u + 1);
+#pragma unroll
+                    for (int grp = 0; grp < ngrp; ++grp)
+                        twl[c * (R - 1) + R - (R >> u) + grp] =
+                            twiddle_fp<false>(itw, (N >> (U0 + u + 1)) * hh + b * ngrp + grp);
+                }
+            }
+        }
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const int col = tid + c * T;
-            const int b = (TF % 64 == 0) ? __builtin_amdgcn_readfirstlane(col / TF) : col / TF;
+            const int b = UNI ? __builtin_amdgcn_readfirstlane(col / TF) : col / TF;
 #pragma unroll
             for (int u = 0; u < NS; ++u) {
                 __builtin_amdgcn_sched_barrier(0);
@@ -179,7 +217,8 @@ struct InvPassFp : InvPass<LOGN, U0, NS> {
                 const int ngrp = R / (2 * half);
 #pragma unroll
                 for (int grp = 0; grp < ngrp; ++grp) {
-                    const double w = twiddle_fp<(TF % 64 == 0)>(itw, mm * hh + b * ngrp + grp);
+                    const double w = UNI ? twiddle_fp<true>(itw, mm * hh + b * ngrp + grp)
+                                         : twl[c * (R - 1) + R - (R >> u) + grp];
 #pragma unroll
                     for (int kk = 0; kk < half; ++kk) {
                         const int k0 = grp * 2 * half + kk;
