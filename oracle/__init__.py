@@ -116,6 +116,13 @@ def _declare(L):
     L.mkref_kklss_rotate.argtypes = [V, I, I, u64p, ctypes.POINTER(u64p), u64p, U, u64p]
     L.mkref_kklss_conjugate.argtypes = [V, I, I, u64p, ctypes.POINTER(u64p), u64p, u64p]
     L.mkref_mkf_ckks_rotate.argtypes = [V, I, I, u64p, ctypes.POINTER(u64p), U, u64p, u64p]
+    L.mkref_gen_secret.argtypes = [V, i64p, u64p]
+    L.mkref_gen_public_key.argtypes = [V, u64p, i64p, u64p, u64p, u64p]
+    L.mkref_gen_switching_key.argtypes = [V, u64p, i64p, u64p]
+    L.mkref_gen_relin_key.argtypes = [V, u64p, u64p, i64p, u64p, u64p, u64p, u64p, u64p]
+    L.mkref_gen_rotation_key.argtypes = [V, I, u64p, i64p, u64p, u64p]
+    L.mkref_gen_conjugation_key.argtypes = [V, u64p, i64p, u64p, u64p]
+    L.mkref_encrypt.argtypes = [V, I, u64p, u64p, i64p, i64p, i64p, u64p, I, I, u64p, u64p]
     L.mkref_prng_seed.argtypes = [ctypes.c_void_p, U]
     L.mkref_prng_next.restype = U
     L.mkref_prng_next.argtypes = [ctypes.c_void_p]
@@ -132,6 +139,11 @@ def _p(a):
 
 def _u64(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.uint64))
+
+
+def _ip(a):
+    assert a.dtype == np.int64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(i64p)
 
 
 # ---- scalar helpers -------------------------------------------------------
@@ -443,6 +455,66 @@ class Ctx:
         o1 = np.zeros_like(ct[0])
         lib().mkref_mkf_ckks_rotate(self._h, ct.shape[1] - 1, ct.shape[0] - 1, _p(ct), arr, gal, _p(o0), _p(o1))
         return o0, o1
+
+
+class KeyGen:
+    """Key generation / encryption restated from mkrlwe (keygen.go, encryptor.go) over an
+    oracle Ctx; every random choice is an input (int64 small polynomials)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def _i64(self, x):
+        return np.ascontiguousarray(np.asarray(x, dtype=np.int64))
+
+    def secret(self, s_small):
+        c = self.ctx
+        out = np.zeros((c.L + c.K, c.N), np.uint64)
+        lib().mkref_gen_secret(c._h, _ip(self._i64(s_small)), _p(out))
+        return out
+
+    def public_key(self, sk, e_small, a0):
+        c = self.ctx
+        pk0 = np.zeros((c.L + c.K, c.N), np.uint64)
+        pk1 = np.zeros_like(pk0)
+        lib().mkref_gen_public_key(c._h, _p(_u64(sk)), _ip(self._i64(e_small)), _p(_u64(a0)), _p(pk0), _p(pk1))
+        return pk0, pk1
+
+    def switching_key(self, skin, errs):
+        c = self.ctx
+        out = np.zeros((c.beta(), c.L + c.K, c.N), np.uint64)
+        lib().mkref_gen_switching_key(c._h, _p(_u64(skin)), _ip(self._i64(errs)), _p(out))
+        return out
+
+    def relin_key(self, sk, r, errs, a, u):
+        c = self.ctx
+        shp = (c.beta(), c.L + c.K, c.N)
+        b, d, v = np.zeros(shp, np.uint64), np.zeros(shp, np.uint64), np.zeros(shp, np.uint64)
+        lib().mkref_gen_relin_key(c._h, _p(_u64(sk)), _p(_u64(r)), _ip(self._i64(errs)), _p(_u64(a)), _p(_u64(u)),
+                                  _p(b), _p(d), _p(v))
+        return b, d, v
+
+    def rotation_key(self, rotidx, sk, errs, a):
+        c = self.ctx
+        out = np.zeros((c.beta(), c.L + c.K, c.N), np.uint64)
+        lib().mkref_gen_rotation_key(c._h, rotidx, _p(_u64(sk)), _ip(self._i64(errs)), _p(_u64(a)), _p(out))
+        return out
+
+    def conjugation_key(self, sk, errs, a):
+        c = self.ctx
+        out = np.zeros((c.beta(), c.L + c.K, c.N), np.uint64)
+        lib().mkref_gen_conjugation_key(c._h, _p(_u64(sk)), _ip(self._i64(errs)), _p(_u64(a)), _p(out))
+        return out
+
+    def encrypt(self, pk0, pk1, v, e0, e1, pt, level=None, pt_ntt=False, ntt_out=False):
+        c = self.ctx
+        level = c.L - 1 if level is None else level
+        c0 = np.zeros((level + 1, c.N), np.uint64)
+        c1 = np.zeros_like(c0)
+        pt = _u64(pt)[:level + 1]
+        lib().mkref_encrypt(c._h, level, _p(_u64(pk0)), _p(_u64(pk1)), _ip(self._i64(v)), _ip(self._i64(e0)),
+                            _ip(self._i64(e1)), _p(np.ascontiguousarray(pt)), int(pt_ntt), int(ntt_out), _p(c0), _p(c1))
+        return c0, c1
 
 
 def galois_element(logN, k):

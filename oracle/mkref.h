@@ -172,6 +172,20 @@ void mkref_kklss_conjugate(const mkref_ctx *c, int level, int k, const u64 *in, 
 void mkref_mkf_ckks_rotate(const mkref_ctx *c, int level, int k, const u64 *in, const u64 *const *keys, u64 gal,
                            u64 *out0, u64 *out1);
 
+/* ---- key generation and encryption (mkrlwe keygen.go / encryptor.go), in mkref_keygen.c ----
+ * Small polynomials (secrets, errors, the encryption's v) are int64 [N] inputs; keys
+ * and secrets are full-level QP polys [L+K][N] (switching keys [beta][L+K][N]). */
+void mkref_gen_secret(const mkref_ctx *c, const int64_t *s, u64 *sk);
+void mkref_gen_public_key(const mkref_ctx *c, const u64 *sk, const int64_t *e, const u64 *a, u64 *pk0, u64 *pk1);
+void mkref_gen_switching_key(const mkref_ctx *c, const u64 *skin, const int64_t *errs, u64 *swk);
+void mkref_gen_relin_key(const mkref_ctx *c, const u64 *sk, const u64 *r, const int64_t *errs, const u64 *a,
+                         const u64 *u, u64 *b, u64 *d, u64 *v);
+void mkref_gen_rotation_key(const mkref_ctx *c, int rotidx, const u64 *sk, const int64_t *errs, const u64 *a,
+                            u64 *rk);
+void mkref_gen_conjugation_key(const mkref_ctx *c, const u64 *sk, const int64_t *errs, const u64 *a, u64 *ck);
+void mkref_encrypt(const mkref_ctx *c, int level, const u64 *pk0, const u64 *pk1, const int64_t *v,
+                   const int64_t *e0, const int64_t *e1, const u64 *pt, int pt_ntt, int ntt_out, u64 *c0, u64 *c1);
+
 /* ---- MK-BFV (software/kklss/mkbfv, software/mkf/mkf_cpu/bfv.go), in mkref_bfv.c ---- */
 /* R = Q u QMul (mkbfv/params.go:36-81): the QP context plus the QMul chain and
  * the plaintext modulus t.  gamma 2 for KKLSS (the decomposition of R polys is
