@@ -76,7 +76,8 @@ void mkhip_ctx_destroy(mkhip_ctx *ctx);
 int mkhip_ctx_info(const mkhip_ctx *ctx, int *alpha, int *beta, size_t *swk_words);
 
 /*
- * Resident evaluation keys (HBM).  CRS index -1 is the KKLSS `u` (params.go:38);
+ * Resident evaluation keys (HBM).  CRS index -1 is the KKLSS `u` (params.go:38),
+ * index 0 its `a` (needed only by the key generator below);
  * the rotation / conjugation CRS indices are described with the Galois key switches.
  * Plans refer to the resident keys: destroy the plans that use a key before
  * removing or replacing it (the host-pointer entry points do this for their own
@@ -281,6 +282,40 @@ typedef struct {
 } mkhip_mkf_rotate_op;
 int mkhip_plan_mkf_rotate(mkhip_ctx *ctx, int nops, const mkhip_mkf_rotate_op *ops, int use_graph,
                           mkhip_plan **out);
+
+/*
+ * Key generation and public-key encryption on the device (mkrlwe/keygen.go,
+ * encryptor.go; the step before the mul+relin path, feeding party registration).
+ * The reference samples secrets, errors and the encryption's v inside these calls
+ * from its crypto PRNG; here they are inputs: small signed polynomials, int64 [N]
+ * (ternary or Gaussian coefficients; the caller samples them).  Secrets are
+ * MForm(NTT(s)) over Q u P, [L+K][N]; keys [beta][L+K][N] as elsewhere.
+ *   mkhip_gen_secret           genSecretKeyFromSampler (keygen.go:44-56)
+ *   mkhip_gen_public_key       GenPublicKey (:88-110): pk0 = NTT(e) - MRed(sk, a0),
+ *                              pk1 = a0 (digit 0 of CRS[0]), both [L+K][N]
+ *   mkhip_gen_relin_key        GenRelinearizationKey(sk, r) (:137-187): errs [3][beta][N]
+ *                              are the errors of b, d, v; uses CRS[0] and CRS[-1]
+ *   mkhip_register_party_keygen the same, registered for `party` without leaving HBM
+ *   mkhip_gen_rotation_key     GenRotationKey (:190-230), errs [beta][N], CRS[rotidx]
+ *   mkhip_gen_conjugation_key  GenConjugationKey (:240-268), errs [beta][N], CRS[-2]
+ *   mkhip_encrypt              Encryptor.Encrypt (encryptor.go:55-119) at `level`:
+ *                              v ternary, e0 / e1 errors, pt [level+1][N] (NTT domain
+ *                              when pt_ntt); ciphertext (c0, c1) [level+1][N], NTT
+ *                              domain when ntt_out
+ * CRS[0] is uploaded with mkhip_set_crs(ctx, 0, a).
+ */
+int mkhip_gen_secret(mkhip_ctx *ctx, const int64_t *s, uint64_t *sk);
+int mkhip_gen_public_key(mkhip_ctx *ctx, const uint64_t *sk, const int64_t *e, const uint64_t *a0, uint64_t *pk0,
+                         uint64_t *pk1);
+int mkhip_gen_relin_key(mkhip_ctx *ctx, const uint64_t *sk, const uint64_t *r, const int64_t *errs, uint64_t *b,
+                        uint64_t *d, uint64_t *v);
+int mkhip_register_party_keygen(mkhip_ctx *ctx, int party, const uint64_t *sk, const uint64_t *r,
+                                const int64_t *errs);
+int mkhip_gen_rotation_key(mkhip_ctx *ctx, int rotidx, const uint64_t *sk, const int64_t *errs, uint64_t *rk);
+int mkhip_gen_conjugation_key(mkhip_ctx *ctx, const uint64_t *sk, const int64_t *errs, uint64_t *ck);
+int mkhip_encrypt(mkhip_ctx *ctx, const uint64_t *pk0, const uint64_t *pk1, int level, const int64_t *v,
+                  const int64_t *e0, const int64_t *e1, const uint64_t *pt, int pt_ntt, int ntt_out, uint64_t *c0,
+                  uint64_t *c1);
 
 /*
  * Tuning knobs (no counterpart in the reference: they choose between equivalent

@@ -44,6 +44,8 @@ EXPORTS = [
     "mkhip_host_free", "mkhip_add_rotation_key", "mkhip_del_rotation_key", "mkhip_add_conjugation_key",
     "mkhip_del_conjugation_key", "mkhip_kklss_rotate", "mkhip_kklss_conjugate", "mkhip_plan_kklss_rotate",
     "mkhip_register_mkf_rotation", "mkhip_mkf_ckks_rotate", "mkhip_plan_mkf_rotate",
+    "mkhip_gen_secret", "mkhip_gen_public_key", "mkhip_gen_relin_key", "mkhip_register_party_keygen",
+    "mkhip_gen_rotation_key", "mkhip_gen_conjugation_key", "mkhip_encrypt",
 ]
 TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64 = 1, 2, 3, 4
 SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3, "rotate": 4, "mkf_rotate": 5}
@@ -192,6 +194,14 @@ def _declare(L):
     L.mkhip_register_mkf_rotation.argtypes = [V, I, I, ctypes.POINTER(u64p)]
     L.mkhip_mkf_ckks_rotate.argtypes = [V, I, u64p, I, I, u64p]
     L.mkhip_plan_mkf_rotate.argtypes = [V, I, ctypes.POINTER(MKFRotateOp), I, ctypes.POINTER(V)]
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    L.mkhip_gen_secret.argtypes = [V, i64p, u64p]
+    L.mkhip_gen_public_key.argtypes = [V, u64p, i64p, u64p, u64p, u64p]
+    L.mkhip_gen_relin_key.argtypes = [V, u64p, u64p, i64p, u64p, u64p, u64p]
+    L.mkhip_register_party_keygen.argtypes = [V, I, u64p, u64p, i64p]
+    L.mkhip_gen_rotation_key.argtypes = [V, I, u64p, i64p, u64p]
+    L.mkhip_gen_conjugation_key.argtypes = [V, u64p, i64p, u64p]
+    L.mkhip_encrypt.argtypes = [V, u64p, u64p, I, i64p, i64p, i64p, u64p, I, I, u64p, u64p]
     L.mkhip_register_bfv_party.argtypes = [V, I, u64p, u64p, u64p, u64p, u64p]
     L.mkhip_kklss_bfv_mulrelin.argtypes = [V, u64p, intp, I, u64p, intp, I, u64p, intp, intp]
     L.mkhip_plan_kklss_bfv.argtypes = [V, I, ctypes.POINTER(KKLSSOp), I, ctypes.POINTER(V)]
@@ -421,6 +431,62 @@ class Context:
         arr = (u64p * len(keys))(*[_p(f) for f in keys])
         check(lib().mkhip_register_mkf_group(self._h, group, len(keys), arr, _p(relin)),
               "mkhip_register_mkf_group")
+
+    # ---- key generation / encryption (small polynomials are int64 [N]) ----
+    def _i64(self, x):
+        a = np.ascontiguousarray(np.asarray(x, dtype=np.int64))
+        return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+    def gen_secret(self, s_small):
+        a, p = self._i64(s_small)
+        out = np.zeros((self.L + self.K, self.N), np.uint64)
+        check(lib().mkhip_gen_secret(self._h, p, _p(out)), "mkhip_gen_secret")
+        return out
+
+    def gen_public_key(self, sk, e_small, a0):
+        a, p = self._i64(e_small)
+        pk0 = np.zeros((self.L + self.K, self.N), np.uint64)
+        pk1 = np.zeros_like(pk0)
+        check(lib().mkhip_gen_public_key(self._h, _p(_u64(sk)), p, _p(_u64(a0)), _p(pk0), _p(pk1)),
+              "mkhip_gen_public_key")
+        return pk0, pk1
+
+    def gen_relin_key(self, sk, r, errs):
+        a, p = self._i64(errs)
+        shp = (self.beta, self.L + self.K, self.N)
+        b, d, v = (np.zeros(shp, np.uint64) for _ in range(3))
+        check(lib().mkhip_gen_relin_key(self._h, _p(_u64(sk)), _p(_u64(r)), p, _p(b), _p(d), _p(v)),
+              "mkhip_gen_relin_key")
+        return b, d, v
+
+    def register_party_keygen(self, pid, sk, r, errs):
+        a, p = self._i64(errs)
+        check(lib().mkhip_register_party_keygen(self._h, pid, _p(_u64(sk)), _p(_u64(r)), p),
+              "mkhip_register_party_keygen")
+
+    def gen_rotation_key(self, rotidx, sk, errs):
+        a, p = self._i64(errs)
+        out = np.zeros((self.beta, self.L + self.K, self.N), np.uint64)
+        check(lib().mkhip_gen_rotation_key(self._h, rotidx, _p(_u64(sk)), p, _p(out)), "mkhip_gen_rotation_key")
+        return out
+
+    def gen_conjugation_key(self, sk, errs):
+        a, p = self._i64(errs)
+        out = np.zeros((self.beta, self.L + self.K, self.N), np.uint64)
+        check(lib().mkhip_gen_conjugation_key(self._h, _p(_u64(sk)), p, _p(out)), "mkhip_gen_conjugation_key")
+        return out
+
+    def encrypt(self, pk0, pk1, v, e0, e1, pt, level=None, pt_ntt=False, ntt_out=False):
+        level = self.L - 1 if level is None else level
+        va, vp = self._i64(v)
+        ea, ep = self._i64(e0)
+        fa, fp = self._i64(e1)
+        pt = np.ascontiguousarray(_u64(pt)[:level + 1])
+        c0 = np.zeros((level + 1, self.N), np.uint64)
+        c1 = np.zeros_like(c0)
+        check(lib().mkhip_encrypt(self._h, _p(_u64(pk0)), _p(_u64(pk1)), level, vp, ep, fp, _p(pt), int(pt_ntt),
+                                  int(ntt_out), _p(c0), _p(c1)), "mkhip_encrypt")
+        return c0, c1
 
     def register_mkf_rotation(self, group, rotation, keys):
         keys = [_u64(x) for x in keys]

@@ -407,8 +407,9 @@ extern "C" int mkhip_set_crs(mkhip_ctx *c, int index, const uint64_t *swk) {
         }
         return upload_key(c, swk, &c->d_crs_u);
     }
-    // rotation CRS (index = rotidx in [1, N/2)) or conjugation CRS (-2)
-    if (index != -2 && (index < 1 || index >= c->N / 2)) return MKHIP_ERR_ARG;
+    // CRS[0] (the a of relinearization and public keys), rotation CRS (index =
+    // rotidx in [1, N/2)) or conjugation CRS (-2)
+    if (index != -2 && (index < 0 || index >= c->N / 2)) return MKHIP_ERR_ARG;
     u64 *p = nullptr;
     int s = upload_key(c, swk, &p);
     if (s != MKHIP_OK) return s;

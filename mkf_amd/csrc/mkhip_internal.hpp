@@ -24,6 +24,8 @@ hipError_t launch_bfv(int what, const DevTables &T, const void *descs, int n, in
 hipError_t launch_fused(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st);
 // kernels_galois.hip
 hipError_t launch_galois(int what, const DevTables &T, const void *descs, int n, hipStream_t st);
+// kernels_keygen.hip
+hipError_t launch_keygen(int what, const DevTables &T, const void *descs, int n, hipStream_t st);
 
 inline int hip_status(hipError_t e) {
     return e == hipSuccess ? MKHIP_OK : (e == hipErrorOutOfMemory ? MKHIP_ERR_OOM : MKHIP_ERR_DEVICE);

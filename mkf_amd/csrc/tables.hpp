@@ -77,6 +77,29 @@ enum KernelId {
     KBFV_QUANTIZE = 16,  // k_quantize             ModDown R -> Q (+ accumulate)
     KINTT_FINISH = 17,   // k_intt_finish          last inverse stage of split (2^15) limbs
     KAUTO = 18,          // k_automorph            Galois automorphism (coefficient or NTT domain)
+    KSMALL_NTT = 19,     // k_small_ntt<LOGN>      small signed polynomial -> residues -> NTT (-> MForm)
+    KKEYOP = 20,         // k_keyop                elementwise key / encryption combination
+};
+
+// A small signed polynomial (secret, error, the encryption's ternary v) reduced
+// modulo ring modulus `mod`, transformed, optionally put in Montgomery form.
+struct SmallNttDesc {
+    const int64_t *src;  // [N]
+    u64 *dst;            // [N]
+    int mod, mform;
+};
+
+// One row of a key or ciphertext, elementwise mod q (ring modulus `mod`):
+//   op 0: out = x + g pmod + y + e - MRed(ka, kb)
+//   op 1: out = q - (x + g pmod + y + e + MRed(ka, kb))   (Lattigo NegLvl: a zero becomes q)
+//   op 2: out = x + g pmod + y + e + MRed(ka, kb)
+// Null operands are absent terms; e is a small signed polynomial.
+struct KeyOpDesc {
+    const u64 *x, *g, *ka, *kb, *y;
+    const int64_t *e;
+    u64 *out;
+    u64 pmod;
+    int mod, op;
 };
 
 // Galois automorphism X -> X^gal of nl limbs, src -> dst (no aliasing).
