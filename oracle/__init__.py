@@ -89,6 +89,8 @@ def _declare(L):
     L.mkref_mkf_switch_one.argtypes = [ctypes.c_void_p, I, u64p, I, u64p, I, u64p, u64p]
     L.mkref_mkf_ckks_mulrelin.argtypes = [ctypes.c_void_p, I, I, u64p, u64p,
                                           ctypes.POINTER(u64p), u64p, u64p, u64p]
+    from . import poly as _poly
+    _poly.declare(L)
     V = ctypes.c_void_p
     L.mkref_bfv_new.restype = V
     L.mkref_bfv_new.argtypes = [I, u64p, I, u64p, u64p, I, U, I, I]
@@ -369,6 +371,21 @@ class Ctx:
                                       _p(_u64(relin_key)), _p(o0), _p(o1))
         return o0, o1
 
+    def mkf_ckks_mulrelin2(self, left, right, fold_keys, relin_key):
+        """MulRelinNew of the MKF polynomial evaluator: operands at their own levels
+        (shapes [1+k][level+1][N]); outputs at the smaller level."""
+        left, right = _u64(left), _u64(right)
+        k = left.shape[0] - 1
+        ll, lr = left.shape[1] - 1, right.shape[1] - 1
+        level = min(ll, lr)
+        keys = [_u64(f) for f in fold_keys]
+        arr = (u64p * k)(*[_p(f) for f in keys])
+        o0 = np.zeros((level + 1, self.N), np.uint64)
+        o1 = np.zeros((level + 1, self.N), np.uint64)
+        lib().mkref_mkf_ckks_mulrelin2(self._h, ll, lr, k, _p(left), _p(right), arr, _p(_u64(relin_key)),
+                                       _p(o0), _p(o1))
+        return o0, o1
+
 
     # ---- Galois key switches (mkref_rot.c) ----
 
@@ -628,3 +645,6 @@ class BfvCtx:
         lib().mkref_mkf_bfv_mulrelin(self._h, k, _p(_u64(left)), _p(_u64(right)), arr, _p(_u64(relin_key)),
                                      _p(o0), _p(o1))
         return o0, o1
+
+
+from .poly import CHEBYSHEV, STANDARD, Ct, MKFError, MKFEvaluator, Polynomial  # noqa: E402,F401

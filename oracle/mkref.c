@@ -772,32 +772,39 @@ void mkref_mkf_switch_one(const mkref_ctx *c, int levelQ, const u64 *in, int in_
     free(D);
 }
 
-/* CKKSCase.Run (mkf_cpu/ckks.go:102-139) up to, not including, the rescale */
-void mkref_mkf_ckks_mulrelin(const mkref_ctx *c, int level, int k,
-                             const u64 *left, const u64 *right,
-                             const u64 *const *fold_keys, const u64 *relin_key,
-                             u64 *out0, u64 *out1) {
+/* CKKSCase.Run (mkf_cpu/ckks.go:102-139) up to, not including, the rescale;
+ * also paperMKFCKKSEvaluator.MulRelinNew (true_mkf_ckks.go:253-303), which folds
+ * each operand at its own level (common.go:109-126: switchOne at in.Level) and
+ * forms the tensor at the smaller level.  out0/out1 have min(level_l, level_r)+1
+ * limbs. */
+void mkref_mkf_ckks_mulrelin2(const mkref_ctx *c, int level_l, int level_r, int k,
+                              const u64 *left, const u64 *right,
+                              const u64 *const *fold_keys, const u64 *relin_key,
+                              u64 *out0, u64 *out1) {
     const mkref_ring *R = &c->ringQP;
-    int N = c->N, nl = level + 1;
-    size_t psz = (size_t)nl * N;
-    u64 *lf[2], *rf[2], *s0 = malloc(sizeof(u64) * psz), *s1 = malloc(sizeof(u64) * psz);
+    int N = c->N, lv[2] = {level_l, level_r};
+    int level = level_l < level_r ? level_l : level_r, nl = level + 1;
+    int nlmax = (level_l > level_r ? level_l : level_r) + 1;
+    size_t psz = (size_t)nl * N, pmax = (size_t)nlmax * N;
+    u64 *lf[2], *rf[2], *s0 = malloc(sizeof(u64) * pmax), *s1 = malloc(sizeof(u64) * pmax);
     const u64 *side[2] = {left, right};
     u64 **dst[2] = {lf, rf};
     for (int s = 0; s < 2; s++) {
-        /* fold (common.go:109-126) */
-        dst[s][0] = malloc(sizeof(u64) * psz);
-        dst[s][1] = calloc(psz, sizeof(u64));
-        memcpy(dst[s][0], side[s], sizeof(u64) * psz);
+        /* fold (common.go:109-126) at the operand's level */
+        size_t ps = (size_t)(lv[s] + 1) * N;
+        dst[s][0] = malloc(sizeof(u64) * ps);
+        dst[s][1] = calloc(ps, sizeof(u64));
+        memcpy(dst[s][0], side[s], sizeof(u64) * ps);
         for (int i = 0; i < k; i++) {
-            mkref_mkf_switch_one(c, level, side[s] + (size_t)(1 + i) * psz, 1, fold_keys[i], 1, s0, s1);
-            for (size_t e = 0; e < psz; e++) {
+            mkref_mkf_switch_one(c, lv[s], side[s] + (size_t)(1 + i) * ps, 1, fold_keys[i], 1, s0, s1);
+            for (size_t e = 0; e < ps; e++) {
                 u64 q = R->q[e / N];
                 dst[s][0][e] = addmod(dst[s][0][e], s0[e], q);
                 dst[s][1][e] = addmod(dst[s][1][e], s1[e], q);
             }
         }
     }
-    /* Karatsuba tensor (ckks.go:120-131) */
+    /* Karatsuba tensor (ckks.go:120-131) on limbs 0..level */
     u64 *mu2 = malloc(sizeof(u64) * psz);
     for (size_t e = 0; e < psz; e++) {
         int l = (int)(e / N);
@@ -821,6 +828,13 @@ void mkref_mkf_ckks_mulrelin(const mkref_ctx *c, int level, int k,
     }
     free(mu2); free(s0); free(s1);
     for (int s = 0; s < 2; s++) { free(dst[s][0]); free(dst[s][1]); }
+}
+
+void mkref_mkf_ckks_mulrelin(const mkref_ctx *c, int level, int k,
+                             const u64 *left, const u64 *right,
+                             const u64 *const *fold_keys, const u64 *relin_key,
+                             u64 *out0, u64 *out1) {
+    mkref_mkf_ckks_mulrelin2(c, level, level, k, left, right, fold_keys, relin_key, out0, out1);
 }
 
 /* ------------------------------------------------------------------------ */

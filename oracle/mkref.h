@@ -148,6 +148,23 @@ void mkref_mkf_ckks_mulrelin(const mkref_ctx *c, int level, int k,
                              const u64 *const *fold_keys, const u64 *relin_key,
                              u64 *out0, u64 *out1);
 void mkref_div_round_last_ntt(const mkref_ctx *c, int level, int n, u64 *poly);
+/* the same with each operand folded at its own level, tensor at the smaller one */
+void mkref_mkf_ckks_mulrelin2(const mkref_ctx *c, int level_l, int level_r, int k,
+                              const u64 *left, const u64 *right,
+                              const u64 *const *fold_keys, const u64 *relin_key,
+                              u64 *out0, u64 *out1);
+
+/* ---- CKKS linear operations of the MKF polynomial evaluator, in mkref_poly.c ---- */
+u64  mkref_scale_up_exact(double value, double n, u64 q);  /* mkckks/utils.go:59-84          */
+u64  mkref_trunc_product_mod(double a, double b, u64 q);   /* trunc(a*b) exact, mod q        */
+u64  mkref_i64_mod(int64_t x, u64 q);                       /* Euclidean residue              */
+/* AddConst / MultByConst constants of the two NTT halves, limbs 0..level */
+void mkref_const_halves(const mkref_ctx *c, int level, double re, double im, double scale, u64 *pos, u64 *neg);
+/* MultByGaussianIntegerAndAdd constants from the residues of the real / imaginary integers */
+void mkref_gauss_halves(const mkref_ctx *c, int level, const u64 *re_res, const u64 *im_res, u64 *pos, u64 *neg);
+void mkref_mul_const_ntt(const mkref_ctx *c, int level, int ncomp, const u64 *in, int in_limbs, const u64 *pos,
+                         const u64 *neg, u64 *out, int out_limbs, int accumulate);
+void mkref_add_const_ntt(const mkref_ctx *c, int level, u64 *c0, const u64 *pos, const u64 *neg);
 
 /* ---- Galois key switches (mkrlwe Rotate/Conjugate, MKF RotateNew), in mkref_rot.c ---- */
 u64  mkref_galois_element(int logN, long long k);   /* 5^(k mod 2N) mod 2N          */
