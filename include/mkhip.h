@@ -284,6 +284,70 @@ int mkhip_plan_mkf_rotate(mkhip_ctx *ctx, int nops, const mkhip_mkf_rotate_op *o
                           mkhip_plan **out);
 
 /*
+ * MKF CKKS polynomial evaluation (paperMKFCKKSEvaluator.EvaluatePoly,
+ * software/mkf/mkf_cpu/true_mkf_ckks_polynomial.go:79-273) and the evaluator's
+ * public linear operations (true_mkf_ckks.go:187-303), on a gamma = 1 CKKS
+ * context.  Ciphertexts are NTT-domain [1+k][level+1][N] (c0, then the k party
+ * components), as paperMKFCKKSCiphertext holds them; scales are float64.
+ *
+ * The polynomial mirrors ckks.Polynomial: coefficients (imaginary parts may be
+ * NULL), MaxDeg, Lead and the basis; A and B (Chebyshev interval) are the
+ * caller's change of variable, applied before the call as in the reference.
+ *   mkhip_plan_mkf_poly            nct ciphertexts (device) through the same
+ *                                  polynomial, as one plan: its schedule of
+ *                                  MulRelin (each operand folded at its own level),
+ *                                  Rescale, Add/Sub, AddConst and
+ *                                  MultByGaussianIntegerAndAdd steps; out[i] receives
+ *                                  [1+k][level_out+1][N].  Errors: empty polynomial
+ *                                  MKHIP_ERR_ARG; depth above the level (the
+ *                                  reference's "levels < polynomial depth")
+ *                                  MKHIP_ERR_LEVEL; rescale errors MKHIP_ERR_SCALE.
+ *   mkhip_mkf_poly_query           output level, scale and MulRelin count only
+ *   mkhip_mkf_ckks_evaluate_poly   one ciphertext, host buffers
+ *   mkhip_mkf_ckks_add             AddNew / SubNew (sub = 1): the operand of smaller
+ *                                  scale first multiplied by floor(ratio) when that is
+ *                                  above 1; output at the smaller level, larger scale
+ *   mkhip_mkf_ckks_add_const       AddConstNew(re + im i): c0 only, scaled by `scale`
+ *   mkhip_mkf_ckks_mult_by_const   MultByConstNew: a constant with a fractional part
+ *                                  is scaled by Q[level] and so is the output scale
+ *   mkhip_mkf_ckks_mult_by_gaussian_integer_and_add
+ *                                  acc (at level_acc, in place) += in (re + im i)
+ *   mkhip_mkf_ckks_rescale         RescaleNew(min_scale): as many primes as the scale
+ *                                  loop divides (mkckks/evaluator.go:360-407)
+ *   mkhip_mkf_ckks_mulrelin_levels MulRelinNew: operands at their own levels, output at
+ *                                  the smaller one, c0 then k copies of Gamma, no
+ *                                  rescale (output scale = product of the scales)
+ * DropLevelNew is a prefix of the limbs of each component: no call.
+ */
+#define MKHIP_BASIS_STANDARD 0
+#define MKHIP_BASIS_CHEBYSHEV 1
+typedef struct {
+    const double *coeffs_re, *coeffs_im; /* ncoeffs each (coeffs_im may be NULL) */
+    int ncoeffs;                         /* degree + 1                           */
+    int max_deg, lead, basis;            /* Polynomial.MaxDeg, .Lead, .Basis      */
+} mkhip_polynomial;
+int mkhip_plan_mkf_poly(mkhip_ctx *ctx, int group, int nct, const uint64_t *const *in, int level, double scale,
+                        const mkhip_polynomial *poly, double target_scale, uint64_t *const *out, int use_graph,
+                        mkhip_plan **plan, int *level_out, double *scale_out);
+int mkhip_mkf_poly_query(mkhip_ctx *ctx, int group, int level, double scale, const mkhip_polynomial *poly,
+                         double target_scale, int *level_out, double *scale_out, int *mulrelins);
+int mkhip_mkf_ckks_evaluate_poly(mkhip_ctx *ctx, int group, const uint64_t *in, int level, double scale,
+                                 const mkhip_polynomial *poly, double target_scale, uint64_t *out, int *level_out,
+                                 double *scale_out);
+int mkhip_mkf_ckks_add(mkhip_ctx *ctx, int k, const uint64_t *a, int level_a, double scale_a, const uint64_t *b,
+                       int level_b, double scale_b, int sub, uint64_t *out, int *level_out, double *scale_out);
+int mkhip_mkf_ckks_add_const(mkhip_ctx *ctx, int k, const uint64_t *in, int level, double scale, double re,
+                             double im, uint64_t *out);
+int mkhip_mkf_ckks_mult_by_const(mkhip_ctx *ctx, int k, const uint64_t *in, int level, double scale, double re,
+                                 double im, uint64_t *out, double *scale_out);
+int mkhip_mkf_ckks_mult_by_gaussian_integer_and_add(mkhip_ctx *ctx, int k, const uint64_t *in, int level_in,
+                                                    int64_t re, int64_t im, uint64_t *acc, int level_acc);
+int mkhip_mkf_ckks_rescale(mkhip_ctx *ctx, int k, const uint64_t *in, int level, double scale, double min_scale,
+                           uint64_t *out, int *level_out, double *scale_out);
+int mkhip_mkf_ckks_mulrelin_levels(mkhip_ctx *ctx, int group, const uint64_t *left, int level_l,
+                                   const uint64_t *right, int level_r, uint64_t *out);
+
+/*
  * Key generation and public-key encryption on the device (mkrlwe/keygen.go,
  * encryptor.go; the step before the mul+relin path, feeding party registration).
  * The reference samples secrets, errors and the encryption's v inside these calls

@@ -46,7 +46,11 @@ EXPORTS = [
     "mkhip_register_mkf_rotation", "mkhip_mkf_ckks_rotate", "mkhip_plan_mkf_rotate",
     "mkhip_gen_secret", "mkhip_gen_public_key", "mkhip_gen_relin_key", "mkhip_register_party_keygen",
     "mkhip_gen_rotation_key", "mkhip_gen_conjugation_key", "mkhip_encrypt",
+    "mkhip_plan_mkf_poly", "mkhip_mkf_poly_query", "mkhip_mkf_ckks_evaluate_poly", "mkhip_mkf_ckks_add",
+    "mkhip_mkf_ckks_add_const", "mkhip_mkf_ckks_mult_by_const", "mkhip_mkf_ckks_mult_by_gaussian_integer_and_add",
+    "mkhip_mkf_ckks_rescale", "mkhip_mkf_ckks_mulrelin_levels",
 ]
+BASIS_STANDARD, BASIS_CHEBYSHEV = 0, 1
 TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64 = 1, 2, 3, 4
 SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3, "rotate": 4, "mkf_rotate": 5}
 MAX_ROTATIONS = 16
@@ -144,6 +148,32 @@ def lib():
     return _lib
 
 
+class PolyStruct(ctypes.Structure):
+    _fields_ = [("coeffs_re", ctypes.POINTER(ctypes.c_double)), ("coeffs_im", ctypes.POINTER(ctypes.c_double)),
+                ("ncoeffs", ctypes.c_int), ("max_deg", ctypes.c_int), ("lead", ctypes.c_int),
+                ("basis", ctypes.c_int)]
+
+
+class Polynomial:
+    """ckks.Polynomial as the evaluator reads it: coefficients (complex), MaxDeg,
+    Lead, Basis (BASIS_STANDARD or BASIS_CHEBYSHEV); a, b the Chebyshev interval the
+    caller maps its input from."""
+
+    def __init__(self, coeffs, max_deg=None, lead=True, basis=BASIS_STANDARD, a=-1.0, b=1.0):
+        self.coeffs = [complex(c) for c in coeffs]
+        self.max_deg = len(self.coeffs) - 1 if max_deg is None else int(max_deg)
+        self.lead, self.basis, self.a, self.b = bool(lead), basis, a, b
+
+    def struct(self):
+        """(PolyStruct, arrays it points to)."""
+        re = np.ascontiguousarray([c.real for c in self.coeffs], np.float64)
+        im = np.ascontiguousarray([c.imag for c in self.coeffs], np.float64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        st = PolyStruct(re.ctypes.data_as(dp), im.ctypes.data_as(dp), len(self.coeffs), self.max_deg,
+                        int(self.lead), int(self.basis))
+        return st, (re, im)
+
+
 def _declare(L):
     I, V, S = ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t
     U, D = ctypes.c_uint64, ctypes.c_double
@@ -202,6 +232,18 @@ def _declare(L):
     L.mkhip_gen_rotation_key.argtypes = [V, I, u64p, i64p, u64p]
     L.mkhip_gen_conjugation_key.argtypes = [V, u64p, i64p, u64p]
     L.mkhip_encrypt.argtypes = [V, u64p, u64p, I, i64p, i64p, i64p, u64p, I, I, u64p, u64p]
+    PP = ctypes.POINTER(PolyStruct)
+    L.mkhip_plan_mkf_poly.argtypes = [V, I, I, ctypes.POINTER(u64p), I, D, PP, D, ctypes.POINTER(u64p), I,
+                                      ctypes.POINTER(V), intp, ctypes.POINTER(D)]
+    L.mkhip_mkf_poly_query.argtypes = [V, I, I, D, PP, D, intp, ctypes.POINTER(D), intp]
+    L.mkhip_mkf_ckks_evaluate_poly.argtypes = [V, I, u64p, I, D, PP, D, u64p, intp, ctypes.POINTER(D)]
+    L.mkhip_mkf_ckks_add.argtypes = [V, I, u64p, I, D, u64p, I, D, I, u64p, intp, ctypes.POINTER(D)]
+    L.mkhip_mkf_ckks_add_const.argtypes = [V, I, u64p, I, D, D, D, u64p]
+    L.mkhip_mkf_ckks_mult_by_const.argtypes = [V, I, u64p, I, D, D, D, u64p, ctypes.POINTER(D)]
+    L.mkhip_mkf_ckks_mult_by_gaussian_integer_and_add.argtypes = [V, I, u64p, I, ctypes.c_int64, ctypes.c_int64,
+                                                                  u64p, I]
+    L.mkhip_mkf_ckks_rescale.argtypes = [V, I, u64p, I, D, D, u64p, intp, ctypes.POINTER(D)]
+    L.mkhip_mkf_ckks_mulrelin_levels.argtypes = [V, I, u64p, I, u64p, I, u64p]
     L.mkhip_register_bfv_party.argtypes = [V, I, u64p, u64p, u64p, u64p, u64p]
     L.mkhip_kklss_bfv_mulrelin.argtypes = [V, u64p, intp, I, u64p, intp, I, u64p, intp, intp]
     L.mkhip_plan_kklss_bfv.argtypes = [V, I, ctypes.POINTER(KKLSSOp), I, ctypes.POINTER(V)]
@@ -525,6 +567,80 @@ class Context:
               "mkhip_mkf_bfv_mulrelin")
         return out
 
+    # ---- MKF polynomial evaluator and linear operations (NTT domain, (1+k, level+1, N)) ----
+    def mkf_poly_query(self, group, level, scale, poly, target_scale):
+        st, keep = poly.struct()
+        lo, so, nm = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        check(lib().mkhip_mkf_poly_query(self._h, group, level, scale, ctypes.byref(st), target_scale,
+                                         ctypes.byref(lo), ctypes.byref(so), ctypes.byref(nm)),
+              "mkhip_mkf_poly_query")
+        return lo.value, so.value, nm.value
+
+    def mkf_evaluate_poly(self, group, ct, scale, poly, target_scale):
+        """EvaluatePoly on one ciphertext -> (out, level_out, scale_out)."""
+        ct = _u64(ct)
+        lo, so, _ = self.mkf_poly_query(group, ct.shape[1] - 1, scale, poly, target_scale)
+        out = np.zeros((ct.shape[0], lo + 1, self.N), np.uint64)
+        st, keep = poly.struct()
+        l2, s2 = ctypes.c_int(), ctypes.c_double()
+        check(lib().mkhip_mkf_ckks_evaluate_poly(self._h, group, _p(ct), ct.shape[1] - 1, scale, ctypes.byref(st),
+                                                 target_scale, _p(out), ctypes.byref(l2), ctypes.byref(s2)),
+              "mkhip_mkf_ckks_evaluate_poly")
+        return out, l2.value, s2.value
+
+    def mkf_add(self, a, scale_a, b, scale_b, sub=False):
+        a, b = _u64(a), _u64(b)
+        k, lv = a.shape[0] - 1, min(a.shape[1], b.shape[1]) - 1
+        out = np.zeros((1 + k, lv + 1, self.N), np.uint64)
+        lo, so = ctypes.c_int(), ctypes.c_double()
+        check(lib().mkhip_mkf_ckks_add(self._h, k, _p(a), a.shape[1] - 1, scale_a, _p(b), b.shape[1] - 1, scale_b,
+                                       int(sub), _p(out), ctypes.byref(lo), ctypes.byref(so)), "mkhip_mkf_ckks_add")
+        return out, lo.value, so.value
+
+    def mkf_add_const(self, a, scale, c):
+        a = _u64(a)
+        c = complex(c)
+        out = np.zeros_like(a)
+        check(lib().mkhip_mkf_ckks_add_const(self._h, a.shape[0] - 1, _p(a), a.shape[1] - 1, scale, c.real, c.imag,
+                                             _p(out)), "mkhip_mkf_ckks_add_const")
+        return out
+
+    def mkf_mult_by_const(self, a, scale, c):
+        a = _u64(a)
+        c = complex(c)
+        out = np.zeros_like(a)
+        so = ctypes.c_double()
+        check(lib().mkhip_mkf_ckks_mult_by_const(self._h, a.shape[0] - 1, _p(a), a.shape[1] - 1, scale, c.real,
+                                                 c.imag, _p(out), ctypes.byref(so)), "mkhip_mkf_ckks_mult_by_const")
+        return out, so.value
+
+    def mkf_gauss_add(self, a, re, im, acc):
+        """acc + a (re + im i) for int64 re, im (MultByGaussianIntegerAndAdd)."""
+        a = _u64(a)
+        acc = _u64(acc).copy()
+        check(lib().mkhip_mkf_ckks_mult_by_gaussian_integer_and_add(self._h, a.shape[0] - 1, _p(a), a.shape[1] - 1,
+                                                                    int(re), int(im), _p(acc), acc.shape[1] - 1),
+              "mkhip_mkf_ckks_mult_by_gaussian_integer_and_add")
+        return acc
+
+    def mkf_rescale(self, a, scale, min_scale):
+        a = _u64(a)
+        out = np.zeros(a.size, np.uint64)
+        lo, so = ctypes.c_int(), ctypes.c_double()
+        check(lib().mkhip_mkf_ckks_rescale(self._h, a.shape[0] - 1, _p(a), a.shape[1] - 1, scale, min_scale,
+                                           _p(out), ctypes.byref(lo), ctypes.byref(so)), "mkhip_mkf_ckks_rescale")
+        n = a.shape[0] * (lo.value + 1) * self.N  # [1+k][level_out+1][N], packed
+        return out[:n].reshape(a.shape[0], lo.value + 1, self.N), lo.value, so.value
+
+    def mkf_mulrelin_levels(self, group, left, right):
+        """MulRelinNew of the paper evaluator: operands at their own levels, no rescale."""
+        left, right = _u64(left), _u64(right)
+        lv = min(left.shape[1], right.shape[1]) - 1
+        out = np.zeros((left.shape[0], lv + 1, self.N), np.uint64)
+        check(lib().mkhip_mkf_ckks_mulrelin_levels(self._h, group, _p(left), left.shape[1] - 1, _p(right),
+                                                   right.shape[1] - 1, _p(out)), "mkhip_mkf_ckks_mulrelin_levels")
+        return out
+
     # ---- tuning ----
     def set_tuning(self, knob, value):
         check(lib().mkhip_set_tuning(self._h, knob, int(value)), "mkhip_set_tuning")
@@ -693,6 +809,26 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class PolyPlan(Plan):
+    """EvaluatePoly over nct device-resident ciphertexts as one plan: ins / outs are
+    device addresses of [1+k][level+1][N] / [1+k][level_out+1][N] words."""
+
+    def __init__(self, ctx, group, ins, level, scale, poly, target_scale, outs, use_graph=True):
+        h = ctypes.c_void_p()
+        st, keep = poly.struct()
+        n = len(ins)
+        ia = (u64p * n)(*[ctypes.cast(ctypes.c_void_p(int(x)), u64p) for x in ins])
+        oa = (u64p * n)(*[ctypes.cast(ctypes.c_void_p(int(x)), u64p) for x in outs])
+        lo, so = ctypes.c_int(), ctypes.c_double()
+        check(lib().mkhip_plan_mkf_poly(ctx.handle, group, n, ia, level, scale, ctypes.byref(st), target_scale, oa,
+                                        int(use_graph), ctypes.byref(h), ctypes.byref(lo), ctypes.byref(so)),
+              "mkhip_plan_mkf_poly")
+        self._h = h
+        self.ctx = ctx
+        self.scheme = "mkf_poly"
+        self.level_out, self.scale_out = lo.value, so.value
 
 
 class Batch:

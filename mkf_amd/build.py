@@ -11,8 +11,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmkhip.so")
 SOURCES = ["kernels.hip", "kernels_mkf.hip", "kernels_bfv.hip", "kernels_fused.hip", "kernels_galois.hip",
-           "kernels_keygen.hip", "mkhip.hip", "mkhip_plan.hip", "mkhip_kklss.hip", "mkhip_mkf.hip", "mkhip_bfv.hip",
-           "mkhip_rotate.hip", "mkhip_keygen.hip", "mkhip_batch.hip", "mkhip_server.hip"]
+           "kernels_keygen.hip", "kernels_poly.hip", "mkhip.hip", "mkhip_plan.hip", "mkhip_kklss.hip", "mkhip_mkf.hip", "mkhip_bfv.hip",
+           "mkhip_rotate.hip", "mkhip_keygen.hip", "mkhip_poly.hip", "mkhip_batch.hip",
+           "mkhip_server.hip"]
 OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -196,6 +196,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
         const u64 g = primitive_root(qi, (flags & MKHIP_ROOT_RULE_SMALLEST2) != 0);
         const u64 psi = h_powmod(g, (qi - 1) / (2 * (u64)N), qi);
         const u64 psiinv = h_inv(psi, qi);
+        c->psi_half.push_back(h_powmod(psi, (u64)N / 2, qi));
         u64 a = 1, b = 1;
         for (int j = 0; j < N; ++j) {
             const u64 idx = bitrev((u64)j, logN);

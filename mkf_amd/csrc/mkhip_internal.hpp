@@ -26,6 +26,7 @@ hipError_t launch_fused(int what, const DevTables &T, const void *descs, int n, 
 hipError_t launch_galois(int what, const DevTables &T, const void *descs, int n, hipStream_t st);
 // kernels_keygen.hip
 hipError_t launch_keygen(int what, const DevTables &T, const void *descs, int n, hipStream_t st);
+hipError_t launch_linear(const DevTables &T, const void *descs, int n, hipStream_t st);
 
 inline int hip_status(hipError_t e) {
     return e == hipSuccess ? MKHIP_OK : (e == hipErrorOutOfMemory ? MKHIP_ERR_OOM : MKHIP_ERR_DEVICE);
@@ -70,6 +71,7 @@ struct mkhip_ctx {
     mkhip::u64 t = 0;
     int flags = 0;
     std::vector<mkhip::u64> Q, P, QMul;  // QMul: BFV auxiliary chain (empty for CKKS)
+    std::vector<mkhip::u64> psi_half;  // psi^(N/2) per ring modulus (a square root of -1; Lattigo NttPsi[1])
     bool bfv = false;
     void *d_tables = nullptr;
     mkhip::DevTables T;
@@ -149,6 +151,8 @@ int finish_rows(mkhip_ctx *c, u64 *base, const int *mods, int nrows, int tscale)
 // Launch stage s (on `st`, default the context stream) / every stage.
 hipError_t plan_stage(mkhip_plan *pl, int s, hipStream_t st = nullptr);
 int plan_launch_raw(mkhip_plan *pl, hipStream_t st = nullptr);
+// Capture every stage of a finished plan into one hipGraph (after one eager launch).
+int plan_capture(mkhip_plan *pl);
 // New plan whose workspace (ws_bytes) is allocated or taken from pm; *out is null
 // on failure.
 int plan_create(mkhip_ctx *c, size_t ws_bytes, mkhip_plan **out, const PlanMem &pm = PlanMem());
@@ -164,6 +168,8 @@ int plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, int use_gr
                    const PlanMem &pm);
 int plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out,
                   const PlanMem &pm);
+int plan_mkf_ckks_lv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int lv_l, int lv_r, int use_graph,
+                     mkhip_plan **out, const PlanMem &pm);
 int plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out,
                  const PlanMem &pm);
 // Upload one switching key (swk_words words) into a new device allocation.

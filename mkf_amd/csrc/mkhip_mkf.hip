@@ -69,11 +69,21 @@ extern "C" int mkhip_unregister_mkf_group(mkhip_ctx *c, int group) {
 int mkhip::plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use_graph, mkhip_plan **out,
                        const PlanMem &pm) {
     if (!c || !ops || nops <= 0 || !out) return MKHIP_ERR_ARG;
+    return plan_mkf_ckks_lv(c, nops, ops, ops[0].level, ops[0].level, use_graph, out, pm);
+}
+
+// As above with the left and right operands at levels lv_l, lv_r >= op.level: each
+// is folded at its own level (common.go:109-126 switches at in.Level), the tensor
+// and relinearization run at op.level (true_mkf_ckks.go:265-290).
+int mkhip::plan_mkf_ckks_lv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int lv_l, int lv_r, int use_graph,
+                            mkhip_plan **out, const PlanMem &pm) {
+    if (!c || !ops || nops <= 0 || !out) return MKHIP_ERR_ARG;
     *out = nullptr;
     if (c->alpha == 0) return MKHIP_ERR_GADGET;
     if (c->gamma != 1) return MKHIP_ERR_SCHEME;  // Lattigo rlwe decomposition: alpha = K
     const int level = ops[0].level, nres = ops[0].nres;
     const int N = c->N, K = c->K;
+    if (lv_l < level || lv_r < level || lv_l >= c->L || lv_r >= c->L) return MKHIP_ERR_LEVEL;
     for (int o = 0; o < nops; ++o) {
         const mkhip_mkf_op &op = ops[o];
         if (!op.left || !op.right || !op.out) return MKHIP_ERR_ARG;
@@ -81,9 +91,14 @@ int mkhip::plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int us
         if (op.nres != nres || nres < 0 || nres > 1 || nres > level) return MKHIP_ERR_SCALE;
         if (!c->mkf_groups.count(op.group)) return MKHIP_ERR_NO_KEY;
     }
+    const bool same = lv_l == lv_r;
+    const int lvs[2] = {lv_l, lv_r};
     const int nl = level + 1, rows = nl + K;
     const int beta = (nl + c->alpha - 1) / c->alpha;
+    const int nlmax = std::max(lv_l, lv_r) + 1, rowsmax = nlmax + K;
+    const int betamax = (nlmax + c->alpha - 1) / c->alpha;
     const size_t limb = (size_t)N * 8, poly = (size_t)nl * limb, qp = (size_t)rows * limb, hoist = (size_t)beta * qp;
+    const size_t polymax = (size_t)nlmax * limb, qpmax = (size_t)rowsmax * limb, hoistmax = (size_t)betamax * qpmax;
 
     struct OpWs {
         size_t coef, D, facc, fold, mu, mu2c, Dr, racc, fin, last;
@@ -97,10 +112,10 @@ int mkhip::plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int us
         const int k = c->mkf_groups[ops[o].group].k;
         w.k = k;
         kmax = std::max(kmax, k);
-        w.coef = cv.take(2 * k * poly);
-        w.D = cv.take(2 * k * hoist);
-        w.facc = cv.take(2 * k * 2 * qp);
-        w.fold = cv.take(4 * poly);
+        w.coef = cv.take(2 * k * polymax);
+        w.D = cv.take(2 * k * hoistmax);
+        w.facc = cv.take(2 * k * 2 * qpmax);
+        w.fold = cv.take(4 * polymax);
         w.mu = cv.take(3 * poly);
         w.mu2c = cv.take(poly);
         w.Dr = cv.take(hoist);
@@ -114,10 +129,11 @@ int mkhip::plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int us
     char *B = (char *)pl->d_mem;
     auto P = [&](size_t off) { return (u64 *)(B + off); };
 
-    std::vector<NttJob> intt_in, intt_mu2, intt_last;
-    std::vector<DecompDesc> dec, dec_mu2;
-    std::vector<Mac2Desc> mac_fold, mac_relin;
-    std::vector<MdNttDesc> md_fold, md_relin;
+    // fold descriptors per side when the sides' levels differ (their grids differ)
+    std::vector<NttJob> intt_in[2], intt_mu2, intt_last;
+    std::vector<DecompDesc> dec[2], dec_mu2;
+    std::vector<Mac2Desc> mac_fold[2], mac_relin;
+    std::vector<MdNttDesc> md_fold[2], md_relin;
     std::vector<KaraDesc> kara;
     std::vector<RescaleNttDesc> resc;
     for (int o = 0; o < nops; ++o) {
@@ -126,42 +142,44 @@ int mkhip::plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int us
         const MkfGroup &g = c->mkf_groups[op.group];
         const int k = w.k;
         const u64 *side[2] = {op.left, op.right};
-        auto coef = [&](int s, int p) { return P(w.coef) + ((size_t)s * k + p) * nl * N; };
-        auto facc = [&](int s, int p, int oo) { return P(w.facc) + (((size_t)s * k + p) * 2 + oo) * rows * N; };
-        auto fold = [&](int s, int oo) { return P(w.fold) + ((size_t)s * 2 + oo) * nl * N; };
+        auto coef = [&](int s, int p) { return P(w.coef) + ((size_t)s * k + p) * nlmax * N; };
+        auto facc = [&](int s, int p, int oo) { return P(w.facc) + (((size_t)s * k + p) * 2 + oo) * rowsmax * N; };
+        auto fold = [&](int s, int oo) { return P(w.fold) + ((size_t)s * 2 + oo) * nlmax * N; };
         auto mu = [&](int i) { return P(w.mu) + (size_t)i * nl * N; };
         for (int s = 0; s < 2; ++s) {
+            const int sv = same ? 0 : s;
+            const int ls = lvs[s], nls = ls + 1, rs = nls + K, bs = (nls + c->alpha - 1) / c->alpha;
             DecompDesc dd = {};
+            dd.D = P(w.D) + (size_t)s * k * betamax * rowsmax * N;
             for (int p = 0; p < k; ++p) {
-                for (int l = 0; l < nl; ++l) {
+                for (int l = 0; l < nls; ++l) {
                     NttJob j;
-                    j.src = side[s] + ((size_t)(1 + p) * nl + l) * N;
+                    j.src = side[s] + ((size_t)(1 + p) * nls + l) * N;
                     j.dst = coef(s, p) + (size_t)l * N;
                     j.mod = l;
                     j.reduce = 0;
-                    intt_in.push_back(j);
+                    intt_in[sv].push_back(j);
                 }
                 dd.comp[p] = coef(s, p);
                 Mac2Desc md = {};
-                md.D = P(w.D) + ((size_t)s * k + p) * beta * rows * N;
+                md.D = dd.D + (size_t)p * bs * rs * N;
                 md.swk = g.fold[p];
                 md.acc[0] = facc(s, p, 0);
                 md.acc[1] = facc(s, p, 1);
-                md.level = level;
-                mac_fold.push_back(md);
+                md.level = ls;
+                mac_fold[sv].push_back(md);
             }
-            dd.D = P(w.D) + (size_t)s * k * beta * rows * N;
             dd.k = k;
-            dd.level = level;
-            dec.push_back(dd);
+            dd.level = ls;
+            dec[sv].push_back(dd);
             for (int oo = 0; oo < 2; ++oo) {
                 MdNttDesc m = {};
                 for (int p = 0; p < k; ++p) m.acc[p] = facc(s, p, oo);
                 m.nacc = k;
                 m.base = oo == 0 ? side[s] : nullptr;  // fold: out[0] starts from c0
                 m.dst = fold(s, oo);
-                m.level = level;
-                md_fold.push_back(m);
+                m.level = ls;
+                md_fold[sv].push_back(m);
             }
         }
         KaraDesc kd = {};
@@ -229,14 +247,25 @@ int mkhip::plan_mkf_ckks(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int us
     DescBlob blob;
     auto &S = pl->stages;
     Finish fin(c);
-    S.push_back(make_stage(KNTT_INV, blob.add(intt_in), (int)intt_in.size(), 0, 0, 1, "intt_in"));
-    fin.jobs(intt_in);
-    fin.emit(S, blob, "intt_in");
-    S.push_back(make_stage(KDECOMP, blob.add(dec), (int)dec.size(), beta, rows, kmax, "decompose"));
-    S.push_back(make_stage(KMAC2, blob.add(mac_fold), (int)mac_fold.size(), beta, rows, 1, "mac_fold"));
-    fin.mac2(mac_fold, c->L, c->K, N);
-    fin.emit(S, blob, "mac_fold");
-    S.push_back(make_stage(KMD_NTT, blob.add(md_fold), (int)md_fold.size(), nl, 0, 1, "moddown_fold"));
+    const int nsides = same ? 1 : 2;
+    for (int sv = 0; sv < nsides; ++sv) {
+        S.push_back(make_stage(KNTT_INV, blob.add(intt_in[sv]), (int)intt_in[sv].size(), 0, 0, 1, "intt_in"));
+        fin.jobs(intt_in[sv]);
+        fin.emit(S, blob, "intt_in");
+    }
+    for (int sv = 0; sv < nsides; ++sv) {
+        const int nls = lvs[sv] + 1, rs = nls + K, bs = (nls + c->alpha - 1) / c->alpha;
+        S.push_back(make_stage(KDECOMP, blob.add(dec[sv]), (int)dec[sv].size(), bs, rs, kmax, "decompose"));
+    }
+    for (int sv = 0; sv < nsides; ++sv) {
+        const int nls = lvs[sv] + 1, rs = nls + K, bs = (nls + c->alpha - 1) / c->alpha;
+        S.push_back(make_stage(KMAC2, blob.add(mac_fold[sv]), (int)mac_fold[sv].size(), bs, rs, 1, "mac_fold"));
+        fin.mac2(mac_fold[sv], c->L, c->K, N);
+        fin.emit(S, blob, "mac_fold");
+    }
+    for (int sv = 0; sv < nsides; ++sv)
+        S.push_back(make_stage(KMD_NTT, blob.add(md_fold[sv]), (int)md_fold[sv].size(), lvs[sv] + 1, 0, 1,
+                               "moddown_fold"));
     S.push_back(make_stage(KKARA, blob.add(kara), (int)kara.size(), nl, 0, 1, "karatsuba"));
     S.push_back(make_stage(KNTT_INV, blob.add(intt_mu2), (int)intt_mu2.size(), 0, 0, 1, "intt_mu2"));
     fin.jobs(intt_mu2);

@@ -16,6 +16,7 @@ hipError_t plan_stage(mkhip_plan *pl, int s, hipStream_t st) {
     if (S.what >= KBFV_LIFT && S.what <= KBFV_QUANTIZE) return launch_bfv(S.what, c->T, S.descs, S.n, S.a, S.b, st);
     if (S.what == KDECOMP_MAC) return launch_fused(S.what, c->T, S.descs, S.n, S.a, S.b, st);
     if (S.what == KAUTO) return launch_galois(S.what, c->T, S.descs, S.n, st);
+    if (S.what == KLINEAR) return launch_linear(c->T, S.descs, S.n, st);
     return launch_kernel(S.what, c->T, S.descs, S.n, S.a, S.b, st, S.c);
 }
 
@@ -83,24 +84,30 @@ int plan_finish(mkhip_plan *&pl, const DescBlob &blob, int use_graph) {
     pl->desc_bytes = blob.host.size();
     for (PlanStage &s : pl->stages) s.descs = (const char *)pl->d_desc + reinterpret_cast<size_t>(s.descs);
     if (use_graph) {
-        // first launch eagerly (sets kernel attributes), then capture the sequence
-        int st = plan_launch_raw(pl);
-        if (st == MKHIP_OK) st = hip_status(hipStreamSynchronize(c->stream));
-        if (st == MKHIP_OK) st = hip_status(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        if (st == MKHIP_OK) {
-            int s2 = plan_launch_raw(pl);
-            hipError_t e2 = hipStreamEndCapture(c->stream, &pl->g);
-            st = s2 != MKHIP_OK ? s2 : hip_status(e2);
-        }
-        if (st == MKHIP_OK) st = hip_status(hipGraphInstantiate(&pl->ge, pl->g, nullptr, nullptr, 0));
+        int st = plan_capture(pl);
         if (st != MKHIP_OK) {
             mkhip_plan_destroy(pl);
             pl = nullptr;
             return st;
         }
-        pl->graph = true;
     }
     return MKHIP_OK;
+}
+
+int plan_capture(mkhip_plan *pl) {
+    mkhip_ctx *c = pl->ctx;
+    // first launch eagerly (sets kernel attributes), then capture the sequence
+    int st = plan_launch_raw(pl);
+    if (st == MKHIP_OK) st = hip_status(hipStreamSynchronize(c->stream));
+    if (st == MKHIP_OK) st = hip_status(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    if (st == MKHIP_OK) {
+        int s2 = plan_launch_raw(pl);
+        hipError_t e2 = hipStreamEndCapture(c->stream, &pl->g);
+        st = s2 != MKHIP_OK ? s2 : hip_status(e2);
+    }
+    if (st == MKHIP_OK) st = hip_status(hipGraphInstantiate(&pl->ge, pl->g, nullptr, nullptr, 0));
+    if (st == MKHIP_OK) pl->graph = true;
+    return st;
 }
 
 int plan_build(mkhip_ctx *c, int scheme, int nops, const void *ops, int use_graph, mkhip_plan **out,

@@ -79,6 +79,24 @@ enum KernelId {
     KAUTO = 18,          // k_automorph            Galois automorphism (coefficient or NTT domain)
     KSMALL_NTT = 19,     // k_small_ntt<LOGN>      small signed polynomial -> residues -> NTT (-> MForm)
     KKEYOP = 20,         // k_keyop                elementwise key / encryption combination
+    KLINEAR = 21,        // k_linear               CKKS linear operations of the polynomial evaluator
+};
+
+// One NTT-domain row (ring modulus `mod`) of a CKKS linear operation; the constant
+// is c_lo on slots [0, N/2) and c_hi on [N/2, N) (a + b psi^(N/2), a - b psi^(N/2)).
+//   LIN_COPY     out = a
+//   LIN_ADD      out = a + b
+//   LIN_SUB      out = a - b
+//   LIN_MULC     out = MRed(a, c)            c in Montgomery form
+//   LIN_MULCADD  out = b + MRed(a, c)        c in Montgomery form (out may be b)
+//   LIN_ADDC     out = a + c                 c a canonical residue
+//   LIN_ZERO     out = 0
+enum LinOp { LIN_COPY = 0, LIN_ADD, LIN_SUB, LIN_MULC, LIN_MULCADD, LIN_ADDC, LIN_ZERO };
+struct LinDesc {
+    const u64 *a, *b;
+    u64 *out;
+    u64 c_lo, c_hi;
+    int mod, op;
 };
 
 // A small signed polynomial (secret, error, the encryption's ternary v) reduced

@@ -252,9 +252,11 @@ class MKFEvaluator:
         if a.level == 0:
             raise MKFError("cannot Rescale: input Ciphertext already at level 0")
         s, nb = a.scale, 0
-        while a.level - nb >= 0 and s / float(self.Q[a.level - nb]) >= min_scale / 2:
+        while s / float(self.Q[a.level - nb]) >= min_scale / 2:
             s /= float(self.Q[a.level - nb])
             nb += 1
+            if a.level - nb < 0:  # the reference indexes below Q[0] next
+                raise MKFError("cannot Rescale: scale divides past Q[0]")
         if nb == 0:
             return Ct(None if a.data is None else a.data.copy(), a.level, a.scale)
         data = None
