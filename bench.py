@@ -335,6 +335,59 @@ def run_rotations(args, rng):
     return out
 
 
+def run_poly(args, rng):
+    """MKF EvaluatePoly (true_mkf_ckks_polynomial.go:79-273) at the MKF headline shape
+    (N = 2^14, 8 Q + 2 P, gamma 1, k = 4): the degree-59 Chebyshev ReLU activation of
+    the reference's ResNet-20 tests (resnet20_activation_test.go:15,
+    resnet20_real_block1_test.go:226-233) on B ciphertexts per plan launch (one
+    hipGraph), scale = the last prime (one prime per rescale)."""
+    import mkf_amd
+
+    logN, L, K, k, B = 14, 8, 2, 4, args.poly_batch
+    N = 1 << logN
+    Q, P = ntt_primes(50, L), ntt_primes(60, K)
+    QP = Q + P
+    ctx = mkf_amd.Context(logN, Q, P, gamma=1)
+    beta = ctx.beta
+
+    def swk():
+        return np.stack([np.stack([uniform(rng, QP, N), uniform(rng, QP, N)]) for _ in range(beta)])
+
+    ctx.register_mkf_group(0, [swk() for _ in range(k)], swk())
+    relu = np.polynomial.chebyshev.chebinterpolate(lambda t: np.maximum(t, 0.0), 59)
+    poly = mkf_amd.Polynomial(list(relu), basis=mkf_amd.BASIS_CHEBYSHEV)
+    D = float(Q[-1])
+    level = L - 1
+    lo, so, nm = ctx.mkf_poly_query(0, level, D, poly, D)
+    words_in, words_out = (1 + k) * L * N, (1 + k) * (lo + 1) * N
+    ins, outs = [], []
+    for _ in range(B):
+        d = ctx.alloc(words_in * 8)
+        ctx.h2d(d, np.concatenate([uniform(rng, Q, N) for _ in range(1 + k)]))
+        ins.append(d)
+        outs.append(ctx.alloc(words_out * 8))
+    t0 = time.perf_counter()
+    plan = mkf_amd.PolyPlan(ctx, 0, ins, level, D, poly, D, outs, use_graph=True)
+    build_s = time.perf_counter() - t0
+    el, lat = time_plan(ctx, plan, args.poly_steps, 1)
+    st = plan.stage_times(1)
+    by = {}
+    for name_, t in zip(plan.stages, st):
+        by[name_] = by.get(name_, 0.0) + t
+    res = {"polynomial": "Chebyshev ReLU, degree 59", "batch": B, "level_in": level, "level_out": lo,
+           "mulrelins_per_eval": nm, "stages_per_launch": len(st),
+           "evals_per_s": round(B * args.poly_steps / el, 1),
+           "mulrelins_per_s": round(B * nm * args.poly_steps / el, 1),
+           "ms_per_step": round(el / args.poly_steps * 1e3, 3), "p99_ms": round(pct(lat, 99) * 1e3, 3),
+           "plan_build_s": round(build_s, 3), "workspace_gb": round(plan.workspace_bytes / 1e9, 3),
+           "stages_ms": {n_: round(t, 3) for n_, t in sorted(by.items(), key=lambda x: -x[1])}}
+    plan.close()
+    for d in ins + outs:
+        ctx.free(d)
+    ctx.close()
+    return res
+
+
 # uneven batch: (logN, L, K, scheme) per ring degree; K = 1 only admits the MKF
 # construction (KKLSS needs alpha = floor(K/2) >= 1)
 UNEVEN_SHAPES = {13: (4, 1, "mkf"), 14: (8, 2, "kklss"), 15: (16, 3, "kklss")}
@@ -641,6 +694,9 @@ def main():
     ap.add_argument("--serve-mem-gb", type=float, default=64.0)
     ap.add_argument("--rot-batch", type=int, default=16, help="ops per launch of the rotation legs (0: skip)")
     ap.add_argument("--rot-steps", type=int, default=5)
+    ap.add_argument("--poly-batch", type=int, default=32,
+                    help="ciphertexts per launch of the polynomial-evaluation leg (0: skip)")
+    ap.add_argument("--poly-steps", type=int, default=3)
     ap.add_argument("--copy-calib-mb", type=float, default=0.0,
                     help="run a plain D2D copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -720,6 +776,10 @@ def main():
     if args.rot_batch > 0:
         line["rotations"] = run_rotations(args, rng)
         print("# rotations: %.1f hoisted rotations/s" % line["rotations"]["kklss_hoisted_1_2_4_8"]["rotations_per_s"],
+              file=sys.stderr, flush=True)
+    if args.poly_batch > 0:
+        line["polynomial"] = run_poly(args, rng)
+        print("# polynomial: %.1f degree-59 evaluations/s" % line["polynomial"]["evals_per_s"],
               file=sys.stderr, flush=True)
     if calib:
         line["copy_calibration"] = calib

@@ -67,6 +67,7 @@ STAGE_KERNELS = {
     "lift": "k_bfv_lift", "quantize": "k_quantize", "moddown_final": "k_moddown<true>",
     "decompose_relin": "k_decomp", "materialize": "k_rescale_ntt", "decompose_mac": "k_decomp_mac",
     "conj_permute": "k_automorph", "permute": "k_automorph", "ext": "k_ext<2>", "moddown": "k_moddown<false>",
+    "linear": "k_linear", "rescale_intt": "k_ntt_batch<inv>", "rescale_div": "k_rescale_ntt",
 }
 # the BFV plans' tensor stage runs the R-basis kernel; MKF-BFV's ModDowns are coefficient-domain
 BFV_STAGE_KERNELS = dict(STAGE_KERNELS, tensor="k_tensor_r")

@@ -647,7 +647,7 @@ int Prog::build(int use_graph, mkhip_plan **out) {
             const size_t before = own.size();
             fin.emit(own, blob, "rescale_intt");
             for (size_t t = before; t < own.size(); ++t) own_at.push_back(s);
-            own.push_back(make_stage(KRESCALE_NTT, blob.add(rd), (int)rd.size(), la, 0, 1, "rescale"));
+            own.push_back(make_stage(KRESCALE_NTT, blob.add(rd), (int)rd.size(), la, 0, 1, "rescale_div"));
             own_at.push_back(s);
         }
     }

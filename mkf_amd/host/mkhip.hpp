@@ -17,6 +17,8 @@
 //                                                      mkbfv/evaluator.go:156-210
 //   mkf::CKKSEvaluator / mkf::BFVEvaluator             mkf_cpu/ckks.go:102-166, bfv.go:212-252
 //   mkf::CKKSEvaluator::RotateNew                      mkf_cpu/true_mkf_ckks.go:323-353
+//   mkf::CKKSEvaluator::EvaluatePoly, AddNew, SubNew, AddConstNew, MultByConstNew,
+//     RescaleNew, DropLevelNew, MulRelinNew           true_mkf_ckks_polynomial.go, true_mkf_ckks.go:187-303
 // The KKLSS evaluators panic on bad input (keyswitch_hoisted.go:53-59, keys.go:
 // 190-198): here they throw mkhip_host::Panic with the reference's message.  The
 // MKF calls return an error value: here a non-empty std::string.
@@ -33,6 +35,7 @@
 #include <algorithm>
 #include <array>
 #include <cstdint>
+#include <complex>
 #include <map>
 #include <set>
 #include <stdexcept>
@@ -633,9 +636,118 @@ class CKKSEvaluator {
         if (st != MKHIP_OK) return {OutputCiphertext(), status_text(st)};
         return {detail2::unflatten(out, k_, N, level_out + 1, true, level_out, scale_out), ""};
     }
+    // ---- paperMKFCKKSEvaluator's public operations (true_mkf_ckks.go:187-303) and
+    // EvaluatePoly (true_mkf_ckks_polynomial.go:79-111); NTT-domain ciphertexts,
+    // errors returned as the reference returns them.
+    struct Polynomial {  // ckks.Polynomial: Coeffs, MaxDeg, Lead, Basis (A, B: the caller's map)
+        std::vector<std::complex<double>> Coeffs;
+        int MaxDeg = 0;
+        bool Lead = true;
+        int Basis = MKHIP_BASIS_STANDARD;
+    };
+    std::pair<MultiCiphertext, std::string> EvaluatePoly(const MultiCiphertext &in, const Polynomial &p,
+                                                         double targetScale) {
+        if (!err_.empty()) return {MultiCiphertext(), err_};
+        if ((int)in.Components.size() != k_) return {MultiCiphertext(), "component count mismatch"};
+        std::vector<double> re, im;
+        for (const auto &c : p.Coeffs) {
+            re.push_back(c.real());
+            im.push_back(c.imag());
+        }
+        mkhip_polynomial mp = {re.data(), im.data(), (int)re.size(), p.MaxDeg, p.Lead ? 1 : 0, p.Basis};
+        int lo = 0;
+        double so = 0;
+        int st = mkhip_mkf_poly_query(ctx_.c, 0, in.Level, in.Scale, &mp, targetScale, &lo, &so, nullptr);
+        if (st != MKHIP_OK) return {MultiCiphertext(), status_text(st)};
+        const std::vector<uint64_t> x = detail2::flatten_mc(in, in.Level + 1);
+        std::vector<uint64_t> out((size_t)(1 + k_) * (lo + 1) * params_.N());
+        st = mkhip_mkf_ckks_evaluate_poly(ctx_.c, 0, x.data(), in.Level, in.Scale, &mp, targetScale, out.data(), &lo,
+                                          &so);
+        if (st != MKHIP_OK) return {MultiCiphertext(), status_text(st)};
+        return {state(out, lo, so), ""};
+    }
+    std::pair<MultiCiphertext, std::string> AddNew(const MultiCiphertext &a, const MultiCiphertext &b) {
+        return add(a, b, 0);
+    }
+    std::pair<MultiCiphertext, std::string> SubNew(const MultiCiphertext &a, const MultiCiphertext &b) {
+        return add(a, b, 1);
+    }
+    std::pair<MultiCiphertext, std::string> AddConstNew(const MultiCiphertext &a, std::complex<double> c) {
+        if (!err_.empty()) return {MultiCiphertext(), err_};
+        const std::vector<uint64_t> x = detail2::flatten_mc(a, a.Level + 1);
+        std::vector<uint64_t> out(x.size());
+        const int st = mkhip_mkf_ckks_add_const(ctx_.c, (int)a.Components.size(), x.data(), a.Level, a.Scale,
+                                                c.real(), c.imag(), out.data());
+        if (st != MKHIP_OK) return {MultiCiphertext(), status_text(st)};
+        return {state(out, a.Level, a.Scale), ""};
+    }
+    std::pair<MultiCiphertext, std::string> MultByConstNew(const MultiCiphertext &a, std::complex<double> c) {
+        if (!err_.empty()) return {MultiCiphertext(), err_};
+        const std::vector<uint64_t> x = detail2::flatten_mc(a, a.Level + 1);
+        std::vector<uint64_t> out(x.size());
+        double so = 0;
+        const int st = mkhip_mkf_ckks_mult_by_const(ctx_.c, (int)a.Components.size(), x.data(), a.Level, a.Scale,
+                                                    c.real(), c.imag(), out.data(), &so);
+        if (st != MKHIP_OK) return {MultiCiphertext(), status_text(st)};
+        return {state(out, a.Level, so), ""};
+    }
+    std::pair<MultiCiphertext, std::string> RescaleNew(const MultiCiphertext &a, double targetScale) {
+        if (!err_.empty()) return {MultiCiphertext(), err_};
+        const std::vector<uint64_t> x = detail2::flatten_mc(a, a.Level + 1);
+        std::vector<uint64_t> out(x.size());
+        int lo = 0;
+        double so = 0;
+        const int st = mkhip_mkf_ckks_rescale(ctx_.c, (int)a.Components.size(), x.data(), a.Level, a.Scale,
+                                              targetScale, out.data(), &lo, &so);
+        if (st != MKHIP_OK) return {MultiCiphertext(), status_text(st)};
+        out.resize((size_t)(1 + a.Components.size()) * (lo + 1) * params_.N());  // packed at the new level
+        return {state(out, lo, so), ""};
+    }
+    std::pair<MultiCiphertext, std::string> DropLevelNew(const MultiCiphertext &a, int levels) {
+        if (levels < 0 || a.Level - levels < 0)
+            return {MultiCiphertext(), "invalid level drop " + std::to_string(levels) + " from level " +
+                                           std::to_string(a.Level)};
+        const std::vector<uint64_t> x = detail2::flatten_mc(a, a.Level - levels + 1);
+        return {state(x, a.Level - levels, a.Scale), ""};
+    }
+    // the evaluator's MulRelinNew: no rescale, operands at their own levels
+    std::pair<MultiCiphertext, std::string> MulRelinNew(const MultiCiphertext &a, const MultiCiphertext &b) {
+        if (!err_.empty()) return {MultiCiphertext(), err_};
+        if ((int)a.Components.size() != k_ || (int)b.Components.size() != k_)
+            return {MultiCiphertext(), "component count mismatch"};
+        const std::vector<uint64_t> x = detail2::flatten_mc(a, a.Level + 1), y = detail2::flatten_mc(b, b.Level + 1);
+        const int lo = std::min(a.Level, b.Level);
+        std::vector<uint64_t> out((size_t)(1 + k_) * (lo + 1) * params_.N());
+        const int st = mkhip_mkf_ckks_mulrelin_levels(ctx_.c, 0, x.data(), a.Level, y.data(), b.Level, out.data());
+        if (st != MKHIP_OK) return {MultiCiphertext(), status_text(st)};
+        return {state(out, lo, a.Scale * b.Scale), ""};
+    }
     mkhip_ctx *handle() const { return ctx_.c; }
 
   private:
+    MultiCiphertext state(const std::vector<uint64_t> &w, int level, double scale) const {
+        const int N = params_.N(), nl = level + 1;
+        const int k = (int)(w.size() / ((size_t)nl * N)) - 1;
+        MultiCiphertext r;
+        r.C0 = Poly::from(w.data(), N, nl, true);
+        for (int p = 0; p < k; ++p) r.Components.push_back(Poly::from(w.data() + (size_t)(1 + p) * nl * N, N, nl, true));
+        r.Level = level;
+        r.Scale = scale;
+        return r;
+    }
+    std::pair<MultiCiphertext, std::string> add(const MultiCiphertext &a, const MultiCiphertext &b, int sub) {
+        if (!err_.empty()) return {MultiCiphertext(), err_};
+        if (a.Components.size() != b.Components.size()) return {MultiCiphertext(), "component count mismatch"};
+        const std::vector<uint64_t> x = detail2::flatten_mc(a, a.Level + 1), y = detail2::flatten_mc(b, b.Level + 1);
+        const int lv = std::min(a.Level, b.Level);
+        std::vector<uint64_t> out((size_t)(1 + a.Components.size()) * (lv + 1) * params_.N());
+        int lo = 0;
+        double so = 0;
+        const int st = mkhip_mkf_ckks_add(ctx_.c, (int)a.Components.size(), x.data(), a.Level, a.Scale, y.data(),
+                                          b.Level, b.Scale, sub, out.data(), &lo, &so);
+        if (st != MKHIP_OK) return {MultiCiphertext(), status_text(st)};
+        return {state(out, lo, so), ""};
+    }
     // GaloisElementForColumnRotationBy: 5^(k mod 2N) mod 2N
     uint64_t galois_element(long long k) const {
         const uint64_t mask = 2 * (uint64_t)params_.N() - 1;

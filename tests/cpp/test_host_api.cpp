@@ -476,6 +476,105 @@ static void test_rotations() {
     std::printf("ok mkf RotateNew (N=2^13, L=4, K=1, k=2)\n");
 }
 
+// The MKF evaluator's public operations through the mirror: MulRelinNew of operands
+// at different levels and RescaleNew against the C oracle, AddNew against the sum,
+// EvaluatePoly against the C entry point, and the error values
+// (true_mkf_ckks.go:187-303, true_mkf_ckks_polynomial.go:79-111).
+static void test_mkf_poly() {
+    const int logN = 13, N = 1 << logN, k = 2, K = 2;
+    mkbfv::Parameters params;
+    params.LogN = logN;
+    params.Q = primes(60, 1);
+    const std::vector<uint64_t> q40 = primes(40, 4, params.Q);
+    params.Q.insert(params.Q.end(), q40.begin(), q40.end());
+    params.P = primes(60, K, params.Q);
+    params.Gamma = 1;
+    const int L = (int)params.Q.size(), beta = (L + K - 1) / K;
+    std::vector<uint64_t> QP = params.Q;
+    QP.insert(QP.end(), params.P.begin(), params.P.end());
+    mkref_prng g;
+    mkref_prng_seed(&g, 505);
+    auto swk_words = [&]() {
+        std::vector<uint64_t> w;
+        for (int d = 0; d < 2 * beta; ++d) {
+            auto r = uniform(g, QP, N);
+            w.insert(w.end(), r.begin(), r.end());
+        }
+        return w;
+    };
+    std::vector<std::vector<uint64_t>> fold = {swk_words(), swk_words()};
+    const std::vector<uint64_t> relin = swk_words();
+    mkf::OnlineKeys keys;
+    for (const auto &f : fold) keys.Fold.push_back(to_swk(f, beta, L, K, N));
+    keys.Relin = to_swk(relin, beta, L, K, N);
+    const double D = (double)(1ull << 40);
+    mkf::CKKSEvaluator eval(params, D);
+    CHECK(eval.SetKeys(keys).empty(), "mkf poly keys");
+    auto make = [&](int level, std::vector<uint64_t> &flat) {
+        mkf::MultiCiphertext m;
+        m.Level = level;
+        m.Scale = D;
+        const std::vector<uint64_t> Ql(params.Q.begin(), params.Q.begin() + level + 1);
+        for (int c = 0; c <= k; ++c) {
+            auto p = uniform(g, Ql, N);
+            flat.insert(flat.end(), p.begin(), p.end());
+            (c == 0 ? m.C0 : (m.Components.emplace_back(), m.Components.back())) = Poly::from(p.data(), N, level + 1, true);
+        }
+        return m;
+    };
+    std::vector<uint64_t> fx, fy;
+    const mkf::MultiCiphertext x = make(L - 1, fx), y = make(L - 2, fy);
+    mkref_ctx ref;
+    mkref_ctx_init(&ref, logN, params.Q.data(), L, params.P.data(), K, 1, 0);
+    // MulRelinNew: x folded at level L-1, y at L-2, output at L-2 (c0, k copies of Gamma)
+    auto m = eval.MulRelinNew(x, y);
+    CHECK(m.second.empty() && m.first.Level == L - 2 && m.first.Scale == D * D, "mkf MulRelinNew");
+    const int nl = L - 1;
+    std::vector<uint64_t> e0((size_t)nl * N), e1((size_t)nl * N);
+    const uint64_t *fp[2] = {fold[0].data(), fold[1].data()};
+    mkref_mkf_ckks_mulrelin2(&ref, L - 1, L - 2, k, fx.data(), fy.data(), fp, relin.data(), e0.data(), e1.data());
+    CHECK(same(m.first.C0, e0.data(), nl, N), "mkf MulRelinNew c0");
+    for (const Poly &c : m.first.Components) CHECK(same(c, e1.data(), nl, N), "mkf MulRelinNew component");
+    // RescaleNew(D): one prime
+    auto r = eval.RescaleNew(m.first, D);
+    CHECK(r.second.empty() && r.first.Level == L - 3, "mkf RescaleNew");
+    mkref_div_round_last_ntt(&ref, L - 2, 1, e0.data());
+    CHECK(same(r.first.C0, e0.data(), nl - 1, N), "mkf RescaleNew c0");
+    // AddNew at equal scales: the sum mod q on the smaller level
+    auto s = eval.AddNew(x, y);
+    CHECK(s.second.empty() && s.first.Level == L - 2, "mkf AddNew");
+    bool ok = true;
+    for (int l = 0; l <= L - 2; ++l)
+        for (int i = 0; i < N; i += 97) {
+            const uint64_t q = params.Q[l];
+            const uint64_t a = fx[(size_t)l * N + i], b = fy[(size_t)l * N + i];
+            ok = ok && s.first.C0.Coeffs[l][i] == (a + b) % q;
+        }
+    CHECK(ok, "mkf AddNew values");
+    // EvaluatePoly through the mirror = the C entry point on the same words
+    mkf::CKKSEvaluator::Polynomial poly;
+    poly.Coeffs = {0.1, 0.2, -0.3, 0.4};
+    poly.MaxDeg = 3;
+    auto ep = eval.EvaluatePoly(x, poly, D);
+    CHECK(ep.second.empty() && ep.first.Level == L - 3, "mkf EvaluatePoly");
+    const double re[4] = {0.1, 0.2, -0.3, 0.4};
+    mkhip_polynomial mp = {re, nullptr, 4, 3, 1, MKHIP_BASIS_STANDARD};
+    std::vector<uint64_t> out((size_t)(1 + k) * (L - 2) * N);
+    int lo = 0;
+    double so = 0;
+    CHECK(mkhip_mkf_ckks_evaluate_poly(eval.handle(), 0, fx.data(), L - 1, D, &mp, D, out.data(), &lo, &so) ==
+              MKHIP_OK && lo == L - 3 && so == D, "mkf evaluate_poly C entry");
+    CHECK(same(ep.first.C0, out.data(), L - 2, N), "mkf EvaluatePoly words");
+    // error values
+    poly.Coeffs.assign(9, 0.0);
+    poly.Coeffs[8] = 1.0;
+    poly.MaxDeg = 8;
+    CHECK(!eval.EvaluatePoly(y, poly, D).second.empty(), "mkf EvaluatePoly depth above level is an error value");
+    CHECK(!eval.DropLevelNew(x, L).second.empty(), "mkf DropLevelNew error value");
+    mkref_ctx_free(&ref);
+    std::printf("ok mkf polynomial evaluator (N=2^13, L=%d, K=%d, k=%d)\n", L, K, k);
+}
+
 int main() {
     std::setvbuf(stdout, nullptr, _IONBF, 0);  // every line reaches the log as it is printed
     try {
@@ -484,6 +583,7 @@ int main() {
         test_mkf(false);
         test_mkf(true);
         test_rotations();
+        test_mkf_poly();
     } catch (const std::exception &e) {
         std::printf("FAIL unexpected exception: %s\n", e.what());
         return 1;
