@@ -51,7 +51,7 @@ EXPORTS = [
     "mkhip_mkf_ckks_rescale", "mkhip_mkf_ckks_mulrelin_levels",
 ]
 BASIS_STANDARD, BASIS_CHEBYSHEV = 0, 1
-TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64 = 1, 2, 3, 4
+TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64, TUNE_SPLIT_LIMB = 1, 2, 3, 4, 5
 SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3, "rotate": 4, "mkf_rotate": 5}
 MAX_ROTATIONS = 16
 CONJUGATE = -2
@@ -656,7 +656,7 @@ class Context:
         returns the candidate times (ms per launch)."""
         arr = _marshal_ops(scheme, ops)
         n = ctypes.c_int()
-        ms = (ctypes.c_double * 8)()
+        ms = (ctypes.c_double * 16)()
         check(lib().mkhip_autotune(self._h, SCHEME_IDS[scheme], len(ops), ctypes.cast(arr, ctypes.c_void_p),
                                    iters, ctypes.byref(n), ms), "mkhip_autotune")
         return list(ms)[:n.value]

@@ -113,10 +113,12 @@ __device__ __forceinline__ int opaque_lane(int t) {
 // forward transform's first stage is computed by both from the two halves, the
 // inverse leaves its last stage to k_intt_finish.  The elements a workgroup
 // holds are h NS + tid + e TH (e < 16).
-template <int LOGN>
+// SPLIT = true at LOGN = 14 is the same halving for kernels that choose it (two
+// workgroups of half the LDS per limb, so that two run on one CU).
+template <int LOGN, bool SPLIT = (LOGN == 15)>
 struct Geo {
-    static constexpr bool split = LOGN == 15;
-    static constexpr int LS = split ? 14 : LOGN;
+    static constexpr bool split = SPLIT;
+    static constexpr int LS = split ? LOGN - 1 : LOGN;
     static constexpr int H = split ? 2 : 1;
     static constexpr int N = 1 << LOGN;
     static constexpr int NS = 1 << LS;
@@ -161,10 +163,10 @@ __device__ __forceinline__ void fwd_transform(u64 (&v)[16], F f, int tid, int h,
 // fwd_transform of a stored limb (row: its N words, values < 3q when red, else
 // < 8q).  The reduction is a wave-uniform branch over all values, not a
 // per-value select, and the loads use the row's scalar base.
-template <int LOGN, int MODE = -1>
+template <int LOGN, int MODE = -1, bool SPLIT = (LOGN == 15)>
 __device__ __forceinline__ void fwd_transform_row(u64 (&v)[16], const u64 *row, bool red, int tid, int h,
                                                   const ulonglong2 *tw, const ModConst &m) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     if constexpr (Gm::split) {
         u64 lo[16], hi[16];
         load_row_cached<Gm::LS>(lo, row, tid);
@@ -221,10 +223,10 @@ __device__ __forceinline__ void fwd_transform_fp(double (&v)[16], F f, int tid, 
     for (int e = 0; e < 16; ++e) v[e] = fp_canon(v[e], fm);
 }
 
-template <int LOGN>
+template <int LOGN, bool SPLIT = (LOGN == 15)>
 __device__ __forceinline__ void fwd_transform_row_fp(double (&v)[16], const u64 *row, bool red, int tid, int h,
                                                      const double *twd, const ModConst &m) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     const FpMod fm = fp_mod(m);
     if constexpr (Gm::split) {
         u64 lo[16], hi[16];

@@ -19,11 +19,11 @@ namespace mkhip {
 // blockIdx -> logical ((side * beta + i) * rows + r) * nops + o (order 0: the ops
 // sharing a key row are adjacent) or ((side * beta + i) * nops + o) * rows + r
 // (order 1: the rows sharing an input limb are adjacent); descs[2 o + side]
-template <int LOGN, int MODE>
+template <int LOGN, int MODE, bool SPLIT>
 __device__ __forceinline__ void decomp_mac_row(const DecompDesc &Dd, int i, int r, int beta, int rows, int h,
                                                bool reduce, size_t krow, int tid, const ulonglong2 *tw,
                                                const ModConst &m) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     constexpr int N = Gm::N;
     constexpr int TH = Gm::TH;
     const size_t hoff = (size_t)h * Gm::NS;
@@ -34,7 +34,8 @@ __device__ __forceinline__ void decomp_mac_row(const DecompDesc &Dd, int i, int 
         u64 v[16];
         __builtin_amdgcn_sched_barrier(0);
         // values below 3 q_src may exceed the transform's bound 8m: reduce then
-        fwd_transform_row<LOGN, MODE>(v, Dd.comp[p] + (size_t)i * N, reduce, opaque_lane(tid), h, opaque(tw), m);
+        fwd_transform_row<LOGN, MODE, SPLIT>(v, Dd.comp[p] + (size_t)i * N, reduce, opaque_lane(tid), h, opaque(tw),
+                                             m);
         __builtin_amdgcn_sched_barrier(0);
         store_row_quad<Gm::LS>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N + hoff, tid);
         const gu64 *key = uniform_ptr(Dd.key[p] + krow);
@@ -53,11 +54,11 @@ __device__ __forceinline__ void decomp_mac_row(const DecompDesc &Dd, int i, int 
 // decomp_mac_row for a modulus below 2^50: FP64 transform and MAC.  The stored
 // digit rows and the accumulator are the same canonical words as the integer
 // form's: x = MForm(sum_p MRed(key_p, D_p)) = sum_p key_p D_p mod q.
-template <int LOGN>
+template <int LOGN, bool SPLIT>
 __device__ __forceinline__ void decomp_mac_row_fp(const DecompDesc &Dd, int i, int r, int beta, int rows, int h,
                                                   bool reduce, size_t krow, int tid, const double *twd,
                                                   const ModConst &m) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     constexpr int N = Gm::N;
     const size_t hoff = (size_t)h * Gm::NS;
     const FpMod fm = fp_mod(m);
@@ -72,7 +73,8 @@ __device__ __forceinline__ void decomp_mac_row_fp(const DecompDesc &Dd, int i, i
         double v[16];
         __builtin_amdgcn_sched_barrier(0);
         if constexpr (Gm::split) {
-            fwd_transform_row_fp<LOGN>(v, Dd.comp[p] + (size_t)i * N, reduce, opaque_lane(tid), h, opaque(twd), m);
+            fwd_transform_row_fp<LOGN, SPLIT>(v, Dd.comp[p] + (size_t)i * N, reduce, opaque_lane(tid), h, opaque(twd),
+                                              m);
         } else {
             const FpMod fm0 = fp_mod(m);
             const int t = opaque_lane(tid);
@@ -120,10 +122,12 @@ __device__ __forceinline__ void decomp_mac_row_fp(const DecompDesc &Dd, int i, i
     store_row_quad<Gm::LS>(out, Dd.xy + krow, tid);
 }
 
-template <int LOGN>
-__global__ void __launch_bounds__(Geo<LOGN>::TH)
+// SPLIT (LOGN = 14): a limb over two workgroups of 512 threads and 64 KiB of LDS,
+// at most 128 registers a lane, so that two workgroups share a CU
+template <int LOGN, bool SPLIT>
+__global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ? 4 : 1))
     k_decomp_mac(DevTables T, const DecompDesc *__restrict__ descs, int nops, int beta, int rows, int order) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     constexpr int N = Gm::N;
     const int tid = threadIdx.x;
     int bid = block_index(T);
@@ -153,28 +157,28 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     // one instance of the loop per butterfly form (a wave-uniform branch), so that
     // neither form's registers are allocated around the other's
     if (fp_ok(m.q) && T.fp64)
-        decomp_mac_row_fp<LOGN>(Dd, i, r, beta, rows, h, reduce, krow, tid, T.twd + (size_t)ri * N, m);
+        decomp_mac_row_fp<LOGN, SPLIT>(Dd, i, r, beta, rows, h, reduce, krow, tid, T.twd + (size_t)ri * N, m);
     else if (fwd_lazy_ok(m.q))
-        decomp_mac_row<LOGN, 1>(Dd, i, r, beta, rows, h, reduce, krow, tid, tw, m);
+        decomp_mac_row<LOGN, 1, SPLIT>(Dd, i, r, beta, rows, h, reduce, krow, tid, tw, m);
     else
-        decomp_mac_row<LOGN, 0>(Dd, i, r, beta, rows, h, reduce, krow, tid, tw, m);
+        decomp_mac_row<LOGN, 0, SPLIT>(Dd, i, r, beta, rows, h, reduce, krow, tid, tw, m);
 }
 
-template <int LOGN>
+template <int LOGN, bool SPLIT = (LOGN == 15)>
 static hipError_t launch_fused_n(int what, const DevTables &T, const void *descs, int n, int a, int b,
                                  hipStream_t st) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     constexpr size_t lds = (size_t)Gm::NS * sizeof(u64);
     static bool init = false;
     if (!init) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_decomp_mac<LOGN>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_decomp_mac<LOGN, SPLIT>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         init = true;
     }
     if (what != KDECOMP_MAC) return hipErrorInvalidValue;
     // n ops (2 descriptors each), a = beta, b = rows
-    hipLaunchKernelGGL((k_decomp_mac<LOGN>), dim3(n * 2 * a * b * Gm::H), dim3(Gm::TH), lds, st, T,
+    hipLaunchKernelGGL((k_decomp_mac<LOGN, SPLIT>), dim3(n * 2 * a * b * Gm::H), dim3(Gm::TH), lds, st, T,
                        (const DecompDesc *)descs, n, a, b, T.decomp_order);
     return hipGetLastError();
 }
@@ -183,7 +187,9 @@ hipError_t launch_fused(int what, const DevTables &T, const void *descs, int n, 
     if (n <= 0) return hipSuccess;
     if (T.alpha != 1) return hipErrorInvalidValue;
     if (T.logN == 13) return launch_fused_n<13>(what, T, descs, n, a, b, st);
-    if (T.logN == 14) return launch_fused_n<14>(what, T, descs, n, a, b, st);
+    if (T.logN == 14)
+        return T.split14 ? launch_fused_n<14, true>(what, T, descs, n, a, b, st)
+                         : launch_fused_n<14, false>(what, T, descs, n, a, b, st);
     if (T.logN == 15) return launch_fused_n<15>(what, T, descs, n, a, b, st);
     return hipErrorInvalidValue;
 }

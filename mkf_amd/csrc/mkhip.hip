@@ -333,6 +333,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     T.bfv = bfv ? 1 : 0;
     T.xcd = 1;
     T.fp64 = 1;
+    T.split14 = 0;
     T.logN = logN;
     T.N = N;
     T.L = L;

@@ -224,6 +224,9 @@ extern "C" int mkhip_set_tuning(mkhip_ctx *c, int knob, int value) {
     case MKHIP_TUNE_FP64:
         c->T.fp64 = value ? 1 : 0;
         break;
+    case MKHIP_TUNE_SPLIT_LIMB:
+        c->T.split14 = value ? 1 : 0;
+        break;
     default:
         return MKHIP_ERR_ARG;
     }
@@ -248,6 +251,9 @@ extern "C" int mkhip_get_tuning(const mkhip_ctx *c, int knob, int *value) {
     case MKHIP_TUNE_FP64:
         *value = c->T.fp64;
         return MKHIP_OK;
+    case MKHIP_TUNE_SPLIT_LIMB:
+        *value = c->T.split14;
+        return MKHIP_OK;
     default:
         return MKHIP_ERR_ARG;
     }
@@ -259,28 +265,38 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
                               double *ms) {
     if (!c || !ops || nops <= 0 || iters <= 0) return MKHIP_ERR_ARG;
     const bool kklss = scheme == MKHIP_SCHEME_KKLSS_CKKS || scheme == MKHIP_SCHEME_KKLSS_BFV;
-    // candidates: (fused, order) for alpha = 1 KKLSS plans -- split, fused order 0,
-    // fused order 1 -- each with and without the XCD remap
+    // candidates: (fused, order) for alpha = 1 KKLSS plans -- unfused, fused order 0,
+    // fused order 1, and at N = 2^14 the fused orders over half-limb workgroups --
+    // each with and without the XCD remap
     struct Cand {
-        int fused, order, xcd;
+        int fused, order, xcd, split;
     };
     std::vector<Cand> cands;
     const int old_fused = c->tune_fused_decomp, old_xcd = c->T.xcd, old_order = c->T.decomp_order;
+    const int old_split = c->T.split14;
     for (int xcd = 1; xcd >= 0; --xcd) {
         if (kklss && c->alpha == 1) {
-            cands.push_back({0, old_order, xcd});
-            cands.push_back({1, 0, xcd});
-            cands.push_back({1, 1, xcd});
+            cands.push_back({0, old_order, xcd, old_split});
+            cands.push_back({1, 0, xcd, 0});
+            cands.push_back({1, 1, xcd, 0});
+            if (c->logN == 14) {
+                cands.push_back({1, 0, xcd, 1});
+                cands.push_back({1, 1, xcd, 1});
+            }
         } else {
-            cands.push_back({old_fused, old_order, xcd});
+            cands.push_back({old_fused, old_order, xcd, old_split});
         }
     }
-    int best = -1, n = 0;
-    double best_ms = 0;
-    for (const Cand &cd : cands) {
+    auto apply = [&](const Cand &cd) {
         mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, cd.fused);
         mkhip_set_tuning(c, MKHIP_TUNE_DECOMP_ORDER, cd.order);
         mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, cd.xcd);
+        mkhip_set_tuning(c, MKHIP_TUNE_SPLIT_LIMB, cd.split);
+    };
+    int best = -1, n = 0;
+    double best_ms = 0;
+    for (const Cand &cd : cands) {
+        apply(cd);
         mkhip_plan *pl = nullptr;
         int s = plan_build(c, scheme, nops, ops, 1, &pl, PlanMem());
         double t = 0;
@@ -288,21 +304,17 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
         if (s == MKHIP_OK) s = mkhip_plan_time(pl, iters, &t);
         mkhip_plan_destroy(pl);
         if (s != MKHIP_OK) {
-            mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, old_fused);
-            mkhip_set_tuning(c, MKHIP_TUNE_DECOMP_ORDER, old_order);
-            mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, old_xcd);
+            apply({old_fused, old_order, old_xcd, old_split});
             return s;
         }
-        if (ms && n < 8) ms[n] = t;
+        if (ms && n < 16) ms[n] = t;
         if (best < 0 || t < best_ms) {
             best = n;
             best_ms = t;
         }
         ++n;
     }
-    mkhip_set_tuning(c, MKHIP_TUNE_FUSED_DECOMP, cands[best].fused);
-    mkhip_set_tuning(c, MKHIP_TUNE_DECOMP_ORDER, cands[best].order);
-    mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, cands[best].xcd);
+    apply(cands[best]);
     if (ncand) *ncand = n;
     return MKHIP_OK;
 }

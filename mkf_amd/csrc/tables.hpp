@@ -46,6 +46,7 @@ struct DevTables {
     int xcd;                  // 1: XCD-contiguous workgroup order (xcd_remap), a tuning knob
     int decomp_order;         // fused decomposition grid order (a tuning knob)
     int fp64;                 // FP64 transforms for moduli below 2^50 (a tuning knob)
+    int split14;              // N = 2^14 fused decomposition over half-limb workgroups (a tuning knob)
 };
 
 // ring index of limb l of an R = Q u QMul polynomial

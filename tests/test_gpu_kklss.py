@@ -226,3 +226,26 @@ def test_fp64_and_integer_arithmetic_agree(setup):
     finally:
         ctx.set_tuning(mk.TUNE_FP64, 1)
     assert np.array_equal(a, b)
+
+
+def test_split_limb_decomposition_agrees(setup):
+    """The N = 2^14 fused decomposition over half-limb workgroups gives the words of
+    the one-workgroup form, in FP64 and in integer arithmetic (other ring sizes
+    ignore the knob)."""
+    name, ctx, ref = setup
+    mk = _mk()
+    _keys(ctx, ref, [0, 1, 2], seed=53)
+    g = O.Prng(54)
+    op0 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(4)])
+    op1 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
+    scale = float(ref.Q[-1])
+    a = ctx.kklss_ckks_mulrelin(op0, [0, 1, 2], op1, [0, 2], scale, scale, scale)[0]
+    ctx.set_tuning(mk.TUNE_SPLIT_LIMB, 1)
+    try:
+        b = ctx.kklss_ckks_mulrelin(op0, [0, 1, 2], op1, [0, 2], scale, scale, scale)[0]
+        ctx.set_tuning(mk.TUNE_FP64, 0)
+        c = ctx.kklss_ckks_mulrelin(op0, [0, 1, 2], op1, [0, 2], scale, scale, scale)[0]
+    finally:
+        ctx.set_tuning(mk.TUNE_SPLIT_LIMB, 0)
+        ctx.set_tuning(mk.TUNE_FP64, 1)
+    assert np.array_equal(a, b) and np.array_equal(a, c)
