@@ -391,8 +391,9 @@ int mkhip_encrypt(mkhip_ctx *ctx, const uint64_t *pk0, const uint64_t *pk1, int 
  *                            adjacent (0) or rows sharing an input limb adjacent (1)
  *   MKHIP_TUNE_FP64          transforms of limbs whose modulus is below 2^50 in FP64
  *                            arithmetic (1, default) or in 64-bit integer arithmetic (0)
- *   MKHIP_TUNE_SPLIT_LIMB    N = 2^14 fused decomposition: each limb over two
- *                            workgroups of half the LDS (1) or one workgroup (0)
+ *   MKHIP_TUNE_SPLIT_LIMB    N = 2^14 one-prime digit decompositions (fused and
+ *                            plain): each limb over two workgroups of half the LDS (1)
+ *                            or one workgroup (0)
  * A knob applies to plans built after it is set.  mkhip_autotune plans and times
  * every applicable combination on the caller's batch (ops: mkhip_kklss_op[] for the
  * KKLSS schemes, mkhip_mkf_op[] for MKF; their outputs are overwritten with the

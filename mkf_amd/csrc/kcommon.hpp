@@ -140,10 +140,10 @@ __device__ __forceinline__ void fwd_ntt_mode(u64 (&v)[16], int tid, const ulongl
 
 // Forward transform of this workgroup's part of a limb whose element c is f(c)
 // (values < 8q); v receives the NTT values at h NS + tid + e TH, canonical.
-template <int LOGN, int MODE = -1, class F>
+template <int LOGN, int MODE = -1, bool SPLIT = (LOGN == 15), class F>
 __device__ __forceinline__ void fwd_transform(u64 (&v)[16], F f, int tid, int h, const ulonglong2 *tw,
                                               const ModConst &m) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     if constexpr (Gm::split) {
         u64 lo[16], hi[16];
 #pragma unroll
@@ -200,10 +200,10 @@ __device__ __forceinline__ FpMod fp_mod(const ModConst &m) {
 }
 
 // fwd_transform in FP64: element c of the limb is f(c) (a word below 2q).
-template <int LOGN, class F>
+template <int LOGN, bool SPLIT = (LOGN == 15), class F>
 __device__ __forceinline__ void fwd_transform_fp(double (&v)[16], F f, int tid, int h, const double *twd,
                                                  const ModConst &m) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     const FpMod fm = fp_mod(m);
     if constexpr (Gm::split) {
         const double w = twiddle_fp<true>(twd, 1);

@@ -108,42 +108,42 @@ __device__ __forceinline__ u64 digit_modup(const DevTables &T, const u64 *__rest
 // itself, reduced mod m only when it may exceed the transform's input bound 8m --
 // source values are below 3 q_src: canonical, or multSum's raw outputs in a BFV
 // lift; otherwise the exact CRT ModUp), forward-transformed, stored at dst.
-template <int LOGN, bool CRT, int MODE>
+template <int LOGN, bool CRT, int MODE, bool SPLIT = (LOGN == 15)>
 __device__ __forceinline__ void decomp_row(const DevTables &T, const u64 *src, u64 *dst, int start, int nd,
                                            bool own_row, int i, int r, int ri, int h, int tid,
                                            const ulonglong2 *tw, const ModConst &m) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     constexpr int N = Gm::N;
     u64 v[16];
     if (!CRT || nd == 1 || own_row) {
         const int sl = own_row ? r : start;
         const bool red = T.mod[r_index(sl, T.L, T.K)].q > 2 * m.q;
-        fwd_transform_row<LOGN, MODE>(v, src + (size_t)sl * N, red, tid, h, tw, m);
+        fwd_transform_row<LOGN, MODE, SPLIT>(v, src + (size_t)sl * N, red, tid, h, tw, m);
     } else {
         const int ebase = i * (kMaxAlpha + 1) + nd;
-        fwd_transform<LOGN, MODE>(v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h,
-                                  tw, m);
+        fwd_transform<LOGN, MODE, SPLIT>(
+            v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h, tw, m);
     }
     store_row_quad<Gm::LS>(v, dst, tid);
 }
 
 // decomp_row in FP64 for a modulus below 2^50 (the CRT rows' ModUp stays integer;
 // its canonical output is transformed in FP64).
-template <int LOGN, bool CRT>
+template <int LOGN, bool CRT, bool SPLIT = (LOGN == 15)>
 __device__ __forceinline__ void decomp_fp(const DevTables &T, const u64 *src, u64 *dst, int start, int nd,
                                           bool own_row, int i, int r, int ri, int h, int tid, const ModConst &m) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     constexpr int N = Gm::N;
     const double *twd = T.twd + (size_t)ri * N;
     double v[16];
     if (!CRT || nd == 1 || own_row) {
         const int sl = own_row ? r : start;
         const bool red = T.mod[r_index(sl, T.L, T.K)].q > 2 * m.q;
-        fwd_transform_row_fp<LOGN>(v, src + (size_t)sl * N, red, tid, h, twd, m);
+        fwd_transform_row_fp<LOGN, SPLIT>(v, src + (size_t)sl * N, red, tid, h, twd, m);
     } else {
         const int ebase = i * (kMaxAlpha + 1) + nd;
-        fwd_transform_fp<LOGN>(v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h,
-                               twd, m);
+        fwd_transform_fp<LOGN, SPLIT>(
+            v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h, twd, m);
     }
     u64 w[16];
 #pragma unroll
@@ -176,10 +176,11 @@ __device__ __forceinline__ void decomp_flat(const DevTables &T, const u64 *src, 
 // One workgroup per (desc, component p, digit i, row r[, half]): logical block
 // (((desc * kmax + p) * beta + i) * rows + r) * H + h, XCD-remapped so the rows of
 // one (component, digit) -- which all read the same input limbs -- share an L2.
-template <int LOGN, bool CRT>
-__global__ void __launch_bounds__(Geo<LOGN>::TH)
+// SPLIT at LOGN = 14: a limb over two workgroups of half the LDS (two per CU)
+template <int LOGN, bool CRT, bool SPLIT = (LOGN == 15)>
+__global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ? 4 : 1))
     k_decomp(DevTables T, const DecompDesc *__restrict__ descs, int kmax, int beta, int rows) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     constexpr int N = Gm::N;
     const int tid = threadIdx.x;
     int bid = block_index(T);
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     const ulonglong2 *tw = T.tw + (size_t)ri * N;
     u64 *dst = Dd.D + (((size_t)p * beta + i) * rows + r) * N + (size_t)h * Gm::NS;
     if (fp_ok(m.q) && T.fp64) {
-        decomp_fp<LOGN, CRT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, m);
+        decomp_fp<LOGN, CRT, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, m);
         return;
     }
     if constexpr (!Gm::split) {
@@ -214,11 +215,11 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
         return;
     }
     if (CRT && !(nd == 1 || own_row)) {  // CRT digit rows: one ModUp per value, then the transform
-        decomp_row<LOGN, true, 0>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
+        decomp_row<LOGN, true, 0, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
     } else if (fwd_lazy_ok(m.q)) {
-        decomp_row<LOGN, false, 1>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
+        decomp_row<LOGN, false, 1, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
     } else {
-        decomp_row<LOGN, false, 0>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
+        decomp_row<LOGN, false, 0, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
     }
 }
 
@@ -454,6 +455,21 @@ static hipError_t launch_all(int what, const DevTables &T, const void *descs, in
         hipLaunchKernelGGL((k_ntt_batch<LOGN, true>), dim3(n * H), block, lds, st, T, (const NttJob *)descs);
         break;
     case KDECOMP:  // n descs, a = beta, b = rows, c = kmax
+        if constexpr (LOGN == 14) {
+            // one-prime digits (copy path) over half-limb workgroups when tuned so
+            if (T.split14 && T.alpha == 1) {
+                static bool init_split = false;
+                constexpr size_t lds2 = lds / 2;
+                if (!init_split) {
+                    hipError_t e = allow_lds(k_decomp<14, false, true>, lds2);
+                    if (e != hipSuccess) return e;
+                    init_split = true;
+                }
+                hipLaunchKernelGGL((k_decomp<14, false, true>), dim3(n * c * a * b * 2), dim3(Gm::TH / 2), lds2, st,
+                                   T, (const DecompDesc *)descs, c, a, b);
+                break;
+            }
+        }
         if (T.alpha > 1)
             hipLaunchKernelGGL((k_decomp<LOGN, true>), dim3(n * c * a * b * H), block, lds, st, T,
                                (const DecompDesc *)descs, c, a, b);

@@ -71,6 +71,28 @@ def test_bfv_mulrelin_parity(setup, shape):
         assert np.array_equal(out[c], exp[c]), c
 
 
+def test_bfv_split_limb_matches_oracle(setup):
+    """The half-limb decomposition workgroups (N = 2^14 tuning knob; other sizes
+    ignore it) give the oracle's words for the two-branch BFV mul+relin."""
+    import mkf_amd
+
+    name, ctx, ref = setup
+    b, d, v, u = _keys(ref, 3, seed=35)
+    for pid in range(3):
+        ctx.register_bfv_party(pid, b[pid], d[pid], v[pid])
+    ctx.set_crs_u(u)
+    g = O.Prng(36)
+    op0 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
+    op1 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
+    exp, eids = ref.mulrelin(op0, [0, 1], op1, [1, 2], b, d, v, u)
+    ctx.set_tuning(mkf_amd.TUNE_SPLIT_LIMB, 1)
+    try:
+        out, ids = ctx.kklss_bfv_mulrelin(op0, [0, 1], op1, [1, 2])
+    finally:
+        ctx.set_tuning(mkf_amd.TUNE_SPLIT_LIMB, 0)
+    assert ids == eids and np.array_equal(out, exp)
+
+
 def test_bfv_batched_plan_matches_oracle(setup):
     import mkf_amd
 
