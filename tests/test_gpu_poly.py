@@ -103,6 +103,8 @@ def _polys(name):
     cases = [([0.1, 0.2, -0.3, 0.4], mkf_amd.BASIS_STANDARD)]  # true_mkf_ckks_test.go:283
     if name == "q40":
         cases += [(list(relu), mkf_amd.BASIS_CHEBYSHEV), (list(odd), mkf_amd.BASIS_CHEBYSHEV),
+                  ([0.7], mkf_amd.BASIS_STANDARD), ([0.1, -0.5], mkf_amd.BASIS_STANDARD),
+                  ([0.1, 0.2, 0.0, 0.0], mkf_amd.BASIS_STANDARD), ([0.3, 0.0, -0.4], mkf_amd.BASIS_CHEBYSHEV),
                   ([0.25 - 0.5j, 1j, 0.3 + 0.1j, 0, -0.2j, 0.15], mkf_amd.BASIS_STANDARD)]
     return cases
 

@@ -268,6 +268,16 @@ def test_evaluate_poly_odd_chebyshev_and_complex_standard():
     assert err2 < 1e-4
 
 
+def test_evaluate_poly_small_degrees():
+    """Degree 0 (a constant: no product), degree 1, zero top coefficients, an even
+    Chebyshev polynomial."""
+    for coeffs, basis in (([0.7], O.STANDARD), ([0.1, -0.5], O.STANDARD), ([0.1, 0.2, 0.0, 0.0], O.STANDARD),
+                          ([0.3, 0.0, -0.4], O.CHEBYSHEV)):
+        ev, out, err = _run_poly(O.Polynomial(coeffs, basis=basis), nq=3)
+        assert err < 1e-4, coeffs
+        assert out.level == 3 - O.Polynomial(coeffs).depth(), coeffs
+
+
 def test_evaluate_poly_errors():
     ctx, keys = _ctx(nq=2)
     ev = O.MKFEvaluator(ctx, 2, keys.fold, keys.relin)
