@@ -701,8 +701,8 @@ def main():
                     help="run a plain D2D copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (ops shard across GPUs "
-              "with zero per-op traffic; see the design notes). Use --gpus 1.", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path (mul+relin ops shard "
+              "across GPUs with zero per-op traffic). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
     schemes = [s for s in args.schemes.split(",") if s]
     for s in schemes:

@@ -6,7 +6,7 @@
  *   KKLSS MK-BFV   mkbfv.Evaluator.MulRelinNew    software/kklss/mkbfv/evaluator.go:85
  *   MKF-CKKS       (*CKKSCase).Run                software/mkf/mkf_cpu/ckks.go:102
  *   MKF-BFV        (*BFVCase).Run                 software/mkf/mkf_cpu/bfv.go:212
- * A Go host binds these through cgo (see INTEGRATION.md); a C++ host through
+ * A Go host binds these through cgo; a C++ host through
  * mkf_amd/host/mkhip.hpp.  Plain pointers and sizes only.
  *
  * Data layout (the reference's fixture/dump layout: little-endian uint64,

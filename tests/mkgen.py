@@ -1,7 +1,7 @@
 """Test-side key generation, encryption and decryption over the CPU oracle.
 
 The reference samples keys with a crypto PRNG (mkrlwe/keygen.go:26), so "keys
-from seed s" is this build's convention: the oracle's xoshiro256** stream.
+from seed s" is the tests' convention: the oracle's xoshiro256** stream.
 The key equations follow mkrlwe/keygen.go:
   secret  s  : ternary, P(0)=1/2 (:58-75), stored as MForm(NTT(s)) over QP (:44-55)
   b_i = MForm(e_i - a_i s)                              (:161-169)

@@ -1,6 +1,6 @@
 """Parameter sets used by the tests and the benchmark.
 
-The BASELINE workloads name synthetic NTT-friendly primes (~50-bit Q, ~60-bit P,
+The benchmark workloads use synthetic NTT-friendly primes (~50-bit Q, ~60-bit P,
 q = 1 mod 2^16 so that every N up to 2^15 is supported); they are generated
 deterministically by scanning down from 2^bits + 1.  P1 is the reference's own
 KKLSS/MKF parameter literal (software/mkf/mkf_cpu/params.go:15-28,
