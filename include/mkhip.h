@@ -394,6 +394,9 @@ int mkhip_encrypt(mkhip_ctx *ctx, const uint64_t *pk0, const uint64_t *pk1, int 
  *   MKHIP_TUNE_SPLIT_LIMB    N = 2^14 one-prime digit decompositions (fused and
  *                            plain): each limb over two workgroups of half the LDS (1)
  *                            or one workgroup (0)
+ *   MKHIP_TUNE_SPLIT_EXT     N = 2^14 external products: each limb over two workgroups
+ *                            of half the LDS, the last inverse stage in k_intt_finish (1),
+ *                            or one workgroup (0)
  * A knob applies to plans built after it is set.  mkhip_autotune plans and times
  * every applicable combination on the caller's batch (ops: mkhip_kklss_op[] for the
  * KKLSS schemes, mkhip_mkf_op[] for MKF; their outputs are overwritten with the
@@ -404,6 +407,7 @@ int mkhip_encrypt(mkhip_ctx *ctx, const uint64_t *pk0, const uint64_t *pk1, int 
 #define MKHIP_TUNE_DECOMP_ORDER 3
 #define MKHIP_TUNE_FP64 4
 #define MKHIP_TUNE_SPLIT_LIMB 5
+#define MKHIP_TUNE_SPLIT_EXT 6
 #define MKHIP_SCHEME_KKLSS_CKKS 0
 #define MKHIP_SCHEME_KKLSS_BFV 1
 #define MKHIP_SCHEME_MKF_CKKS 2

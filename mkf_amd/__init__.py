@@ -51,7 +51,7 @@ EXPORTS = [
     "mkhip_mkf_ckks_rescale", "mkhip_mkf_ckks_mulrelin_levels",
 ]
 BASIS_STANDARD, BASIS_CHEBYSHEV = 0, 1
-TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64, TUNE_SPLIT_LIMB = 1, 2, 3, 4, 5
+TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64, TUNE_SPLIT_LIMB, TUNE_SPLIT_EXT = 1, 2, 3, 4, 5, 6
 SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3, "rotate": 4, "mkf_rotate": 5}
 MAX_ROTATIONS = 16
 CONJUGATE = -2

@@ -266,10 +266,10 @@ __device__ __forceinline__ void fwd_transform_row_fp(double (&v)[16], const u64 
 // Inverse transform of this workgroup's elements (v[e] = element h NS + tid + e TH,
 // values < 4q) with the N^-1 (or t N^-1) scaling given; for a split limb the last
 // stage and the scaling are left to k_intt_finish.
-template <int LOGN>
+template <int LOGN, bool SPLIT = (LOGN == 15)>
 __device__ __forceinline__ void inv_transform(u64 (&v)[16], int tid, int h, const ulonglong2 *itw, u64 q,
                                               u64 ninv, u64 ninvp) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     if constexpr (Gm::split)
         ntt_inv<Gm::LS>(v, g_lds, tid, itw, q, ninv, ninvp, 2 + h, false);
     else
@@ -278,10 +278,10 @@ __device__ __forceinline__ void inv_transform(u64 (&v)[16], int tid, int h, cons
 
 // inv_transform choosing the FP64 arithmetic for moduli below 2^50 (a wave-uniform
 // branch).  tscale: t N^-1 instead of N^-1.  Same words as the integer form.
-template <int LOGN>
+template <int LOGN, bool SPLIT = (LOGN == 15)>
 __device__ __forceinline__ void inv_transform_sel(u64 (&v)[16], int tid, int h, const DevTables &T, int ri,
                                                   const ModConst &m, bool tscale) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     if (fp_ok(m.q) && T.fp64) {
         const double *itwd = opaque(T.itwd + (size_t)ri * Gm::N);
         if constexpr (Gm::split)
@@ -289,8 +289,8 @@ __device__ __forceinline__ void inv_transform_sel(u64 (&v)[16], int tid, int h, 
         else
             ntt_inv_fp<LOGN>(v, g_lds, tid, itwd, fp_mod(m), fp_from_u64(tscale ? m.tninv : m.ninv), 1, true);
     } else {
-        inv_transform<LOGN>(v, tid, h, opaque(T.itw + (size_t)ri * Gm::N), m.q, tscale ? m.tninv : m.ninv,
-                            tscale ? m.tninvp : m.ninvp);
+        inv_transform<LOGN, SPLIT>(v, tid, h, opaque(T.itw + (size_t)ri * Gm::N), m.q, tscale ? m.tninv : m.ninv,
+                                   tscale ? m.tninvp : m.ninvp);
     }
 }
 

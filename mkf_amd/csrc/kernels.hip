@@ -291,10 +291,12 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
 // NKEY keys share one hoisted decomposition D (the nested products with v_i and
 // u, keyswitch_hoisted.go:177-193): one pass over D feeds both accumulators; the
 // second is parked in its output row while the first is transformed.
-template <int LOGN, int NKEY>
-__global__ void __launch_bounds__(Geo<LOGN>::TH)
+// SPLIT at LOGN = 14: half-limb workgroups (two per CU); the last inverse stage
+// is left to k_intt_finish as for 2^15
+template <int LOGN, int NKEY, bool SPLIT = (LOGN == 15)>
+__global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ? 4 : 1))
     k_ext(DevTables T, const ExtDesc *__restrict__ descs, int beta, int rows) {
-    typedef Geo<LOGN> Gm;
+    typedef Geo<LOGN, SPLIT> Gm;
     constexpr int N = Gm::N;
     const int tid = threadIdx.x;
     int bid = block_index(T);
@@ -326,12 +328,12 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
                                NKEY > 1 ? Dd.key[1] + (size_t)ri * N + hoff : nullptr, kstride, beta, tid, m);
         // opaque(): the second transform reloads its twiddles instead of keeping the
         // first one's live across it
-        inv_transform_sel<LOGN>(a0, opaque_lane(tid), h, T, ri, m, false);
+        inv_transform_sel<LOGN, SPLIT>(a0, opaque_lane(tid), h, T, ri, m, false);
     }
     store_row<Gm::LS>(a0, Dd.acc[0] + (size_t)r * N + hoff, tid);
     if (NKEY > 1) {
         load_row_quad<Gm::LS>(a0, o1, tid);
-        inv_transform_sel<LOGN>(a0, opaque_lane(tid), h, T, ri, m, false);
+        inv_transform_sel<LOGN, SPLIT>(a0, opaque_lane(tid), h, T, ri, m, false);
         store_row<Gm::LS>(a0, o1, tid);
     }
 }
@@ -481,10 +483,30 @@ static hipError_t launch_all(int what, const DevTables &T, const void *descs, in
         hipLaunchKernelGGL((k_tensor<LOGN>), dim3(n * a * H), block, lds, st, T, (const TensorDesc *)descs, a);
         break;
     case KEXT1:
-        hipLaunchKernelGGL((k_ext<LOGN, 1>), dim3(n * b * H), block, lds, st, T, (const ExtDesc *)descs, a, b);
-        break;
     case KEXT2:
-        hipLaunchKernelGGL((k_ext<LOGN, 2>), dim3(n * b * H), block, lds, st, T, (const ExtDesc *)descs, a, b);
+        if constexpr (LOGN == 14) {
+            if (ext_split(T)) {
+                static bool init_split = false;
+                constexpr size_t lds2 = lds / 2;
+                if (!init_split) {
+                    hipError_t e;
+                    if ((e = allow_lds(k_ext<14, 1, true>, lds2)) != hipSuccess) return e;
+                    if ((e = allow_lds(k_ext<14, 2, true>, lds2)) != hipSuccess) return e;
+                    init_split = true;
+                }
+                if (what == KEXT1)
+                    hipLaunchKernelGGL((k_ext<14, 1, true>), dim3(n * b * 2), dim3(Gm::TH / 2), lds2, st, T,
+                                       (const ExtDesc *)descs, a, b);
+                else
+                    hipLaunchKernelGGL((k_ext<14, 2, true>), dim3(n * b * 2), dim3(Gm::TH / 2), lds2, st, T,
+                                       (const ExtDesc *)descs, a, b);
+                break;
+            }
+        }
+        if (what == KEXT1)
+            hipLaunchKernelGGL((k_ext<LOGN, 1>), dim3(n * b * H), block, lds, st, T, (const ExtDesc *)descs, a, b);
+        else
+            hipLaunchKernelGGL((k_ext<LOGN, 2>), dim3(n * b * H), block, lds, st, T, (const ExtDesc *)descs, a, b);
         break;
     default:
         return hipErrorInvalidValue;

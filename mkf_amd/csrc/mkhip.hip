@@ -334,6 +334,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     T.xcd = 1;
     T.fp64 = 1;
     T.split14 = 0;
+    T.split14_ext = 0;
     T.logN = logN;
     T.N = N;
     T.L = L;
@@ -570,8 +571,8 @@ struct DevScratch {
 
 // Split (2^15) inverse transforms: the last stage and the scaling of the nrows
 // rows at base + r N (ring modulus mods[r]); synchronous, nothing below 2^15.
-int mkhip::finish_rows(mkhip_ctx *c, u64 *base, const int *mods, int nrows, int tscale) {
-    if (c->logN != 15 || nrows <= 0) return MKHIP_OK;
+int mkhip::finish_rows(mkhip_ctx *c, u64 *base, const int *mods, int nrows, int tscale, bool split) {
+    if ((c->logN != 15 && !split) || nrows <= 0) return MKHIP_OK;
     std::vector<FinishDesc> f(nrows);
     for (int r = 0; r < nrows; ++r) {
         f[r].row = base + (size_t)r * c->N;
@@ -699,7 +700,7 @@ extern "C" int mkhip_external_product(mkhip_ctx *c, const uint64_t *a_hoisted, c
     {
         std::vector<int> mods(rows);
         for (int r = 0; r < rows; ++r) mods[r] = r < nl ? r : c->L + r - nl;
-        const int st = finish_rows(c, dacc, mods.data(), rows, 0);
+        const int st = finish_rows(c, dacc, mods.data(), rows, 0, ext_split(c->T));
         if (st != MKHIP_OK) return st;
     }
     HIP_TRY(launch_kernel(KMODDOWN, c->T, dm, 1, 0, 0, c->stream));

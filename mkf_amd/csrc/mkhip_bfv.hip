@@ -280,16 +280,17 @@ int mkhip::plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, int
     fin.emit(S, blob, "tensor");
     S.push_back(make_stage(KBFV_QUANTIZE, blob.add(quant), (int)quant.size(), 0, 0, 1, "quantize"));
     S.push_back(make_stage(KEXT1, blob.add(ext1), (int)ext1.size(), 2 * beta, rows, 1, "ext_xy"));
-    for (const ExtDesc &e : ext1) fin.qp(e.acc[0], nl, c->L, K, N);
-    fin.emit(S, blob, "ext_xy");
+    Finish fext(c, ext_split(c->T));  // the external products may run on half-limb workgroups
+    for (const ExtDesc &e : ext1) fext.qp(e.acc[0], nl, c->L, K, N);
+    fext.emit(S, blob, "ext_xy");
     S.push_back(make_stage(KMODDOWN, blob.add(md1), (int)md1.size(), 0, 0, 1, "moddown_xy"));
     S.push_back(make_stage(KDECOMP, blob.add(dec2), (int)dec2.size(), beta, rows, kmax2, "decompose_t"));
     S.push_back(make_stage(KEXT2, blob.add(ext2), (int)ext2.size(), beta, rows, 1, "ext_vu"));
     for (const ExtDesc &e : ext2) {
-        fin.qp(e.acc[0], nl, c->L, K, N);
-        fin.qp(e.acc[1], nl, c->L, K, N);
+        fext.qp(e.acc[0], nl, c->L, K, N);
+        fext.qp(e.acc[1], nl, c->L, K, N);
     }
-    fin.emit(S, blob, "ext_vu");
+    fext.emit(S, blob, "ext_vu");
     S.push_back(make_stage(KMODDOWN_FINAL, blob.add(md2), (int)md2.size(), 0, 0, 1, "moddown_final"));
     st = plan_finish(pl, blob, use_graph);
     if (st != MKHIP_OK) return st;

@@ -147,7 +147,7 @@ struct DescBlob {
 };
 
 // mkhip.hip: last stage + scaling of split inverse transforms (see Finish below)
-int finish_rows(mkhip_ctx *c, u64 *base, const int *mods, int nrows, int tscale);
+int finish_rows(mkhip_ctx *c, u64 *base, const int *mods, int nrows, int tscale, bool split = false);
 // Launch stage s (on `st`, default the context stream) / every stage.
 hipError_t plan_stage(mkhip_plan *pl, int s, hipStream_t st = nullptr);
 int plan_launch_raw(mkhip_plan *pl, hipStream_t st = nullptr);
@@ -204,6 +204,8 @@ struct Finish {
     bool on = false;
     std::vector<FinishDesc> d;
     explicit Finish(const mkhip_ctx *c) : on(c->logN == 15) {}
+    // split: the stage's kernel ran on half-limb workgroups at this ring size too
+    Finish(const mkhip_ctx *c, bool split) : on(c->logN == 15 || split) {}
     void row(u64 *r, int mod, int tscale = 0) {
         if (!on) return;
         FinishDesc f;

@@ -227,6 +227,9 @@ extern "C" int mkhip_set_tuning(mkhip_ctx *c, int knob, int value) {
     case MKHIP_TUNE_SPLIT_LIMB:
         c->T.split14 = value ? 1 : 0;
         break;
+    case MKHIP_TUNE_SPLIT_EXT:
+        c->T.split14_ext = value ? 1 : 0;
+        break;
     default:
         return MKHIP_ERR_ARG;
     }
@@ -254,6 +257,9 @@ extern "C" int mkhip_get_tuning(const mkhip_ctx *c, int knob, int *value) {
     case MKHIP_TUNE_SPLIT_LIMB:
         *value = c->T.split14;
         return MKHIP_OK;
+    case MKHIP_TUNE_SPLIT_EXT:
+        *value = c->T.split14_ext;
+        return MKHIP_OK;
     default:
         return MKHIP_ERR_ARG;
     }
@@ -269,22 +275,24 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
     // fused order 1, and at N = 2^14 the fused orders over half-limb workgroups --
     // each with and without the XCD remap
     struct Cand {
-        int fused, order, xcd, split;
+        int fused, order, xcd, split, split_ext;
     };
     std::vector<Cand> cands;
     const int old_fused = c->tune_fused_decomp, old_xcd = c->T.xcd, old_order = c->T.decomp_order;
-    const int old_split = c->T.split14;
+    const int old_split = c->T.split14, old_split_ext = c->T.split14_ext;
     for (int xcd = 1; xcd >= 0; --xcd) {
         if (kklss && c->alpha == 1) {
-            cands.push_back({0, old_order, xcd, old_split});
-            cands.push_back({1, 0, xcd, 0});
-            cands.push_back({1, 1, xcd, 0});
+            cands.push_back({0, old_order, xcd, old_split, old_split_ext});
+            cands.push_back({1, 0, xcd, 0, old_split_ext});
+            cands.push_back({1, 1, xcd, 0, old_split_ext});
             if (c->logN == 14) {
-                cands.push_back({1, 0, xcd, 1});
-                cands.push_back({1, 1, xcd, 1});
+                // (half-limb external products are not tried: their extra k_intt_finish
+                // pass costs more than the occupancy gains, measured on MI355X)
+                cands.push_back({1, 0, xcd, 1, old_split_ext});
+                cands.push_back({1, 1, xcd, 1, old_split_ext});
             }
         } else {
-            cands.push_back({old_fused, old_order, xcd, old_split});
+            cands.push_back({old_fused, old_order, xcd, old_split, old_split_ext});
         }
     }
     auto apply = [&](const Cand &cd) {
@@ -292,6 +300,7 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
         mkhip_set_tuning(c, MKHIP_TUNE_DECOMP_ORDER, cd.order);
         mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, cd.xcd);
         mkhip_set_tuning(c, MKHIP_TUNE_SPLIT_LIMB, cd.split);
+        mkhip_set_tuning(c, MKHIP_TUNE_SPLIT_EXT, cd.split_ext);
     };
     int best = -1, n = 0;
     double best_ms = 0;
@@ -304,7 +313,7 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
         if (s == MKHIP_OK) s = mkhip_plan_time(pl, iters, &t);
         mkhip_plan_destroy(pl);
         if (s != MKHIP_OK) {
-            apply({old_fused, old_order, old_xcd, old_split});
+            apply({old_fused, old_order, old_xcd, old_split, old_split_ext});
             return s;
         }
         if (ms && n < 16) ms[n] = t;

@@ -338,11 +338,12 @@ int mkhip::plan_kklss_rotate(mkhip_ctx *c, int nops, const mkhip_rotate_op *ops,
     if (!dec.empty()) S.push_back(make_stage(KDECOMP, blob.add(dec), (int)dec.size(), beta, rows, kmax, "decompose"));
     if (!ext.empty()) {
         S.push_back(make_stage(KEXT2, blob.add(ext), (int)ext.size(), beta, rows, 1, "ext"));
+        Finish fext(c, ext_split(c->T));  // the external products may run on half-limb workgroups
         for (const ExtDesc &e : ext) {
-            fin.qp(e.acc[0], nl, c->L, K, N);
-            fin.qp(e.acc[1], nl, c->L, K, N);
+            fext.qp(e.acc[0], nl, c->L, K, N);
+            fext.qp(e.acc[1], nl, c->L, K, N);
         }
-        fin.emit(S, blob, "ext");
+        fext.emit(S, blob, "ext");
     }
     if (!md.empty()) S.push_back(make_stage(KMODDOWN, blob.add(md), (int)md.size(), 0, 0, 1, "moddown"));
     if (!post.empty()) S.push_back(make_stage(KAUTO, blob.add(post), (int)post.size(), 0, 0, 1, "permute"));

@@ -46,8 +46,14 @@ struct DevTables {
     int xcd;                  // 1: XCD-contiguous workgroup order (xcd_remap), a tuning knob
     int decomp_order;         // fused decomposition grid order (a tuning knob)
     int fp64;                 // FP64 transforms for moduli below 2^50 (a tuning knob)
-    int split14;              // N = 2^14 fused decomposition over half-limb workgroups (a tuning knob)
+    int split14;              // N = 2^14 one-prime decompositions over half-limb workgroups (a tuning knob)
+    int split14_ext;          // N = 2^14 external products over half-limb workgroups (a tuning knob)
 };
+
+// N = 2^14 external products over half-limb workgroups: a plan that emits KEXT
+// stages then finishes their inverse transforms with k_intt_finish, as at 2^15.
+inline bool ext_split(const DevTables &T) { return T.logN == 14 && T.split14_ext != 0; }
+
 
 // ring index of limb l of an R = Q u QMul polynomial
 __host__ __device__ __forceinline__ int r_index(int l, int L, int K) { return l < L ? l : K + l; }

@@ -249,3 +249,31 @@ def test_split_limb_decomposition_agrees(setup):
         ctx.set_tuning(mk.TUNE_SPLIT_LIMB, 0)
         ctx.set_tuning(mk.TUNE_FP64, 1)
     assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_split_external_products_agree(setup):
+    """N = 2^14 external products over half-limb workgroups (their last inverse
+    stage in k_intt_finish) give the words of the one-workgroup form, for the
+    plan path and the external-product primitive."""
+    name, ctx, ref = setup
+    mk = _mk()
+    _keys(ctx, ref, [0, 1], seed=55)
+    g = O.Prng(56)
+    op0 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
+    op1 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
+    scale = float(ref.Q[-1])
+    level = ref.L - 1
+    a_h = ref.decompose(g.uniform(ref.Q, ref.N), level)
+    key = np.stack([g.uniform(ref.QP, ref.N) for _ in range(ctx.beta)])
+    a = ctx.kklss_ckks_mulrelin(op0, [0, 1], op1, [0, 1], scale, scale, scale)[0]
+    e0 = ctx.external_product(a_h, key, level)
+    ctx.set_tuning(mk.TUNE_SPLIT_EXT, 1)
+    ctx.set_tuning(mk.TUNE_SPLIT_LIMB, 1)
+    try:
+        b = ctx.kklss_ckks_mulrelin(op0, [0, 1], op1, [0, 1], scale, scale, scale)[0]
+        e1 = ctx.external_product(a_h, key, level)
+    finally:
+        ctx.set_tuning(mk.TUNE_SPLIT_EXT, 0)
+        ctx.set_tuning(mk.TUNE_SPLIT_LIMB, 0)
+    assert np.array_equal(a, b) and np.array_equal(e0, e1)
+    assert np.array_equal(e1, ref.external_product(a_h, key, level))
