@@ -443,7 +443,7 @@ def run_uneven(args, rng):
                 ops.append(dict(left=a, right=b, out=out, group=k, level=L - 1, nres=nres))
         tune = []
         if args.autotune:
-            tune = ctx.autotune(ops[:args.uneven_tune_ops], scheme=scheme, iters=2)
+            tune = ctx.autotune(ops[:args.uneven_tune_ops], scheme=scheme, iters=3)
         branches.append((logN, scheme, ctx, ops, tune))
     setup_s = time.perf_counter() - t_setup
     batch = mkf_amd.Batch()
@@ -680,7 +680,7 @@ def main():
     ap.add_argument("--uneven-steps", type=int, default=5)
     ap.add_argument("--uneven-budget-gb", type=float, default=24.0,
                     help="workspace budget of each ring degree's branch of the uneven batch")
-    ap.add_argument("--uneven-tune-ops", type=int, default=4,
+    ap.add_argument("--uneven-tune-ops", type=int, default=16,
                     help="ops of each branch the autotuner times")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="length of each Poisson arrival run of the serving loop; 0 skips serving")
