@@ -397,6 +397,9 @@ int mkhip_encrypt(mkhip_ctx *ctx, const uint64_t *pk0, const uint64_t *pk1, int 
  *   MKHIP_TUNE_SPLIT_EXT     N = 2^14 external products: each limb over two workgroups
  *                            of half the LDS, the last inverse stage in k_intt_finish (1),
  *                            or one workgroup (0)
+ *   MKHIP_TUNE_DIGIT_PRE     MKF CKKS plans with CRT digits (alpha >= 2): the digit-only
+ *                            part of the ModUp (y_l, the overflow estimate) computed once
+ *                            per digit by k_digit_pre (1) or by every target row (0)
  * A knob applies to plans built after it is set.  mkhip_autotune plans and times
  * every applicable combination on the caller's batch (ops: mkhip_kklss_op[] for the
  * KKLSS schemes, mkhip_mkf_op[] for MKF; their outputs are overwritten with the
@@ -408,6 +411,7 @@ int mkhip_encrypt(mkhip_ctx *ctx, const uint64_t *pk0, const uint64_t *pk1, int 
 #define MKHIP_TUNE_FP64 4
 #define MKHIP_TUNE_SPLIT_LIMB 5
 #define MKHIP_TUNE_SPLIT_EXT 6
+#define MKHIP_TUNE_DIGIT_PRE 7
 #define MKHIP_SCHEME_KKLSS_CKKS 0
 #define MKHIP_SCHEME_KKLSS_BFV 1
 #define MKHIP_SCHEME_MKF_CKKS 2

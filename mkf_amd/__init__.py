@@ -51,7 +51,8 @@ EXPORTS = [
     "mkhip_mkf_ckks_rescale", "mkhip_mkf_ckks_mulrelin_levels",
 ]
 BASIS_STANDARD, BASIS_CHEBYSHEV = 0, 1
-TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64, TUNE_SPLIT_LIMB, TUNE_SPLIT_EXT = 1, 2, 3, 4, 5, 6
+TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64, TUNE_SPLIT_LIMB, TUNE_SPLIT_EXT, TUNE_DIGIT_PRE = \
+    1, 2, 3, 4, 5, 6, 7
 SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3, "rotate": 4, "mkf_rotate": 5}
 MAX_ROTATIONS = 16
 CONJUGATE = -2
@@ -68,6 +69,7 @@ STAGE_KERNELS = {
     "decompose_relin": "k_decomp", "materialize": "k_rescale_ntt", "decompose_mac": "k_decomp_mac",
     "conj_permute": "k_automorph", "permute": "k_automorph", "ext": "k_ext<2>", "moddown": "k_moddown<false>",
     "linear": "k_linear", "rescale_intt": "k_ntt_batch<inv>", "rescale_div": "k_rescale_ntt",
+    "digit_pre": "k_digit_pre", "digit_pre_mu2": "k_digit_pre",
 }
 # the BFV plans' tensor stage runs the R-basis kernel; MKF-BFV's ModDowns are coefficient-domain
 BFV_STAGE_KERNELS = dict(STAGE_KERNELS, tensor="k_tensor_r")

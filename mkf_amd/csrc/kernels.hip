@@ -104,6 +104,50 @@ __device__ __forceinline__ u64 digit_modup(const DevTables &T, const u64 *__rest
     return reduce64(r, m.q, m.mu);
 }
 
+// digit_modup with the digit-only part (y_l, v) read from k_digit_pre's rows
+__device__ __forceinline__ u64 digit_modup_pre(const DevTables &T, const u64 *__restrict__ pre, int c, int nd,
+                                               int ebase, int ri, const ModConst &m) {
+    const int N = T.N;
+    const u64 *qs = T.dc_qstar + ((size_t)ebase * (T.L + T.K) + ri) * kMaxAlpha;
+    const u64 *vt = T.dc_vt + ((size_t)ebase * (T.L + T.K) + ri) * (kMaxAlpha + 1);
+    u64 hi = 0, lo = 0;
+    for (int l = 0; l < nd; ++l) mac128(hi, lo, G(pre)[(size_t)l * N + c], qs[l]);
+    const u64 v = G(pre)[(size_t)T.alpha * N + c];
+    u64 r = mont_reduce128(hi, lo, m.q, m.qinv) + vt[v];
+    return reduce64(r, m.q, m.mu);
+}
+
+// The part of digit_modup every target row of a CRT digit repeats -- y_l =
+// MRed(x_l, (D/q_l)^-1 R) and v = trunc(sum_l y_l / q_l), same operations in the
+// same order -- once per (component, digit, coefficient): rows l < nd of the
+// digit's block hold y_l, row alpha holds v.  grid (N / 256, descs x kmax x beta).
+__global__ void __launch_bounds__(256) k_digit_pre(DevTables T, const DecompDesc *__restrict__ descs, int kmax,
+                                                   int beta) {
+    const int N = T.N;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    int b = blockIdx.y;
+    const int i = b % beta;
+    b /= beta;
+    const int p = b % kmax;
+    const DecompDesc &Dd = descs[b / kmax];
+    if (p >= Dd.k || !Dd.pre) return;
+    const int level = Dd.level, alpha = T.alpha, start = i * alpha;
+    const int nd = (level > alpha * (i + 1) - 1) ? alpha : (level % alpha) + 1;
+    if (nd == 1) return;  // a one-prime digit is copied, not extended
+    const int ebase = i * (kMaxAlpha + 1) + nd;
+    const u64 *yc = T.dc_y + (size_t)ebase * kMaxAlpha;
+    u64 *pre = Dd.pre + ((size_t)p * beta + i) * (alpha + 1) * N;
+    const u64 *src = Dd.comp[p];
+    double vi = 0.0;
+    for (int l = 0; l < nd; ++l) {
+        const ModConst ml = T.mod[start + l];
+        const u64 y = mred(G(src)[(size_t)(start + l) * N + c], yc[l], ml.q, ml.qinv);
+        vi += (double)y / (double)ml.q;
+        G(pre)[(size_t)l * N + c] = y;
+    }
+    G(pre)[(size_t)alpha * N + c] = (u64)vi;
+}
+
 // One digit row: the digit's value (copy path / the digit's own rows: the residue
 // itself, reduced mod m only when it may exceed the transform's input bound 8m --
 // source values are below 3 q_src: canonical, or multSum's raw outputs in a BFV
@@ -111,7 +155,7 @@ __device__ __forceinline__ u64 digit_modup(const DevTables &T, const u64 *__rest
 template <int LOGN, bool CRT, int MODE, bool SPLIT = (LOGN == 15)>
 __device__ __forceinline__ void decomp_row(const DevTables &T, const u64 *src, u64 *dst, int start, int nd,
                                            bool own_row, int i, int r, int ri, int h, int tid,
-                                           const ulonglong2 *tw, const ModConst &m) {
+                                           const ulonglong2 *tw, const ModConst &m, const u64 *pre) {
     typedef Geo<LOGN, SPLIT> Gm;
     constexpr int N = Gm::N;
     u64 v[16];
@@ -121,8 +165,12 @@ __device__ __forceinline__ void decomp_row(const DevTables &T, const u64 *src, u
         fwd_transform_row<LOGN, MODE, SPLIT>(v, src + (size_t)sl * N, red, tid, h, tw, m);
     } else {
         const int ebase = i * (kMaxAlpha + 1) + nd;
-        fwd_transform<LOGN, MODE, SPLIT>(
-            v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h, tw, m);
+        if (pre)
+            fwd_transform<LOGN, MODE, SPLIT>(
+                v, [&](int c) { return digit_modup_pre(T, pre, c, nd, ebase, ri, m); }, tid, h, tw, m);
+        else
+            fwd_transform<LOGN, MODE, SPLIT>(
+                v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h, tw, m);
     }
     store_row_quad<Gm::LS>(v, dst, tid);
 }
@@ -131,7 +179,8 @@ __device__ __forceinline__ void decomp_row(const DevTables &T, const u64 *src, u
 // its canonical output is transformed in FP64).
 template <int LOGN, bool CRT, bool SPLIT = (LOGN == 15)>
 __device__ __forceinline__ void decomp_fp(const DevTables &T, const u64 *src, u64 *dst, int start, int nd,
-                                          bool own_row, int i, int r, int ri, int h, int tid, const ModConst &m) {
+                                          bool own_row, int i, int r, int ri, int h, int tid, const ModConst &m,
+                                          const u64 *pre) {
     typedef Geo<LOGN, SPLIT> Gm;
     constexpr int N = Gm::N;
     const double *twd = T.twd + (size_t)ri * N;
@@ -142,8 +191,12 @@ __device__ __forceinline__ void decomp_fp(const DevTables &T, const u64 *src, u6
         fwd_transform_row_fp<LOGN, SPLIT>(v, src + (size_t)sl * N, red, tid, h, twd, m);
     } else {
         const int ebase = i * (kMaxAlpha + 1) + nd;
-        fwd_transform_fp<LOGN, SPLIT>(
-            v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h, twd, m);
+        if (pre)
+            fwd_transform_fp<LOGN, SPLIT>(
+                v, [&](int c) { return digit_modup_pre(T, pre, c, nd, ebase, ri, m); }, tid, h, twd, m);
+        else
+            fwd_transform_fp<LOGN, SPLIT>(
+                v, [&](int c) { return digit_modup(T, src, c, start, nd, ebase, ri, m); }, tid, h, twd, m);
     }
     u64 w[16];
 #pragma unroll
@@ -155,7 +208,7 @@ __device__ __forceinline__ void decomp_fp(const DevTables &T, const u64 *src, u6
 template <int LOGN, bool CRT, int MODE>
 __device__ __forceinline__ void decomp_flat(const DevTables &T, const u64 *src, u64 *dst, int start, int nd,
                                             bool own_row, int i, int r, int ri, int tid, const ulonglong2 *tw,
-                                            const ModConst &m) {
+                                            const ModConst &m, const u64 *pre) {
     constexpr int N = 1 << LOGN;
     u64 v[16];
     if (!CRT || nd == 1 || own_row) {
@@ -167,7 +220,10 @@ __device__ __forceinline__ void decomp_flat(const DevTables &T, const u64 *src, 
         }
     } else {
         const int ebase = i * (kMaxAlpha + 1) + nd;
-        for (int e = 0; e < 16; ++e) v[e] = digit_modup(T, src, tid + e * (N / 16), start, nd, ebase, ri, m);
+        if (pre)
+            for (int e = 0; e < 16; ++e) v[e] = digit_modup_pre(T, pre, tid + e * (N / 16), nd, ebase, ri, m);
+        else
+            for (int e = 0; e < 16; ++e) v[e] = digit_modup(T, src, tid + e * (N / 16), start, nd, ebase, ri, m);
     }
     fwd_ntt_mode<LOGN, MODE>(v, tid, tw, m, 1);
     store_row_quad<LOGN>(v, dst, tid);
@@ -202,24 +258,25 @@ __global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ?
     const u64 *src = Dd.comp[p];
     const ulonglong2 *tw = T.tw + (size_t)ri * N;
     u64 *dst = Dd.D + (((size_t)p * beta + i) * rows + r) * N + (size_t)h * Gm::NS;
+    const u64 *pre = (CRT && Dd.pre) ? Dd.pre + ((size_t)p * beta + i) * (alpha + 1) * N : nullptr;
     if (fp_ok(m.q) && T.fp64) {
-        decomp_fp<LOGN, CRT, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, m);
+        decomp_fp<LOGN, CRT, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, m, pre);
         return;
     }
     if constexpr (!Gm::split) {
         const bool crt_row = CRT && !(nd == 1 || own_row);
         if (!crt_row && fwd_lazy_ok(m.q))
-            decomp_flat<LOGN, CRT, 1>(T, src, dst, start, nd, own_row, i, r, ri, tid, tw, m);
+            decomp_flat<LOGN, CRT, 1>(T, src, dst, start, nd, own_row, i, r, ri, tid, tw, m, pre);
         else
-            decomp_flat<LOGN, CRT, 0>(T, src, dst, start, nd, own_row, i, r, ri, tid, tw, m);
+            decomp_flat<LOGN, CRT, 0>(T, src, dst, start, nd, own_row, i, r, ri, tid, tw, m, pre);
         return;
     }
     if (CRT && !(nd == 1 || own_row)) {  // CRT digit rows: one ModUp per value, then the transform
-        decomp_row<LOGN, true, 0, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
+        decomp_row<LOGN, true, 0, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m, pre);
     } else if (fwd_lazy_ok(m.q)) {
-        decomp_row<LOGN, false, 1, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
+        decomp_row<LOGN, false, 1, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m, pre);
     } else {
-        decomp_row<LOGN, false, 0, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m);
+        decomp_row<LOGN, false, 0, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m, pre);
     }
 }
 
@@ -539,6 +596,11 @@ hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
 hipError_t launch_kernel(int what, const DevTables &T, const void *descs, int n, int a, int b, hipStream_t st,
                          int c) {
     if (n <= 0) return hipSuccess;
+    if (what == KDIGIT_PRE) {  // n descs, a = beta, c = kmax
+        hipLaunchKernelGGL(k_digit_pre, dim3(T.N / 256, n * c * a), dim3(256), 0, st, T, (const DecompDesc *)descs,
+                           c, a);
+        return hipGetLastError();
+    }
     if (what == KMAC_XY) {
         hipLaunchKernelGGL(k_mac_xy, dim3(T.N / 256, a * b, 2), dim3(256), 0, st, T, (const MacDesc *)descs, n,
                            a, b);

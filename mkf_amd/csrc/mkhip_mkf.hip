@@ -101,17 +101,22 @@ int mkhip::plan_mkf_ckks_lv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int
     const size_t polymax = (size_t)nlmax * limb, qpmax = (size_t)rowsmax * limb, hoistmax = (size_t)betamax * qpmax;
 
     struct OpWs {
-        size_t coef, D, facc, fold, mu, mu2c, Dr, racc, fin, last;
+        size_t coef, D, facc, fold, mu, mu2c, Dr, racc, fin, last, pre, pre2;
         int k;
     };
     std::vector<OpWs> ws(nops);
     Carve cv;
     int kmax = 1;
+    // CRT digits (alpha >= 2): the digit-only part of the ModUp once per digit
+    const bool crt = c->alpha > 1 && c->T.digit_pre;
+    const size_t preblk = (size_t)(c->alpha + 1) * limb;
     for (int o = 0; o < nops; ++o) {
         OpWs &w = ws[o];
         const int k = c->mkf_groups[ops[o].group].k;
         w.k = k;
         kmax = std::max(kmax, k);
+        w.pre = crt ? cv.take(2 * k * betamax * preblk) : 0;
+        w.pre2 = crt ? cv.take(beta * preblk) : 0;
         w.coef = cv.take(2 * k * polymax);
         w.D = cv.take(2 * k * hoistmax);
         w.facc = cv.take(2 * k * 2 * qpmax);
@@ -151,6 +156,7 @@ int mkhip::plan_mkf_ckks_lv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int
             const int ls = lvs[s], nls = ls + 1, rs = nls + K, bs = (nls + c->alpha - 1) / c->alpha;
             DecompDesc dd = {};
             dd.D = P(w.D) + (size_t)s * k * betamax * rowsmax * N;
+            if (crt) dd.pre = P(w.pre) + (size_t)s * k * betamax * (c->alpha + 1) * N;
             for (int p = 0; p < k; ++p) {
                 for (int l = 0; l < nls; ++l) {
                     NttJob j;
@@ -202,6 +208,7 @@ int mkhip::plan_mkf_ckks_lv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int
         DecompDesc d2 = {};
         d2.comp[0] = P(w.mu2c);
         d2.D = P(w.Dr);
+        if (crt) d2.pre = P(w.pre2);
         d2.k = 1;
         d2.level = level;
         dec_mu2.push_back(d2);
@@ -255,7 +262,9 @@ int mkhip::plan_mkf_ckks_lv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int
     }
     for (int sv = 0; sv < nsides; ++sv) {
         const int nls = lvs[sv] + 1, rs = nls + K, bs = (nls + c->alpha - 1) / c->alpha;
-        S.push_back(make_stage(KDECOMP, blob.add(dec[sv]), (int)dec[sv].size(), bs, rs, kmax, "decompose"));
+        const size_t off = blob.add(dec[sv]);
+        if (crt) S.push_back(make_stage(KDIGIT_PRE, off, (int)dec[sv].size(), bs, 0, kmax, "digit_pre"));
+        S.push_back(make_stage(KDECOMP, off, (int)dec[sv].size(), bs, rs, kmax, "decompose"));
     }
     for (int sv = 0; sv < nsides; ++sv) {
         const int nls = lvs[sv] + 1, rs = nls + K, bs = (nls + c->alpha - 1) / c->alpha;
@@ -270,7 +279,11 @@ int mkhip::plan_mkf_ckks_lv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int
     S.push_back(make_stage(KNTT_INV, blob.add(intt_mu2), (int)intt_mu2.size(), 0, 0, 1, "intt_mu2"));
     fin.jobs(intt_mu2);
     fin.emit(S, blob, "intt_mu2");
-    S.push_back(make_stage(KDECOMP, blob.add(dec_mu2), (int)dec_mu2.size(), beta, rows, 1, "decompose_mu2"));
+    {
+        const size_t off = blob.add(dec_mu2);
+        if (crt) S.push_back(make_stage(KDIGIT_PRE, off, (int)dec_mu2.size(), beta, 0, 1, "digit_pre_mu2"));
+        S.push_back(make_stage(KDECOMP, off, (int)dec_mu2.size(), beta, rows, 1, "decompose_mu2"));
+    }
     S.push_back(make_stage(KMAC2, blob.add(mac_relin), (int)mac_relin.size(), beta, rows, 1, "mac_relin"));
     fin.mac2(mac_relin, c->L, c->K, N);
     fin.emit(S, blob, "mac_relin");

@@ -230,6 +230,9 @@ extern "C" int mkhip_set_tuning(mkhip_ctx *c, int knob, int value) {
     case MKHIP_TUNE_SPLIT_EXT:
         c->T.split14_ext = value ? 1 : 0;
         break;
+    case MKHIP_TUNE_DIGIT_PRE:
+        c->T.digit_pre = value ? 1 : 0;
+        break;
     default:
         return MKHIP_ERR_ARG;
     }
@@ -260,6 +263,9 @@ extern "C" int mkhip_get_tuning(const mkhip_ctx *c, int knob, int *value) {
     case MKHIP_TUNE_SPLIT_EXT:
         *value = c->T.split14_ext;
         return MKHIP_OK;
+    case MKHIP_TUNE_DIGIT_PRE:
+        *value = c->T.digit_pre;
+        return MKHIP_OK;
     default:
         return MKHIP_ERR_ARG;
     }
@@ -275,24 +281,28 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
     // fused order 1, and at N = 2^14 the fused orders over half-limb workgroups --
     // each with and without the XCD remap
     struct Cand {
-        int fused, order, xcd, split, split_ext;
+        int fused, order, xcd, split, split_ext, pre;
     };
     std::vector<Cand> cands;
     const int old_fused = c->tune_fused_decomp, old_xcd = c->T.xcd, old_order = c->T.decomp_order;
-    const int old_split = c->T.split14, old_split_ext = c->T.split14_ext;
+    const int old_split = c->T.split14, old_split_ext = c->T.split14_ext, old_pre = c->T.digit_pre;
+    const bool mkf_crt = scheme == MKHIP_SCHEME_MKF_CKKS && c->alpha > 1;
     for (int xcd = 1; xcd >= 0; --xcd) {
         if (kklss && c->alpha == 1) {
-            cands.push_back({0, old_order, xcd, old_split, old_split_ext});
-            cands.push_back({1, 0, xcd, 0, old_split_ext});
-            cands.push_back({1, 1, xcd, 0, old_split_ext});
+            cands.push_back({0, old_order, xcd, old_split, old_split_ext, old_pre});
+            cands.push_back({1, 0, xcd, 0, old_split_ext, old_pre});
+            cands.push_back({1, 1, xcd, 0, old_split_ext, old_pre});
             if (c->logN == 14) {
                 // (half-limb external products are not tried: their extra k_intt_finish
                 // pass costs more than the occupancy gains, measured on MI355X)
-                cands.push_back({1, 0, xcd, 1, old_split_ext});
-                cands.push_back({1, 1, xcd, 1, old_split_ext});
+                cands.push_back({1, 0, xcd, 1, old_split_ext, old_pre});
+                cands.push_back({1, 1, xcd, 1, old_split_ext, old_pre});
             }
+        } else if (mkf_crt) {  // MKF CKKS with CRT digits: with and without the digit pass
+            cands.push_back({old_fused, old_order, xcd, old_split, old_split_ext, 0});
+            cands.push_back({old_fused, old_order, xcd, old_split, old_split_ext, 1});
         } else {
-            cands.push_back({old_fused, old_order, xcd, old_split, old_split_ext});
+            cands.push_back({old_fused, old_order, xcd, old_split, old_split_ext, old_pre});
         }
     }
     auto apply = [&](const Cand &cd) {
@@ -301,6 +311,7 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
         mkhip_set_tuning(c, MKHIP_TUNE_XCD_REMAP, cd.xcd);
         mkhip_set_tuning(c, MKHIP_TUNE_SPLIT_LIMB, cd.split);
         mkhip_set_tuning(c, MKHIP_TUNE_SPLIT_EXT, cd.split_ext);
+        mkhip_set_tuning(c, MKHIP_TUNE_DIGIT_PRE, cd.pre);
     };
     int best = -1, n = 0;
     double best_ms = 0;
@@ -313,7 +324,7 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
         if (s == MKHIP_OK) s = mkhip_plan_time(pl, iters, &t);
         mkhip_plan_destroy(pl);
         if (s != MKHIP_OK) {
-            apply({old_fused, old_order, old_xcd, old_split, old_split_ext});
+            apply({old_fused, old_order, old_xcd, old_split, old_split_ext, old_pre});
             return s;
         }
         if (ms && n < 16) ms[n] = t;

@@ -48,6 +48,7 @@ struct DevTables {
     int fp64;                 // FP64 transforms for moduli below 2^50 (a tuning knob)
     int split14;              // N = 2^14 one-prime decompositions over half-limb workgroups (a tuning knob)
     int split14_ext;          // N = 2^14 external products over half-limb workgroups (a tuning knob)
+    int digit_pre;            // MKF CRT digits: digit-only ModUp part once per digit (a tuning knob)
 };
 
 // N = 2^14 external products over half-limb workgroups: a plan that emits KEXT
@@ -87,6 +88,7 @@ enum KernelId {
     KSMALL_NTT = 19,     // k_small_ntt<LOGN>      small signed polynomial -> residues -> NTT (-> MForm)
     KKEYOP = 20,         // k_keyop                elementwise key / encryption combination
     KLINEAR = 21,        // k_linear               CKKS linear operations of the polynomial evaluator
+    KDIGIT_PRE = 22,     // k_digit_pre            digit-only part of the CRT ModUp, once per digit
 };
 
 // One NTT-domain row (ring modulus `mod`) of a CKKS linear operation; the constant
@@ -232,7 +234,8 @@ struct DecompDesc {
     u64 *D;                        // [k][beta][rows][N]
     u64 *xy;                       // [beta][rows][N] (MForm), may be null
     int k, level;
-    int pad[2];
+    u64 *pre;                      // [k][beta][alpha+1][N]: y_l rows and v of the CRT digits
+                                   // (k_digit_pre), or null: the rows compute them
 };
 
 // Generic batched limb NTT job.

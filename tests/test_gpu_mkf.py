@@ -93,3 +93,31 @@ def test_mkf_batched_plan_matches_oracle(setup):
     for p in bufs:
         ctx.free(p)
     ctx.unregister_mkf_group(3)
+
+
+def test_digit_pass_matches_oracle(setup):
+    """With the digit-only part of the CRT ModUp computed once per digit
+    (k_digit_pre), mul+relin and a product of operands at different levels keep
+    the oracle's words."""
+    import mkf_amd
+
+    name, ctx, ref = setup
+    k = 2
+    fold, relin = _keys(ref, k, seed=47)
+    ctx.register_mkf_group(11, fold, relin)
+    g = O.Prng(48)
+    left = np.stack([g.uniform(ref.Q, ref.N) for _ in range(1 + k)])
+    right = np.stack([g.uniform(ref.Q, ref.N) for _ in range(1 + k)])
+    scale = float(ref.Q[-1])
+    o0, o1 = ref.mkf_ckks_mulrelin(left, right, fold, relin)
+    p0, p1 = ref.mkf_ckks_mulrelin2(left, right[:, :ref.L - 1].copy(), fold, relin)
+    ctx.set_tuning(mkf_amd.TUNE_DIGIT_PRE, 1)
+    try:
+        out, level, _ = ctx.mkf_ckks_mulrelin(11, left, right, scale, scale, scale)
+        lv = ctx.mkf_mulrelin_levels(11, left, right[:, :ref.L - 1])
+    finally:
+        ctx.set_tuning(mkf_amd.TUNE_DIGIT_PRE, 0)
+    assert np.array_equal(out[0], ref.div_round_last(o0, 1, ntt=True))
+    assert np.array_equal(out[1], ref.div_round_last(o1, 1, ntt=True))
+    assert np.array_equal(lv[0], p0) and np.array_equal(lv[1], p1)
+    ctx.unregister_mkf_group(11)
