@@ -382,11 +382,14 @@ __device__ __forceinline__ void ext_rows_fp(double (&a0)[16], u64 *park1, const 
         const u64 *dr = d + i * dstride, *k0r = k0 + i * kstride, *k1r = NK > 1 ? k1 + i * kstride : nullptr;
         // products below 0.6q: reduce every fourth digit
         const bool red = (i & 3) == 3;
+        // CH values of every operand are loaded before they are used (more loads in
+        // flight per wave); CH is bounded by the registers the accumulators leave
+        constexpr int CH = NK > 1 ? 8 : 16;
 #pragma unroll
-        for (int c = 0; c < 16; c += 4) {
-            u64 x[4], y0[4], y1[4];
+        for (int c = 0; c < 16; c += CH) {
+            u64 x[CH], y0[CH], y1[CH];
 #pragma unroll
-            for (int e = 0; e < 4; e += 2) {
+            for (int e = 0; e < CH; e += 2) {
                 const int off = Qd::elem(tid, c + e);
                 const u64v2 xd = *(const gv2 *)(uniform_ptr(dr) + off), yd = *(const gv2 *)(uniform_ptr(k0r) + off);
                 x[e] = xd.x;
@@ -400,7 +403,7 @@ __device__ __forceinline__ void ext_rows_fp(double (&a0)[16], u64 *park1, const 
                 }
             }
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
+            for (int e = 0; e < CH; ++e) {
                 const double xv = fp_from_u64(x[e]);
                 a0[c + e] += fp_mulmod(fp_from_u64(y0[e]), xv, fm);
                 if (NK > 1) a1[c + e] += fp_mulmod(fp_from_u64(y1[e]), xv, fm);
