@@ -20,11 +20,15 @@ def _chain(name):
         Q = [int(q) for q in O.gen_primes(60, 13, 1)] + [int(q) for q in O.gen_primes(40, 13, 6)]
         P = [int(p) for p in O.gen_primes(60, 13, 2, exclude=Q)]
         return 13, Q, P, float(2 ** 40)
+    if name == "q40n15":  # N = 2^15 (limbs over two workgroups): one 60-bit and three 40-bit primes
+        Q = [int(q) for q in O.gen_primes(60, 15, 1)] + [int(q) for q in O.gen_primes(40, 15, 3)]
+        P = [int(p) for p in O.gen_primes(60, 15, 2, exclude=Q)]
+        return 15, Q, P, float(2 ** 40)
     logN, Q, P = config("P1")  # the reference's KKLSS / MKF parameter literal, 52-bit scale primes
     return logN, [int(q) for q in Q], [int(p) for p in P], float(2 ** 52)
 
 
-@pytest.fixture(scope="module", params=["q40", "P1"])
+@pytest.fixture(scope="module", params=["q40", "P1", "q40n15"])
 def env(request):
     import mkf_amd
 
