@@ -248,6 +248,23 @@ def run_scheme(scheme, args, rng):
                         "p99_ms": round(pct(plat, 99) * 1e3, 3)}
         res["batched_over_plain"] = round(ops_s / res["plain"]["ops_per_s"], 3)
         pplan.close()
+    if scheme == "kklss_ckks" and args.integer_leg:
+        # the same batch with every transform in 64-bit integer arithmetic (the
+        # FP64 transforms off; same words)
+        import mkf_amd
+
+        w.ctx.set_tuning(mkf_amd.TUNE_FP64, 0)
+        try:
+            iplan = w.plan(ops)
+            iel, ilat = time_plan(w.ctx, iplan, args.steps, args.warmup)
+            ist = iplan.stage_times(args.stage_iters)
+            res["integer_arithmetic"] = {
+                "ops_per_s": round(B * args.steps / iel, 2), "ms_per_step": round(iel / args.steps * 1e3, 3),
+                "p99_ms": round(pct(ilat, 99) * 1e3, 3),
+                "stages_ms": {iplan.stages[s_]: round(ist[s_], 4) for s_ in range(len(ist))}}
+            iplan.close()
+        finally:
+            w.ctx.set_tuning(mkf_amd.TUNE_FP64, 1)
     extra = (w, plan, stage_ms, names)
     return res, extra
 
@@ -699,6 +716,8 @@ def main():
     ap.add_argument("--poly-batch", type=int, default=32,
                     help="ciphertexts per launch of the polynomial-evaluation leg (0: skip)")
     ap.add_argument("--poly-steps", type=int, default=3)
+    ap.add_argument("--no-integer-leg", dest="integer_leg", action="store_false",
+                    help="skip the headline batch re-timed with integer-only transforms")
     ap.add_argument("--copy-calib-mb", type=float, default=0.0,
                     help="run a plain D2D copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -753,6 +772,7 @@ def main():
                         "ops_per_launch": B},
             "plain": res.get("plain"), "batched_over_plain": res.get("batched_over_plain"),
             "stages_ms": res["stages_ms"], "workspace_gb": res["workspace_gb"],
+            "integer_arithmetic": res.get("integer_arithmetic"),
             "roofline": roofline, "cpu_baseline": cpu,
         })
         plan.close()
