@@ -761,6 +761,9 @@ def main():
                     "valu_busy_pct": pmc.get("valu_busy_pct"),
                     "kernel": kern, "stage": names[dom], "kernel_ms": round(stage_ms[dom], 4),
                     "alg_bytes_per_launch": dom_bytes}
+        # every stage's algorithmic bytes over its measured time, as a fraction of the HBM peak
+        roofline["stages_frac"] = {n_: round(alg[n_] / (stage_ms[i_] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                   for i_, n_ in enumerate(names) if n_ in alg and stage_ms[i_] > 0}
         cpu = None
         if not args.no_cpu:
             cpu = cpu_baseline(w.logN, w.Q, w.P, k, w.keys, w.u, args.cpu_seconds)
