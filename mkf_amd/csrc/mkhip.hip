@@ -332,7 +332,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     for (int i = 0; i < 9; ++i) *bqt[i] = bfv ? (const u64 *)(base + o_bq[i]) : nullptr;
     T.bfv = bfv ? 1 : 0;
     T.xcd = 1;
-    T.fp64 = 1;
+    T.fp64 = 0;  // integer transforms unless the FP64 knob is set
     T.split14 = 0;
     T.split14_ext = 0;
     T.digit_pre = 0;

@@ -109,6 +109,11 @@ struct PlanMem {
 
 struct mkhip_plan {
     mkhip_ctx *ctx = nullptr;
+    // the context's tables and tuning knobs as they were when the plan was built:
+    // every stage launches with this copy, so a knob changed later (which may
+    // change a stage's geometry, e.g. half-limb external products that leave their
+    // last inverse stage to a finish stage) cannot reach an existing plan
+    mkhip::DevTables T;
     void *d_mem = nullptr;   // workspace
     void *d_desc = nullptr;  // descriptor tables
     size_t bytes = 0;        // workspace + descriptor bytes

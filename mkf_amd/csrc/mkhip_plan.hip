@@ -10,14 +10,14 @@ namespace mkhip {
 
 hipError_t plan_stage(mkhip_plan *pl, int s, hipStream_t st) {
     const PlanStage &S = pl->stages[s];
-    mkhip_ctx *c = pl->ctx;
-    if (!st) st = c->stream;
-    if (S.what >= KMAC2 && S.what <= KRESCALE_NTT) return launch_mkf(S.what, c->T, S.descs, S.n, S.a, S.b, st);
-    if (S.what >= KBFV_LIFT && S.what <= KBFV_QUANTIZE) return launch_bfv(S.what, c->T, S.descs, S.n, S.a, S.b, st);
-    if (S.what == KDECOMP_MAC) return launch_fused(S.what, c->T, S.descs, S.n, S.a, S.b, st);
-    if (S.what == KAUTO) return launch_galois(S.what, c->T, S.descs, S.n, st);
-    if (S.what == KLINEAR) return launch_linear(c->T, S.descs, S.n, st);
-    return launch_kernel(S.what, c->T, S.descs, S.n, S.a, S.b, st, S.c);
+    if (!st) st = pl->ctx->stream;
+    const DevTables &T = pl->T;  // knobs as built
+    if (S.what >= KMAC2 && S.what <= KRESCALE_NTT) return launch_mkf(S.what, T, S.descs, S.n, S.a, S.b, st);
+    if (S.what >= KBFV_LIFT && S.what <= KBFV_QUANTIZE) return launch_bfv(S.what, T, S.descs, S.n, S.a, S.b, st);
+    if (S.what == KDECOMP_MAC) return launch_fused(S.what, T, S.descs, S.n, S.a, S.b, st);
+    if (S.what == KAUTO) return launch_galois(S.what, T, S.descs, S.n, st);
+    if (S.what == KLINEAR) return launch_linear(T, S.descs, S.n, st);
+    return launch_kernel(S.what, T, S.descs, S.n, S.a, S.b, st, S.c);
 }
 
 int plan_launch_raw(mkhip_plan *pl, hipStream_t st) {
@@ -29,6 +29,7 @@ int plan_create(mkhip_ctx *c, size_t ws_bytes, mkhip_plan **out, const PlanMem &
     *out = nullptr;
     mkhip_plan *pl = new mkhip_plan();
     pl->ctx = c;
+    pl->T = c->T;
     pl->mem = pm;
     pl->bytes = ws_bytes;
     if (pm.query) {

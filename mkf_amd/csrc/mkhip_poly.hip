@@ -575,6 +575,7 @@ int Prog::build(int use_graph, mkhip_plan **out) {
     const size_t ws_off = cv.take(std::max<size_t>(ws_need, 256));
     mkhip_plan *pl = new mkhip_plan();
     pl->ctx = c;
+    pl->T = c->T;
     pl->bytes = cv.off;
     hipError_t e = hipMalloc(&pl->d_mem, std::max<size_t>(cv.off, 256));
     if (e != hipSuccess) {

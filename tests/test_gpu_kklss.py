@@ -200,37 +200,42 @@ def test_unregistered_party_is_not_served_from_the_cached_plan(setup):
 
 
 def test_fp64_and_integer_arithmetic_agree(setup):
-    """Limbs whose modulus is below 2^50 run the FP64 transforms; extreme inputs
-    (all q-1, alternating q-1 / 0, a single q-1) match the oracle, and a whole
-    mul+relin gives the same words with the integer arithmetic forced."""
+    """The opt-in FP64 transforms (limbs whose modulus is below 2^50) and the
+    default integer transforms give the same words: extreme inputs (all q-1,
+    alternating q-1 / 0, a single q-1) match the oracle in both, and a whole
+    mul+relin gives the same words either way."""
     name, ctx, ref = setup
     mk = _mk()
     N, Q = ref.N, ref.Q
+    assert ctx.get_tuning(mk.TUNE_FP64) == 0  # integer arithmetic by default
     x = np.stack([np.full(N, q - 1, np.uint64) if i % 3 == 0 else
                   np.where(np.arange(N) % 2 == 0, q - 1, 0).astype(np.uint64) if i % 3 == 1 else
                   np.where(np.arange(N) == N - 1, q - 1, 0).astype(np.uint64) for i, q in enumerate(Q)])
     rows = list(range(len(Q)))
-    fwd = ctx.ntt(x, rows)
-    assert np.array_equal(fwd, np.stack([ref.ntt(x[i], i) for i in rows]))
-    assert np.array_equal(ctx.ntt(x, rows, inverse=True), np.stack([ref.ntt(x[i], i, inverse=True) for i in rows]))
-    assert np.array_equal(ctx.ntt(fwd, rows, inverse=True), x)
+    exp_f = np.stack([ref.ntt(x[i], i) for i in rows])
+    exp_i = np.stack([ref.ntt(x[i], i, inverse=True) for i in rows])
     _keys(ctx, ref, [0, 1], seed=43)
     g = O.Prng(44)
     op0 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
     op1 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
     scale = float(ref.Q[-1])
-    a = ctx.kklss_ckks_mulrelin(op0, [0, 1], op1, [0, 1], scale, scale, scale)[0]
-    ctx.set_tuning(mk.TUNE_FP64, 0)
-    try:
-        b = ctx.kklss_ckks_mulrelin(op0, [0, 1], op1, [0, 1], scale, scale, scale)[0]
-    finally:
-        ctx.set_tuning(mk.TUNE_FP64, 1)
-    assert np.array_equal(a, b)
+    res = []
+    for fp in (0, 1):
+        ctx.set_tuning(mk.TUNE_FP64, fp)
+        try:
+            fwd = ctx.ntt(x, rows)
+            assert np.array_equal(fwd, exp_f), fp
+            assert np.array_equal(ctx.ntt(x, rows, inverse=True), exp_i), fp
+            assert np.array_equal(ctx.ntt(fwd, rows, inverse=True), x), fp
+            res.append(ctx.kklss_ckks_mulrelin(op0, [0, 1], op1, [0, 1], scale, scale, scale)[0])
+        finally:
+            ctx.set_tuning(mk.TUNE_FP64, 0)
+    assert np.array_equal(res[0], res[1])
 
 
 def test_split_limb_decomposition_agrees(setup):
     """The N = 2^14 fused decomposition over half-limb workgroups gives the words of
-    the one-workgroup form, in FP64 and in integer arithmetic (other ring sizes
+    the one-workgroup form, in integer and in FP64 arithmetic (other ring sizes
     ignore the knob)."""
     name, ctx, ref = setup
     mk = _mk()
@@ -243,12 +248,54 @@ def test_split_limb_decomposition_agrees(setup):
     ctx.set_tuning(mk.TUNE_SPLIT_LIMB, 1)
     try:
         b = ctx.kklss_ckks_mulrelin(op0, [0, 1, 2], op1, [0, 2], scale, scale, scale)[0]
-        ctx.set_tuning(mk.TUNE_FP64, 0)
+        ctx.set_tuning(mk.TUNE_FP64, 1)
         c = ctx.kklss_ckks_mulrelin(op0, [0, 1, 2], op1, [0, 2], scale, scale, scale)[0]
     finally:
         ctx.set_tuning(mk.TUNE_SPLIT_LIMB, 0)
-        ctx.set_tuning(mk.TUNE_FP64, 1)
+        ctx.set_tuning(mk.TUNE_FP64, 0)
     assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_plan_keeps_the_knobs_it_was_built_with(setup):
+    """A plan launches with the knob settings of its build: flipping the half-limb
+    external products (whose plans carry an extra finish stage) or the arithmetic
+    after the build changes nothing in an eager plan, a graph plan or its stage
+    timing launches."""
+    name, ctx, ref = setup
+    mk = _mk()
+    keys, u = _keys(ctx, ref, [0, 1], seed=57)
+    g = O.Prng(58)
+    N, nl = ref.N, ref.L
+    op0 = np.stack([g.uniform(ref.Q, N) for _ in range(3)])
+    op1 = np.stack([g.uniform(ref.Q, N) for _ in range(3)])
+    exp = _oracle_mulrelin(ref, op0, [0, 1], op1, [0, 1], keys, u, 1)[0]
+    d0, d1, dout = ctx.alloc(op0.nbytes), ctx.alloc(op1.nbytes), ctx.alloc(3 * (nl - 1) * N * 8)
+    ctx.h2d(d0, op0)
+    ctx.h2d(d1, op1)
+    ops = [dict(op0=d0, op1=d1, out=dout, ids0=[0, 1], ids1=[0, 1], level=nl - 1, nres=1)]
+    try:
+        for built, flipped in ((0, 1), (1, 0)):
+            ctx.set_tuning(mk.TUNE_SPLIT_EXT, built)
+            ctx.set_tuning(mk.TUNE_SPLIT_LIMB, built)
+            plans = [mk.Plan(ctx, ops, use_graph=False), mk.Plan(ctx, ops, use_graph=True)]
+            ctx.set_tuning(mk.TUNE_SPLIT_EXT, flipped)
+            ctx.set_tuning(mk.TUNE_SPLIT_LIMB, flipped)
+            ctx.set_tuning(mk.TUNE_FP64, flipped)
+            for pl in plans:
+                for launch in (pl.launch, lambda: pl.stage_times(1)):
+                    ctx.h2d(dout, np.zeros((3, nl - 1, N), np.uint64))
+                    launch()
+                    ctx.sync()
+                    got = ctx.d2h(np.zeros((3, nl - 1, N), np.uint64), dout)
+                    assert np.array_equal(got, exp), (built, pl)
+                pl.close()
+            ctx.set_tuning(mk.TUNE_FP64, 0)
+    finally:
+        ctx.set_tuning(mk.TUNE_SPLIT_EXT, 0)
+        ctx.set_tuning(mk.TUNE_SPLIT_LIMB, 0)
+        ctx.set_tuning(mk.TUNE_FP64, 0)
+        for p in (d0, d1, dout):
+            ctx.free(p)
 
 
 def test_split_external_products_agree(setup):
