@@ -88,26 +88,81 @@ def pct(xs, p):
 
 
 def stage_alg_bytes(B, k, L, K, beta, N, W=8):
-    """Algorithmic bytes per launch of each KKLSS MK-CKKS plan stage (each datum once).
+    """Algorithmic HBM bytes per launch of each KKLSS MK-CKKS plan stage, every
+    datum counted once: an op's input limbs once per (op, side, party), the
+    parties' keys once per launch (the batch shares them), the x / y keys once
+    per op (written once, read once), every output once.  In limbs of N W bytes:
 
-    Per-unit figures: a limb transform (NTT / INTT) moves 2 N W bytes (read +
-    write); a key-switch MAC moves (digit words + key words + output words) W;
-    a ModDown reads (L + K) rows and writes L rows.  Keys shared by the batch
-    count once per launch."""
+      decompose_mac   inputs 2kL B + keys (d, b) 2k beta rows + digits written
+                      2k beta rows B + x / y written 2 beta rows B
+      decompose       inputs + digits;  mac_xy: digits + keys + x / y
+      tensor_ntt      the two c0s read and written (4 L B)
+      tensor          c0s + the parties' diagonal digit rows (2 L + 2k L) B
+                      read, (1 + k) L B written
+      ext_xy          digits of both sides 2k beta rows B + x / y 2 beta rows B
+                      read, 2k rows B accumulators written
+      moddown_xy      2k rows B read + k L B accumulation targets read,
+                      2k L B written
+      decompose_t     t_i k L B read, digits k beta rows B written
+      ext_vu          digits k beta rows B + keys (v_i, u) (k + 1) beta rows
+                      read, 2k rows B written
+      moddown_rescale 2k rows B + (1 + k) L B read, (1 + k)(L - 1) B written"""
     rows = L + K
     limb = N * W
+    dig = 2 * k * beta * rows
     return {
-        "decompose": B * 2 * k * beta * rows * 2 * limb,                        # limb NTTs
-        "mac_xy": (B * 2 * k * beta * rows + 2 * k * beta * rows + B * 2 * beta * rows) * limb,
-        "decompose_mac": (B * 2 * k * beta * rows * 2 + 2 * k * beta * rows + B * 2 * beta * rows) * limb,
-        "tensor_ntt": B * 2 * L * 2 * limb,                 # NTT of the c0s (the other inputs are digit rows)
-        "tensor": B * (1 + k) * L * 2 * limb,                                   # tensor + INTT
-        "ext_xy": B * 2 * k * (2 * beta * rows + rows) * limb,                  # D + x|y + out
-        "moddown_xy": B * 2 * k * (rows + L) * limb,
-        "decompose_t": B * k * beta * rows * 2 * limb,
-        "ext_vu": (B * k * (beta * rows + 2 * rows) + (k + 1) * beta * rows) * limb,  # keys once
-        "moddown_rescale": B * (2 * k * (rows + L) + (1 + k) * (L + L - 1)) * limb,
+        "decompose_mac": (2 * k * L * B + dig + dig * B + 2 * beta * rows * B) * limb,
+        "decompose": (2 * k * L * B + dig * B) * limb,
+        "mac_xy": (dig * B + dig + 2 * beta * rows * B) * limb,
+        "tensor_ntt": 4 * L * B * limb,
+        "tensor": ((2 + 2 * k) * L + (1 + k) * L) * B * limb,
+        "ext_xy": (dig + 2 * beta * rows + 2 * k * rows) * B * limb,
+        "moddown_xy": (2 * k * rows + k * L + 2 * k * L) * B * limb,
+        "decompose_t": (k * L + k * beta * rows) * B * limb,
+        "ext_vu": ((k * beta * rows + 2 * k * rows) * B + (k + 1) * beta * rows) * limb,
+        "moddown_rescale": (2 * k * rows + (1 + k) * L + (1 + k) * (L - 1)) * B * limb,
     }
+
+
+def stage_valu_work(B, k, L, K, beta, N):
+    """Modular-arithmetic work per launch of the transform-bearing KKLSS stages:
+    (forward butterflies on Q rows -- the unreduced form --, butterflies of the
+    reduced form -- forward on P rows, every inverse butterfly --, Montgomery
+    products of the MACs).  A limb transform is N/2 log2(N) butterflies."""
+    rows = L + K
+    nb = N // 2 * (N.bit_length() - 1)
+    dig = 2 * k * beta
+    return {
+        "decompose_mac": (dig * L * nb * B, dig * K * nb * B, dig * rows * N * B),
+        "decompose": (dig * L * nb * B, dig * K * nb * B, 0),
+        "mac_xy": (0, 0, dig * rows * N * B),
+        "tensor_ntt": (2 * L * nb * B, 0, 0),
+        "tensor": (0, (1 + k) * L * nb * B, (1 + 2 * k) * L * N * B),
+        "ext_xy": (0, 2 * k * rows * nb * B, dig * rows * N * B),
+        "decompose_t": (k * beta * L * nb * B, k * beta * K * nb * B, 0),
+        "ext_vu": (0, 2 * k * rows * nb * B, dig * rows * N * B),
+    }
+
+
+def measure_peaks(iters=5):
+    """This GPU's peaks, measured: device copy bandwidth over 1 GiB buffers and the
+    in-register issue rates of the integer primitives (mkhip_probe_peak)."""
+    import mkf_amd
+
+    ctx = mkf_amd.Context(13, ntt_primes(50, 2), ntt_primes(60, 1), gamma=1)
+    try:
+        lib_copy = ctx.probe_peak(mkf_amd.PROBE_COPY, 1 << 30, iters) / 1e9
+        stream_copy = ctx.probe_peak(mkf_amd.PROBE_COPY_STREAM, 1 << 30, iters) / 1e9
+        # the measured HBM peak: the faster of the library's copy and the streaming copy
+        out = {"copy_gbs": round(max(lib_copy, stream_copy), 1), "copy_kernel_gbs": round(lib_copy, 1),
+               "copy_stream_gbs": round(stream_copy, 1)}
+        for name, kind in (("mul_lo_u32_per_s", mkf_amd.PROBE_MUL32), ("mad_u64_u32_per_s", mkf_amd.PROBE_MAD64),
+                           ("bfly_lazy_per_s", mkf_amd.PROBE_BFLY_LAZY), ("bfly_reduced_per_s", mkf_amd.PROBE_BFLY_RED),
+                           ("mred_add_per_s", mkf_amd.PROBE_MRED)):
+            out[name] = float("%.4g" % ctx.probe_peak(kind, 4096, iters))
+    finally:
+        ctx.close()
+    return out
 
 
 def op_alg_bytes(scheme, B, k, L, K, beta, N, W=8):
@@ -248,25 +303,48 @@ def run_scheme(scheme, args, rng):
                         "p99_ms": round(pct(plat, 99) * 1e3, 3)}
         res["batched_over_plain"] = round(ops_s / res["plain"]["ops_per_s"], 3)
         pplan.close()
-    if scheme == "kklss_ckks" and args.integer_leg:
-        # the same batch with every transform in 64-bit integer arithmetic (the
-        # FP64 transforms off; same words)
+    if scheme == "kklss_ckks" and args.fp64_leg:
+        # the same batch with the opt-in FP64 transforms for moduli below 2^50 (the
+        # same words; the north-star arithmetic is the integer form above)
         import mkf_amd
 
-        w.ctx.set_tuning(mkf_amd.TUNE_FP64, 0)
+        w.ctx.set_tuning(mkf_amd.TUNE_FP64, 1)
         try:
-            iplan = w.plan(ops)
-            iel, ilat = time_plan(w.ctx, iplan, args.steps, args.warmup)
-            ist = iplan.stage_times(args.stage_iters)
-            res["integer_arithmetic"] = {
-                "ops_per_s": round(B * args.steps / iel, 2), "ms_per_step": round(iel / args.steps * 1e3, 3),
-                "p99_ms": round(pct(ilat, 99) * 1e3, 3),
-                "stages_ms": {iplan.stages[s_]: round(ist[s_], 4) for s_ in range(len(ist))}}
-            iplan.close()
+            fplan = w.plan(ops)
+            fel, flat = time_plan(w.ctx, fplan, args.steps, args.warmup)
+            fst = fplan.stage_times(args.stage_iters)
+            res["fp64_arithmetic"] = {
+                "ops_per_s": round(B * args.steps / fel, 2), "ms_per_step": round(fel / args.steps * 1e3, 3),
+                "p99_ms": round(pct(flat, 99) * 1e3, 3),
+                "stages_ms": {fplan.stages[s_]: round(fst[s_], 4) for s_ in range(len(fst))}}
+            fplan.close()
         finally:
-            w.ctx.set_tuning(mkf_amd.TUNE_FP64, 1)
+            w.ctx.set_tuning(mkf_amd.TUNE_FP64, 0)
+    if scheme == "kklss_ckks" and args.batch_sweep:
+        res["batch_sweep"] = batch_sweep(w, ops, args, rng, res.get("plain"))
     extra = (w, plan, stage_ms, names)
     return res, extra
+
+
+def batch_sweep(w, ops, args, rng, plain):
+    """The headline plan at smaller batches (the first b ops of the batch, same
+    knobs): throughput and latency per launch, and the highest-throughput batch
+    whose p99 launch latency stays under 10 ms, against the plain per-op path."""
+    rows = []
+    for b in args.batch_sweep:
+        if b >= len(ops):
+            continue
+        pl = w.plan(ops[:b])
+        el, lat = time_plan(w.ctx, pl, args.steps, args.warmup)
+        pl.close()
+        rows.append({"batch": b, "ops_per_s": round(b * args.steps / el, 2), "p50_ms": round(pct(lat, 50) * 1e3, 3),
+                     "p99_ms": round(pct(lat, 99) * 1e3, 3)})
+    ok = [r for r in rows if r["p99_ms"] < 10.0]
+    best = max(ok, key=lambda r: r["ops_per_s"]) if ok else None
+    out = {"sizes": rows, "best_batch_under_p99_10ms": best}
+    if best and plain and plain.get("ops_per_s"):
+        out["best_over_plain"] = round(best["ops_per_s"] / plain["ops_per_s"], 3)
+    return out
 
 
 ROT_STEPS = (1, 2, 4, 8)
@@ -626,28 +704,78 @@ def run_serving_headline(args, rng):
     return _serve_sweep(args, rng, [("kklss", 14, 8, 2)], 4, 4, 4, args.serve_headline_load)
 
 
-def cpu_baseline(logN, Q, P, k, keys, u, budget_s):
-    """The CPU oracle (C restatement of the reference path, 1 thread) on a bounded sample."""
+def _cpu_threads(requested):
+    if requested > 0:
+        return requested
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(logN, Q, P, k, keys, u, threads, warm=5, timed=10):
+    """The CPU oracle (C restatement of the reference path) under the reference's
+    timing policy (profile_exact_test.go:45-61: 5 warm-up runs, then 10 timed
+    runs): one thread, and `threads` threads running independent ops at once
+    (each thread 5 warm-up + 10 timed ops; the timed region starts when every
+    thread has finished its warm-up).  The C calls release the interpreter lock."""
+    import threading
+
     import oracle as O
 
-    ref = O.Ctx(logN, Q, P, gamma=2)
-    rng = np.random.default_rng(1234)
+    N = 1 << logN
     ids = list(range(k))
-    n, t0 = 0, time.perf_counter()
-    while True:
-        op0 = np.stack([uniform(rng, Q, 1 << logN) for _ in range(1 + k)])
-        op1 = np.stack([uniform(rng, Q, 1 << logN) for _ in range(1 + k)])
-        out, _ = ref.kklss_mulrelin(op0, ids, op1, ids, [kk[0] for kk in keys], [kk[1] for kk in keys],
-                                    [kk[2] for kk in keys], u)
-        for i in range(out.shape[0]):
-            ref.div_round_last(out[i], 1)
-        n += 1
+    kb, kd, kv = [kk[0] for kk in keys], [kk[1] for kk in keys], [kk[2] for kk in keys]
+
+    def leg(nthr):
+        refs = [O.Ctx(logN, Q, P, gamma=2) for _ in range(nthr)]
+        inputs = []
+        for t in range(nthr):
+            r = np.random.default_rng(1234 + t)
+            inputs.append([(np.stack([uniform(r, Q, N) for _ in range(1 + k)]),
+                            np.stack([uniform(r, Q, N) for _ in range(1 + k)])) for _ in range(2)])
+        bar = threading.Barrier(nthr + 1)
+        done = threading.Barrier(nthr + 1)
+
+        def one(t, i):
+            op0, op1 = inputs[t][i % 2]
+            out, _ = refs[t].kklss_mulrelin(op0, ids, op1, ids, kb, kd, kv, u)
+            for c in range(out.shape[0]):
+                refs[t].div_round_last(out[c], 1)
+
+        def worker(t):
+            for i in range(warm):
+                one(t, i)
+            bar.wait()
+            for i in range(timed):
+                one(t, i)
+            done.wait()
+
+        th = [threading.Thread(target=worker, args=(t,)) for t in range(nthr)]
+        for x in th:
+            x.start()
+        bar.wait()
+        t0 = time.perf_counter()
+        done.wait()
         el = time.perf_counter() - t0
-        if el >= budget_s or n >= 64:
-            break
-    return {"value": round(n / el, 3), "unit": "ops/s", "cores": 1, "kind": "port",
-            "sample": "%d KKLSS MK-CKKS mul+relin+rescale ops (N=2^14, L=8, K=2, k=4) with the C oracle, "
-                      "single thread, %.1f s" % (n, el)}
+        for x in th:
+            x.join()
+        return nthr * timed / el, el
+
+    v1, el1 = leg(1)
+    res = {"value": round(v1, 3), "unit": "ops/s", "cores": 1, "kind": "port",
+           "sample": "KKLSS MK-CKKS mul+relin+rescale (N=2^14, L=8, K=2, k=4) with the C oracle, one thread: "
+                     "%d warm-up + %d timed ops (%.1f s timed)" % (warm, timed, el1)}
+    if threads > 1:
+        vn, eln = leg(threads)
+        res["multi_thread"] = {"value": round(vn, 3), "unit": "ops/s", "cores": threads,
+                               "sample": "%d threads, each %d warm-up + %d timed independent ops (%.1f s timed)"
+                                         % (threads, warm, timed, eln)}
+    res["reference_published"] = {"value_ms_per_op": 536.0, "note": "the reference's Go KKLSS-CKKS mul+relin, "
+                                  "P1 parameters, 16 parties, Apple arm64, one thread (different hardware and "
+                                  "parameters)"}
+    return res
 
 
 def copy_calibration(mb):
@@ -686,7 +814,6 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--bfv-batch", type=int, default=32)
     ap.add_argument("--plain-ops", type=int, default=32)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-iters", type=int, default=3)
     ap.add_argument("--schemes", default=",".join(ALL_SCHEMES))
@@ -716,8 +843,14 @@ def main():
     ap.add_argument("--poly-batch", type=int, default=32,
                     help="ciphertexts per launch of the polynomial-evaluation leg (0: skip)")
     ap.add_argument("--poly-steps", type=int, default=3)
-    ap.add_argument("--no-integer-leg", dest="integer_leg", action="store_false",
-                    help="skip the headline batch re-timed with integer-only transforms")
+    ap.add_argument("--no-fp64-leg", dest="fp64_leg", action="store_false",
+                    help="skip the headline batch re-timed with the opt-in FP64 transforms")
+    ap.add_argument("--batch-sweep", type=lambda x: [int(v) for v in x.split(",") if v], default=[8, 16, 24, 32, 48],
+                    help="headline batch sizes timed for the best batch under p99 < 10 ms ('' skips)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the op-parallel CPU leg (0: the cores this process may use, at most 16)")
+    ap.add_argument("--no-peaks", dest="peaks", action="store_false",
+                    help="skip the measured copy bandwidth and integer issue rates")
     ap.add_argument("--copy-calib-mb", type=float, default=0.0,
                     help="run a plain D2D copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -731,6 +864,9 @@ def main():
             ap.error("unknown scheme %s" % s)
 
     calib = copy_calibration(args.copy_calib_mb) if args.copy_calib_mb > 0 else None
+    peaks = measure_peaks() if args.peaks else None
+    if peaks:
+        print("# peaks: %s" % json.dumps(peaks), file=sys.stderr, flush=True)
     rng = np.random.default_rng(0)
     results, head = {}, None
     for s in schemes:
@@ -751,6 +887,7 @@ def main():
         res, (w, plan, stage_ms, names) = head
         B, k, L, K, beta, N = w.B, w.k, w.L, w.K, w.beta, w.N
         alg = stage_alg_bytes(B, k, L, K, beta, N)
+        work = stage_valu_work(B, k, L, K, beta, N)
         dom = max(range(len(stage_ms)), key=lambda s: stage_ms[s])
         dom_bytes = alg[names[dom]]
         achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
@@ -758,15 +895,39 @@ def main():
         pmc = load_pmc_summary(kern)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc.get("bytes_per_launch"),
-                    "valu_busy_pct": pmc.get("valu_busy_pct"),
                     "kernel": kern, "stage": names[dom], "kernel_ms": round(stage_ms[dom], 4),
                     "alg_bytes_per_launch": dom_bytes}
+        if pmc.get("bytes_per_launch"):
+            roofline["traffic_over_alg"] = round(pmc["bytes_per_launch"] / dom_bytes, 3)
+            roofline["traffic_kernel_ms"] = pmc.get("kernel_ms")
+        if pmc.get("valu_busy_pct") is not None:
+            roofline["valu_busy_pct"] = pmc["valu_busy_pct"]
         # every stage's algorithmic bytes over its measured time, as a fraction of the HBM peak
         roofline["stages_frac"] = {n_: round(alg[n_] / (stage_ms[i_] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
                                    for i_, n_ in enumerate(names) if n_ in alg and stage_ms[i_] > 0}
+        if peaks:
+            roofline["measured_copy_gbs"] = peaks["copy_gbs"]
+            roofline["frac_of_measured_copy"] = round(achieved / peaks["copy_gbs"], 4)
+
+            def valu_frac(stage, ms):
+                lz, rd, mr = work[stage]
+                t_min = lz / peaks["bfly_lazy_per_s"] + rd / peaks["bfly_reduced_per_s"] + mr / peaks["mred_add_per_s"]
+                return t_min / (ms * 1e-3)
+
+            # the integer-VALU roofline: the stage's butterflies and Montgomery products
+            # at the measured in-register rates of the same primitives, over its time
+            if names[dom] in work:
+                lz, rd, mr = work[names[dom]]
+                roofline["valu"] = {
+                    "bound": "int32-valu", "butterflies_unreduced": lz, "butterflies_reduced": rd,
+                    "mred_products": mr, "peak_bfly_unreduced_per_s": peaks["bfly_lazy_per_s"],
+                    "peak_bfly_reduced_per_s": peaks["bfly_reduced_per_s"], "peak_mred_per_s": peaks["mred_add_per_s"],
+                    "frac": round(valu_frac(names[dom], stage_ms[dom]), 4)}
+            roofline["stages_valu_frac"] = {n_: round(valu_frac(n_, stage_ms[i_]), 4)
+                                            for i_, n_ in enumerate(names) if n_ in work and stage_ms[i_] > 0}
         cpu = None
         if not args.no_cpu:
-            cpu = cpu_baseline(w.logN, w.Q, w.P, k, w.keys, w.u, args.cpu_seconds)
+            cpu = cpu_baseline(w.logN, w.Q, w.P, k, w.keys, w.u, _cpu_threads(args.cpu_threads))
         line.update({
             "value": res["ops_per_s"], "unit": "ops/s", "ms_per_step": res["ms_per_step"],
             "config": {"workload": "kklss-mkckks-mulrelin-rescale batched", "logN": w.logN, "L": L, "K": K,
@@ -775,8 +936,8 @@ def main():
                         "ops_per_launch": B},
             "plain": res.get("plain"), "batched_over_plain": res.get("batched_over_plain"),
             "stages_ms": res["stages_ms"], "workspace_gb": res["workspace_gb"],
-            "integer_arithmetic": res.get("integer_arithmetic"),
-            "roofline": roofline, "cpu_baseline": cpu,
+            "fp64_arithmetic": res.get("fp64_arithmetic"), "batch_sweep": res.get("batch_sweep"),
+            "peaks": peaks, "roofline": roofline, "cpu_baseline": cpu,
         })
         plan.close()
         w.close()

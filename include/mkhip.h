@@ -114,6 +114,26 @@ int mkhip_d2d(mkhip_ctx *ctx, void *dst, const void *src, size_t bytes);
 int mkhip_sync(mkhip_ctx *ctx);
 
 /*
+ * Peak-rate probes of this GPU (no counterpart in the reference: measurement).
+ * Runs `iters` timed launches (HIP events on the context's stream) of a probe and
+ * writes its rate: MKHIP_PROBE_COPY bytes read + written per second by the
+ * library's device copy over two buffers of `size` bytes (allocated and freed by
+ * the call, size >= 1 MiB), MKHIP_PROBE_COPY_STREAM the same with a streaming
+ * copy (several non-temporal 16-byte loads in flight a lane); the others count operations per second of work held
+ * in registers, `size` steps per lane (0: 4096) -- v_mul_lo_u32, v_mad_u64_u32,
+ * forward NTT butterflies (unreduced form of moduli below 2^57 / reduced form of
+ * moduli up to 2^61) and Montgomery products with a modular add.
+ */
+#define MKHIP_PROBE_COPY 0
+#define MKHIP_PROBE_MUL32 1
+#define MKHIP_PROBE_MAD64 2
+#define MKHIP_PROBE_BFLY_LAZY 3
+#define MKHIP_PROBE_BFLY_RED 4
+#define MKHIP_PROBE_MRED 5
+#define MKHIP_PROBE_COPY_STREAM 6
+int mkhip_probe_peak(mkhip_ctx *ctx, int kind, size_t size, int iters, double *rate);
+
+/*
  * Batched KKLSS MK-CKKS mul + relin over device-resident operands.  A plan fixes
  * the batch shape (ops, parties, levels) and owns the descriptors and the
  * workspace; launching it replays one hipGraph of the whole batch.
@@ -389,8 +409,9 @@ int mkhip_encrypt(mkhip_ctx *ctx, const uint64_t *pk0, const uint64_t *pk1, int 
  *   MKHIP_TUNE_XCD_REMAP     workgroups ordered XCD-contiguously (1) or as issued (0)
  *   MKHIP_TUNE_DECOMP_ORDER  fused decomposition grid order: ops sharing a key row
  *                            adjacent (0) or rows sharing an input limb adjacent (1)
- *   MKHIP_TUNE_FP64          transforms of limbs whose modulus is below 2^50 in FP64
- *                            arithmetic (1, default) or in 64-bit integer arithmetic (0)
+ *   MKHIP_TUNE_FP64          transforms of limbs whose modulus is below 2^50 in 64-bit
+ *                            integer arithmetic (0, default) or in exact FP64
+ *                            two-product arithmetic (1, opt-in; the same words)
  *   MKHIP_TUNE_SPLIT_LIMB    N = 2^14 one-prime digit decompositions (fused and
  *                            plain): each limb over two workgroups of half the LDS (1)
  *                            or one workgroup (0)
@@ -400,7 +421,8 @@ int mkhip_encrypt(mkhip_ctx *ctx, const uint64_t *pk0, const uint64_t *pk1, int 
  *   MKHIP_TUNE_DIGIT_PRE     MKF CKKS plans with CRT digits (alpha >= 2): the digit-only
  *                            part of the ModUp (y_l, the overflow estimate) computed once
  *                            per digit by k_digit_pre (1) or by every target row (0)
- * A knob applies to plans built after it is set.  mkhip_autotune plans and times
+ * A knob applies to plans built after it is set: a plan keeps the settings it was
+ * built with for every later launch, graph capture and stage timing.  mkhip_autotune plans and times
  * every applicable combination on the caller's batch (ops: mkhip_kklss_op[] for the
  * KKLSS schemes, mkhip_mkf_op[] for MKF; their outputs are overwritten with the
  * results), keeps the fastest, and writes up to 16 candidate times (ms per launch).

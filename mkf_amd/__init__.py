@@ -48,11 +48,13 @@ EXPORTS = [
     "mkhip_gen_rotation_key", "mkhip_gen_conjugation_key", "mkhip_encrypt",
     "mkhip_plan_mkf_poly", "mkhip_mkf_poly_query", "mkhip_mkf_ckks_evaluate_poly", "mkhip_mkf_ckks_add",
     "mkhip_mkf_ckks_add_const", "mkhip_mkf_ckks_mult_by_const", "mkhip_mkf_ckks_mult_by_gaussian_integer_and_add",
-    "mkhip_mkf_ckks_rescale", "mkhip_mkf_ckks_mulrelin_levels",
+    "mkhip_mkf_ckks_rescale", "mkhip_mkf_ckks_mulrelin_levels", "mkhip_probe_peak",
 ]
 BASIS_STANDARD, BASIS_CHEBYSHEV = 0, 1
 TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64, TUNE_SPLIT_LIMB, TUNE_SPLIT_EXT, TUNE_DIGIT_PRE = \
     1, 2, 3, 4, 5, 6, 7
+PROBE_COPY, PROBE_MUL32, PROBE_MAD64, PROBE_BFLY_LAZY, PROBE_BFLY_RED, PROBE_MRED, PROBE_COPY_STREAM = \
+    0, 1, 2, 3, 4, 5, 6
 SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3, "rotate": 4, "mkf_rotate": 5}
 MAX_ROTATIONS = 16
 CONJUGATE = -2
@@ -196,6 +198,7 @@ def _declare(L):
     L.mkhip_h2d.argtypes = [V, V, V, S]
     L.mkhip_d2h.argtypes = [V, V, V, S]
     L.mkhip_d2d.argtypes = [V, V, V, S]
+    L.mkhip_probe_peak.argtypes = [V, I, S, I, dblp]
     L.mkhip_sync.argtypes = [V]
     L.mkhip_plan_kklss_ckks.argtypes = [V, I, ctypes.POINTER(KKLSSOp), I, ctypes.POINTER(V)]
     L.mkhip_plan_launch.argtypes = [V]
@@ -643,6 +646,13 @@ class Context:
         check(lib().mkhip_mkf_ckks_mulrelin_levels(self._h, group, _p(left), left.shape[1] - 1, _p(right),
                                                    right.shape[1] - 1, _p(out)), "mkhip_mkf_ckks_mulrelin_levels")
         return out
+
+    def probe_peak(self, kind, size=0, iters=5):
+        """Measured peak rate of this GPU (mkhip_probe_peak): bytes/s for PROBE_COPY
+        (two buffers of `size` bytes), operations/s for the register probes."""
+        r = ctypes.c_double()
+        check(lib().mkhip_probe_peak(self._h, kind, int(size), int(iters), ctypes.byref(r)), "mkhip_probe_peak")
+        return r.value
 
     # ---- tuning ----
     def set_tuning(self, knob, value):
