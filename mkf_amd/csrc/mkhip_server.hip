@@ -221,30 +221,39 @@ bool launch_batch(mkhip_server *S, Class &C, Slot &sl) {
         std::lock_guard<std::mutex> g(S->mu);
         largest = S->pool.largest();
     }
-    // the longest FIFO prefix whose block fits the largest free extent
+    // the longest FIFO prefix whose block fits the largest free extent; a prefix
+    // that a request cannot be planned in (say, a party unregistered after the
+    // enqueue) is shortened past it, so that only that request fails
     size_t n = cand.size(), io = 0, ws = 0, desc = 0;
+    int front_status = MKHIP_OK;
     while (n > 0) {
-        if (batch_bytes(C, cand, n, &io, &ws, &desc) != MKHIP_OK) {
-            n = 0;
-            break;
+        const int qs = batch_bytes(C, cand, n, &io, &ws, &desc);
+        if (qs != MKHIP_OK) {
+            if (n == 1) {
+                front_status = qs;
+                n = 0;
+                break;
+            }
+            --n;
+            continue;
         }
         if (Pool::align(io) + Pool::align(ws) + Pool::align(desc) <= largest) break;
         n = n > 8 ? n * 3 / 4 : n - 1;
     }
     if (n == 0) {
-        // a request that cannot fit even alone in an idle pool can never run
+        // the oldest request fails alone when it cannot be planned, or when its
+        // block does not fit even the whole pool with nothing in flight
         bool idle = true;
         for (const Slot &o : S->slots) idle = idle && !o.busy;
-        if (idle && !C.wait.empty()) {
+        if (!C.wait.empty() && (front_status != MKHIP_OK || idle)) {
             Req *q = C.wait.front();
-            size_t a = 0, b = 0, d = 0;
-            const int st = batch_bytes(C, cand, 1, &a, &b, &d);
             C.wait.pop_front();
             std::lock_guard<std::mutex> g(S->mu);
             q->t_done = Clock::now();
-            q->status = st != MKHIP_OK ? st : MKHIP_ERR_OOM;
+            q->status = front_status != MKHIP_OK ? front_status : MKHIP_ERR_OOM;
             ++S->n_done;
             S->cv_done.notify_all();
+            return true;
         }
         return false;
     }
