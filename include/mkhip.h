@@ -49,6 +49,7 @@ extern "C" {
 #define MKHIP_ERR_SCALE (-9)         /* Rescale: minScale <= 0, scale 0, level 0      */
 #define MKHIP_ERR_SCHEME (-10)       /* call not valid for this context's scheme      */
 #define MKHIP_ERR_TOO_MANY (-11)     /* more parties than MKHIP_MAX_PARTIES           */
+#define MKHIP_ERR_SCRATCH (-12)      /* work larger than the allocation-free scratch  */
 
 #define MKHIP_MAX_PARTIES 32
 
@@ -149,6 +150,24 @@ typedef struct {
 typedef struct mkhip_plan mkhip_plan;
 
 int mkhip_plan_kklss_ckks(mkhip_ctx *ctx, int nops, const mkhip_kklss_op *ops, int use_graph, mkhip_plan **out);
+
+/*
+ * Allocation-free form of mkhip_plan_kklss_ckks + mkhip_plan_launch (eager) for
+ * the same ops, returning when the outputs are written.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  The call allocates nothing on the
+ * device: the plan's workspace and descriptor tables live in one fixed array of
+ * MKHIP_NOALLOC_SCRATCH_BYTES per GPU (held by each GPU the library runs on until
+ * the process ends); operands and outputs are the caller's device buffers.  Ops
+ * whose workspace + descriptors exceed it fail with MKHIP_ERR_SCRATCH (one op of
+ * the N = 2^14, 8 Q + 2 P, k = 4 shape takes about 180 MB).  Every call on a GPU
+ * shares that GPU's array and leaves its contents behind (nothing needs
+ * clearing: no kernel reads workspace it has not written in the same call);
+ * calls that overlap on one GPU -- from two threads, or from contexts whose
+ * streams run at once -- race.
+ */
+#define MKHIP_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int mkhip_kklss_ckks_mulrelin_noalloc(mkhip_ctx *ctx, int nops, const mkhip_kklss_op *ops);
 int mkhip_plan_launch(mkhip_plan *plan);
 /* average device time of the plan's kernels over `iters` launches (HIP events) */
 int mkhip_plan_time(mkhip_plan *plan, int iters, double *ms_per_launch);

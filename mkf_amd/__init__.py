@@ -23,7 +23,9 @@ STATUS = {
     0: "MKHIP_OK", -1: "MKHIP_ERR_ARG", -2: "MKHIP_ERR_PARAMS", -3: "MKHIP_ERR_GADGET",
     -4: "MKHIP_ERR_UNKNOWN_PARTY", -5: "MKHIP_ERR_LEVEL", -6: "MKHIP_ERR_NO_KEY", -7: "MKHIP_ERR_OOM",
     -8: "MKHIP_ERR_DEVICE", -9: "MKHIP_ERR_SCALE", -10: "MKHIP_ERR_SCHEME", -11: "MKHIP_ERR_TOO_MANY",
+    -12: "MKHIP_ERR_SCRATCH",
 }
+NOALLOC_SCRATCH_BYTES = 256 << 20
 
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = [
@@ -49,6 +51,7 @@ EXPORTS = [
     "mkhip_plan_mkf_poly", "mkhip_mkf_poly_query", "mkhip_mkf_ckks_evaluate_poly", "mkhip_mkf_ckks_add",
     "mkhip_mkf_ckks_add_const", "mkhip_mkf_ckks_mult_by_const", "mkhip_mkf_ckks_mult_by_gaussian_integer_and_add",
     "mkhip_mkf_ckks_rescale", "mkhip_mkf_ckks_mulrelin_levels", "mkhip_probe_peak",
+    "mkhip_kklss_ckks_mulrelin_noalloc",
 ]
 BASIS_STANDARD, BASIS_CHEBYSHEV = 0, 1
 TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64, TUNE_SPLIT_LIMB, TUNE_SPLIT_EXT, TUNE_DIGIT_PRE = \
@@ -199,6 +202,7 @@ def _declare(L):
     L.mkhip_d2h.argtypes = [V, V, V, S]
     L.mkhip_d2d.argtypes = [V, V, V, S]
     L.mkhip_probe_peak.argtypes = [V, I, S, I, dblp]
+    L.mkhip_kklss_ckks_mulrelin_noalloc.argtypes = [V, I, ctypes.POINTER(KKLSSOp)]
     L.mkhip_sync.argtypes = [V]
     L.mkhip_plan_kklss_ckks.argtypes = [V, I, ctypes.POINTER(KKLSSOp), I, ctypes.POINTER(V)]
     L.mkhip_plan_launch.argtypes = [V]
@@ -646,6 +650,14 @@ class Context:
         check(lib().mkhip_mkf_ckks_mulrelin_levels(self._h, group, _p(left), left.shape[1] - 1, _p(right),
                                                    right.shape[1] - 1, _p(out)), "mkhip_mkf_ckks_mulrelin_levels")
         return out
+
+    def kklss_ckks_mulrelin_noalloc(self, ops):
+        """KKLSS MK-CKKS mul+relin+rescale of device-resident ops (dicts as for Plan)
+        with no device allocation: workspace and descriptors in the library's fixed
+        scratch (mkhip_kklss_ckks_mulrelin_noalloc).  Returns when the outputs are
+        written."""
+        arr = _marshal_ops("kklss", ops)
+        check(lib().mkhip_kklss_ckks_mulrelin_noalloc(self._h, len(ops), arr), "mkhip_kklss_ckks_mulrelin_noalloc")
 
     def probe_peak(self, kind, size=0, iters=5):
         """Measured peak rate of this GPU (mkhip_probe_peak): bytes/s for PROBE_COPY

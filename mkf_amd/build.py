@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "libmkhip.so")
 SOURCES = ["kernels.hip", "kernels_mkf.hip", "kernels_bfv.hip", "kernels_fused.hip", "kernels_galois.hip",
            "kernels_keygen.hip", "kernels_poly.hip", "mkhip.hip", "mkhip_plan.hip", "mkhip_kklss.hip", "mkhip_mkf.hip", "mkhip_bfv.hip",
            "mkhip_rotate.hip", "mkhip_keygen.hip", "mkhip_poly.hip", "mkhip_batch.hip",
-           "mkhip_server.hip", "mkhip_peaks.hip"]
+           "mkhip_server.hip", "mkhip_peaks.hip", "mkhip_noalloc.hip"]
 OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

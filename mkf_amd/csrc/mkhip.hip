@@ -129,6 +129,7 @@ extern "C" const char *mkhip_status_string(int s) {
     case MKHIP_ERR_SCALE: return "cannot Rescale (minScale <= 0, scale 0 or level 0)";
     case MKHIP_ERR_SCHEME: return "operation not valid for this context";
     case MKHIP_ERR_TOO_MANY: return "too many parties";
+    case MKHIP_ERR_SCRATCH: return "work larger than the allocation-free entry point's fixed scratch";
     default: return "unknown status";
     }
 }
@@ -343,6 +344,10 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     T.alpha = alpha;
     T.beta_max = c->beta_max;
     T.nmod = nmod;
+    // look up this device's allocation-free scratch once, with the device current
+    // (a failure is kept and reported by the allocation-free entry point only)
+    void *scratch_ptr = nullptr;
+    (void)noalloc_scratch(&scratch_ptr);
     *out = c;
     return MKHIP_OK;
 }

@@ -28,6 +28,10 @@ hipError_t launch_galois(int what, const DevTables &T, const void *descs, int n,
 hipError_t launch_keygen(int what, const DevTables &T, const void *descs, int n, hipStream_t st);
 hipError_t launch_linear(const DevTables &T, const void *descs, int n, hipStream_t st);
 
+// mkhip_noalloc.hip: device address of the current device's allocation-free
+// scratch, looked up once per device
+int noalloc_scratch(void **ptr);
+
 inline int hip_status(hipError_t e) {
     return e == hipSuccess ? MKHIP_OK : (e == hipErrorOutOfMemory ? MKHIP_ERR_OOM : MKHIP_ERR_DEVICE);
 }
