@@ -35,7 +35,7 @@ sys.path.insert(0, ROOT)
 METRIC = "MK-CKKS mul+relin/s at N=2^14 k=4 (batched vs plain); p50/p99 latency"
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 ALL_SCHEMES = ["kklss_ckks", "mkf_ckks", "kklss_bfv", "mkf_bfv"]
-TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic_kklss_ckks_n14_l8_k4_b64.json")
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic_kklss_ckks_mulrelin_n14_l8_k4_b64.json")
 
 # The reference's MK-BFV parameter literal PN14QP439 (software/kklss/mkbfv/mkbfv_test.go:76-107)
 BFV_Q = [0x1fffffffe30001, 0x1fffffffd10001, 0x1fffffffbf0001, 0x1fffffffb60001, 0x1fffffff920001,
