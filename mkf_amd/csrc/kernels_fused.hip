@@ -42,10 +42,19 @@ __device__ __forceinline__ void decomp_mac_row(const DecompDesc &Dd, int i, int 
 #pragma unroll
         for (int e = 0; e < 16; e += 2) {  // key words in the transform's Quad layout
             const u64v2 kw = *(const gv2 *)(key + Quad<Gm::LS>::elem(tid, e));
-            acc[e] = add_mod(acc[e], mred(kw.x, v[e], m.q, m.qinv), m.q);
-            acc[e + 1] = add_mod(acc[e + 1], mred(kw.y, v[e + 1], m.q, m.qinv), m.q);
+            if constexpr (MODE == 1) {
+                // q < 2^57: products in (0, 2q) summed unreduced, below 64 q for
+                // at most 32 parties
+                acc[e] += mred_lazy(kw.x, v[e], m.q, m.qinv);
+                acc[e + 1] += mred_lazy(kw.y, v[e + 1], m.q, m.qinv);
+            } else {
+                acc[e] = add_mod(acc[e], mred(kw.x, v[e], m.q, m.qinv), m.q);
+                acc[e + 1] = add_mod(acc[e + 1], mred(kw.y, v[e + 1], m.q, m.qinv), m.q);
+            }
         }
     }
+    // MForm of the sum; an unreduced sum below 64 q keeps acc r2 < q 2^64, so the
+    // result is canonical either way
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = mred(acc[e], m.r2, m.q, m.qinv);
     store_row_quad<Gm::LS>(acc, Dd.xy + krow, tid);

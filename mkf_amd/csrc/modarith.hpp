@@ -73,8 +73,23 @@ __device__ __forceinline__ u64 mulhi64_approx(u64 a, u64 b) {
 
 // a * w mod q in [0, 4q) for any a < 2^64 and w < q, w' = floor(w 2^64 / q):
 // the exact Shoup quotient gives [0, 2q); the approximate one is at most 2 lower.
+// The result a w - qh q is formed mod 2^64 as a w + qh (2^64 - q): the two low
+// partial products chain through one multiply-add each (v_mad_u64_u32 takes the
+// other as its addend) and the four cross products land in the high word, so no
+// 64-bit subtraction (and its borrow chain) is needed.
 __device__ __forceinline__ u64 shoup_lazy4(u64 a, u64 w, u64 wp, u64 q) {
-    return a * w - mulhi64_approx(a, wp) * q;
+    const u64 qh = mulhi64_approx(a, wp);
+    const u64 nq = 0 - q;
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), w0 = (u32)w, w1 = (u32)(w >> 32);
+    const u32 h0 = (u32)qh, h1 = (u32)(qh >> 32), n0 = (u32)nq, n1 = (u32)(nq >> 32);
+    const u64 lo = (u64)h0 * n0 + (u64)a0 * w0;
+    // the cross products stay 32-bit multiplies summed by two three-input adds
+    // (opaque to the combiner, which would otherwise widen them into 64-bit
+    // multiply-adds and a 64-bit add)
+    u32 c0 = a0 * w1, c1 = a1 * w0, c2 = h0 * n1, c3 = h1 * n0;
+    asm("" : "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3));
+    const u32 hi = (u32)(lo >> 32) + c0 + c1 + c2 + c3;
+    return ((u64)hi << 32) | (u32)lo;
 }
 
 MKHIP_HD u64 shoup(u64 a, u64 w, u64 wp, u64 q) {
@@ -87,6 +102,28 @@ MKHIP_HD u64 reduce64(u64 x, u64 q, u64 mu) {
     u64 r = x - mulhi64(x, mu) * q;  // r < 2q + q for q < 2^63
     r = r >= q ? r - q : r;
     return r >= q ? r - q : r;
+}
+
+// x mod q for any x < 2^64 and 2^32 <= q (so that mu = floor(2^64 / q) < 2^32):
+// the quotient estimate hi32(x_hi * mu) is at most 2 below floor(x / q), so
+// x - est q < 3q and two conditional subtractions finish.  One 32-bit multiply
+// for the estimate and one multiply-add (x + est (2^64 - q)) for the remainder.
+__device__ __forceinline__ u64 reduce_big(u64 x, u64 q, u64 mu) {
+    const u32 est = __umulhi((u32)(x >> 32), (u32)mu);
+    const u64 nq = 0 - q;
+    u64 r = (u64)est * (u32)nq + x;
+    r += (u64)(est * (u32)(nq >> 32)) << 32;
+    const u64 q2 = q << 1;
+    r = r >= q2 ? r - q2 : r;
+    return r >= q ? r - q : r;
+}
+
+// a * b * 2^-64 mod q in (0, 2q) for a * b < q * 2^64 (MRed without its final
+// conditional subtraction).
+__device__ __forceinline__ u64 mred_lazy(u64 a, u64 b, u64 q, u64 qinv) {
+    u64 hi, lo;
+    mul_wide(a, b, hi, lo);
+    return hi - mulhi64(lo * qinv, q) + q;
 }
 
 MKHIP_HD u64 add_mod(u64 a, u64 b, u64 q) {

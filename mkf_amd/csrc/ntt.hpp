@@ -41,8 +41,8 @@ __device__ __forceinline__ void fwd_bfly_lazy(u64 &x, u64 &y, u64 w, u64 wp, u64
     y = u + q4 - t;
 }
 
-// The moduli whose forward transforms run unreduced.
-__device__ __forceinline__ bool fwd_lazy_ok(u64 q) { return q < (1ull << 57); }
+// The moduli whose forward transforms run unreduced (and end with reduce_big).
+__device__ __forceinline__ bool fwd_lazy_ok(u64 q) { return q >= (1ull << 32) && q < (1ull << 57); }
 
 // In: x, y in [0, 4q).  Out: x, y in [0, 4q).
 __device__ __forceinline__ void inv_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q4, u64 q) {
@@ -249,8 +249,8 @@ struct Quad {
 // Forward NTT; v in Linear layout in, Quad layout out; returns canonical values.
 // With hh = 2 + h it is the transform of half h of a limb of 2^(LOGN+1) after
 // that limb's first stage (fwd_first_stage).
-// LAZY (q < 2^57, mu = floor(2^64 / q)): unreduced butterflies, one Barrett
-// reduction per value at the end.
+// LAZY (2^32 <= q < 2^57, mu = floor(2^64 / q)): unreduced butterflies, one
+// reduction per value at the end (reduce_big).
 template <int LOGN, bool LAZY = false>
 __device__ __forceinline__ void ntt_fwd(u64 (&v)[16], u64 *s, int tid, const ulonglong2 *__restrict__ tw,
                                         u64 q, int hh = 1, u64 mu = 0) {
@@ -266,7 +266,7 @@ __device__ __forceinline__ void ntt_fwd(u64 (&v)[16], u64 *s, int tid, const ulo
     if (LAZY) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            v[i] = reduce64(v[i], q, mu);
+            v[i] = reduce_big(v[i], q, mu);
             __builtin_amdgcn_sched_barrier(0);
         }
     } else {
