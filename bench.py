@@ -280,6 +280,7 @@ def run_scheme(scheme, args, rng):
     ops_s = B * args.steps / elapsed
     stage_ms = plan.stage_times(args.stage_iters)
     names = plan.stages
+    per_run = [plan.stage_times(1) for _ in range(args.ref_runs)]
     res = {
         "ops_per_s": round(ops_s, 2), "ms_per_step": round(elapsed / args.steps * 1e3, 3), "batch": B,
         "p50_ms": round(pct(lat, 50) * 1e3, 3), "p99_ms": round(pct(lat, 99) * 1e3, 3),
@@ -292,6 +293,9 @@ def run_scheme(scheme, args, rng):
         "shape": {"logN": w.logN, "L": w.L, "K": w.K, "k": w.k, "alpha": w.ctx.alpha, "beta": w.beta,
                   "nres": w.nres, "t": w.t},
     }
+    res["reference_stages_ms"] = {g: round(t, 4) for g, t in ref_stage_split(names, stage_ms).items()}
+    param = "N%dL%dK%d-B%d" % (w.logN, w.L, w.K, B)
+    res["mkfbench_rows"] = mkfbench_rows(scheme, param, w.k, B, names, per_run, args.warmup)
     alg = op_alg_bytes(scheme, B, w.k, w.L, w.K, w.beta, w.N)
     res["alg_bytes_per_op"] = int(alg)
     res["op_gbs"] = round(alg * ops_s / 1e9, 1)
@@ -324,6 +328,77 @@ def run_scheme(scheme, args, rng):
         res["batch_sweep"] = batch_sweep(w, ops, args, rng, res.get("plain"))
     extra = (w, plan, stage_ms, names)
     return res, extra
+
+
+# The reference's stage names (software/mkf/stageprof/stageprof.go:17-25) and the
+# plan stages whose device time each one covers.  The MKF profile fields of
+# mkfbench (mkf_cpu/profile.go:8-19, attributed in common.go:79-139 and
+# ckks.go:102-152) use the same split: Decomposition (DecomposeNTT, with its
+# inverse transform of the NTT-domain input), External (KeyswitchHoistedNoModDown
+# and the fold's additions), Modulus (ModDown, Rescale), Tensor, Memory (the k
+# copies of Gamma).
+REF_STAGES = {
+    "tensor_product": ("tensor_ntt", "tensor", "karatsuba", "lift"),
+    "gadget_decomposition": ("decompose_mac", "decompose", "decompose_t", "intt_in", "digit_pre", "intt_mu2",
+                             "digit_pre_mu2", "decompose_mu2", "decompose_relin"),
+    "external_product_accumulation": ("ext_xy", "ext_vu", "mac_xy", "mac_fold", "mac_relin"),
+    "modup_moddown_rescale": ("moddown_xy", "moddown_rescale", "moddown_final", "moddown_fold", "moddown_relin",
+                              "intt_last", "rescale", "quantize"),
+    "memory_data_movement": ("materialize",),
+}
+REF_FIELD = {"tensor_product": "tensor_ns", "gadget_decomposition": "decomposition_ns",
+             "external_product_accumulation": "external_product_ns",
+             "modup_moddown_rescale": "modup_moddown_rescale_ns", "memory_data_movement": "memory_copy_ns"}
+MKFBENCH_HEADER = ["scheme", "parameter", "parties", "run", "warmup", "gomaxprocs",
+                   "total_ns", "tensor_ns", "decomposition_ns", "external_product_ns",
+                   "modup_moddown_rescale_ns", "memory_copy_ns", "other_ns",
+                   "decomposition_count", "external_product_poly_count",
+                   "validation_passed", "expected", "observed", "max_error",
+                   "go_version", "goos", "goarch", "timestamp_utc"]
+
+
+def ref_stage_split(names, times):
+    """Plan stage times grouped under the reference's stage names; stages no group
+    names (the N = 2^15 finish passes) count as other_synchronization."""
+    out = {k: 0.0 for k in list(REF_STAGES) + ["other_synchronization"]}
+    for n_, t in zip(names, times):
+        base = n_[:-len("_finish")] if n_.endswith("_finish") else None
+        grp = next((g for g, members in REF_STAGES.items() if n_ in members), None)
+        if grp is None or base is not None:
+            grp = "other_synchronization"
+        out[grp] += t
+    return out
+
+
+def switch_counts(scheme, k):
+    """Decompositions and external-product polynomials per op, counted as the
+    reference's profile counts them: MKF one decomposition and 2 output polys per
+    switch, 2k fold switches + 1 relinearization (common.go:84-89); KKLSS the 2k
+    operand decompositions + k for the t_i, and 4k external products."""
+    if scheme.startswith("mkf"):
+        return 2 * k + 1, 2 * (2 * k + 1)
+    return 3 * k, 4 * k
+
+
+def mkfbench_rows(scheme, param, k, B, names, runs, warmup):
+    """Rows in the reference's mkfbench CSV format (cmd/mkfbench/main.go:17-24),
+    one per timed launch; times are per op (launch time / B, device time of each
+    stage from HIP events).  Validation columns are left empty: the bench does not
+    decrypt (bit-exact parity is checked by the tests)."""
+    import datetime
+
+    ndec, next_ = switch_counts(scheme, k)
+    rows = []
+    label = {"kklss_ckks": "KKLSS-CKKS", "kklss_bfv": "KKLSS-BFV", "mkf_ckks": "MKF-CKKS", "mkf_bfv": "MKF-BFV"}[scheme]
+    for i, times in enumerate(runs):
+        g = ref_stage_split(names, times)
+        per = {f: int(round(g[st] * 1e6 / B)) for st, f in REF_FIELD.items()}
+        total = int(round(sum(times) * 1e6 / B))
+        rows.append([label, param, k, i + 1, warmup, 1, total, per["tensor_ns"], per["decomposition_ns"],
+                     per["external_product_ns"], per["modup_moddown_rescale_ns"], per["memory_copy_ns"],
+                     int(round(g["other_synchronization"] * 1e6 / B)), ndec, next_, "", "", "", "",
+                     "hip", "linux", "gfx950", datetime.datetime.utcnow().isoformat() + "Z"])
+    return rows
 
 
 def batch_sweep(w, ops, args, rng, plain):
@@ -849,6 +924,10 @@ def main():
                     help="headline batch sizes timed for the best batch under p99 < 10 ms ('' skips)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the op-parallel CPU leg (0: the cores this process may use, at most 16)")
+    ap.add_argument("--ref-runs", type=int, default=10,
+                    help="timed launches written as mkfbench rows per scheme (the reference's -runs)")
+    ap.add_argument("--mkfbench-csv", default="",
+                    help="write the mkfbench-format rows (cmd/mkfbench/main.go:17-24) of every scheme to this file")
     ap.add_argument("--no-peaks", dest="peaks", action="store_false",
                     help="skip the measured copy bandwidth and integer issue rates")
     ap.add_argument("--copy-calib-mb", type=float, default=0.0,
@@ -935,12 +1014,23 @@ def main():
             "batched": {"ops_per_s": res["ops_per_s"], "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"],
                         "ops_per_launch": B},
             "plain": res.get("plain"), "batched_over_plain": res.get("batched_over_plain"),
-            "stages_ms": res["stages_ms"], "workspace_gb": res["workspace_gb"],
+            "stages_ms": res["stages_ms"], "reference_stages_ms": res["reference_stages_ms"],
+            "workspace_gb": res["workspace_gb"],
             "fp64_arithmetic": res.get("fp64_arithmetic"), "batch_sweep": res.get("batch_sweep"),
             "peaks": peaks, "roofline": roofline, "cpu_baseline": cpu,
         })
         plan.close()
         w.close()
+    rows = []
+    for r in results.values():
+        rows += r.pop("mkfbench_rows", [])
+    if args.mkfbench_csv:
+        import csv
+
+        with open(args.mkfbench_csv, "w", newline="") as f:
+            cw = csv.writer(f)
+            cw.writerow(MKFBENCH_HEADER)
+            cw.writerows(rows)
     line["schemes"] = {s: r for s, r in results.items() if s != "kklss_ckks"}
     if args.serve_seconds > 0:
         line["serving_headline"] = run_serving_headline(args, rng)
