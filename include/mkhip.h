@@ -143,8 +143,15 @@ typedef struct {
     const uint64_t *op0, *op1; /* device pointers, layout as above           */
     uint64_t *out;             /* device pointer                              */
     int ids0[MKHIP_MAX_PARTIES], ids1[MKHIP_MAX_PARTIES];
-    int k0, k1, level;         /* both operands at `level` (same limb count)  */
+    int k0, k1, level;         /* level of the product                        */
     int nres;                  /* rescales to apply (from mkhip_rescale_count) */
+    int level0, level1;        /* KKLSS MK-CKKS: levels of op0 / op1 when above
+                                  `level` (components of level0 + 1 / level1 + 1
+                                  limbs, each operand decomposed at its own level,
+                                  mkckks/evaluator.go:433-439, the product at
+                                  `level`, :577-581); a value below `level` (e.g. 0
+                                  from zero-initialisation) means `level`.  The
+                                  KKLSS MK-BFV plans ignore them. */
 } mkhip_kklss_op;
 
 typedef struct mkhip_plan mkhip_plan;

@@ -119,6 +119,7 @@ class KKLSSOp(ctypes.Structure):
         ("op0", ctypes.c_void_p), ("op1", ctypes.c_void_p), ("out", ctypes.c_void_p),
         ("ids0", ctypes.c_int * MAX_PARTIES), ("ids1", ctypes.c_int * MAX_PARTIES),
         ("k0", ctypes.c_int), ("k1", ctypes.c_int), ("level", ctypes.c_int), ("nres", ctypes.c_int),
+        ("level0", ctypes.c_int), ("level1", ctypes.c_int),
     ]
 
 
@@ -752,6 +753,7 @@ def _marshal_ops(scheme, ops):
             for j, v in enumerate(o["ids1"]):
                 a.ids1[j] = v
             a.k0, a.k1, a.level, a.nres = len(o["ids0"]), len(o["ids1"]), o["level"], o["nres"]
+            a.level0, a.level1 = o.get("level0", 0), o.get("level1", 0)
         return arr
     if scheme == "rotate":
         arr = (RotateOp * len(ops))()

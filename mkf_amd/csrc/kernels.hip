@@ -131,8 +131,8 @@ __global__ void __launch_bounds__(256) k_digit_pre(DevTables T, const DecompDesc
     const int p = b % kmax;
     const DecompDesc &Dd = descs[b / kmax];
     if (p >= Dd.k || !Dd.pre) return;
-    const int level = Dd.level, alpha = T.alpha, start = i * alpha;
-    const int nd = (level > alpha * (i + 1) - 1) ? alpha : (level % alpha) + 1;
+    const int alpha = T.alpha, start = i * alpha;
+    const int nd = digit_primes(i, max(Dd.src_level, Dd.level), alpha);
     if (nd == 1) return;  // a one-prime digit is copied, not extended
     const int ebase = i * (kMaxAlpha + 1) + nd;
     const u64 *yc = T.dc_y + (size_t)ebase * kMaxAlpha;
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ?
     const ModConst m = T.mod[ri];
     const int alpha = T.alpha;
     const int start = i * alpha;
-    const int nd = (level > alpha * (i + 1) - 1) ? alpha : (level % alpha) + 1;
+    const int nd = digit_primes(i, max(Dd.src_level, level), alpha);
     const bool own_row = (r <= level) && (r >= start) && (r < start + nd);
     const u64 *src = Dd.comp[p];
     const ulonglong2 *tw = T.tw + (size_t)ri * N;

@@ -45,6 +45,7 @@ int mkhip::plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, in
         if (!op.op0 || !op.op1 || !op.out) return MKHIP_ERR_ARG;
         if (op.level != level) return MKHIP_ERR_LEVEL;
         if (op.level < 0 || op.level >= c->L) return MKHIP_ERR_LEVEL;
+        if (op.level0 >= c->L || op.level1 >= c->L) return MKHIP_ERR_LEVEL;
         if (op.k0 < 0 || op.k1 < 0 || op.k0 > MKHIP_MAX_PARTIES || op.k1 > MKHIP_MAX_PARTIES) return MKHIP_ERR_TOO_MANY;
         if (op.nres < 0 || op.nres > op.level) return MKHIP_ERR_SCALE;
         for (int i = 0; i < op.k0; ++i)
@@ -106,11 +107,17 @@ int mkhip::plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, in
     for (int o = 0; o < nops; ++o) {
         const mkhip_kklss_op &op = ops[o];
         const OpWs &w = ws[o];
+        // each operand at its own level (its components' limb count); decomposed
+        // there, read at the product's level
+        const int lv0 = std::max(op.level0, level), lv1 = std::max(op.level1, level);
+        const size_t cs0 = (size_t)(lv0 + 1) * N, cs1 = (size_t)(lv1 + 1) * N;
         // decompositions of both sides; x / y accumulations (key-stationary)
         DecompDesc s0 = {}, s1 = {};
         MacDesc m0 = {}, m1 = {};
+        s0.src_level = lv0;
+        s1.src_level = lv1;
         for (int p = 0; p < op.k0; ++p) {
-            s0.comp[p] = op.op0 + (size_t)(1 + p) * nl * N;
+            s0.comp[p] = op.op0 + (size_t)(1 + p) * cs0;
             m0.key[p] = s0.key[p] = c->parties[op.ids0[p]].d;
         }
         s0.D = P(w.D0);
@@ -119,7 +126,7 @@ int mkhip::plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, in
         s0.level = m0.level = level;
         m0.xy = s0.xy = P(w.x);
         for (int p = 0; p < op.k1; ++p) {
-            s1.comp[p] = op.op1 + (size_t)(1 + p) * nl * N;
+            s1.comp[p] = op.op1 + (size_t)(1 + p) * cs1;
             m1.key[p] = s1.key[p] = c->parties[op.ids1[p]].b;
         }
         s1.D = P(w.D1);
@@ -143,7 +150,7 @@ int mkhip::plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, in
         const bool diag = c->alpha == 1;
         for (int comp = 0; comp < nc0 + nc1; ++comp) {
             if (diag && comp != 0 && comp != nc0) continue;
-            const u64 *src = comp < nc0 ? op.op0 + (size_t)comp * nl * N : op.op1 + (size_t)(comp - nc0) * nl * N;
+            const u64 *src = comp < nc0 ? op.op0 + (size_t)comp * cs0 : op.op1 + (size_t)(comp - nc0) * cs1;
             for (int l = 0; l < nl; ++l) {
                 NttJob j;
                 j.src = src + (size_t)l * N;
@@ -314,10 +321,13 @@ extern "C" int mkhip_kklss_ckks_mulrelin(mkhip_ctx *c, const uint64_t *op0, int 
     if (k0 > MKHIP_MAX_PARTIES || k1 > MKHIP_MAX_PARTIES) return MKHIP_ERR_TOO_MANY;
     if (level0 < 0 || level1 < 0 || level0 >= c->L || level1 >= c->L) return MKHIP_ERR_LEVEL;
     if (c->alpha == 0) return MKHIP_ERR_GADGET;
-    // DropLevel to the common level (evaluator.go:577-581).  With alpha > 1 the
-    // reference decomposes each operand at its own level, which this path does not model.
+    // The product is taken at the common level (evaluator.go:577-581) and each
+    // operand is decomposed at its own level (:433-439).  With one prime per digit
+    // the digits do not depend on the level, so the operands are staged at the
+    // common level; with alpha > 1 a digit can span limbs above it, so every limb
+    // of each operand is staged.
     const int level = std::min(level0, level1);
-    if (level0 != level1 && c->alpha != 1) return MKHIP_ERR_LEVEL;
+    const int lv0 = c->alpha == 1 ? level : level0, lv1 = c->alpha == 1 ? level : level1;
     // mulRelinHoisted ignores Rescale's error return (evaluator.go:585): a
     // rejected rescale (level 0, minScale <= 0) leaves the product unrescaled.
     int nres = 0;
@@ -337,10 +347,11 @@ extern "C" int mkhip_kklss_ckks_mulrelin(mkhip_ctx *c, const uint64_t *op0, int 
         if (find_id(ids, ku, ids1[i]) < 0) ids[ku++] = ids1[i];
     if (ku > MKHIP_MAX_PARTIES) return MKHIP_ERR_TOO_MANY;
     std::sort(ids, ids + ku);
-    const size_t in0 = (size_t)(1 + k0) * nl * N, in1 = (size_t)(1 + k1) * nl * N, ow = (size_t)(1 + ku) * nl * N;
+    const size_t in0 = (size_t)(1 + k0) * (lv0 + 1) * N, in1 = (size_t)(1 + k1) * (lv1 + 1) * N;
+    const size_t ow = (size_t)(1 + ku) * nl * N;
     const size_t words = in0 + in1 + ow;
     // shape signature of the cached plan
-    std::vector<int> sig = {0, k0, k1, level, nres};
+    std::vector<int> sig = {0, k0, k1, level, nres, lv0, lv1};
     sig.insert(sig.end(), ids0, ids0 + k0);
     sig.insert(sig.end(), ids1, ids1 + k1);
     if (!c->single || c->single->sig != sig || c->single_io_words < words) {
@@ -363,19 +374,22 @@ extern "C" int mkhip_kklss_ckks_mulrelin(mkhip_ctx *c, const uint64_t *op0, int 
         op.k1 = k1;
         op.level = level;
         op.nres = nres;
+        op.level0 = lv0;
+        op.level1 = lv1;
         mkhip_plan *pl = nullptr;
         s = mkhip_plan_kklss_ckks(c, 1, &op, 0, &pl);
         if (s != MKHIP_OK) return s;
         pl->sig = sig;
         c->single = pl;
     }
-    // stage the operands at the common level (limbs 0..level of each component)
+    // stage limbs 0..lv of each component
     for (int comp = 0; comp <= k0; ++comp)
-        HIP_TRY(hipMemcpyAsync(c->single_io + (size_t)comp * nl * N, op0 + (size_t)comp * (level0 + 1) * N,
-                               (size_t)nl * N * 8, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->single_io + (size_t)comp * (lv0 + 1) * N, op0 + (size_t)comp * (level0 + 1) * N,
+                               (size_t)(lv0 + 1) * N * 8, hipMemcpyHostToDevice, c->stream));
     for (int comp = 0; comp <= k1; ++comp)
-        HIP_TRY(hipMemcpyAsync(c->single_io + in0 + (size_t)comp * nl * N, op1 + (size_t)comp * (level1 + 1) * N,
-                               (size_t)nl * N * 8, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->single_io + in0 + (size_t)comp * (lv1 + 1) * N,
+                               op1 + (size_t)comp * (level1 + 1) * N, (size_t)(lv1 + 1) * N * 8,
+                               hipMemcpyHostToDevice, c->stream));
     s = mkhip_plan_launch(c->single);
     if (s != MKHIP_OK) return s;
     const size_t outw = (size_t)(1 + ku) * (nl - nres) * N;

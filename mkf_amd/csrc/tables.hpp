@@ -59,6 +59,12 @@ inline bool ext_split(const DevTables &T) { return T.logN == 14 && T.split14_ext
 // ring index of limb l of an R = Q u QMul polynomial
 __host__ __device__ __forceinline__ int r_index(int l, int L, int K) { return l < L ? l : K + l; }
 
+// primes of gadget digit i of components decomposed at level src (Decomposer's
+// decompLvl + 2, basis_extension.go:440-451)
+__host__ __device__ __forceinline__ int digit_primes(int i, int src, int alpha) {
+    return (src > alpha * (i + 1) - 1) ? alpha : (src % alpha) + 1;
+}
+
 __device__ __forceinline__ int dc_base(const DevTables &T, int digit, int nd) {
     return digit * (kMaxAlpha + 1) + nd;
 }
@@ -234,6 +240,11 @@ struct DecompDesc {
     u64 *D;                        // [k][beta][rows][N]
     u64 *xy;                       // [beta][rows][N] (MForm), may be null
     int k, level;
+    int src_level;                 // level the components were decomposed at (their
+                                   // limb count - 1), above `level` for an operand
+                                   // above the product's level: digit i spans limbs
+                                   // i alpha .. min(i alpha + alpha - 1, src_level);
+                                   // a value below `level` means `level`
     u64 *pre;                      // [k][beta][alpha+1][N]: y_l rows and v of the CRT digits
                                    // (k_digit_pre), or null: the rows compute them
 };

@@ -81,6 +81,11 @@ def _declare(L):
     L.mkref_kklss_mulrelin.argtypes = [ctypes.c_void_p, I, u64p, intp, I, u64p, intp, I,
                                        ctypes.POINTER(u64p), ctypes.POINTER(u64p),
                                        ctypes.POINTER(u64p), u64p, u64p, intp]
+    L.mkref_kklss_mulrelin_lv.restype = I
+    L.mkref_kklss_mulrelin_lv.argtypes = [ctypes.c_void_p, I, I, I, u64p, intp, I, u64p, intp, I,
+                                          ctypes.POINTER(u64p), ctypes.POINTER(u64p),
+                                          ctypes.POINTER(u64p), u64p, u64p, intp]
+    L.mkref_decompose_src.argtypes = [ctypes.c_void_p, I, I, u64p, u64p]
     L.mkref_rescale_count.restype = I
     L.mkref_rescale_count.argtypes = [u64p, I, ctypes.c_double, ctypes.c_double,
                                       ctypes.POINTER(ctypes.c_double)]
@@ -338,6 +343,38 @@ class Ctx:
                                         b, d, v, _p(u), _p(out), iout)
         assert nc == ku + 1
         return out, list(iout)
+
+    def kklss_mulrelin_levels(self, op0, ids0, op1, ids1, rlk_b, rlk_d, rlk_v, crs_u):
+        """KKLSS MK-CKKS mul+relin before rescale of operands at their own levels
+        (op = (1+k, lv+1, N)): each decomposed at its own level, the product at the
+        lower one (mkckks/evaluator.go:433-439, 577-581).  Returns (out, ids)."""
+        lv0, lv1 = op0.shape[1] - 1, op1.shape[1] - 1
+        level = min(lv0, lv1)
+        op0, op1 = _u64(op0), _u64(op1)
+        keys = [_u64(k) for k in (list(rlk_b) + list(rlk_d) + list(rlk_v))]
+        n = len(rlk_b)
+        arr = (u64p * n)
+        b = arr(*[_p(k) for k in keys[:n]])
+        d = arr(*[_p(k) for k in keys[n:2 * n]])
+        v = arr(*[_p(k) for k in keys[2 * n:]])
+        u = _u64(crs_u)
+        k0, k1 = len(ids0), len(ids1)
+        i0 = (ctypes.c_int * max(k0, 1))(*ids0)
+        i1 = (ctypes.c_int * max(k1, 1))(*ids1)
+        ku = len(set(ids0) | set(ids1))
+        out = np.zeros((1 + ku, level + 1, self.N), np.uint64)
+        iout = (ctypes.c_int * ku)()
+        nc = lib().mkref_kklss_mulrelin_lv(self._h, level, lv0, lv1, _p(op0), i0, k0, _p(op1), i1, k1,
+                                           b, d, v, _p(u), _p(out), iout)
+        assert nc == ku + 1
+        return out, list(iout)
+
+    def decompose_src(self, a, level):
+        """Digits of a (src_level + 1 limbs) at its own level, read at `level`."""
+        a = _u64(a)
+        out = np.zeros((self.beta(level), level + 1 + self.K, self.N), np.uint64)
+        lib().mkref_decompose_src(self._h, level, a.shape[0] - 1, _p(a), _p(out))
+        return out
 
     def rescale_count(self, level, scale, min_scale):
         q = _u64(self.Q)

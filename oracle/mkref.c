@@ -437,15 +437,23 @@ static int qp_ring_index(const mkref_ctx *c, int levelQ, int r) {
  * followed by the NTT of every row (keyswitch.go:22-32).  The digit's integer
  * value a mod D (D = product of the digit's primes) is reconstructed by the
  * exact basis extension and written in every row of Q[0..levelQ] u P.
+ *
+ * The digit is formed from the limbs of a polynomial at level src_level >=
+ * levelQ (digit i = limbs i alpha .. min(i alpha + alpha - 1, src_level)) while
+ * the output has the rows of level levelQ: an operand decomposed at its own
+ * level (mkckks/evaluator.go:433-439) and read at the product's lower level,
+ * where MulAndRelinHoisted uses rows 0..level and the P rows of the first
+ * Beta(level) digits (keyswitch_hoisted.go:84-123).  src_level = levelQ is the
+ * ordinary decomposition.
  */
-void mkref_decompose_digit(const mkref_ctx *c, int levelQ, int digit, const u64 *a, u64 *out) {
+void mkref_decompose_digit_src(const mkref_ctx *c, int levelQ, int src_level, int digit, const u64 *a, u64 *out) {
     int N = c->N, K = c->K;
     int alpha = mkref_alpha(c);
     int rows = levelQ + 1 + K;
     int start = digit * alpha;
     int decompLvl;
-    if (levelQ > alpha * (digit + 1) - 1) decompLvl = alpha - 2;
-    else decompLvl = (levelQ % alpha) - 1;
+    if (src_level > alpha * (digit + 1) - 1) decompLvl = alpha - 2;
+    else decompLvl = (src_level % alpha) - 1;
 
     if (decompLvl == -1) {
         /* copy path (:443-451): the limb's residue is copied unreduced into every row */
@@ -466,8 +474,10 @@ void mkref_decompose_digit(const mkref_ctx *c, int levelQ, int digit, const u64 
             for (int r = 0; r <= levelQ; r++) out[(size_t)r * N + x] = tmp[r];
             for (int j = 0; j < K; j++) out[(size_t)(levelQ + 1 + j) * N + x] = tmp[c->L + j];
             /* digit rows keep the input residue (:487); the reference later recomputes
-             * them from the same extension, which is congruent (D = 0 mod q_j there) */
-            for (int i = 0; i < nd; i++) out[(size_t)(start + i) * N + x] = a[(size_t)(start + i) * N + x];
+             * them from the same extension, which is congruent (D = 0 mod q_j there).
+             * Rows of the digit above levelQ are not part of the output. */
+            for (int i = 0; i < nd && start + i <= levelQ; i++)
+                out[(size_t)(start + i) * N + x] = a[(size_t)(start + i) * N + x];
         }
         mkref_modup_free(&m);
     }
@@ -477,6 +487,17 @@ void mkref_decompose_digit(const mkref_ctx *c, int levelQ, int digit, const u64 
         for (int x = 0; x < N; x++) row[x] %= c->ringQP.q[ri];
         mkref_ntt_limb(&c->ringQP, ri, row);
     }
+}
+
+void mkref_decompose_digit(const mkref_ctx *c, int levelQ, int digit, const u64 *a, u64 *out) {
+    mkref_decompose_digit_src(c, levelQ, levelQ, digit, a, out);
+}
+
+/* KeySwitcher.Decompose at src_level, read at levelQ (see mkref_decompose_digit_src) */
+void mkref_decompose_src(const mkref_ctx *c, int levelQ, int src_level, const u64 *a, u64 *out) {
+    int beta = mkref_beta(c, levelQ);
+    size_t dsz = (size_t)(levelQ + 1 + c->K) * c->N;
+    for (int i = 0; i < beta; i++) mkref_decompose_digit_src(c, levelQ, src_level, i, a, out + i * dsz);
 }
 
 /* KeySwitcher.Decompose (keyswitch.go:50-76), input coefficient domain */
@@ -581,17 +602,27 @@ void mkref_mform_qp(const mkref_ctx *c, int levelQ, u64 *x, int ndigits) {
  * (:561-569) -> mulRelinHoisted (:575-586) -> KeySwitcher.MulAndRelinHoisted
  * (keyswitch_hoisted.go:49-194).  The rescale (:585) is separate.
  */
-int mkref_kklss_mulrelin(const mkref_ctx *c, int level,
-                         const u64 *op0, const int *ids0, int k0,
-                         const u64 *op1, const int *ids1, int k1,
-                         const u64 *const *rlk_b, const u64 *const *rlk_d,
-                         const u64 *const *rlk_v, const u64 *crs_u,
-                         u64 *out, int *ids_out) {
+/*
+ * Operands at their own levels lv0, lv1 >= level (components of lv + 1 limbs):
+ * each is decomposed at its own level (evaluator.go:433-439) and the product is
+ * taken at `level` (:577-581), which reads the operands' limbs 0..level.
+ */
+int mkref_kklss_mulrelin_lv(const mkref_ctx *c, int level, int lv0, int lv1,
+                            const u64 *op0_in, const int *ids0, int k0,
+                            const u64 *op1_in, const int *ids1, int k1,
+                            const u64 *const *rlk_b, const u64 *const *rlk_d,
+                            const u64 *const *rlk_v, const u64 *crs_u,
+                            u64 *out, int *ids_out) {
     int N = c->N, K = c->K;
     int nl = level + 1, rows = nl + K, beta = mkref_beta(c, level);
     size_t psz = (size_t)nl * N, dsz = (size_t)rows * N, hsz = (size_t)beta * dsz;
+    size_t psz0 = (size_t)(lv0 + 1) * N, psz1 = (size_t)(lv1 + 1) * N;
     int ku = union_ids(ids0, k0, ids1, k1, ids_out);
     const mkref_ring *R = &c->ringQP;
+    /* the operands' limbs 0..level, contiguous */
+    u64 *op0 = malloc(sizeof(u64) * psz * (1 + k0)), *op1 = malloc(sizeof(u64) * psz * (1 + k1));
+    for (int i = 0; i <= k0; i++) memcpy(op0 + (size_t)i * psz, op0_in + (size_t)i * psz0, sizeof(u64) * psz);
+    for (int j = 0; j <= k1; j++) memcpy(op1 + (size_t)j * psz, op1_in + (size_t)j * psz1, sizeof(u64) * psz);
 
     u64 *D0 = malloc(sizeof(u64) * hsz * (k0 ? k0 : 1));
     u64 *D1 = malloc(sizeof(u64) * hsz * (k1 ? k1 : 1));
@@ -602,8 +633,8 @@ int mkref_kklss_mulrelin(const mkref_ctx *c, int level,
     u64 *Dt = malloc(sizeof(u64) * hsz);
 
     /* hoisted decompositions (evaluator.go:432-438) */
-    for (int i = 0; i < k0; i++) mkref_decompose(c, level, op0 + (size_t)(1 + i) * psz, D0 + i * hsz);
-    for (int j = 0; j < k1; j++) mkref_decompose(c, level, op1 + (size_t)(1 + j) * psz, D1 + j * hsz);
+    for (int i = 0; i < k0; i++) mkref_decompose_src(c, level, lv0, op0_in + (size_t)(1 + i) * psz0, D0 + i * hsz);
+    for (int j = 0; j < k1; j++) mkref_decompose_src(c, level, lv1, op1_in + (size_t)(1 + j) * psz1, D1 + j * hsz);
 
     /* x = sum d_i (.) D(op0_i), y = sum b_j (.) D(op1_j), then MForm (:84-123) */
     for (int i = 0; i < k0; i++) mac_keyed(c, level, rlk_d[ids0[i]], D0 + i * hsz, x);
@@ -672,7 +703,18 @@ int mkref_kklss_mulrelin(const mkref_ctx *c, int level,
     }
 
     free(D0); free(D1); free(x); free(y); free(pool0); free(pool1); free(pool2); free(t); free(Dt);
+    free(op0); free(op1);
     return ku + 1;
+}
+
+int mkref_kklss_mulrelin(const mkref_ctx *c, int level,
+                         const u64 *op0, const int *ids0, int k0,
+                         const u64 *op1, const int *ids1, int k1,
+                         const u64 *const *rlk_b, const u64 *const *rlk_d,
+                         const u64 *const *rlk_v, const u64 *crs_u,
+                         u64 *out, int *ids_out) {
+    return mkref_kklss_mulrelin_lv(c, level, level, level, op0, ids0, k0, op1, ids1, k1, rlk_b, rlk_d, rlk_v,
+                                   crs_u, out, ids_out);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -103,6 +103,11 @@ void mkref_decompose_digit(const mkref_ctx *c, int levelQ, int digit, const u64 
 
 /* KKLSS Decompose: all beta digits, out is beta x (levelQ+1+K) x N */
 void mkref_decompose(const mkref_ctx *c, int levelQ, const u64 *a, u64 *out);
+/* the digits of a poly at src_level >= levelQ (a: src_level+1 limbs; digit i =
+ * limbs i alpha .. min(i alpha + alpha - 1, src_level)), the first Beta(levelQ)
+ * of them, with the rows of level levelQ (Q[0..levelQ] u P) */
+void mkref_decompose_digit_src(const mkref_ctx *c, int levelQ, int src_level, int digit, const u64 *a, u64 *out);
+void mkref_decompose_src(const mkref_ctx *c, int levelQ, int src_level, const u64 *a, u64 *out);
 
 /* KKLSS ExternalProductHoisted: out (levelQ+1 limbs, coefficient) =
  * ModDown(INTT(sum_beta a_hoisted[b] (.) key[b])).  key rows for a given digit are
@@ -130,6 +135,15 @@ int mkref_kklss_mulrelin(const mkref_ctx *c, int level,
                          const u64 *const *rlk_b, const u64 *const *rlk_d,
                          const u64 *const *rlk_v, const u64 *crs_u,
                          u64 *out, int *ids_out);
+/* the same with operands at their own levels lv0, lv1 >= level (components of
+ * lv + 1 limbs), each decomposed at its own level (evaluator.go:433-439), the
+ * product taken at `level` */
+int mkref_kklss_mulrelin_lv(const mkref_ctx *c, int level, int lv0, int lv1,
+                            const u64 *op0, const int *ids0, int k0,
+                            const u64 *op1, const int *ids1, int k1,
+                            const u64 *const *rlk_b, const u64 *const *rlk_d,
+                            const u64 *const *rlk_v, const u64 *crs_u,
+                            u64 *out, int *ids_out);
 
 /* Rescale as KKLSS Evaluator.Rescale: number of divisions chosen by the float64
  * scale loop; returns nbRescales and writes the new scale. */

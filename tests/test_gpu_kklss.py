@@ -324,3 +324,50 @@ def test_split_external_products_agree(setup):
         ctx.set_tuning(mk.TUNE_SPLIT_LIMB, 0)
     assert np.array_equal(a, b) and np.array_equal(e0, e1)
     assert np.array_equal(e1, ref.external_product(a_h, key, level))
+
+
+def test_operands_at_their_own_levels_alpha2():
+    """alpha = 2 (4 P primes): operands at levels 4 and 2 are each decomposed at
+    their own level (mkckks/evaluator.go:433-439) and multiplied at level 2, so
+    op0's last digit spans limbs 2 and 3 -- through the host entry point and a
+    plan with the operand levels set, equal to the oracle's words; at level 3 (a
+    digit boundary) too."""
+    from params import primes
+
+    mk = _mk()
+    logN, Q, P = 13, primes(50, 5), primes(60, 4)
+    ctx, ref = mk.Context(logN, Q, P, gamma=2), O.Ctx(logN, Q, P, gamma=2)
+    try:
+        assert ctx.alpha == 2
+        keys, u = _keys(ctx, ref, [0, 1, 2], seed=61)
+        pids = sorted(keys)
+        kb, kd, kv = ([keys[p][j] for p in pids] for j in range(3))
+        g = O.Prng(62)
+        N = ref.N
+        op0 = np.stack([g.uniform(ref.Q, N) for _ in range(3)])  # level 4
+        for lv1 in (2, 3):
+            op1 = np.stack([g.uniform(ref.Q[:lv1 + 1], N) for _ in range(3)])
+            scale = float(ref.Q[lv1])
+            out, ids, level, _ = ctx.kklss_ckks_mulrelin(op0, [0, 1], op1, [1, 2], scale, scale, scale)
+            nres = lv1 - level
+            assert nres >= 1 and ids == [0, 1, 2]
+            exp, eids = ref.kklss_mulrelin_levels(op0, [0, 1], op1, [1, 2], kb, kd, kv, u)
+            exp = np.stack([ref.div_round_last(exp[i], nres) for i in range(exp.shape[0])])
+            assert eids == [0, 1, 2]
+            assert np.array_equal(out, exp), lv1
+            # the plan path with the operands' own levels
+            d0, d1, do = ctx.alloc(op0.nbytes), ctx.alloc(op1.nbytes), ctx.alloc(exp.nbytes)
+            try:
+                ctx.h2d(d0, op0)
+                ctx.h2d(d1, op1)
+                pl = mk.Plan(ctx, [dict(op0=d0, op1=d1, out=do, ids0=[0, 1], ids1=[1, 2], level=lv1, nres=nres,
+                                        level0=4, level1=lv1)], use_graph=True)
+                pl.launch()
+                ctx.sync()
+                pl.close()
+                assert np.array_equal(ctx.d2h(np.zeros(exp.shape, np.uint64), do), exp), lv1
+            finally:
+                for p in (d0, d1, do):
+                    ctx.free(p)
+    finally:
+        ctx.close()

@@ -290,6 +290,65 @@ def test_kklss_mulrelin_decrypts_to_the_product():
     assert max(abs(a - b) for a, b in zip(got2, exp2)) < 2 ** 12
 
 
+def test_decomposition_at_the_operands_own_level():
+    """An operand above the product's level is decomposed at its own level
+    (mkckks/evaluator.go:433-439): with alpha = 2 the last digit read at level 2
+    is the CRT integer of limbs 2 and 3 of the level-4 operand, not limb 2 alone,
+    and every row 0..2 and P holds its NTT; digits entirely below the level are
+    those of the ordinary decomposition."""
+    logN = 9
+    Q, P = O.gen_primes(50, 15, 5), O.gen_primes(60, 15, 4)
+    ctx = O.Ctx(logN, Q, P, gamma=2)
+    assert ctx.alpha == 2
+    a = O.Prng(31).uniform(Q, ctx.N)
+    level = 2
+    D = ctx.decompose_src(a, level)
+    plain = ctx.decompose(a[:level + 1], level)
+    assert D.shape == plain.shape == (2, level + 1 + len(P), ctx.N)
+    assert np.array_equal(D[0], plain[0])
+    assert not np.array_equal(D[1], plain[1])
+    X = _crt(a[2:4], Q[2:4])
+    for r, m in enumerate(Q[:level + 1] + P):
+        ri = r if r <= level else len(Q) + r - level - 1
+        row = np.array([v % m for v in X], np.uint64)
+        assert np.array_equal(D[1][r], ctx.ntt(row, ri)), r
+    # src level = level: the ordinary decomposition
+    assert np.array_equal(ctx.decompose_src(a[:level + 1], level), plain)
+
+
+def test_kklss_mulrelin_operands_at_their_own_levels():
+    """MulRelinNew of operands at levels 4 and 2 with alpha = 2: op0's last digit
+    at level 2 spans limbs 2 and 3, so the words differ from dropping op0 to level 2
+    first, and both decrypt to m0 m1 (each is a valid key switch mod Q_2); with
+    alpha = 1, or when level + 1 is a multiple of alpha, the two agree."""
+    logN = 9
+    Q, P = O.gen_primes(50, 15, 5), O.gen_primes(60, 15, 4)
+    ctx = O.Ctx(logN, Q, P, gamma=2)
+    keys = G.KKLSSKeys(ctx, 2, seed=33)
+    g = O.Prng(34)
+    N, D = ctx.N, 1 << 30
+    x = (np.array([int(v) for v in g.uniform([15], N)[0]]) - 7).astype(np.int64)
+    y = (np.array([int(v) for v in g.uniform([15], N)[0]]) - 7).astype(np.int64)
+    op0 = np.stack(keys.encrypt(0, O.small_to_rns(x * D, Q), g))
+    op1 = np.stack(keys.encrypt(1, O.small_to_rns(y * D, Q), g))
+    exp = G.negacyclic_mul([int(v) * D for v in x], [int(v) * D for v in y])
+    args = (keys.b, keys.d, keys.v, keys.u)
+    for lv1, differs in ((2, True), (3, False)):
+        own, ids = ctx.kklss_mulrelin_levels(op0, [0], op1[:, :lv1 + 1], [1], *args)
+        drop, ids2 = ctx.kklss_mulrelin(op0[:, :lv1 + 1], [0], op1[:, :lv1 + 1], [1], *args, level=lv1)
+        assert ids == ids2 == [0, 1]
+        assert (not np.array_equal(own, drop)) == differs, lv1
+        for out in (own, drop):
+            got = G.centered_crt(keys.decrypt(out, ids), Q[:lv1 + 1])
+            assert max(abs(a - b) for a, b in zip(got, exp)) < 2 ** 60, lv1
+    # alpha = 1: the digit of one limb does not depend on the level
+    ctx1 = O.Ctx(logN, Q, P[:2], gamma=2)
+    k1 = G.KKLSSKeys(ctx1, 2, seed=35)
+    own, _ = ctx1.kklss_mulrelin_levels(op0, [0], op1[:, :3], [1], k1.b, k1.d, k1.v, k1.u)
+    drop, _ = ctx1.kklss_mulrelin(op0[:, :3], [0], op1[:, :3], [1], k1.b, k1.d, k1.v, k1.u, level=2)
+    assert np.array_equal(own, drop)
+
+
 def test_planted_relin_mistake_is_caught():
     """Swapping the roles of d and b keys (a transposed operand) breaks decryption."""
     logN = 9
