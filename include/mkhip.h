@@ -428,6 +428,44 @@ int mkhip_encrypt(mkhip_ctx *ctx, const uint64_t *pk0, const uint64_t *pk1, int 
                   uint64_t *c1);
 
 /*
+ * KKLSS MK-BFV relinearization keys (mkbfv/keygen.go:24-162), a BFV context with
+ * gamma 2 (one prime per digit): errs [5][beta][N] are the errors of b_1, b_2,
+ * d_1, d_2 and v; CRS[0] (a_1), CRS[-3] (a_2, mkhip_set_crs(ctx, -3, a)) and u
+ * must be uploaded.
+ *   b_br = MForm(e) - MRed(a_br, s)                                  (:46-63)
+ *   d_br = MForm(G_br,i s + e) - MRed(a_br, r), GenBFVSwitchingKey's gadget
+ *          G_1,i = (Q QMul / q_i) t [(Q QMul / q_i)^-1]_q_i P / QMul over the Q
+ *          primes and G_2,i likewise over the QMul primes        (:65-73, 91-162)
+ *   v    = -(MForm(e + P r g_i) + MRed(u, s))                         (:75-85)
+ * b and d hold both branches (2 beta digits: branch 1 then 2) as
+ * mkhip_register_bfv_party takes them; mkhip_register_bfv_party_keygen generates
+ * and registers them for `party` without leaving HBM.
+ */
+int mkhip_gen_bfv_relin_key(mkhip_ctx *ctx, const uint64_t *sk, const uint64_t *r, const int64_t *errs, uint64_t *b,
+                            uint64_t *d, uint64_t *v);
+int mkhip_register_bfv_party_keygen(mkhip_ctx *ctx, int party, const uint64_t *sk, const uint64_t *r,
+                                    const int64_t *errs);
+
+/*
+ * MKF keys (mkf_cpu/common.go:32-51, Lattigo rlwe.KeyGenerator), a gamma-1 context:
+ *   mkhip_gen_mkf_switching_key  GenSwitchingKey(s_in, s_out): [beta][2][L+K][N] =
+ *                                (MForm(e + P s_in g_i) - MRed(a_i, s_out), a_i); a
+ *                                [beta][L+K][N] uniform (the reference samples it
+ *                                directly in NTT and Montgomery form), errs [beta][N]
+ *   mkhip_gen_mkf_relin_key      GenRelinearizationKey(s, 1): the switching key s^2 -> s
+ *   mkhip_register_mkf_group_keygen generateOnlineKeys for the k secrets sks
+ *                                [k][L+K][N]: fold keys s_i -> s = sum s_i (aggregateSecret)
+ *                                and the relinearization key of s, registered as `group`
+ *                                without leaving HBM; a [k+1][beta][L+K][N] and errs
+ *                                [k+1][beta][N] (fold keys in party order, then relin)
+ */
+int mkhip_gen_mkf_switching_key(mkhip_ctx *ctx, const uint64_t *s_in, const uint64_t *s_out, const uint64_t *a,
+                                const int64_t *errs, uint64_t *swk);
+int mkhip_gen_mkf_relin_key(mkhip_ctx *ctx, const uint64_t *s, const uint64_t *a, const int64_t *errs, uint64_t *swk);
+int mkhip_register_mkf_group_keygen(mkhip_ctx *ctx, int group, int k, const uint64_t *sks, const uint64_t *a,
+                                    const int64_t *errs);
+
+/*
  * Tuning knobs (no counterpart in the reference: they choose between equivalent
  * kernel schedules; every setting gives the same bits).
  *   MKHIP_TUNE_FUSED_DECOMP  alpha = 1 KKLSS plans: the gadget decomposition fused

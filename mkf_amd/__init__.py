@@ -47,6 +47,8 @@ EXPORTS = [
     "mkhip_del_conjugation_key", "mkhip_kklss_rotate", "mkhip_kklss_conjugate", "mkhip_plan_kklss_rotate",
     "mkhip_register_mkf_rotation", "mkhip_mkf_ckks_rotate", "mkhip_plan_mkf_rotate",
     "mkhip_gen_secret", "mkhip_gen_public_key", "mkhip_gen_relin_key", "mkhip_register_party_keygen",
+    "mkhip_gen_bfv_relin_key", "mkhip_register_bfv_party_keygen", "mkhip_gen_mkf_switching_key",
+    "mkhip_gen_mkf_relin_key", "mkhip_register_mkf_group_keygen",
     "mkhip_gen_rotation_key", "mkhip_gen_conjugation_key", "mkhip_encrypt",
     "mkhip_plan_mkf_poly", "mkhip_mkf_poly_query", "mkhip_mkf_ckks_evaluate_poly", "mkhip_mkf_ckks_add",
     "mkhip_mkf_ckks_add_const", "mkhip_mkf_ckks_mult_by_const", "mkhip_mkf_ckks_mult_by_gaussian_integer_and_add",
@@ -240,6 +242,11 @@ def _declare(L):
     L.mkhip_gen_public_key.argtypes = [V, u64p, i64p, u64p, u64p, u64p]
     L.mkhip_gen_relin_key.argtypes = [V, u64p, u64p, i64p, u64p, u64p, u64p]
     L.mkhip_register_party_keygen.argtypes = [V, I, u64p, u64p, i64p]
+    L.mkhip_gen_bfv_relin_key.argtypes = [V, u64p, u64p, i64p, u64p, u64p, u64p]
+    L.mkhip_register_bfv_party_keygen.argtypes = [V, I, u64p, u64p, i64p]
+    L.mkhip_gen_mkf_switching_key.argtypes = [V, u64p, u64p, u64p, i64p, u64p]
+    L.mkhip_gen_mkf_relin_key.argtypes = [V, u64p, u64p, i64p, u64p]
+    L.mkhip_register_mkf_group_keygen.argtypes = [V, I, I, u64p, u64p, i64p]
     L.mkhip_gen_rotation_key.argtypes = [V, I, u64p, i64p, u64p]
     L.mkhip_gen_conjugation_key.argtypes = [V, u64p, i64p, u64p]
     L.mkhip_encrypt.argtypes = [V, u64p, u64p, I, i64p, i64p, i64p, u64p, I, I, u64p, u64p]
@@ -516,6 +523,47 @@ class Context:
         a, p = self._i64(errs)
         check(lib().mkhip_register_party_keygen(self._h, pid, _p(_u64(sk)), _p(_u64(r)), p),
               "mkhip_register_party_keygen")
+
+    def gen_bfv_relin_key(self, sk, r, errs):
+        """KKLSS MK-BFV GenRelinearizationKey (mkbfv/keygen.go:24-162): errs (5, beta, N)
+        of b_1, b_2, d_1, d_2, v; CRS[0], CRS[-3] and u uploaded.  Returns b, d (both
+        branches, (2 beta, L+K, N)) and v ((beta, L+K, N))."""
+        a, p = self._i64(errs)
+        shp = (self.beta, self.L + self.K, self.N)
+        b, d = np.zeros((2,) + shp, np.uint64), np.zeros((2,) + shp, np.uint64)
+        v = np.zeros(shp, np.uint64)
+        check(lib().mkhip_gen_bfv_relin_key(self._h, _p(_u64(sk)), _p(_u64(r)), p, _p(b), _p(d), _p(v)),
+              "mkhip_gen_bfv_relin_key")
+        return b.reshape((2 * self.beta,) + shp[1:]), d.reshape((2 * self.beta,) + shp[1:]), v
+
+    def register_bfv_party_keygen(self, pid, sk, r, errs):
+        a, p = self._i64(errs)
+        check(lib().mkhip_register_bfv_party_keygen(self._h, pid, _p(_u64(sk)), _p(_u64(r)), p),
+              "mkhip_register_bfv_party_keygen")
+
+    def gen_mkf_switching_key(self, s_in, s_out, a_uniform, errs):
+        """Lattigo GenSwitchingKey(s_in, s_out) as MKF uses it (mkf_cpu/common.go:44-51):
+        a_uniform (beta, L+K, N), errs (beta, N); returns (beta, 2, L+K, N)."""
+        a, p = self._i64(errs)
+        out = np.zeros((self.beta, 2, self.L + self.K, self.N), np.uint64)
+        check(lib().mkhip_gen_mkf_switching_key(self._h, _p(_u64(s_in)), _p(_u64(s_out)), _p(_u64(a_uniform)), p,
+                                                _p(out)), "mkhip_gen_mkf_switching_key")
+        return out
+
+    def gen_mkf_relin_key(self, s, a_uniform, errs):
+        a, p = self._i64(errs)
+        out = np.zeros((self.beta, 2, self.L + self.K, self.N), np.uint64)
+        check(lib().mkhip_gen_mkf_relin_key(self._h, _p(_u64(s)), _p(_u64(a_uniform)), p, _p(out)),
+              "mkhip_gen_mkf_relin_key")
+        return out
+
+    def register_mkf_group_keygen(self, group, sks, a_uniform, errs):
+        """Fold keys s_i -> sum s_i and the relinearization key, generated and registered
+        in HBM: sks (k, L+K, N), a_uniform (k+1, beta, L+K, N), errs (k+1, beta, N)."""
+        sks = _u64(sks)
+        a, p = self._i64(errs)
+        check(lib().mkhip_register_mkf_group_keygen(self._h, group, sks.shape[0], _p(sks), _p(_u64(a_uniform)), p),
+              "mkhip_register_mkf_group_keygen")
 
     def gen_rotation_key(self, rotidx, sk, errs):
         a, p = self._i64(errs)

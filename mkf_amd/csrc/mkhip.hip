@@ -417,8 +417,9 @@ extern "C" int mkhip_set_crs(mkhip_ctx *c, int index, const uint64_t *swk) {
         return upload_key(c, swk, &c->d_crs_u);
     }
     // CRS[0] (the a of relinearization and public keys), rotation CRS (index =
-    // rotidx in [1, N/2)) or conjugation CRS (-2)
-    if (index != -2 && (index < 0 || index >= c->N / 2)) return MKHIP_ERR_ARG;
+    // rotidx in [1, N/2)), conjugation CRS (-2) or the second a of the MK-BFV
+    // relinearization keys (-3, mkbfv/keygen.go:37)
+    if (index != -2 && index != -3 && (index < 0 || index >= c->N / 2)) return MKHIP_ERR_ARG;
     u64 *p = nullptr;
     int s = upload_key(c, swk, &p);
     if (s != MKHIP_OK) return s;

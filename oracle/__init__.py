@@ -21,6 +21,7 @@ Parity status of each restated function (what pins it):
   kklss / mkf mul+relin     decrypt-and-compare tests (mkckks_test.go:352-358 bound)
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -127,6 +128,10 @@ def _declare(L):
     L.mkref_gen_public_key.argtypes = [V, u64p, i64p, u64p, u64p, u64p]
     L.mkref_gen_switching_key.argtypes = [V, u64p, i64p, u64p]
     L.mkref_gen_relin_key.argtypes = [V, u64p, u64p, i64p, u64p, u64p, u64p, u64p, u64p]
+    L.mkref_gen_bfv_relin_key.argtypes = [V, u64p, u64p, u64p, u64p, i64p, u64p, u64p, u64p, u64p, u64p, u64p]
+    L.mkref_gen_mkf_switching_key.argtypes = [V, u64p, u64p, u64p, i64p, u64p]
+    L.mkref_gen_mkf_relin_key.argtypes = [V, u64p, u64p, i64p, u64p]
+    L.mkref_aggregate_secret.argtypes = [V, I, u64p, u64p]
     L.mkref_gen_rotation_key.argtypes = [V, I, u64p, i64p, u64p, u64p]
     L.mkref_gen_conjugation_key.argtypes = [V, u64p, i64p, u64p, u64p]
     L.mkref_encrypt.argtypes = [V, I, u64p, u64p, i64p, i64p, i64p, u64p, I, I, u64p, u64p]
@@ -547,6 +552,66 @@ class KeyGen:
         lib().mkref_gen_relin_key(c._h, _p(_u64(sk)), _p(_u64(r)), _ip(self._i64(errs)), _p(_u64(a)), _p(_u64(u)),
                                   _p(b), _p(d), _p(v))
         return b, d, v
+
+    def bfv_gadget(self, Q, QMul, P, t):
+        """GenBFVSwitchingKey's gadget values G_br,i (mkbfv/keygen.go:91-162), with
+        exact integers as the reference computes them, reduced mod every row's
+        modulus: ([beta][L+K] for branch 1, the same for branch 2)."""
+        c = self.ctx
+        QP = [int(q) for q in c.QP]
+        alpha = c.alpha
+        Qp, QMp, Pp = math.prod(Q), math.prod(QMul), math.prod(P)
+        QQ = Qp * QMp
+        out = []
+        for primes in (Q, QMul):
+            tab = np.zeros((c.beta(), len(QP)), np.uint64)
+            for i in range(c.beta()):
+                Qi = math.prod(primes[i * alpha:(i + 1) * alpha])
+                Gi = (QQ // Qi) * t * pow(QQ // Qi, -1, Qi) * Pp // QMp
+                tab[i] = [Gi % q for q in QP]
+            out.append(tab)
+        return out
+
+    def bfv_relin_key(self, Q, QMul, P, t, sk, r, errs, a1, a2, u):
+        """KKLSS MK-BFV GenRelinearizationKey (mkbfv/keygen.go:24-162) over this Q u P
+        context: errs (5, beta, N) of b_1, b_2, d_1, d_2, v.  Returns b, d (2 beta
+        digits: branch 1 then 2) and v."""
+        c = self.ctx
+        gm1, gm2 = self.bfv_gadget(Q, QMul, P, t)
+        shp = (c.beta(), c.L + c.K, c.N)
+        b, d = np.zeros((2,) + shp, np.uint64), np.zeros((2,) + shp, np.uint64)
+        v = np.zeros(shp, np.uint64)
+        lib().mkref_gen_bfv_relin_key(c._h, _p(gm1), _p(gm2), _p(_u64(sk)), _p(_u64(r)), _ip(self._i64(errs)),
+                                      _p(_u64(a1)), _p(_u64(a2)), _p(_u64(u)), _p(b), _p(d), _p(v))
+        return b.reshape((2 * shp[0],) + shp[1:]), d.reshape((2 * shp[0],) + shp[1:]), v
+
+    def mkf_switching_key(self, s_in, s_out, a, errs):
+        """Lattigo GenSwitchingKey(s_in, s_out) (mkf_cpu/common.go:44-51): (beta, 2, L+K, N)."""
+        c = self.ctx
+        out = np.zeros((c.beta(), 2, c.L + c.K, c.N), np.uint64)
+        lib().mkref_gen_mkf_switching_key(c._h, _p(_u64(s_in)), _p(_u64(s_out)), _p(_u64(a)),
+                                          _ip(self._i64(errs)), _p(out))
+        return out
+
+    def mkf_relin_key(self, s, a, errs):
+        c = self.ctx
+        out = np.zeros((c.beta(), 2, c.L + c.K, c.N), np.uint64)
+        lib().mkref_gen_mkf_relin_key(c._h, _p(_u64(s)), _p(_u64(a)), _ip(self._i64(errs)), _p(out))
+        return out
+
+    def aggregate_secret(self, sks):
+        c = self.ctx
+        sks = _u64(sks)
+        out = np.zeros((c.L + c.K, c.N), np.uint64)
+        lib().mkref_aggregate_secret(c._h, sks.shape[0], _p(sks), _p(out))
+        return out
+
+    def mkf_group_keys(self, sks, a, errs):
+        """generateOnlineKeys (common.go:44-51): fold keys s_i -> sum s_i, then the
+        relinearization key of the sum; a (k+1, beta, L+K, N), errs (k+1, beta, N)."""
+        s = self.aggregate_secret(sks)
+        fold = [self.mkf_switching_key(sks[i], s, a[i], errs[i]) for i in range(len(sks))]
+        return fold, self.mkf_relin_key(s, a[len(sks)], errs[len(sks)])
 
     def rotation_key(self, rotidx, sk, errs, a):
         c = self.ctx

@@ -284,4 +284,15 @@ void mkref_modup_exact_simple(int N, const u64 *in, const u64 *Q, int nq, u64 *o
 #ifdef __cplusplus
 }
 #endif
+
+/* MK-BFV relinearization keys (mkbfv/keygen.go:24-162) and MKF keys
+ * (mkf_cpu/common.go:32-51), in mkref_keygen.c */
+void mkref_gen_bfv_relin_key(const mkref_ctx *c, const u64 *gm1, const u64 *gm2, const u64 *sk, const u64 *r,
+                             const int64_t *errs, const u64 *a1, const u64 *a2, const u64 *u, u64 *b, u64 *d,
+                             u64 *v);
+void mkref_gen_mkf_switching_key(const mkref_ctx *c, const u64 *skin, const u64 *skout, const u64 *a,
+                                 const int64_t *errs, u64 *swk);
+void mkref_gen_mkf_relin_key(const mkref_ctx *c, const u64 *sk, const u64 *a, const int64_t *errs, u64 *swk);
+void mkref_aggregate_secret(const mkref_ctx *c, int k, const u64 *sks, u64 *out);
+
 #endif

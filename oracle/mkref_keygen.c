@@ -15,6 +15,14 @@
  *   software/kklss/mkrlwe/keygen.go:240-268   GenConjugationKey
  *   software/kklss/mkrlwe/keygen.go:270-327   GenSwitchingKey
  *   software/kklss/mkrlwe/encryptor.go:55-119 Encryptor.Encrypt
+ *   software/kklss/mkbfv/keygen.go:24-162     MK-BFV GenRelinearizationKey and
+ *                                             GenBFVSwitchingKey (the gadget values
+ *                                             G_br,i mod q come in as a table: the
+ *                                             caller computes them with exact big
+ *                                             integers, as the reference does)
+ *   software/mkf/mkf_cpu/common.go:32-51      aggregateSecret / generateOnlineKeys
+ *                                             over Lattigo rlwe GenSwitchingKey and
+ *                                             GenRelinearizationKey(s, 1)
  *
  * Lattigo v2.3.0 primitives they call (absent; restated as published):
  * ExtendBasisSmallNormAndCenter (a small coefficient's residue in every modulus of
@@ -243,4 +251,111 @@ void mkref_encrypt(const mkref_ctx *c, int level, const u64 *pk0, const u64 *pk1
     }
     free(t);
     free(vm);
+}
+
+/* ---------------------------------------------------------------------------
+ * KKLSS MK-BFV relinearization keys (mkbfv/keygen.go:24-162).  errs [5][beta][N]
+ * are the errors of b_1, b_2, d_1, d_2, v; gm1 / gm2 [beta][L+K] are G_br,i mod q_r
+ * of GenBFVSwitchingKey (:91-162); a1 = CRS[0], a2 = CRS[-3], u = CRS[-1].
+ *   b_br = MForm(e - InvMForm(MRed(a_br, s)))                  (:46-63)
+ *   d_br = MForm(InvMForm(s) G_br,i + e) - MRed(a_br, r)       (:65-73)
+ *   v    = NegLvl(GenSwitchingKey(r) + MRed(u, s))             (:75-85)
+ * b and d hold branch 1 then branch 2 (2 beta digits each).
+ */
+void mkref_gen_bfv_relin_key(const mkref_ctx *c, const u64 *gm1, const u64 *gm2, const u64 *sk, const u64 *r,
+                             const int64_t *errs, const u64 *a1, const u64 *a2, const u64 *u, u64 *b, u64 *d,
+                             u64 *v) {
+    const int N = c->N, L = c->L, K = c->K, R = L + K, beta = mkref_beta(c, L - 1);
+    const size_t ksz = (size_t)R * N;
+    u64 *tmp = malloc(sizeof(u64) * ksz);
+    for (int br = 0; br < 2; br++) {
+        const u64 *a = br ? a2 : a1, *gm = br ? gm2 : gm1;
+        for (int i = 0; i < beta; i++) {
+            /* b_br */
+            small_ntt_qp(c, errs + ((size_t)br * beta + i) * N, 0, tmp);
+            for (int rr = 0; rr < R; rr++) {
+                u64 q = qp_mod(c, rr), qinv = c->ringQP.qinv[rr];
+                for (int x = 0; x < N; x++) {
+                    size_t o = i * ksz + (size_t)rr * N + x, sx = (size_t)rr * N + x;
+                    u64 m = mkref_mred(mkref_mred(a[o], sk[sx], q, qinv), 1, q, qinv);
+                    b[br * beta * ksz + o] = mkref_mform(subm(tmp[sx], m, q), q);
+                }
+            }
+            /* d_br */
+            small_ntt_qp(c, errs + ((size_t)(2 + br) * beta + i) * N, 0, tmp);
+            for (int rr = 0; rr < R; rr++) {
+                u64 q = qp_mod(c, rr), qinv = c->ringQP.qinv[rr];
+                u64 g = gm[(size_t)i * R + rr] % q;
+                for (int x = 0; x < N; x++) {
+                    size_t o = i * ksz + (size_t)rr * N + x, sx = (size_t)rr * N + x;
+                    u64 splain = mkref_mred(sk[sx], 1, q, qinv);
+                    u64 y = addm((u64)((u128)splain * g % q), tmp[sx], q);
+                    y = mkref_mform(y, q);
+                    d[br * beta * ksz + o] = subm(y, mkref_mred(a[o], r[sx], q, qinv), q);
+                }
+            }
+        }
+    }
+    mkref_gen_switching_key(c, r, errs + (size_t)4 * beta * N, v);
+    for (int i = 0; i < beta; i++)
+        for (int rr = 0; rr < R; rr++) {
+            u64 q = qp_mod(c, rr), qinv = c->ringQP.qinv[rr];
+            for (int x = 0; x < N; x++) {
+                size_t o = i * ksz + (size_t)rr * N + x;
+                u64 y = addm(v[o], mkref_mred(u[o], sk[(size_t)rr * N + x], q, qinv), q);
+                v[o] = q - y; /* NegLvl */
+            }
+        }
+    free(tmp);
+}
+
+/* ---------------------------------------------------------------------------
+ * Lattigo rlwe GenSwitchingKey(skIn, skOut) as the MKF keys use it
+ * (mkf_cpu/common.go:44-51): swk [beta][2][L+K][N] with
+ *   swk[i][0] = MForm(NTT(e_i)) + P skIn on digit i's Q limbs - MRed(a_i, skOut)
+ *   swk[i][1] = a_i    (uniform, taken as already in NTT and Montgomery form)
+ * a [beta][L+K][N], errs [beta][N].
+ */
+void mkref_gen_mkf_switching_key(const mkref_ctx *c, const u64 *skin, const u64 *skout, const u64 *a,
+                                 const int64_t *errs, u64 *swk) {
+    const int N = c->N, R = c->L + c->K, beta = mkref_beta(c, c->L - 1);
+    const size_t ksz = (size_t)R * N;
+    u64 *k0 = malloc(sizeof(u64) * ksz * beta);
+    mkref_gen_switching_key(c, skin, errs, k0);
+    sub_mred(c, k0, a, skout, beta);
+    for (int i = 0; i < beta; i++) {
+        memcpy(swk + (2 * (size_t)i) * ksz, k0 + (size_t)i * ksz, sizeof(u64) * ksz);
+        memcpy(swk + (2 * (size_t)i + 1) * ksz, a + (size_t)i * ksz, sizeof(u64) * ksz);
+    }
+    free(k0);
+}
+
+/* GenRelinearizationKey(s, 1): the switching key from s^2 (MRed(s, s) of the MForm
+ * secret is MForm(s^2)) to s */
+void mkref_gen_mkf_relin_key(const mkref_ctx *c, const u64 *sk, const u64 *a, const int64_t *errs, u64 *swk) {
+    const int N = c->N, R = c->L + c->K;
+    u64 *s2 = malloc(sizeof(u64) * (size_t)R * N);
+    for (int rr = 0; rr < R; rr++) {
+        u64 q = qp_mod(c, rr), qinv = c->ringQP.qinv[rr];
+        for (int x = 0; x < N; x++) {
+            size_t o = (size_t)rr * N + x;
+            s2[o] = mkref_mred(sk[o], sk[o], q, qinv);
+        }
+    }
+    mkref_gen_mkf_switching_key(c, s2, sk, a, errs, swk);
+    free(s2);
+}
+
+/* aggregateSecret (common.go:32-42): sum of the k secrets over Q u P */
+void mkref_aggregate_secret(const mkref_ctx *c, int k, const u64 *sks, u64 *out) {
+    const int N = c->N, R = c->L + c->K;
+    memset(out, 0, sizeof(u64) * (size_t)R * N);
+    for (int i = 0; i < k; i++)
+        for (int rr = 0; rr < R; rr++) {
+            u64 q = qp_mod(c, rr);
+            for (int x = 0; x < N; x++) {
+                size_t o = (size_t)rr * N + x;
+                out[o] = addm(out[o], sks[(size_t)i * R * N + o], q);
+            }
+        }
 }

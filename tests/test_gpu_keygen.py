@@ -99,3 +99,75 @@ def test_encrypt(kg, ntt_out, pt_ntt):
         got = ctx.encrypt(pk0, pk1, v, e0, e1, pt, level=level, pt_ntt=pt_ntt, ntt_out=ntt_out)
         exp = okg.encrypt(pk0, pk1, v, e0, e1, pt, level=level, pt_ntt=pt_ntt, ntt_out=ntt_out)
         assert np.array_equal(got[0], exp[0]) and np.array_equal(got[1], exp[1]), level
+
+
+@pytest.mark.parametrize("name", ["bsmall", "b3"])
+def test_bfv_relin_key_and_registration(name):
+    """KKLSS MK-BFV relinearization keys (mkbfv/keygen.go:24-162: both gadget
+    branches of d, b over CRS[0] and CRS[-3], v) equal the oracle's, and keys
+    generated and registered in HBM serve mul + relin like uploaded ones."""
+    import mkf_amd
+
+    from params import bfv_config
+
+    logN, Q, QM, P, t = bfv_config(name)
+    ctx = mkf_amd.Context(logN, Q, P, gamma=2, t=t, qmul=QM)
+    ref = O.Ctx(logN, Q, P, gamma=2)
+    okg = O.KeyGen(ref)
+    try:
+        g = O.Prng(311)
+        N, beta = ref.N, ref.beta()
+        a1, a2, u = _crs(ref, g), _crs(ref, g), _crs(ref, g)
+        ctx.set_crs(0, a1)
+        ctx.set_crs(-3, a2)
+        ctx.set_crs_u(u)
+        sks, rs, errs = [], [], []
+        for p in range(2):
+            sks.append(okg.secret(g.ternary(N)))
+            rs.append(okg.secret(g.ternary(N)))
+            errs.append(np.stack([np.stack([g.gaussian(N) for _ in range(beta)]) for _ in range(5)]))
+            got = ctx.gen_bfv_relin_key(sks[p], rs[p], errs[p])
+            exp = okg.bfv_relin_key(Q, QM, P, t, sks[p], rs[p], errs[p], a1, a2, u)
+            for x, y in zip(got, exp):
+                assert np.array_equal(x, y), (name, p)
+            ctx.register_bfv_party(10 + p, *exp)
+            ctx.register_bfv_party_keygen(20 + p, sks[p], rs[p], errs[p])
+        ops = [np.stack([g.uniform(Q, N) for _ in range(3)]) for _ in range(2)]
+        want = ctx.kklss_bfv_mulrelin(ops[0], [10, 11], ops[1], [10, 11])[0]
+        got = ctx.kklss_bfv_mulrelin(ops[0], [20, 21], ops[1], [20, 21])[0]
+        assert np.array_equal(got, want)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("name", ["small", "c2"])
+def test_mkf_keys_and_group_registration(name):
+    """MKF keys (mkf_cpu/common.go:32-51 over Lattigo GenSwitchingKey /
+    GenRelinearizationKey(s, 1)) equal the oracle's; a group generated and
+    registered in HBM (aggregate secret, fold keys, relinearization key) serves
+    MKF mul + relin like the same keys uploaded."""
+    import mkf_amd
+
+    logN, Q, P = config(name)
+    ctx = mkf_amd.Context(logN, Q, P, gamma=1)
+    ref = O.Ctx(logN, Q, P, gamma=1)
+    okg = O.KeyGen(ref)
+    try:
+        g = O.Prng(312)
+        N, QP, beta, k = ref.N, ref.QP, ref.beta(), 3
+        sks = np.stack([okg.secret(g.ternary(N)) for _ in range(k)])
+        a = np.stack([np.stack([g.uniform(QP, N) for _ in range(beta)]) for _ in range(k + 1)])
+        errs = np.stack([np.stack([g.gaussian(N) for _ in range(beta)]) for _ in range(k + 1)])
+        assert np.array_equal(ctx.gen_mkf_switching_key(sks[0], sks[1], a[0], errs[0]),
+                              okg.mkf_switching_key(sks[0], sks[1], a[0], errs[0]))
+        assert np.array_equal(ctx.gen_mkf_relin_key(sks[2], a[1], errs[1]), okg.mkf_relin_key(sks[2], a[1], errs[1]))
+        fold, relin = okg.mkf_group_keys(sks, a, errs)
+        ctx.register_mkf_group(1, fold, relin)
+        ctx.register_mkf_group_keygen(2, sks, a, errs)
+        left = np.stack([g.uniform(Q, N) for _ in range(1 + k)])
+        right = np.stack([g.uniform(Q, N) for _ in range(1 + k)])
+        want = ctx.mkf_ckks_mulrelin(1, left, right)[0]
+        got = ctx.mkf_ckks_mulrelin(2, left, right)[0]
+        assert np.array_equal(got, want)
+    finally:
+        ctx.close()
