@@ -307,6 +307,8 @@ def run_scheme(scheme, args, rng):
                         "p99_ms": round(pct(plat, 99) * 1e3, 3)}
         res["batched_over_plain"] = round(ops_s / res["plain"]["ops_per_s"], 3)
         pplan.close()
+        if scheme == "kklss_ckks":
+            res["plain_host"] = plain_host_rate(w, rng, args.plain_ops)
     if scheme == "kklss_ckks" and args.fp64_leg:
         # the same batch with the opt-in FP64 transforms for moduli below 2^50 (the
         # same words; the north-star arithmetic is the integer form above)
@@ -399,6 +401,27 @@ def mkfbench_rows(scheme, param, k, B, names, runs, warmup):
                      int(round(g["other_synchronization"] * 1e6 / B)), ndec, next_, "", "", "", "",
                      "hip", "linux", "gfx950", datetime.datetime.utcnow().isoformat() + "Z"])
     return rows
+
+
+def plain_host_rate(w, rng, n):
+    """The host-pointer entry point (mkhip_kklss_ckks_mulrelin: operands copied in
+    from host memory, the result copied back, one op per call) at the headline
+    shape: ops/s and latency, operands in ordinary (pageable) host memory."""
+    N, k, L = w.N, w.k, w.L
+    ids = list(range(k))
+    op0 = np.stack([uniform(rng, w.Q, N) for _ in range(1 + k)])
+    op1 = np.stack([uniform(rng, w.Q, N) for _ in range(1 + k)])
+    scale = float(w.Q[-1])
+    for _ in range(3):
+        w.ctx.kklss_ckks_mulrelin(op0, ids, op1, ids, scale, scale, scale)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(n):
+        s = time.perf_counter()
+        w.ctx.kklss_ckks_mulrelin(op0, ids, op1, ids, scale, scale, scale)
+        lat.append(time.perf_counter() - s)
+    el = time.perf_counter() - t0
+    return {"ops_per_s": round(n / el, 2), "p50_ms": round(pct(lat, 50) * 1e3, 3), "p99_ms": round(pct(lat, 99) * 1e3, 3)}
 
 
 def batch_sweep(w, ops, args, rng, plain):
@@ -715,10 +738,10 @@ def _serve_run(classes, args, rng, rate, duration, streams, nparty, kmin, kmax):
         pids = sorted(int(x) for x in rng.choice(nparty, size=k, replace=False))
         s0, s1 = (int(x) for x in rng.integers(0, c["npool"] - k, size=2))
         out = c["outs"][i % len(c["outs"])].a[:1 + k]
-        reqs.append((ids[ci], c, pids, s0, s1, out, k))
+        reqs.append((ids[ci], c, pids, s0, s1, out, k, ci))
     tickets, t0 = [], time.perf_counter()
     due = t0
-    for (cl, c, pids, s0, s1, out, k), gap in zip(reqs, gaps):
+    for (cl, c, pids, s0, s1, out, k, ci), gap in zip(reqs, gaps):
         due += gap
         while True:
             now = time.perf_counter()
@@ -729,27 +752,35 @@ def _serve_run(classes, args, rng, rate, duration, streams, nparty, kmin, kmax):
         tickets.append(srv.enqueue(cl, pa[s0:s0 + 1 + k], pa[s1:s1 + 1 + k], out, ids0=pids, ids1=pids,
                                    nres=c["nres"]))
     lat, qd, bad = [], [], 0
-    for t in tickets:
+    by_class = {}
+    for t, r in zip(tickets, reqs):
         st, l, q = srv.wait(t)
         bad += st != 0
         lat.append(l)
         qd.append(q)
+        by_class.setdefault(r[7], []).append(l)
     el = time.perf_counter() - t0
     stats = srv.stats()
     srv.close()
-    return {"offered_rps": rate, "requests": n, "errors": bad, "achieved_ops_per_s": round(n / el, 2),
-            "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3),
-            "queue_p50_ms": round(pct(qd, 50), 3), "mean_batch": round(stats["mean_batch"], 2),
-            "max_batch": stats["max_batch"], "max_inflight": stats["max_inflight"]}
+    res = {"offered_rps": rate, "requests": n, "errors": bad, "achieved_ops_per_s": round(n / el, 2),
+           "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3),
+           "queue_p50_ms": round(pct(qd, 50), 3), "mean_batch": round(stats["mean_batch"], 2),
+           "max_batch": stats["max_batch"], "max_inflight": stats["max_inflight"]}
+    if len(classes) > 1:
+        res["per_class"] = {"%s_N2^%d" % (classes[ci]["scheme"], classes[ci]["logN"]):
+                            {"requests": len(v), "p50_ms": round(pct(v, 50), 3), "p99_ms": round(pct(v, 99), 3)}
+                            for ci, v in sorted(by_class.items())}
+    return res
 
 
-def _serve_sweep(args, rng, specs, nparty, kmin, kmax, loads):
+def _serve_sweep(args, rng, specs, nparty, kmin, kmax, loads, burst):
     t_setup = time.perf_counter()
     classes = _serve_setup(args, rng, specs, nparty, kmax)
     setup_s = time.perf_counter() - t_setup
     run = lambda rate, dur: _serve_run(classes, args, rng, rate, dur, args.serve_streams, nparty, kmin, kmax)
-    # capacity: a saturating burst (every request at once)
-    sat = run(1e6, args.serve_burst / 1e6)
+    # capacity: a saturating burst (every request at once), after a warm-up burst
+    run(1e6, min(burst, 64) / 1e6)
+    sat = run(1e6, burst / 1e6)
     sweep = []
     for f in loads:
         sweep.append(run(sat["achieved_ops_per_s"] * f, args.serve_seconds))
@@ -770,13 +801,15 @@ def run_serving(args, rng):
     """Serving loop: Poisson arrivals of KKLSS MK-CKKS / MK-BFV mul+relin requests,
     k uniform in 2..16, N in {2^14, 2^15}, operands from registered parties whose
     keys stay resident; continuous batching, admission by device memory."""
-    return _serve_sweep(args, rng, SERVE_CLASSES, args.serve_parties, 2, args.serve_kmax, args.serve_load)
+    return _serve_sweep(args, rng, SERVE_CLASSES, args.serve_parties, 2, args.serve_kmax, args.serve_load,
+                        args.serve_burst)
 
 
 def run_serving_headline(args, rng):
     """The serving loop at the headline shape (KKLSS MK-CKKS, N=2^14, L=8, K=2,
     k=4): Poisson arrivals at fractions of the saturated rate."""
-    return _serve_sweep(args, rng, [("kklss", 14, 8, 2)], 4, 4, 4, args.serve_headline_load)
+    return _serve_sweep(args, rng, [("kklss", 14, 8, 2)], 4, 4, 4, args.serve_headline_load,
+                        args.serve_headline_burst)
 
 
 def _cpu_threads(requested):
@@ -905,9 +938,11 @@ def main():
                     help="length of each Poisson arrival run of the serving loop; 0 skips serving")
     ap.add_argument("--serve-load", type=lambda x: [float(v) for v in x.split(",")], default=[0.3, 0.6, 0.9],
                     help="offered load as fractions of the measured saturated rate")
-    ap.add_argument("--serve-burst", type=int, default=96, help="requests of the saturating burst")
+    ap.add_argument("--serve-burst", type=int, default=256, help="requests of the saturating burst")
     ap.add_argument("--serve-headline-load", type=lambda x: [float(v) for v in x.split(",")],
-                    default=[0.5, 0.7, 0.85, 0.95])
+                    default=[0.5, 0.6, 0.7, 0.8, 0.85, 0.9, 0.95])
+    ap.add_argument("--serve-headline-burst", type=int, default=384,
+                    help="requests of the saturating burst at the headline shape")
     ap.add_argument("--serve-parties", type=int, default=32)
     ap.add_argument("--serve-kmax", type=int, default=16)
     ap.add_argument("--serve-streams", type=int, default=4)
@@ -1014,6 +1049,7 @@ def main():
             "batched": {"ops_per_s": res["ops_per_s"], "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"],
                         "ops_per_launch": B},
             "plain": res.get("plain"), "batched_over_plain": res.get("batched_over_plain"),
+            "plain_host": res.get("plain_host"),
             "stages_ms": res["stages_ms"], "reference_stages_ms": res["reference_stages_ms"],
             "workspace_gb": res["workspace_gb"],
             "fp64_arithmetic": res.get("fp64_arithmetic"), "batch_sweep": res.get("batch_sweep"),
@@ -1040,7 +1076,12 @@ def main():
         best = max(ok, key=lambda r: r["achieved_ops_per_s"]) if ok else None
         sh["best_under_p99_10ms"] = best
         if best and plain.get("ops_per_s"):
+            # against the device-resident plain path (operands already in HBM) and the
+            # host-pointer plain path (operands copied in and out per op, as served)
             sh["served_over_plain"] = round(best["achieved_ops_per_s"] / plain["ops_per_s"], 3)
+            ph = line.get("plain_host") or {}
+            if ph.get("ops_per_s"):
+                sh["served_over_plain_host"] = round(best["achieved_ops_per_s"] / ph["ops_per_s"], 3)
         print("# serving (headline shape): saturated %.1f ops/s" % sh["saturated"]["achieved_ops_per_s"],
               file=sys.stderr, flush=True)
         line["serving"] = run_serving(args, rng)

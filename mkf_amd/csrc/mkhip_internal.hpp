@@ -108,6 +108,10 @@ struct PlanMem {
     hipStream_t stream = nullptr;
     bool query = false;
     size_t *ws_need = nullptr, *desc_need = nullptr;
+    // pinned host staging for the descriptor upload into desc (at least the
+    // descriptor bytes): the copy is then asynchronous for the calling thread
+    void *desc_host = nullptr;
+    size_t desc_host_bytes = 0;
 };
 }  // namespace mkhip
 

@@ -69,9 +69,15 @@ int plan_finish(mkhip_plan *&pl, const DescBlob &blob, int use_graph) {
             return MKHIP_ERR_OOM;
         }
         pl->d_desc = pm.desc;
-        pl->host_desc = blob.host;
-        e = hipMemcpyAsync(pl->d_desc, pl->host_desc.data(), pl->host_desc.size(), hipMemcpyHostToDevice,
-                           pm.stream ? pm.stream : c->stream);
+        const void *src = nullptr;
+        if (pm.desc_host && pm.desc_host_bytes >= blob.host.size()) {
+            std::memcpy(pm.desc_host, blob.host.data(), blob.host.size());
+            src = pm.desc_host;
+        } else {
+            pl->host_desc = blob.host;
+            src = pl->host_desc.data();
+        }
+        e = hipMemcpyAsync(pl->d_desc, src, blob.host.size(), hipMemcpyHostToDevice, pm.stream ? pm.stream : c->stream);
     } else {
         e = hipMalloc(&pl->d_desc, db);
         if (e == hipSuccess) e = hipMemcpy(pl->d_desc, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice);

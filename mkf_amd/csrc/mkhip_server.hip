@@ -10,9 +10,15 @@
 //   * continuous batching -- whenever one of the server's streams is free, the
 //     class (context) with the oldest waiting request is batched: its waiting
 //     requests, oldest first, up to max_batch ops and the free memory;
-//   * overlap -- batches on different streams run concurrently on the device,
-//     so requests that arrive while others are on the GPU start as soon as a
-//     stream and memory free up, not when the GPU drains.
+//   * overlap -- batches on different streams run concurrently on the device
+//     (their operand copies, kernels and result copies each on the batch's own
+//     stream, so one batch's copies run beside another's kernels), and requests
+//     that arrive while others are on the GPU start as soon as a stream and
+//     memory free up, not when the GPU drains.  Descriptor tables go up from
+//     pinned staging, so the scheduler thread never waits on a copy.  (One
+//     upload and one download stream shared in admission order was measured
+//     slower on MI355X: the host's enqueue latency between copies then leaves
+//     the link idle, where copies on several streams overlap it.)
 // Results do not depend on the mix: every op of a batch reads its own operands
 // and keys and writes its own output (the plans share no accumulator across ops).
 #include "mkhip_internal.hpp"
@@ -96,6 +102,8 @@ struct Class {
 struct Slot {
     hipStream_t st = nullptr;
     hipEvent_t done = nullptr;
+    void *desc_host = nullptr;          // pinned staging of the descriptor tables
+    size_t desc_host_bytes = 0;
     bool busy = false;
     mkhip_plan *plan = nullptr;
     size_t off = 0, bytes = 0;
@@ -276,6 +284,14 @@ bool launch_batch(mkhip_server *S, Class &C, Slot &sl) {
     for (Req *q : sl.reqs) q->t_start = now;
     std::vector<char> ops = make_ops(C, sl.reqs, n, d_io);
     int st = MKHIP_OK;
+    // pinned staging for the descriptor tables (grown, never per batch in steady state)
+    if (sl.desc_host_bytes < desc) {
+        if (sl.desc_host) (void)hipHostFree(sl.desc_host);
+        sl.desc_host = nullptr;
+        sl.desc_host_bytes = 0;
+        const size_t cap = std::max<size_t>(desc, 1 << 20);
+        if (hipHostMalloc(&sl.desc_host, cap, hipHostMallocDefault) == hipSuccess) sl.desc_host_bytes = cap;
+    }
     // operands in
     u64 *p = d_io;
     for (Req *q : sl.reqs) {
@@ -292,6 +308,8 @@ bool launch_batch(mkhip_server *S, Class &C, Slot &sl) {
     pm.desc = d_desc;
     pm.desc_bytes = desc;
     pm.stream = sl.st;
+    pm.desc_host = sl.desc_host;
+    pm.desc_host_bytes = sl.desc_host_bytes;
     if (st == MKHIP_OK) st = plan_build(C.c, C.scheme, (int)n, ops.data(), 0, &sl.plan, pm);
     if (st == MKHIP_OK) st = plan_launch_raw(sl.plan, sl.st);
     // results out
@@ -531,7 +549,9 @@ extern "C" void mkhip_server_destroy(mkhip_server *S) {
     for (Slot &sl : S->slots) {
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.st) (void)hipStreamDestroy(sl.st);
+        if (sl.desc_host) (void)hipHostFree(sl.desc_host);
     }
+
     if (S->d_pool) (void)hipFree(S->d_pool);
     delete S;
 }
