@@ -371,3 +371,51 @@ def test_operands_at_their_own_levels_alpha2():
                     ctx.free(p)
     finally:
         ctx.close()
+
+
+def test_fused_decomposition_grid_orders_agree():
+    """The fused decomposition's grid orders (ops adjacent, rows adjacent) with one
+    and two workgroups per limb give the same words on a 20-op batch of mixed
+    party sets; two ops are checked against the oracle."""
+    mk = _mk()
+    logN, Q, P = config("c2")
+    ctx, ref = mk.Context(logN, Q, P, gamma=2), O.Ctx(logN, Q, P, gamma=2)
+    bufs = []
+    try:
+        keys, u = _keys(ctx, ref, [0, 1, 2], seed=63)
+        g = O.Prng(64)
+        N, nl, B = ref.N, ref.L, 20
+        ops, hosts = [], []
+        for b in range(B):
+            ids0, ids1 = ([0, 1], [1, 2]) if b % 2 else ([0, 1, 2], [2])
+            op0 = np.stack([g.uniform(ref.Q, N) for _ in range(1 + len(ids0))])
+            op1 = np.stack([g.uniform(ref.Q, N) for _ in range(1 + len(ids1))])
+            d0, d1, do = ctx.alloc(op0.nbytes), ctx.alloc(op1.nbytes), ctx.alloc(4 * (nl - 1) * N * 8)
+            ctx.h2d(d0, op0)
+            ctx.h2d(d1, op1)
+            bufs += [d0, d1, do]
+            hosts.append((op0, ids0, op1, ids1, do))
+            ops.append(dict(op0=d0, op1=d1, out=do, ids0=ids0, ids1=ids1, level=nl - 1, nres=1))
+        results = []
+        for split in (0, 1):
+            for order in (0, 1):
+                ctx.set_tuning(mk.TUNE_SPLIT_LIMB, split)
+                ctx.set_tuning(mk.TUNE_DECOMP_ORDER, order)
+                pl = mk.Plan(ctx, ops, use_graph=False)
+                pl.launch()
+                ctx.sync()
+                pl.close()
+                results.append([ctx.d2h(np.zeros((4, nl - 1, N), np.uint64), h[4]) for h in hosts])
+        for r in results[1:]:
+            for a, b in zip(results[0], r):
+                assert np.array_equal(a, b)
+        for b in (0, 19):
+            op0, ids0, op1, ids1, _ = hosts[b]
+            exp, _ = _oracle_mulrelin(ref, op0, ids0, op1, ids1, keys, u, 1)
+            assert np.array_equal(results[-1][b], exp), b
+    finally:
+        ctx.set_tuning(mk.TUNE_SPLIT_LIMB, 0)
+        ctx.set_tuning(mk.TUNE_DECOMP_ORDER, 0)
+        for p in bufs:
+            ctx.free(p)
+        ctx.close()
