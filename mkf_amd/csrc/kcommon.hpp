@@ -329,7 +329,10 @@ __device__ __forceinline__ u64 redc_sum(u64 hi, u64 lo, const ModConst &m) {
 // row, where it waits while a0 is transformed).  d_i = d + i dstride, k_i = k + i
 // kstride.  Digits are walked in the outer loop (all 16 values of a row per
 // digit, four loads in flight per chunk).
-template <int LOGN, int NK>
+// LAZY (for 2^32 <= q and ndig q < 2^63, see ext_lazy_ok): the products'
+// MRed values are summed unreduced in (0, 2q) each and brought to canonical once,
+// after the last digit.
+template <int LOGN, int NK, bool LAZY = false>
 __device__ __forceinline__ void ext_rows(u64 (&a0)[16], u64 *park1, const u64 *d, size_t dstride, const u64 *k0,
                                          const u64 *k1, size_t kstride, int ndig, int tid, const ModConst &m) {
     typedef Quad<LOGN> Qd;
@@ -357,13 +360,32 @@ __device__ __forceinline__ void ext_rows(u64 (&a0)[16], u64 *park1, const u64 *d
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                a0[c + e] = add_mod(a0[c + e], mred(y0[e], x[e], m.q, m.qinv), m.q);
-                if (NK > 1) a1[c + e] = add_mod(a1[c + e], mred(y1[e], x[e], m.q, m.qinv), m.q);
+                if constexpr (LAZY) {
+                    a0[c + e] += mred_lazy(y0[e], x[e], m.q, m.qinv);
+                    if (NK > 1) a1[c + e] += mred_lazy(y1[e], x[e], m.q, m.qinv);
+                } else {
+                    a0[c + e] = add_mod(a0[c + e], mred(y0[e], x[e], m.q, m.qinv), m.q);
+                    if (NK > 1) a1[c + e] = add_mod(a1[c + e], mred(y1[e], x[e], m.q, m.qinv), m.q);
+                }
             }
             __builtin_amdgcn_sched_barrier(0);
         }
     }
+    if constexpr (LAZY) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            a0[e] = reduce_big(a0[e], m.q, m.mu);
+            if (NK > 1) a1[e] = reduce_big(a1[e], m.q, m.mu);
+        }
+    }
     if constexpr (NK > 1) store_row_quad<LOGN>(a1, park1, tid);
+}
+
+// Whether ext_rows may accumulate lazily: ndig sums of values below 2q stay
+// below 2^64, and reduce_big's quotient estimate needs q >= 2^32.
+__device__ __forceinline__ bool ext_lazy_ok(u64 q, int ndig) {
+    const u64 n = (u64)ndig;
+    return q >= ((u64)1 << 32) && mulhi64(n, q) == 0 && n * q < ((u64)1 << 63);
 }
 
 // ext_rows in FP64 (modulus below 2^50): a0[e] = sum_i MRed(k0_i[e], d_i[e]) as a

@@ -380,9 +380,12 @@ __global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ?
         else
             ntt_inv_fp_d<LOGN>(d0, a0, g_lds, t, itwd, fp_mod(m), fp_from_u64(m.ninv), 1, true);
     } else {
-        ext_rows<Gm::LS, NKEY>(a0, o1, Dd.D + (size_t)r * N + hoff, (size_t)rows * N,
-                               Dd.key[0] + (size_t)ri * N + hoff,
-                               NKEY > 1 ? Dd.key[1] + (size_t)ri * N + hoff : nullptr, kstride, beta, tid, m);
+        const u64 *dr = Dd.D + (size_t)r * N + hoff, *k0 = Dd.key[0] + (size_t)ri * N + hoff;
+        const u64 *k1 = NKEY > 1 ? Dd.key[1] + (size_t)ri * N + hoff : nullptr;
+        if (ext_lazy_ok(m.q, beta))
+            ext_rows<Gm::LS, NKEY, true>(a0, o1, dr, (size_t)rows * N, k0, k1, kstride, beta, tid, m);
+        else
+            ext_rows<Gm::LS, NKEY>(a0, o1, dr, (size_t)rows * N, k0, k1, kstride, beta, tid, m);
         // opaque(): the second transform reloads its twiddles instead of keeping the
         // first one's live across it
         inv_transform_sel<LOGN, SPLIT>(a0, opaque_lane(tid), h, T, ri, m, false);

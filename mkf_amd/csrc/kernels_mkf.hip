@@ -30,8 +30,11 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     u64 a0[16];
     u64 *o0 = Dd.acc[0] + (size_t)r * N + hoff, *o1 = Dd.acc[1] + (size_t)r * N + hoff;
     // swk digit b: output 0 at (2b) kstride, output 1 at (2b+1) kstride
-    ext_rows<Gm::LS, 2>(a0, o1, Dd.D + (size_t)r * N + hoff, (size_t)rows * N, Dd.swk + (size_t)ri * N + hoff,
-                        Dd.swk + kstride + (size_t)ri * N + hoff, 2 * kstride, beta, tid, m);
+    const u64 *dr = Dd.D + (size_t)r * N + hoff, *k0 = Dd.swk + (size_t)ri * N + hoff, *k1 = k0 + kstride;
+    if (ext_lazy_ok(m.q, beta))
+        ext_rows<Gm::LS, 2, true>(a0, o1, dr, (size_t)rows * N, k0, k1, 2 * kstride, beta, tid, m);
+    else
+        ext_rows<Gm::LS, 2>(a0, o1, dr, (size_t)rows * N, k0, k1, 2 * kstride, beta, tid, m);
     if (r > Dd.level || Dd.intt_all) {
         inv_transform_sel<LOGN>(a0, opaque_lane(tid), h, T, ri, m, false);
         store_row<Gm::LS>(a0, o0, tid);
