@@ -151,7 +151,10 @@ __device__ __forceinline__ void fwd_transform(u64 (&v)[16], F f, int tid, int h,
             lo[e] = f(tid + e * Gm::TH);
             hi[e] = f(Gm::NS + tid + e * Gm::TH);
         }
-        fwd_first_stage(v, lo, hi, tw, m.q, h);
+        if constexpr (MODE == 1)
+            fwd_first_stage_lazy(v, lo, hi, tw, m.q, h);
+        else
+            fwd_first_stage(v, lo, hi, tw, m.q, h);
         fwd_ntt_mode<Gm::LS, MODE>(v, tid, tw, m, 2 + h);
     } else {
 #pragma unroll
@@ -178,7 +181,10 @@ __device__ __forceinline__ void fwd_transform_row(u64 (&v)[16], const u64 *row, 
                 hi[e] = reduce64(hi[e], m.q, m.mu);
             }
         }
-        fwd_first_stage(v, lo, hi, tw, m.q, h);
+        if constexpr (MODE == 1)
+            fwd_first_stage_lazy(v, lo, hi, tw, m.q, h);
+        else
+            fwd_first_stage(v, lo, hi, tw, m.q, h);
         fwd_ntt_mode<Gm::LS, MODE>(v, tid, tw, m, 2 + h);
     } else {
         load_row_cached<LOGN>(v, row, tid);

@@ -325,6 +325,22 @@ __device__ __forceinline__ void fwd_first_stage(u64 (&v)[16], const u64 (&lo)[16
     }
 }
 
+// fwd_first_stage for the unreduced transforms (2^32 <= q < 2^57): only the kept
+// result is formed, with no conditional subtraction; out < lo + 4q (< 12q), after
+// which the 14 or fewer remaining unreduced stages stay below 68q < 2^64.
+__device__ __forceinline__ void fwd_first_stage_lazy(u64 (&v)[16], const u64 (&lo)[16], const u64 (&hi)[16],
+                                                     const ulonglong2 *__restrict__ tw, u64 q, int h) {
+    const ulonglong2 w = twiddle<true>(tw, 1);
+    const u64 q4 = q << 2;
+    if (__builtin_amdgcn_readfirstlane(h)) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = lo[i] + q4 - shoup_lazy4(hi[i], w.x, w.y, q);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = lo[i] + shoup_lazy4(hi[i], w.x, w.y, q);
+    }
+}
+
 // The same stage keeping both results: lo <- x + w y, hi <- x - w y.
 __device__ __forceinline__ void fwd_first_stage_both(u64 (&lo)[16], u64 (&hi)[16], const ulonglong2 *__restrict__ tw,
                                                      u64 q) {
