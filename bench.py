@@ -22,6 +22,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu
                        [--schemes kklss_ckks,mkf_ckks,kklss_bfv,mkf_bfv]
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -739,18 +740,27 @@ def _serve_run(classes, args, rng, rate, duration, streams, nparty, kmin, kmax):
         s0, s1 = (int(x) for x in rng.integers(0, c["npool"] - k, size=2))
         out = c["outs"][i % len(c["outs"])].a[:1 + k]
         reqs.append((ids[ci], c, pids, s0, s1, out, k, ci))
-    tickets, t0 = [], time.perf_counter()
-    due = t0
-    for (cl, c, pids, s0, s1, out, k, ci), gap in zip(reqs, gaps):
-        due += gap
-        while True:
-            now = time.perf_counter()
-            if now >= due:
-                break
-            time.sleep(min(due - now, 0.0005))
-        pa = c["pool"].a
-        tickets.append(srv.enqueue(cl, pa[s0:s0 + 1 + k], pa[s1:s1 + 1 + k], out, ids0=pids, ids1=pids,
-                                   nres=c["nres"]))
+    # operand views made before the clock starts; the collector is off while the
+    # schedule is replayed, so that a collection pause does not turn into a burst
+    views = [(c["pool"].a[s0:s0 + 1 + k], c["pool"].a[s1:s1 + 1 + k]) for (_, c, _, s0, s1, _, k, _) in reqs]
+    lag = []
+    gc.collect()
+    gc.disable()
+    try:
+        tickets, t0 = [], time.perf_counter()
+        due = t0
+        for (cl, c, pids, s0, s1, out, k, ci), (v0, v1), gap in zip(reqs, views, gaps):
+            due += gap
+            while True:
+                now = time.perf_counter()
+                if now >= due:
+                    break
+                if due - now > 0.0003:
+                    time.sleep(min(due - now - 0.0002, 0.0005))
+            lag.append(now - due)
+            tickets.append(srv.enqueue(cl, v0, v1, out, ids0=pids, ids1=pids, nres=c["nres"]))
+    finally:
+        gc.enable()
     lat, qd, bad = [], [], 0
     by_class = {}
     for t, r in zip(tickets, reqs):
@@ -764,7 +774,8 @@ def _serve_run(classes, args, rng, rate, duration, streams, nparty, kmin, kmax):
     srv.close()
     res = {"offered_rps": rate, "requests": n, "errors": bad, "achieved_ops_per_s": round(n / el, 2),
            "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3),
-           "queue_p50_ms": round(pct(qd, 50), 3), "mean_batch": round(stats["mean_batch"], 2),
+           "queue_p50_ms": round(pct(qd, 50), 3), "arrival_lag_p99_ms": round(pct(lag, 99) * 1e3, 3),
+           "arrival_lag_max_ms": round(max(lag) * 1e3, 3), "mean_batch": round(stats["mean_batch"], 2),
            "max_batch": stats["max_batch"], "max_inflight": stats["max_inflight"]}
     if len(classes) > 1:
         res["per_class"] = {"%s_N2^%d" % (classes[ci]["scheme"], classes[ci]["logN"]):
