@@ -1025,6 +1025,11 @@ def main():
         if pmc.get("bytes_per_launch"):
             roofline["traffic_over_alg"] = round(pmc["bytes_per_launch"] / dom_bytes, 3)
             roofline["traffic_kernel_ms"] = pmc.get("kernel_ms")
+            # the profiled instance (the counters are per template instance: whole or
+            # half-limb workgroups) beside the one this run's tuning launched
+            roofline["traffic_instance"] = pmc.get("instance")
+            if kern == "k_decomp_mac":
+                roofline["instance"] = "k_decomp_mac<%d, %s>" % (w.logN, "true" if res["tuning"]["split_limb"] else "false")
         if pmc.get("valu_busy_pct") is not None:
             roofline["valu_busy_pct"] = pmc["valu_busy_pct"]
         # every stage's algorithmic bytes over its measured time, as a fraction of the HBM peak
@@ -1065,6 +1070,7 @@ def main():
             "workspace_gb": res["workspace_gb"],
             "fp64_arithmetic": res.get("fp64_arithmetic"), "batch_sweep": res.get("batch_sweep"),
             "peaks": peaks, "roofline": roofline, "cpu_baseline": cpu,
+            "tuning": res["tuning"], "autotune_ms": res["autotune_ms"],
         })
         plan.close()
         w.close()

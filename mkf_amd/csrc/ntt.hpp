@@ -331,14 +331,12 @@ __device__ __forceinline__ void fwd_first_stage(u64 (&v)[16], const u64 (&lo)[16
 __device__ __forceinline__ void fwd_first_stage_lazy(u64 (&v)[16], const u64 (&lo)[16], const u64 (&hi)[16],
                                                      const ulonglong2 *__restrict__ tw, u64 q, int h) {
     const ulonglong2 w = twiddle<true>(tw, 1);
-    const u64 q4 = q << 2;
-    if (__builtin_amdgcn_readfirstlane(h)) {
+    // half 1 keeps lo + 4q - t = lo + (t ^ ~0) + 4q + 1, half 0 lo + t: one form
+    // with a uniform mask, so that each value is formed as soon as its product is
+    const u64 mask = 0 - (u64)(__builtin_amdgcn_readfirstlane(h) & 1);
+    const u64 add = mask & ((q << 2) + 1);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) v[i] = lo[i] + q4 - shoup_lazy4(hi[i], w.x, w.y, q);
-    } else {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) v[i] = lo[i] + shoup_lazy4(hi[i], w.x, w.y, q);
-    }
+    for (int i = 0; i < 16; ++i) v[i] = lo[i] + (shoup_lazy4(hi[i], w.x, w.y, q) ^ mask) + add;
 }
 
 // The same stage keeping both results: lo <- x + w y, hi <- x - w y.
