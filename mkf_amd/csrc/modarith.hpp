@@ -118,12 +118,28 @@ __device__ __forceinline__ u64 reduce_big(u64 x, u64 q, u64 mu) {
     return r >= q ? r - q : r;
 }
 
-// a * b * 2^-64 mod q in (0, 2q) for a * b < q * 2^64 (MRed without its final
-// conditional subtraction).
+// (u64)a * b + c: one v_mad_u64_u32.
+__device__ __forceinline__ u64 mad32(u32 a, u32 b, u64 c) { return (u64)a * b + c; }
+
+// a * b * 2^-64 mod q in (0, 2q) for a, b < q < 2^61 (MRed without its final
+// conditional subtraction: hi(ab) - hi(m q) + q with m = lo(ab) q^-1 mod 2^64).
+// Both 128-bit products are formed from 32-bit multiply-adds whose addends are
+// the running partial sums, so that no 64-bit addend has to be assembled from a
+// 32-bit half and a zero: with a1, b1, q1 < 2^29 no partial sum exceeds 2^64
+// (the middle sum of m q, which can, is split at bit 32).
 __device__ __forceinline__ u64 mred_lazy(u64 a, u64 b, u64 q, u64 qinv) {
-    u64 hi, lo;
-    mul_wide(a, b, hi, lo);
-    return hi - mulhi64(lo * qinv, q) + q;
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    const u64 p00 = (u64)a0 * b0;
+    const u64 mid = mad32(a0, b1, mad32(a1, b0, p00 >> 32));  // a0 b1 + a1 b0 + hi32(a0 b0)
+    const u64 hi = mad32(a1, b1, mid >> 32);                  // hi64(a b)
+    const u32 l0 = (u32)p00, l1 = (u32)mid;                   // lo64(a b)
+    const u32 q0 = (u32)q, q1 = (u32)(q >> 32), i0 = (u32)qinv, i1 = (u32)(qinv >> 32);
+    const u64 mm = (u64)l0 * i0;
+    const u32 m0 = (u32)mm, m1 = (u32)(mm >> 32) + l0 * i1 + l1 * i0;
+    const u64 s = mad32(m1, q0, ((u64)m0 * q0) >> 32);        // m1 q0 + hi32(m0 q0) < 2^64
+    const u64 t = mad32(m0, q1, (u32)s);                      // m0 q1 + lo32(s) < 2^62
+    const u64 mqh = mad32(m1, q1, s >> 32) + (t >> 32);       // hi64(m q)
+    return hi + q - mqh;
 }
 
 MKHIP_HD u64 add_mod(u64 a, u64 b, u64 q) {
