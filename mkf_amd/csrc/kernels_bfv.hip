@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
 template <int LOGN>
 static hipError_t launch_tensor_r(const DevTables &T, const void *descs, int n, int nl, hipStream_t st) {
     typedef Geo<LOGN> Gm;
-    constexpr size_t lds = (size_t)Gm::NS * sizeof(u64);
+    constexpr size_t lds = (size_t)lds_words(Gm::NS) * sizeof(u64);
     static bool init = false;
     if (!init) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_tensor_r<LOGN>),

@@ -177,7 +177,7 @@ template <int LOGN, bool SPLIT = (LOGN == 15)>
 static hipError_t launch_fused_n(int what, const DevTables &T, const void *descs, int n, int a, int b,
                                  hipStream_t st) {
     typedef Geo<LOGN, SPLIT> Gm;
-    constexpr size_t lds = (size_t)Gm::NS * sizeof(u64);
+    constexpr size_t lds = (size_t)lds_words(Gm::NS) * sizeof(u64);
     static bool init = false;
     if (!init) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_decomp_mac<LOGN, SPLIT>),

@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(256) k_keyop(DevTables T, const KeyOpDesc *__r
 template <int LOGN>
 static hipError_t launch_small_ntt(const DevTables &T, const void *descs, int n, hipStream_t st) {
     typedef Geo<LOGN> Gm;
-    constexpr size_t lds = (size_t)Gm::NS * sizeof(u64);
+    constexpr size_t lds = (size_t)lds_words(Gm::NS) * sizeof(u64);
     static bool init = false;
     if (!init) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_small_ntt<LOGN>),

@@ -166,7 +166,7 @@ template <int LOGN>
 static hipError_t launch_mkf_n(int what, const DevTables &T, const void *descs, int n, int a, int b,
                                hipStream_t st) {
     typedef Geo<LOGN> Gm;
-    constexpr size_t lds = (size_t)Gm::NS * sizeof(u64);
+    constexpr size_t lds = (size_t)lds_words(Gm::NS) * sizeof(u64);
     constexpr int H = Gm::H;
     static bool init = false;
     if (!init) {

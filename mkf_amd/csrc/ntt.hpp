@@ -11,7 +11,8 @@
 //
 // Implementation: every thread holds 16 coefficients in registers.  Stages are
 // processed in passes of up to 4 (radix-16 in registers); between passes the
-// limb is exchanged through LDS (N * 8 bytes: 128 KiB at N = 2^14).  Butterflies
+// limb is exchanged through LDS (N * 8 bytes plus one pad word per 32: 132 KiB
+// at N = 2^14).  Butterflies
 // are Harvey's lazy forms with Shoup twiddles whose quotient is approximated
 // (shoup_lazy4: product in [0, 4q)); values stay in [0, 8q) forward and
 // [0, 4q) inverse (q < 2^61) and are reduced to canonical residues on the way out.
@@ -83,6 +84,15 @@ __device__ __forceinline__ ulonglong2 twiddle(const ulonglong2 *__restrict__ tw,
     return w;
 }
 
+// Closed form of the padded-LDS condition (the CPU tests check it
+// exhaustively) for a pass of line length L (TL or TF) and radix R on T lanes:
+// with L a power of two dividing T and T R a multiple of 32, base mod 32 is below
+// min(L, 32) (plus multiples of R L when R L < 32) and off mod 32 is a multiple of
+// L below 32 minus that bound, so base mod 32 + off mod 32 never reaches 32.
+__host__ __device__ constexpr bool pad_split_ok(int L, int T, int R) {
+    return L > 0 && (L & (L - 1)) == 0 && T % L == 0 && (T * R) % 32 == 0;
+}
+
 // ---- pass geometry -------------------------------------------------------
 // A forward pass (S0, NS) covers stages S0..S0+NS-1.  Its "columns" are the sets
 // of 2^NS coefficients that interact within the pass: column col has base
@@ -96,12 +106,16 @@ struct FwdPass {
     static constexpr int TL = N >> (S0 + NS);
     static constexpr int R = 1 << NS;
     static constexpr int C = 16 / R;
-    __device__ static __forceinline__ int elem(int tid, int i) {
+    __host__ __device__ static constexpr int elem(int tid, int i) {
         const int c = i / R, k = i % R;
         const int col = tid + c * T;
         const int b = col / TL, r = col % TL;
         return b * R * TL + r + k * TL;
     }
+    // elem = base(tid) + off(i) (TL divides T): the lane's part and a constant
+    __host__ __device__ static constexpr int base(int tid) { return (tid / TL) * R * TL + tid % TL; }
+    __host__ __device__ static constexpr int off(int i) { return (i / R) * T * R + (i % R) * TL; }
+    static constexpr bool pad_ok = pad_split_ok(TL, T, R);
     // hh: twiddle multiplier -- 1 for a whole transform; 2 + h for half h of a
     // transform of twice this size (its stage s twiddles start at 2^(s+1) + h 2^s)
     template <bool LAZY = false>
@@ -143,12 +157,15 @@ struct InvPass {
     static constexpr int TF = 1 << U0;
     static constexpr int R = 1 << NS;
     static constexpr int C = 16 / R;
-    __device__ static __forceinline__ int elem(int tid, int i) {
+    __host__ __device__ static constexpr int elem(int tid, int i) {
         const int c = i / R, k = i % R;
         const int col = tid + c * T;
         const int b = col / TF, r = col % TF;
         return b * R * TF + r + k * TF;
     }
+    __host__ __device__ static constexpr int base(int tid) { return (tid / TF) * R * TF + tid % TF; }
+    __host__ __device__ static constexpr int off(int i) { return (i / R) * T * R + (i % R) * TF; }
+    static constexpr bool pad_ok = pad_split_ok(TF, T, R);
     __device__ static __forceinline__ void run(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ itw, u64 q,
                                                int hh) {
         const u64 q4 = q << 2;
@@ -176,18 +193,28 @@ struct InvPass {
     }
 };
 
-// LDS word of element idx: bits 4..7 are folded into the bank bits 0..3, so the
-// strided patterns of the radix-16 / radix-4 passes (element strides of 4, 32 and
-// 64 words across the lanes) spread over all banks instead of a quarter of them.
-__device__ __forceinline__ int lds_swizzle(int idx) { return idx ^ ((idx >> 4) & 15); }
+// LDS word of element idx: one pad word after every 32, so that the strided
+// patterns of the passes (lanes 32 or 64 words apart within a wave) start on
+// different banks.  Every pass layout splits as elem = base(tid) + off(i) with
+// base mod 32 + off mod 32 < 32, so lds_pad(elem) = lds_pad(base) + lds_pad(off):
+// one address register per lane and compile-time offsets for the 16 values
+// (pad_ok asserts the closed form of this condition at compile time; the CPU
+// tests check every lane).
+__host__ __device__ constexpr int lds_pad(int idx) { return idx + (idx >> 5); }
+
+// LDS words a workgroup holding ns elements uses.
+__host__ __device__ constexpr int lds_words(int ns) { return ns + ns / 32; }
 
 template <class From, class To>
 __device__ __forceinline__ void lds_exchange(u64 (&v)[16], u64 *s, int tid) {
+    static_assert(From::pad_ok && To::pad_ok, "padded LDS layout");
+    u64 *w = s + lds_pad(From::base(tid));
 #pragma unroll
-    for (int i = 0; i < 16; ++i) s[lds_swizzle(From::elem(tid, i))] = v[i];
+    for (int i = 0; i < 16; ++i) w[lds_pad(From::off(i))] = v[i];
     __syncthreads();
+    const u64 *r = s + lds_pad(To::base(tid));
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = s[lds_swizzle(To::elem(tid, i))];
+    for (int i = 0; i < 16; ++i) v[i] = r[lds_pad(To::off(i))];
     __syncthreads();
 }
 

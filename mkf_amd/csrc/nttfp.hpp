@@ -126,12 +126,15 @@ struct FwdPassFp : FwdPass<LOGN, S0, NS> {
 
 template <class From, class To>
 __device__ __forceinline__ void lds_exchange_fp(double (&v)[16], u64 *s, int tid) {
+    static_assert(From::pad_ok && To::pad_ok, "padded LDS layout");
     double *d = reinterpret_cast<double *>(s);
+    double *w = d + lds_pad(From::base(tid));
 #pragma unroll
-    for (int i = 0; i < 16; ++i) d[lds_swizzle(From::elem(tid, i))] = v[i];
+    for (int i = 0; i < 16; ++i) w[lds_pad(From::off(i))] = v[i];
     __syncthreads();
+    const double *r = d + lds_pad(To::base(tid));
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = d[lds_swizzle(To::elem(tid, i))];
+    for (int i = 0; i < 16; ++i) v[i] = r[lds_pad(To::off(i))];
     __syncthreads();
 }
 
