@@ -90,6 +90,12 @@ __device__ __forceinline__ void store_row_quad(const u64 (&v)[16], u64 *__restri
     }
 }
 
+// Wave-uniform copy of an index the compiler cannot prove uniform (one derived from
+// blockIdx through a run-time division, which it computes on the VALU): the
+// descriptor loads and row addresses that depend on it then stay in scalar
+// registers instead of being rebuilt with readfirstlane before every access.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // A wave-uniform pointer made opaque to the optimizer: table loads that use it
 // stay where they are written instead of being hoisted out of an enclosing loop
 // and held in registers across its iterations.

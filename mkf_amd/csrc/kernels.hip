@@ -242,10 +242,10 @@ __global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ?
     int bid = block_index(T);
     const int h = Gm::split ? bid % 2 : 0;
     bid /= Gm::H;
-    const int r = bid % rows;
-    const int i = (bid / rows) % beta;
-    const int p = (bid / (rows * beta)) % kmax;
-    const int d = bid / (rows * beta * kmax);
+    const int r = uni(bid % rows);
+    const int i = uni((bid / rows) % beta);
+    const int p = uni((bid / (rows * beta)) % kmax);
+    const int d = uni(bid / (rows * beta * kmax));
     const DecompDesc &Dd = descs[d];
     if (p >= Dd.k) return;
     const int level = Dd.level;
@@ -317,8 +317,8 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     int bid = block_index(T);
     const int h = Gm::split ? bid % 2 : 0;
     bid /= Gm::H;
-    const int l = bid % nl;
-    const TensorDesc &Dd = descs[bid / nl];
+    const int l = uni(bid % nl);
+    const TensorDesc &Dd = descs[uni(bid / nl)];
     const ModConst m = T.mod[l];
     const size_t off = (size_t)l * N + (size_t)h * Gm::NS;
     const size_t uoff = (size_t)l * (Dd.ustride > 1 ? Dd.ustride : 1) * N + (size_t)h * Gm::NS;
@@ -360,8 +360,8 @@ __global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ?
     const int h = Gm::split ? bid % 2 : 0;
     bid /= Gm::H;
     const int ndesc = gridDim.x / (rows * Gm::H);
-    const int r = bid / ndesc;
-    const ExtDesc &Dd = descs[bid % ndesc];
+    const int r = uni(bid / ndesc);
+    const ExtDesc &Dd = descs[uni(bid % ndesc)];
     const int ri = ring_index(r, Dd.level, T.L);
     const ModConst m = T.mod[ri];
     const size_t kstride = (size_t)(T.L + T.K) * N;

@@ -140,8 +140,8 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     int bid = block_index(T);
     const int h = Gm::split ? bid % 2 : 0;
     bid /= Gm::H;
-    const int l = bid % nl;
-    const TensorDesc &Dd = descs[bid / nl];
+    const int l = uni(bid % nl);
+    const TensorDesc &Dd = descs[uni(bid / nl)];
     const int ri = r_index(l, T.L, T.K);
     const ModConst m = T.mod[ri];
     const size_t off = (size_t)l * N + (size_t)h * Gm::NS;

@@ -154,8 +154,10 @@ __global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ?
         o = bid % nops;
         bid /= nops;
     }
-    const int i = bid % beta;
-    const int side = bid / beta;
+    o = uni(o);
+    r = uni(r);
+    const int i = uni(bid % beta);
+    const int side = uni(bid / beta);
     const DecompDesc &Dd = descs[2 * o + side];
     const int ri = ring_index(r, Dd.level, T.L);
     const ModConst m = T.mod[ri];

@@ -21,8 +21,8 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     int bid = block_index(T);
     const int h = Gm::split ? bid % 2 : 0;
     bid /= Gm::H;
-    const int r = bid % rows;
-    const Mac2Desc &Dd = descs[bid / rows];
+    const int r = uni(bid % rows);
+    const Mac2Desc &Dd = descs[uni(bid / rows)];
     const int ri = ring_index(r, Dd.level, T.L);
     const ModConst m = T.mod[ri];
     const size_t kstride = (size_t)(T.L + T.K) * N;
@@ -56,8 +56,8 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     int bid = block_index(T);
     const int h = Gm::split ? bid % 2 : 0;
     bid /= Gm::H;
-    const int j = bid % nl;
-    const MdNttDesc &Dd = descs[bid / nl];
+    const int j = uni(bid % nl);
+    const MdNttDesc &Dd = descs[uni(bid / nl)];
     const ModConst m = T.mod[j];
     const ulonglong2 *tw = T.tw + (size_t)j * N;
     const u64 pinv = T.md_pinv[j];
@@ -122,8 +122,8 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     int bid = block_index(T);
     const int h = Gm::split ? bid % 2 : 0;
     bid /= Gm::H;
-    const int j = bid % nlo;
-    const RescaleNttDesc &Dd = descs[bid / nlo];
+    const int j = uni(bid % nlo);
+    const RescaleNttDesc &Dd = descs[uni(bid / nlo)];
     const ModConst m = T.mod[j];
     const size_t hoff = (size_t)h * Gm::NS;
     u64 v[16];
