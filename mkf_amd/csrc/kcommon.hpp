@@ -133,11 +133,12 @@ struct Geo {
 
 // MODE: -1 picks the unreduced butterflies at run time when the modulus allows
 // them; 1 / 0 force the unreduced / reduced form (the caller checked fwd_lazy_ok).
+// canon = false: the unreduced form (MODE 1) leaves its output below 68q.
 template <int LOGN, int MODE = -1>
 __device__ __forceinline__ void fwd_ntt_mode(u64 (&v)[16], int tid, const ulonglong2 *tw, const ModConst &m,
-                                             int hh) {
+                                             int hh, bool canon = true) {
     if constexpr (MODE == 1)
-        ntt_fwd<LOGN, true>(v, g_lds, tid, tw, m.q, hh, m.mu);
+        ntt_fwd<LOGN, true>(v, g_lds, tid, tw, m.q, hh, m.mu, canon);
     else if constexpr (MODE == 0)
         ntt_fwd<LOGN, false>(v, g_lds, tid, tw, m.q, hh);
     else
@@ -174,7 +175,7 @@ __device__ __forceinline__ void fwd_transform(u64 (&v)[16], F f, int tid, int h,
 // per-value select, and the loads use the row's scalar base.
 template <int LOGN, int MODE = -1, bool SPLIT = (LOGN == 15)>
 __device__ __forceinline__ void fwd_transform_row(u64 (&v)[16], const u64 *row, bool red, int tid, int h,
-                                                  const ulonglong2 *tw, const ModConst &m) {
+                                                  const ulonglong2 *tw, const ModConst &m, bool canon = true) {
     typedef Geo<LOGN, SPLIT> Gm;
     if constexpr (Gm::split) {
         u64 lo[16], hi[16];
@@ -191,14 +192,14 @@ __device__ __forceinline__ void fwd_transform_row(u64 (&v)[16], const u64 *row, 
             fwd_first_stage_lazy(v, lo, hi, tw, m.q, h);
         else
             fwd_first_stage(v, lo, hi, tw, m.q, h);
-        fwd_ntt_mode<Gm::LS, MODE>(v, tid, tw, m, 2 + h);
+        fwd_ntt_mode<Gm::LS, MODE>(v, tid, tw, m, 2 + h, canon);
     } else {
         load_row_cached<LOGN>(v, row, tid);
         if (red) {
 #pragma unroll
             for (int e = 0; e < 16; ++e) v[e] = reduce64(v[e], m.q, m.mu);
         }
-        fwd_ntt_mode<LOGN, MODE>(v, tid, tw, m, 1);
+        fwd_ntt_mode<LOGN, MODE>(v, tid, tw, m, 1, canon);
     }
 }
 
@@ -399,6 +400,14 @@ __device__ __forceinline__ bool ext_lazy_ok(u64 q, int ndig) {
     const u64 n = (u64)ndig;
     return q >= ((u64)1 << 32) && mulhi64(n, q) == 0 && n * q < ((u64)1 << 63);
 }
+
+// Whether a digit row of modulus q may be stored and used unreduced (below 68q,
+// the bound of the unreduced forward transform): its consumers take it as the
+// second operand of a Montgomery product with a canonical first operand
+// (mred_lazy, mred, mulmod_m), which stays exact for operands below 2^63.1 and
+// moduli below 2^61 -- 68q < 2^63.1 holds for every q the unreduced transform
+// takes (fwd_lazy_ok: q < 2^57).
+__device__ __forceinline__ bool lazy_digits_ok(u64 q) { return fwd_lazy_ok(q); }
 
 // ext_rows in FP64 (modulus below 2^50): a0[e] = sum_i MRed(k0_i[e], d_i[e]) as a
 // double congruent residue (|a0| < 2.6q), computed as (sum_i k0_i d_i) 2^-64; with

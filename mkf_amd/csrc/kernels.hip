@@ -155,14 +155,14 @@ __global__ void __launch_bounds__(256) k_digit_pre(DevTables T, const DecompDesc
 template <int LOGN, bool CRT, int MODE, bool SPLIT = (LOGN == 15)>
 __device__ __forceinline__ void decomp_row(const DevTables &T, const u64 *src, u64 *dst, int start, int nd,
                                            bool own_row, int i, int r, int ri, int h, int tid,
-                                           const ulonglong2 *tw, const ModConst &m, const u64 *pre) {
+                                           const ulonglong2 *tw, const ModConst &m, const u64 *pre, bool canon) {
     typedef Geo<LOGN, SPLIT> Gm;
     constexpr int N = Gm::N;
     u64 v[16];
     if (!CRT || nd == 1 || own_row) {
         const int sl = own_row ? r : start;
         const bool red = T.mod[r_index(sl, T.L, T.K)].q > 2 * m.q;
-        fwd_transform_row<LOGN, MODE, SPLIT>(v, src + (size_t)sl * N, red, tid, h, tw, m);
+        fwd_transform_row<LOGN, MODE, SPLIT>(v, src + (size_t)sl * N, red, tid, h, tw, m, canon);
     } else {
         const int ebase = i * (kMaxAlpha + 1) + nd;
         if (pre)
@@ -208,7 +208,7 @@ __device__ __forceinline__ void decomp_fp(const DevTables &T, const u64 *src, u6
 template <int LOGN, bool CRT, int MODE>
 __device__ __forceinline__ void decomp_flat(const DevTables &T, const u64 *src, u64 *dst, int start, int nd,
                                             bool own_row, int i, int r, int ri, int tid, const ulonglong2 *tw,
-                                            const ModConst &m, const u64 *pre) {
+                                            const ModConst &m, const u64 *pre, bool canon) {
     constexpr int N = 1 << LOGN;
     u64 v[16];
     if (!CRT || nd == 1 || own_row) {
@@ -225,7 +225,7 @@ __device__ __forceinline__ void decomp_flat(const DevTables &T, const u64 *src, 
         else
             for (int e = 0; e < 16; ++e) v[e] = digit_modup(T, src, tid + e * (N / 16), start, nd, ebase, ri, m);
     }
-    fwd_ntt_mode<LOGN, MODE>(v, tid, tw, m, 1);
+    fwd_ntt_mode<LOGN, MODE>(v, tid, tw, m, 1, canon);
     store_row_quad<LOGN>(v, dst, tid);
 }
 
@@ -259,6 +259,8 @@ __global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ?
     const ulonglong2 *tw = T.tw + (size_t)ri * N;
     u64 *dst = Dd.D + (((size_t)p * beta + i) * rows + r) * N + (size_t)h * Gm::NS;
     const u64 *pre = (CRT && Dd.pre) ? Dd.pre + ((size_t)p * beta + i) * (alpha + 1) * N : nullptr;
+    // unreduced digit rows (below 68q) for the plans' Montgomery consumers
+    const bool canon = !(Dd.lazy_out && lazy_digits_ok(m.q));
     if (fp_ok(m.q) && T.fp64) {
         decomp_fp<LOGN, CRT, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, m, pre);
         return;
@@ -266,17 +268,17 @@ __global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ?
     if constexpr (!Gm::split) {
         const bool crt_row = CRT && !(nd == 1 || own_row);
         if (!crt_row && fwd_lazy_ok(m.q))
-            decomp_flat<LOGN, CRT, 1>(T, src, dst, start, nd, own_row, i, r, ri, tid, tw, m, pre);
+            decomp_flat<LOGN, CRT, 1>(T, src, dst, start, nd, own_row, i, r, ri, tid, tw, m, pre, canon);
         else
-            decomp_flat<LOGN, CRT, 0>(T, src, dst, start, nd, own_row, i, r, ri, tid, tw, m, pre);
+            decomp_flat<LOGN, CRT, 0>(T, src, dst, start, nd, own_row, i, r, ri, tid, tw, m, pre, true);
         return;
     }
     if (CRT && !(nd == 1 || own_row)) {  // CRT digit rows: one ModUp per value, then the transform
-        decomp_row<LOGN, true, 0, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m, pre);
+        decomp_row<LOGN, true, 0, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m, pre, true);
     } else if (fwd_lazy_ok(m.q)) {
-        decomp_row<LOGN, false, 1, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m, pre);
+        decomp_row<LOGN, false, 1, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m, pre, canon);
     } else {
-        decomp_row<LOGN, false, 0, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m, pre);
+        decomp_row<LOGN, false, 0, SPLIT>(T, src, dst, start, nd, own_row, i, r, ri, h, tid, tw, m, pre, true);
     }
 }
 

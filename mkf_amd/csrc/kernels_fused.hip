@@ -33,9 +33,11 @@ __device__ __forceinline__ void decomp_mac_row(const DecompDesc &Dd, int i, int 
     for (int p = 0; p < Dd.k; ++p) {
         u64 v[16];
         __builtin_amdgcn_sched_barrier(0);
-        // values below 3 q_src may exceed the transform's bound 8m: reduce then
+        // values below 3 q_src may exceed the transform's bound 8m: reduce then.
+        // MODE 1: the digit row is stored and multiplied unreduced (below 68q,
+        // lazy_digits_ok); its consumers are Montgomery products
         fwd_transform_row<LOGN, MODE, SPLIT>(v, Dd.comp[p] + (size_t)i * N, reduce, opaque_lane(tid), h, opaque(tw),
-                                             m);
+                                             m, MODE != 1);
         __builtin_amdgcn_sched_barrier(0);
         store_row_quad<Gm::LS>(v, Dd.D + (((size_t)p * beta + i) * rows + r) * N + hoff, tid);
         const gu64 *key = uniform_ptr(Dd.key[p] + krow);
@@ -43,8 +45,9 @@ __device__ __forceinline__ void decomp_mac_row(const DecompDesc &Dd, int i, int 
         for (int e = 0; e < 16; e += 2) {  // key words in the transform's Quad layout
             const u64v2 kw = *(const gv2 *)(key + Quad<Gm::LS>::elem(tid, e));
             if constexpr (MODE == 1) {
-                // q < 2^57: products in (0, 2q) summed unreduced, below 64 q for
-                // at most 32 parties
+                // q < 2^57: digits below 68q < 2^63.1 are valid mred_lazy operands;
+                // products in (0, 2q) summed unreduced, below 64 q for at most 32
+                // parties
                 acc[e] += mred_lazy(kw.x, v[e], m.q, m.qinv);
                 acc[e + 1] += mred_lazy(kw.y, v[e + 1], m.q, m.qinv);
             } else {

@@ -129,6 +129,7 @@ int mkhip::plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, int
         }
         DecompDesc s0 = {}, s1 = {};
         MacDesc m0 = {}, m1 = {};
+        s0.lazy_out = s1.lazy_out = 1;  // digits feed Montgomery products only
         for (int p = 0; p < op.k0; ++p) {
             s0.comp[p] = lifted(0, 1 + p);
             m0.key[p] = s0.key[p] = c->parties[op.ids0[p]].d;
@@ -218,6 +219,7 @@ int mkhip::plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, int
             md1.push_back(md);
         }
         DecompDesc s2 = {};
+        s2.lazy_out = 1;
         for (int p = 0; p < op.k0; ++p) s2.comp[p] = P(w.t) + (size_t)p * nl * N;
         s2.D = P(w.Dt);
         s2.k = op.k0;

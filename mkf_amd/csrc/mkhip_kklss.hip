@@ -114,6 +114,7 @@ int mkhip::plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, in
         // decompositions of both sides; x / y accumulations (key-stationary)
         DecompDesc s0 = {}, s1 = {};
         MacDesc m0 = {}, m1 = {};
+        s0.lazy_out = s1.lazy_out = 1;  // digits feed Montgomery products only
         s0.src_level = lv0;
         s1.src_level = lv1;
         for (int p = 0; p < op.k0; ++p) {
@@ -224,6 +225,7 @@ int mkhip::plan_kklss_ckks(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, in
         }
         // D(t_i)
         DecompDesc s2 = {};
+        s2.lazy_out = 1;
         for (int p = 0; p < op.k0; ++p) s2.comp[p] = P(w.t) + (size_t)p * nl * N;
         s2.D = P(w.Dt);
         s2.k = op.k0;

@@ -155,6 +155,7 @@ int mkhip::plan_mkf_ckks_lv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int
             const int sv = same ? 0 : s;
             const int ls = lvs[s], nls = ls + 1, rs = nls + K, bs = (nls + c->alpha - 1) / c->alpha;
             DecompDesc dd = {};
+            dd.lazy_out = 1;  // digits feed Montgomery products only
             dd.D = P(w.D) + (size_t)s * k * betamax * rowsmax * N;
             if (crt) dd.pre = P(w.pre) + (size_t)s * k * betamax * (c->alpha + 1) * N;
             for (int p = 0; p < k; ++p) {
@@ -206,6 +207,7 @@ int mkhip::plan_mkf_ckks_lv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int
             intt_mu2.push_back(j);
         }
         DecompDesc d2 = {};
+        d2.lazy_out = 1;
         d2.comp[0] = P(w.mu2c);
         d2.D = P(w.Dr);
         if (crt) d2.pre = P(w.pre2);
@@ -442,6 +444,7 @@ int mkhip::plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use
         auto prod = [&](int i) { return P(w.p) + (size_t)i * nl * N; };
         for (int s = 0; s < 2; ++s) {
             DecompDesc dd = {};
+            dd.lazy_out = 1;  // digits feed Montgomery products only
             for (int p = 0; p < k; ++p) {
                 dd.comp[p] = side[s] + (size_t)(1 + p) * nl * N;
                 Mac2Desc md = {};
@@ -499,6 +502,7 @@ int mkhip::plan_mkf_bfv(mkhip_ctx *c, int nops, const mkhip_mkf_op *ops, int use
             quant.push_back(qd);
         }
         DecompDesc d2 = {};
+        d2.lazy_out = 1;
         d2.comp[0] = prod(2);
         d2.D = P(w.Dr);
         d2.k = 1;

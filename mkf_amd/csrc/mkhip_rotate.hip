@@ -253,6 +253,7 @@ int mkhip::plan_kklss_rotate(mkhip_ctx *c, int nops, const mkhip_rotate_op *ops,
         auto compw = [&](u64 *base, int p) { return base + (size_t)p * nl * N; };
         if (W.din) {
             DecompDesc d = {};
+            d.lazy_out = 1;  // digits feed Montgomery products only
             for (int p = 0; p < k; ++p) d.comp[p] = comp(op.in, 1 + p);
             d.D = P(W.Din);
             d.k = k;
@@ -277,6 +278,7 @@ int mkhip::plan_kklss_rotate(mkhip_ctx *c, int nops, const mkhip_rotate_op *ops,
                 }
                 if (k) {
                     DecompDesc d = {};
+                    d.lazy_out = 1;
                     for (int p = 0; p < k; ++p) d.comp[p] = comp(pc, 1 + p);
                     d.D = P(w.Dc);
                     d.k = k;
@@ -516,6 +518,7 @@ int mkhip::plan_mkf_rotate(mkhip_ctx *c, int nops, const mkhip_mkf_rotate_op *op
             perm.push_back(a);
         }
         DecompDesc d = {};
+        d.lazy_out = 1;  // digits feed Montgomery products only
         MdNttDesc m0 = {}, m1 = {};
         for (int p = 0; p < k; ++p) {
             u64 *coef = P(w.coef) + (size_t)p * nl * N;

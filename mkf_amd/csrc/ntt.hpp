@@ -277,10 +277,12 @@ struct Quad {
 // With hh = 2 + h it is the transform of half h of a limb of 2^(LOGN+1) after
 // that limb's first stage (fwd_first_stage).
 // LAZY (2^32 <= q < 2^57, mu = floor(2^64 / q)): unreduced butterflies, one
-// reduction per value at the end (reduce_big).
+// reduction per value at the end (reduce_big); with canon = false that reduction
+// is left out and the values stay below 68q (< 2^63.1), for consumers that take
+// them as Montgomery operands (see lazy_digits_ok).
 template <int LOGN, bool LAZY = false>
 __device__ __forceinline__ void ntt_fwd(u64 (&v)[16], u64 *s, int tid, const ulonglong2 *__restrict__ tw,
-                                        u64 q, int hh = 1, u64 mu = 0) {
+                                        u64 q, int hh = 1, u64 mu = 0, bool canon = true) {
     typedef NttPlan<LOGN> P;
     static_assert(P::F0::TL == Linear<LOGN>::T, "first pass must be the linear layout");
     P::F0::template run<LAZY>(v, tid, tw, q, hh);
@@ -291,10 +293,12 @@ __device__ __forceinline__ void ntt_fwd(u64 (&v)[16], u64 *s, int tid, const ulo
     lds_exchange<typename P::F2, typename P::F3>(v, s, tid);
     P::F3::template run<LAZY>(v, tid, tw, q, hh);
     if (LAZY) {
+        if (canon) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            v[i] = reduce_big(v[i], q, mu);
-            __builtin_amdgcn_sched_barrier(0);
+            for (int i = 0; i < 16; ++i) {
+                v[i] = reduce_big(v[i], q, mu);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     } else {
 #pragma unroll

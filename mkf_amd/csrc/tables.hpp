@@ -247,6 +247,9 @@ struct DecompDesc {
                                    // a value below `level` means `level`
     u64 *pre;                      // [k][beta][alpha+1][N]: y_l rows and v of the CRT digits
                                    // (k_digit_pre), or null: the rows compute them
+    int lazy_out;                  // 1: rows of moduli with lazy_digits_ok may be stored
+                                   // unreduced (below 68q; every plan-internal consumer is a
+                                   // Montgomery product), 0: canonical (the public entry points)
 };
 
 // Generic batched limb NTT job.
