@@ -290,7 +290,8 @@ def run_scheme(scheme, args, rng):
         "autotune_ms": [round(t, 3) for t in tune_ms],
         "tuning": {"fused_decomp": w.ctx.get_tuning(1), "xcd_remap": w.ctx.get_tuning(2),
                    "decomp_order": w.ctx.get_tuning(3), "split_limb": w.ctx.get_tuning(5),
-                   "split_ext": w.ctx.get_tuning(6), "digit_pre": w.ctx.get_tuning(7)},
+                   "split_ext": w.ctx.get_tuning(6), "digit_pre": w.ctx.get_tuning(7),
+                   "xcd_ext": w.ctx.get_tuning(8)},
         "shape": {"logN": w.logN, "L": w.L, "K": w.K, "k": w.k, "alpha": w.ctx.alpha, "beta": w.beta,
                   "nres": w.nres, "t": w.t},
     }
@@ -652,7 +653,8 @@ def run_uneven(args, rng):
                                  "autotune_ms": [round(t, 3) for t in tune],
                                  "tuning": {"fused_decomp": ctx.get_tuning(1), "xcd_remap": ctx.get_tuning(2),
                                             "decomp_order": ctx.get_tuning(3), "split_limb": ctx.get_tuning(5),
-                                            "split_ext": ctx.get_tuning(6), "digit_pre": ctx.get_tuning(7)}}
+                                            "split_ext": ctx.get_tuning(6), "digit_pre": ctx.get_tuning(7),
+                                            "xcd_ext": ctx.get_tuning(8)}}
     batch.finalize()
     for _ in range(args.warmup):
         batch.submit()

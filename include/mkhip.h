@@ -471,8 +471,12 @@ int mkhip_register_mkf_group_keygen(mkhip_ctx *ctx, int group, int k, const uint
  *   MKHIP_TUNE_FUSED_DECOMP  alpha = 1 KKLSS plans: the gadget decomposition fused
  *                            with the x / y key MAC (1) or as two launches (0)
  *   MKHIP_TUNE_XCD_REMAP     workgroups ordered XCD-contiguously (1) or as issued (0)
+ *                            (every limb kernel but the external products)
  *   MKHIP_TUNE_DECOMP_ORDER  fused decomposition grid order: ops sharing a key row
- *                            adjacent (0) or rows sharing an input limb adjacent (1)
+ *                            adjacent (0), rows sharing an input limb adjacent (1), or
+ *                            XCD-tiled (2: each XCD walks every op of one (side,
+ *                            digit, block of up to 5 rows), so input limbs and key
+ *                            rows are fetched from HBM about once each)
  *   MKHIP_TUNE_FP64          transforms of limbs whose modulus is below 2^50 in 64-bit
  *                            integer arithmetic (0, default) or in exact FP64
  *                            two-product arithmetic (1, opt-in; the same words)
@@ -485,9 +489,12 @@ int mkhip_register_mkf_group_keygen(mkhip_ctx *ctx, int group, int k, const uint
  *   MKHIP_TUNE_DIGIT_PRE     MKF CKKS plans with CRT digits (alpha >= 2): the digit-only
  *                            part of the ModUp (y_l, the overflow estimate) computed once
  *                            per digit by k_digit_pre (1) or by every target row (0)
+ *   MKHIP_TUNE_XCD_EXT       external-product kernels (k_ext, k_mac2) in XCD-contiguous
+ *                            order (1, default) or as issued (0)
  * A knob applies to plans built after it is set: a plan keeps the settings it was
  * built with for every later launch, graph capture and stage timing.  mkhip_autotune plans and times
- * every applicable combination on the caller's batch (ops: mkhip_kklss_op[] for the
+ * every applicable combination of the decomposition knobs on the caller's batch, then
+ * the best with MKHIP_TUNE_XCD_EXT flipped (ops: mkhip_kklss_op[] for the
  * KKLSS schemes, mkhip_mkf_op[] for MKF; their outputs are overwritten with the
  * results), keeps the fastest, and writes up to 16 candidate times (ms per launch).
  */
@@ -498,6 +505,7 @@ int mkhip_register_mkf_group_keygen(mkhip_ctx *ctx, int group, int k, const uint
 #define MKHIP_TUNE_SPLIT_LIMB 5
 #define MKHIP_TUNE_SPLIT_EXT 6
 #define MKHIP_TUNE_DIGIT_PRE 7
+#define MKHIP_TUNE_XCD_EXT 8
 #define MKHIP_SCHEME_KKLSS_CKKS 0
 #define MKHIP_SCHEME_KKLSS_BFV 1
 #define MKHIP_SCHEME_MKF_CKKS 2

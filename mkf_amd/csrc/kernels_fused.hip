@@ -18,7 +18,14 @@ namespace mkhip {
 
 // blockIdx -> logical ((side * beta + i) * rows + r) * nops + o (order 0: the ops
 // sharing a key row are adjacent) or ((side * beta + i) * nops + o) * rows + r
-// (order 1: the rows sharing an input limb are adjacent); descs[2 o + side]
+// (order 1: the rows sharing an input limb are adjacent); descs[2 o + side].
+// Order 2 (XCD-tiled): each XCD takes a contiguous range of
+// (((side * beta + i) * nrb + rb) * nops + o) * rbs + r', row r = rb rbs + r' --
+// the rows are cut into nrb blocks of rbs, so an XCD walks every op of one
+// (side, digit, row block): the op's input limbs serve the block's rbs rows from
+// its L2 and the block's key rows (rbs x k rows, about 2.5 MB at k = 4) stay
+// there across the ops, so that inputs are fetched about nrb times and keys about
+// once per launch.
 template <int LOGN, int MODE, bool SPLIT>
 __device__ __forceinline__ void decomp_mac_row(const DecompDesc &Dd, int i, int r, int beta, int rows, int h,
                                                bool reduce, size_t krow, int tid, const ulonglong2 *tw,
@@ -138,15 +145,24 @@ __device__ __forceinline__ void decomp_mac_row_fp(const DecompDesc &Dd, int i, i
 // at most 128 registers a lane, so that two workgroups share a CU
 template <int LOGN, bool SPLIT>
 __global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ? 4 : 1))
-    k_decomp_mac(DevTables T, const DecompDesc *__restrict__ descs, int nops, int beta, int rows, int order) {
+    k_decomp_mac(DevTables T, const DecompDesc *__restrict__ descs, int nops, int beta, int rows, int order, int rbs) {
     typedef Geo<LOGN, SPLIT> Gm;
     constexpr int N = Gm::N;
     const int tid = threadIdx.x;
-    int bid = block_index(T);
+    int bid = order == 2 ? xcd_remap(blockIdx.x, gridDim.x) : block_index(T);
     const int h = Gm::split ? bid % 2 : 0;
     bid /= Gm::H;
     int o, r;
-    if (order == 0) {
+    if (order == 2) {
+        const int rin = bid % rbs;
+        bid /= rbs;
+        o = bid % nops;
+        bid /= nops;
+        const int nrb = (rows + rbs - 1) / rbs;
+        r = (bid % nrb) * rbs + rin;
+        bid /= nrb;
+        if (r >= rows) return;  // the last row block's unused slots (whole workgroup)
+    } else if (order == 0) {
         o = bid % nops;
         bid /= nops;
         r = bid % rows;
@@ -191,9 +207,15 @@ static hipError_t launch_fused_n(int what, const DevTables &T, const void *descs
         init = true;
     }
     if (what != KDECOMP_MAC) return hipErrorInvalidValue;
-    // n ops (2 descriptors each), a = beta, b = rows
-    hipLaunchKernelGGL((k_decomp_mac<LOGN, SPLIT>), dim3(n * 2 * a * b * Gm::H), dim3(Gm::TH), lds, st, T,
-                       (const DecompDesc *)descs, n, a, b, T.decomp_order);
+    // n ops (2 descriptors each), a = beta, b = rows; order 2 cuts the rows into
+    // blocks of at most 5 (their key rows fit one XCD's 4 MB L2 at k = 4)
+    int rbs = b, nrb = 1;
+    if (T.decomp_order == 2) {
+        nrb = (b + 4) / 5;
+        rbs = (b + nrb - 1) / nrb;
+    }
+    hipLaunchKernelGGL((k_decomp_mac<LOGN, SPLIT>), dim3(n * 2 * a * nrb * rbs * Gm::H), dim3(Gm::TH), lds, st, T,
+                       (const DecompDesc *)descs, n, a, b, T.decomp_order, rbs);
     return hipGetLastError();
 }
 

@@ -18,7 +18,7 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     typedef Geo<LOGN> Gm;
     constexpr int N = Gm::N;
     const int tid = threadIdx.x;
-    int bid = block_index(T);
+    int bid = block_index_ext(T);
     const int h = Gm::split ? bid % 2 : 0;
     bid /= Gm::H;
     const int r = uni(bid % rows);

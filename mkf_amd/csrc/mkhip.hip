@@ -337,6 +337,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     T.split14 = 0;
     T.split14_ext = 0;
     T.digit_pre = 0;
+    T.xcd_ext = 1;
     T.logN = logN;
     T.N = N;
     T.L = L;

@@ -226,7 +226,8 @@ extern "C" int mkhip_set_tuning(mkhip_ctx *c, int knob, int value) {
         c->T.xcd = value ? 1 : 0;
         break;
     case MKHIP_TUNE_DECOMP_ORDER:
-        c->T.decomp_order = value ? 1 : 0;
+        if (value < 0 || value > 2) return MKHIP_ERR_ARG;
+        c->T.decomp_order = value;
         break;
     case MKHIP_TUNE_FP64:
         c->T.fp64 = value ? 1 : 0;
@@ -239,6 +240,9 @@ extern "C" int mkhip_set_tuning(mkhip_ctx *c, int knob, int value) {
         break;
     case MKHIP_TUNE_DIGIT_PRE:
         c->T.digit_pre = value ? 1 : 0;
+        break;
+    case MKHIP_TUNE_XCD_EXT:
+        c->T.xcd_ext = value ? 1 : 0;
         break;
     default:
         return MKHIP_ERR_ARG;
@@ -273,6 +277,9 @@ extern "C" int mkhip_get_tuning(const mkhip_ctx *c, int knob, int *value) {
     case MKHIP_TUNE_DIGIT_PRE:
         *value = c->T.digit_pre;
         return MKHIP_OK;
+    case MKHIP_TUNE_XCD_EXT:
+        *value = c->T.xcd_ext;
+        return MKHIP_OK;
     default:
         return MKHIP_ERR_ARG;
     }
@@ -288,28 +295,31 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
     // fused order 1, and at N = 2^14 the fused orders over half-limb workgroups --
     // each with and without the XCD remap
     struct Cand {
-        int fused, order, xcd, split, split_ext, pre;
+        int fused, order, xcd, split, split_ext, pre, xe;
     };
     std::vector<Cand> cands;
     const int old_fused = c->tune_fused_decomp, old_xcd = c->T.xcd, old_order = c->T.decomp_order;
     const int old_split = c->T.split14, old_split_ext = c->T.split14_ext, old_pre = c->T.digit_pre;
+    const int old_xe = c->T.xcd_ext;
     const bool mkf_crt = scheme == MKHIP_SCHEME_MKF_CKKS && c->alpha > 1;
     for (int xcd = 1; xcd >= 0; --xcd) {
         if (kklss && c->alpha == 1) {
-            cands.push_back({0, old_order, xcd, old_split, old_split_ext, old_pre});
-            cands.push_back({1, 0, xcd, 0, old_split_ext, old_pre});
-            cands.push_back({1, 1, xcd, 0, old_split_ext, old_pre});
+            cands.push_back({0, old_order, xcd, old_split, old_split_ext, old_pre, old_xe});
+            cands.push_back({1, 0, xcd, 0, old_split_ext, old_pre, old_xe});
+            cands.push_back({1, 1, xcd, 0, old_split_ext, old_pre, old_xe});
+            cands.push_back({1, 2, xcd, 0, old_split_ext, old_pre, old_xe});
             if (c->logN == 14) {
                 // (half-limb external products are not tried: their extra k_intt_finish
                 // pass costs more than the occupancy gains, measured on MI355X)
-                cands.push_back({1, 0, xcd, 1, old_split_ext, old_pre});
-                cands.push_back({1, 1, xcd, 1, old_split_ext, old_pre});
+                cands.push_back({1, 0, xcd, 1, old_split_ext, old_pre, old_xe});
+                cands.push_back({1, 1, xcd, 1, old_split_ext, old_pre, old_xe});
+                cands.push_back({1, 2, xcd, 1, old_split_ext, old_pre, old_xe});
             }
         } else if (mkf_crt) {  // MKF CKKS with CRT digits: with and without the digit pass
-            cands.push_back({old_fused, old_order, xcd, old_split, old_split_ext, 0});
-            cands.push_back({old_fused, old_order, xcd, old_split, old_split_ext, 1});
+            cands.push_back({old_fused, old_order, xcd, old_split, old_split_ext, 0, old_xe});
+            cands.push_back({old_fused, old_order, xcd, old_split, old_split_ext, 1, old_xe});
         } else {
-            cands.push_back({old_fused, old_order, xcd, old_split, old_split_ext, old_pre});
+            cands.push_back({old_fused, old_order, xcd, old_split, old_split_ext, old_pre, old_xe});
         }
     }
     auto apply = [&](const Cand &cd) {
@@ -319,24 +329,37 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
         mkhip_set_tuning(c, MKHIP_TUNE_SPLIT_LIMB, cd.split);
         mkhip_set_tuning(c, MKHIP_TUNE_SPLIT_EXT, cd.split_ext);
         mkhip_set_tuning(c, MKHIP_TUNE_DIGIT_PRE, cd.pre);
+        mkhip_set_tuning(c, MKHIP_TUNE_XCD_EXT, cd.xe);
     };
-    int best = -1, n = 0;
-    double best_ms = 0;
-    for (const Cand &cd : cands) {
+    auto time_cand = [&](const Cand &cd, double *t) {
         apply(cd);
         mkhip_plan *pl = nullptr;
         int s = plan_build(c, scheme, nops, ops, 1, &pl, PlanMem());
-        double t = 0;
         if (s == MKHIP_OK) s = mkhip_plan_launch(pl);  // warm-up
-        if (s == MKHIP_OK) s = mkhip_plan_time(pl, iters, &t);
+        if (s == MKHIP_OK) s = mkhip_plan_time(pl, iters, t);
         mkhip_plan_destroy(pl);
+        return s;
+    };
+    // the decomposition candidates, then the best of them with the external
+    // products' XCD order flipped (that knob touches only the other kernels)
+    int best = -1, n = 0;
+    double best_ms = 0;
+    const int nfirst = (int)cands.size();
+    for (int ci = 0; ci <= nfirst; ++ci) {
+        if (ci == nfirst) {
+            Cand flip = cands[best];
+            flip.xe = 1 - flip.xe;
+            cands.push_back(flip);
+        }
+        double t = 0;
+        const int s = time_cand(cands[ci], &t);
         if (s != MKHIP_OK) {
-            apply({old_fused, old_order, old_xcd, old_split, old_split_ext, old_pre});
+            apply({old_fused, old_order, old_xcd, old_split, old_split_ext, old_pre, old_xe});
             return s;
         }
         if (ms && n < 16) ms[n] = t;
         if (best < 0 || t < best_ms) {
-            best = n;
+            best = ci;
             best_ms = t;
         }
         ++n;

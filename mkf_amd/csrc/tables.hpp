@@ -49,6 +49,8 @@ struct DevTables {
     int split14;              // N = 2^14 one-prime decompositions over half-limb workgroups (a tuning knob)
     int split14_ext;          // N = 2^14 external products over half-limb workgroups (a tuning knob)
     int digit_pre;            // MKF CRT digits: digit-only ModUp part once per digit (a tuning knob)
+    int xcd_ext;              // 1: external-product kernels (k_ext, k_mac2) in XCD-contiguous
+                              // order, a tuning knob separate from xcd
 };
 
 // N = 2^14 external products over half-limb workgroups: a plan that emits KEXT
@@ -227,6 +229,13 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 // Logical index of this workgroup: XCD-contiguous when the context's knob is on.
 __device__ __forceinline__ int block_index(const DevTables &T) {
     return T.xcd ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+}
+
+// The same for the external-product kernels, whose knob is tuned separately (their
+// adjacent workgroups share the x / y or key rows of one op, the decompositions'
+// share input limbs).
+__device__ __forceinline__ int block_index_ext(const DevTables &T) {
+    return T.xcd_ext ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
 }
 
 // ---- descriptors -----------------------------------------------------------

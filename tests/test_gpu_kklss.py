@@ -373,18 +373,14 @@ def test_operands_at_their_own_levels_alpha2():
         ctx.close()
 
 
-def test_fused_decomposition_grid_orders_agree():
-    """The fused decomposition's grid orders (ops adjacent, rows adjacent) with one
-    and two workgroups per limb give the same words on a 20-op batch of mixed
-    party sets; two ops are checked against the oracle."""
+def _grid_orders_agree(logN, Q, P, B, splits, orders, seed):
     mk = _mk()
-    logN, Q, P = config("c2")
     ctx, ref = mk.Context(logN, Q, P, gamma=2), O.Ctx(logN, Q, P, gamma=2)
     bufs = []
     try:
-        keys, u = _keys(ctx, ref, [0, 1, 2], seed=63)
-        g = O.Prng(64)
-        N, nl, B = ref.N, ref.L, 20
+        keys, u = _keys(ctx, ref, [0, 1, 2], seed=seed)
+        g = O.Prng(seed + 1)
+        N, nl = ref.N, ref.L
         ops, hosts = [], []
         for b in range(B):
             ids0, ids1 = ([0, 1], [1, 2]) if b % 2 else ([0, 1, 2], [2])
@@ -397,8 +393,8 @@ def test_fused_decomposition_grid_orders_agree():
             hosts.append((op0, ids0, op1, ids1, do))
             ops.append(dict(op0=d0, op1=d1, out=do, ids0=ids0, ids1=ids1, level=nl - 1, nres=1))
         results = []
-        for split in (0, 1):
-            for order in (0, 1):
+        for split in splits:
+            for order in orders:
                 ctx.set_tuning(mk.TUNE_SPLIT_LIMB, split)
                 ctx.set_tuning(mk.TUNE_DECOMP_ORDER, order)
                 pl = mk.Plan(ctx, ops, use_graph=False)
@@ -406,16 +402,40 @@ def test_fused_decomposition_grid_orders_agree():
                 ctx.sync()
                 pl.close()
                 results.append([ctx.d2h(np.zeros((4, nl - 1, N), np.uint64), h[4]) for h in hosts])
+        # the external products in issue order instead of XCD-contiguous
+        ctx.set_tuning(mk.TUNE_XCD_EXT, 0)
+        pl = mk.Plan(ctx, ops, use_graph=False)
+        pl.launch()
+        ctx.sync()
+        pl.close()
+        results.append([ctx.d2h(np.zeros((4, nl - 1, N), np.uint64), h[4]) for h in hosts])
         for r in results[1:]:
             for a, b in zip(results[0], r):
                 assert np.array_equal(a, b)
-        for b in (0, 19):
+        for b in (0, B - 1):
             op0, ids0, op1, ids1, _ = hosts[b]
             exp, _ = _oracle_mulrelin(ref, op0, ids0, op1, ids1, keys, u, 1)
             assert np.array_equal(results[-1][b], exp), b
     finally:
         ctx.set_tuning(mk.TUNE_SPLIT_LIMB, 0)
         ctx.set_tuning(mk.TUNE_DECOMP_ORDER, 0)
+        ctx.set_tuning(mk.TUNE_XCD_EXT, 1)
         for p in bufs:
             ctx.free(p)
         ctx.close()
+
+
+def test_fused_decomposition_grid_orders_agree():
+    """The fused decomposition's grid orders (ops adjacent, rows adjacent, XCD-tiled
+    row blocks) with one and two workgroups per limb, and the external products in
+    either workgroup order, give the same words on a 20-op batch of mixed party
+    sets; two ops are checked against the oracle."""
+    logN, Q, P = config("c2")
+    _grid_orders_agree(logN, Q, P, 20, (0, 1), (0, 1, 2), 63)
+
+
+def test_fused_decomposition_tiled_order_with_a_partial_row_block():
+    """The XCD-tiled order on 5 Q + 2 P rows (two blocks of 4 rows, the last slot
+    of each op's second block unused) gives the rows-adjacent order's words."""
+    Q, P = O.gen_primes(50, 15, 5), O.gen_primes(60, 15, 2)
+    _grid_orders_agree(13, Q, P, 6, (0,), (1, 2), 71)
