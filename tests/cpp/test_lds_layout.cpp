@@ -1,7 +1,7 @@
 // Exhaustive host-side check of the transforms' padded LDS layout: for every pass
 // of every ring degree the kernels use, every lane's 16 element positions split
 // exactly into a lane base plus a compile-time offset, that split survives the
-// padding (one pad word after every 32), the padded words stay inside the
+// padding (one pad word after every 16), the padded words stay inside the
 // workgroup's allocation, and no two (lane, value) pairs share a padded word.
 // The kernels assert only the closed form of this condition (pad_ok).
 #include <cstdio>

@@ -1,4 +1,4 @@
-"""The transforms' padded LDS layout (one pad word after every 32): for every
+"""The transforms' padded LDS layout (one pad word after every 16): for every
 pass of every ring degree, each lane's element positions split exactly into a
 lane base plus compile-time offsets that survive the padding, fit the
 workgroup's allocation and never collide.  Host-only (tests/cpp/test_lds_layout)."""
