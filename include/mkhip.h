@@ -484,8 +484,9 @@ int mkhip_register_mkf_group_keygen(mkhip_ctx *ctx, int group, int k, const uint
  *                            plain): each limb over two workgroups of half the LDS (1)
  *                            or one workgroup (0)
  *   MKHIP_TUNE_SPLIT_EXT     N = 2^14 external products: each limb over two workgroups
- *                            of half the LDS, the last inverse stage in k_intt_finish (1),
- *                            or one workgroup (0)
+ *                            of half the LDS, the last inverse stage done by the ModDown
+ *                            that reads the rows (1), or one workgroup (0); at N = 2^15
+ *                            the external products always run so
  *   MKHIP_TUNE_DIGIT_PRE     MKF CKKS plans with CRT digits (alpha >= 2): the digit-only
  *                            part of the ModUp (y_l, the overflow estimate) computed once
  *                            per digit by k_digit_pre (1) or by every target row (0)

@@ -484,7 +484,23 @@ struct PUp {
     u64 v;
 };
 
-__device__ __forceinline__ PUp modup_p_prepare(const DevTables &T, const u64 *__restrict__ src, int nl, size_t c) {
+// Word c of row `row` (modulus index mod) of a QP polynomial.  unf: the row's
+// inverse transform ran as two half-limb transforms and left the last stage
+// (t = N/2) and the N^-1 scaling undone (values below 4q) -- they are done here,
+// from the word and its partner c ^ N/2, as k_intt_finish would: the lower word
+// becomes (x + y) N^-1, the upper (x - y) psiinv[1] N^-1 (x lower, y upper).
+__device__ __forceinline__ u64 row_word(const DevTables &T, const u64 *__restrict__ src, int row, int mod, size_t c,
+                                        bool unf) {
+    const u64 x = G(src)[(size_t)row * T.N + c];
+    if (!unf) return x;
+    const size_t half = (size_t)T.N >> 1;
+    const u64 y = G(src)[(size_t)row * T.N + (c ^ half)];
+    const ModConst &mm = T.mod[mod];
+    return c < half ? shoup(x + y, mm.ninv, mm.ninvp, mm.q) : shoup(y + (mm.q << 2) - x, mm.wn, mm.wnp, mm.q);
+}
+
+__device__ __forceinline__ PUp modup_p_prepare(const DevTables &T, const u64 *__restrict__ src, int nl, size_t c,
+                                               bool unf = false) {
     PUp u;
     double vi = 0.0;
 #pragma unroll
@@ -492,7 +508,7 @@ __device__ __forceinline__ PUp modup_p_prepare(const DevTables &T, const u64 *__
         u.y[k] = 0;
         if (k < T.K) {
             const ModConst mp = T.mod[T.L + k];
-            u.y[k] = mred(G(src)[(size_t)(nl + k) * T.N + c], T.md_y[k], mp.q, mp.qinv);
+            u.y[k] = mred(row_word(T, src, nl + k, T.L + k, c, unf), T.md_y[k], mp.q, mp.qinv);
             vi += (double)u.y[k] / (double)mp.q;
         }
     }

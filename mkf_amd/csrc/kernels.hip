@@ -432,15 +432,16 @@ __global__ void __launch_bounds__(256) k_moddown(DevTables T, const ModDownDesc 
     u64 res[kMaxQ];
 #pragma unroll
     for (int j = 0; j < kMaxQ; ++j) res[j] = (Dd.accumulate && j < nl) ? G(Dd.dst)[(size_t)j * N + c] : 0;
+    const bool unf = Dd.unfinished != 0;
     for (int s = 0; s < Dd.nsrc; ++s) {
         const u64 *src = Dd.src[s];
-        const PUp up = modup_p_prepare(T, src, nl, c);
+        const PUp up = modup_p_prepare(T, src, nl, c, unf);
 #pragma unroll
         for (int j = 0; j < kMaxQ; ++j) {
             if (j < nl) {
                 const ModConst mq = T.mod[j];
                 const u64 z = modup_p_to_row(T, up, j, mq);
-                const u64 x = G(src)[(size_t)j * N + c];
+                const u64 x = row_word(T, src, j, j, c, unf);
                 const u64 md = mred(z + 2 * mq.q - x, T.md_pinv[j], mq.q, mq.qinv);
                 res[j] = add_mod(res[j], md, mq.q);
             }

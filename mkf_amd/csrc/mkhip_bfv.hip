@@ -281,18 +281,17 @@ int mkhip::plan_kklss_bfv(mkhip_ctx *c, int nops, const mkhip_kklss_op *ops, int
         for (int l = 0; l < nr; ++l) fin.row(t.out + (size_t)l * N, l < c->L ? l : K + l, 1);
     fin.emit(S, blob, "tensor");
     S.push_back(make_stage(KBFV_QUANTIZE, blob.add(quant), (int)quant.size(), 0, 0, 1, "quantize"));
+    // external products on half-limb workgroups (N = 2^15, or the knob at 2^14) leave
+    // their last inverse stage and N^-1 scaling to the ModDowns, the only readers
+    // of their rows (no finish pass over the accumulators)
+    if (Finish(c, ext_split(c->T)).on) {
+        for (ModDownDesc &m : md1) m.unfinished = 1;
+        for (ModDownDesc &m : md2) m.unfinished = 1;
+    }
     S.push_back(make_stage(KEXT1, blob.add(ext1), (int)ext1.size(), 2 * beta, rows, 1, "ext_xy"));
-    Finish fext(c, ext_split(c->T));  // the external products may run on half-limb workgroups
-    for (const ExtDesc &e : ext1) fext.qp(e.acc[0], nl, c->L, K, N);
-    fext.emit(S, blob, "ext_xy");
     S.push_back(make_stage(KMODDOWN, blob.add(md1), (int)md1.size(), 0, 0, 1, "moddown_xy"));
     S.push_back(make_stage(KDECOMP, blob.add(dec2), (int)dec2.size(), beta, rows, kmax2, "decompose_t"));
     S.push_back(make_stage(KEXT2, blob.add(ext2), (int)ext2.size(), beta, rows, 1, "ext_vu"));
-    for (const ExtDesc &e : ext2) {
-        fext.qp(e.acc[0], nl, c->L, K, N);
-        fext.qp(e.acc[1], nl, c->L, K, N);
-    }
-    fext.emit(S, blob, "ext_vu");
     S.push_back(make_stage(KMODDOWN_FINAL, blob.add(md2), (int)md2.size(), 0, 0, 1, "moddown_final"));
     st = plan_finish(pl, blob, use_graph);
     if (st != MKHIP_OK) return st;

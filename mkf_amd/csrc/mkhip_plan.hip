@@ -309,8 +309,6 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
             cands.push_back({1, 1, xcd, 0, old_split_ext, old_pre, old_xe});
             cands.push_back({1, 2, xcd, 0, old_split_ext, old_pre, old_xe});
             if (c->logN == 14) {
-                // (half-limb external products are not tried: their extra k_intt_finish
-                // pass costs more than the occupancy gains, measured on MI355X)
                 cands.push_back({1, 0, xcd, 1, old_split_ext, old_pre, old_xe});
                 cands.push_back({1, 1, xcd, 1, old_split_ext, old_pre, old_xe});
                 cands.push_back({1, 2, xcd, 1, old_split_ext, old_pre, old_xe});
@@ -340,15 +338,20 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
         mkhip_plan_destroy(pl);
         return s;
     };
-    // the decomposition candidates, then the best of them with the external
-    // products' XCD order flipped (that knob touches only the other kernels)
+    // the decomposition candidates, then the best of them with each knob of the
+    // external products flipped in turn (XCD order; at N = 2^14 half-limb
+    // workgroups, whose last inverse stage the ModDowns then do), kept when faster
     int best = -1, n = 0;
     double best_ms = 0;
     const int nfirst = (int)cands.size();
-    for (int ci = 0; ci <= nfirst; ++ci) {
-        if (ci == nfirst) {
+    const int nflip = (kklss && c->alpha == 1 && c->logN == 14) ? 2 : 1;
+    for (int ci = 0; ci < nfirst + nflip; ++ci) {
+        if (ci >= nfirst) {
             Cand flip = cands[best];
-            flip.xe = 1 - flip.xe;
+            if (ci == nfirst)
+                flip.xe = 1 - flip.xe;
+            else
+                flip.split_ext = 1 - flip.split_ext;
             cands.push_back(flip);
         }
         double t = 0;

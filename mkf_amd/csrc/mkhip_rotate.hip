@@ -340,12 +340,10 @@ int mkhip::plan_kklss_rotate(mkhip_ctx *c, int nops, const mkhip_rotate_op *ops,
     if (!dec.empty()) S.push_back(make_stage(KDECOMP, blob.add(dec), (int)dec.size(), beta, rows, kmax, "decompose"));
     if (!ext.empty()) {
         S.push_back(make_stage(KEXT2, blob.add(ext), (int)ext.size(), beta, rows, 1, "ext"));
-        Finish fext(c, ext_split(c->T));  // the external products may run on half-limb workgroups
-        for (const ExtDesc &e : ext) {
-            fext.qp(e.acc[0], nl, c->L, K, N);
-            fext.qp(e.acc[1], nl, c->L, K, N);
-        }
-        fext.emit(S, blob, "ext");
+        // on half-limb workgroups the last inverse stage is left to the ModDowns,
+        // the only readers of the accumulators
+        if (Finish(c, ext_split(c->T)).on)
+            for (ModDownDesc &m : md) m.unfinished = 1;
     }
     if (!md.empty()) S.push_back(make_stage(KMODDOWN, blob.add(md), (int)md.size(), 0, 0, 1, "moddown"));
     if (!post.empty()) S.push_back(make_stage(KAUTO, blob.add(post), (int)post.size(), 0, 0, 1, "permute"));

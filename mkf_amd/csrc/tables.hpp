@@ -308,7 +308,9 @@ struct ModDownDesc {
     const u64 *base;       // added before writing (may be null)
     u64 *out;              // final output (when nres >= 0 and final)
     int nsrc, level, accumulate, final_, nres;
-    int pad[3];
+    int unfinished;        // 1: the sources' inverse transforms ran on half-limb workgroups
+                           // and left their last stage and N^-1 scaling to this kernel
+    int pad[2];
 };
 
 }  // namespace mkhip
