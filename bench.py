@@ -294,7 +294,7 @@ def run_scheme(scheme, args, rng):
             w.ctx.set_tuning(TUNING_KNOBS[knob], val)
         tune_ms = []
     else:
-        tune_ms = w.ctx.autotune(ops, scheme=sch, iters=2) if args.autotune else []
+        tune_ms = w.ctx.autotune(ops, scheme=sch, iters=args.autotune_iters) if args.autotune else []
     plan = w.plan(ops)
     elapsed, lat = time_plan(w.ctx, plan, args.steps, args.warmup)
     ops_s = B * args.steps / elapsed
@@ -959,6 +959,8 @@ def main():
     ap.add_argument("--schemes", default=",".join(ALL_SCHEMES))
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the library's autotuner (plans use the default knobs)")
+    ap.add_argument("--autotune-iters", type=int, default=5,
+                    help="timed launches per autotuner candidate (candidates a few %% apart need several)")
     ap.add_argument("--tuning", type=parse_tuning, default={},
                     help="fixed knobs for the headline scheme instead of the autotuner, e.g. "
                          "'decomp_order=1,split_limb=1,xcd_remap=0,xcd_ext=1' (reproducible profiles)")
