@@ -516,13 +516,17 @@ __device__ __forceinline__ PUp modup_p_prepare(const DevTables &T, const u64 *__
     return u;
 }
 
-__device__ __forceinline__ u64 modup_p_to_row(const DevTables &T, const PUp &u, int j, const ModConst &mq) {
+// The same residue below 3q, not reduced (for sums that are reduced once).
+__device__ __forceinline__ u64 modup_p_to_row_lazy(const DevTables &T, const PUp &u, int j, const ModConst &mq) {
     u64 hi = 0, lo = 0;
 #pragma unroll
     for (int k = 0; k < kMaxK; ++k)
         if (k < T.K) mac128(hi, lo, u.y[k], T.md_qstar[j * T.K + k]);
-    const u64 z = mont_reduce128(hi, lo, mq.q, mq.qinv) + T.md_vt[j * (T.K + 1) + u.v];
-    return reduce64(z, mq.q, mq.mu);
+    return mont_reduce128(hi, lo, mq.q, mq.qinv) + T.md_vt[j * (T.K + 1) + u.v];  // (0, 2q) + [0, q)
+}
+
+__device__ __forceinline__ u64 modup_p_to_row(const DevTables &T, const PUp &u, int j, const ModConst &mq) {
+    return reduce64(modup_p_to_row_lazy(T, u, j, mq), mq.q, mq.mu);
 }
 
 }  // namespace mkhip

@@ -440,7 +440,9 @@ __global__ void __launch_bounds__(256) k_moddown(DevTables T, const ModDownDesc 
         for (int j = 0; j < kMaxQ; ++j) {
             if (j < nl) {
                 const ModConst mq = T.mod[j];
-                const u64 z = modup_p_to_row(T, up, j, mq);
+                // ModUp below 3q, x canonical: the difference is in (q, 5q), a valid
+                // Montgomery operand, so ModUp needs no reduction of its own
+                const u64 z = modup_p_to_row_lazy(T, up, j, mq);
                 const u64 x = row_word(T, src, j, j, c, unf);
                 const u64 md = mred(z + 2 * mq.q - x, T.md_pinv[j], mq.q, mq.qinv);
                 res[j] = add_mod(res[j], md, mq.q);
