@@ -492,6 +492,10 @@ int mkhip_register_mkf_group_keygen(mkhip_ctx *ctx, int group, int k, const uint
  *                            per digit by k_digit_pre (1) or by every target row (0)
  *   MKHIP_TUNE_XCD_EXT       external-product kernels (k_ext, k_mac2) in XCD-contiguous
  *                            order (1, default) or as issued (0)
+ *   MKHIP_TUNE_EXT_CS        external-product MACs (k_ext, k_mac2): each sum of key x digit
+ *                            products kept as an exact carry-save 128-bit sum and reduced
+ *                            once (1, default, for rows whose sums fit) or every product
+ *                            Montgomery-reduced (0); the same words
  * A knob applies to plans built after it is set: a plan keeps the settings it was
  * built with for every later launch, graph capture and stage timing.  mkhip_autotune plans and times
  * every applicable combination of the decomposition knobs on the caller's batch, then
@@ -507,6 +511,7 @@ int mkhip_register_mkf_group_keygen(mkhip_ctx *ctx, int group, int k, const uint
 #define MKHIP_TUNE_SPLIT_EXT 6
 #define MKHIP_TUNE_DIGIT_PRE 7
 #define MKHIP_TUNE_XCD_EXT 8
+#define MKHIP_TUNE_EXT_CS 9
 #define MKHIP_SCHEME_KKLSS_CKKS 0
 #define MKHIP_SCHEME_KKLSS_BFV 1
 #define MKHIP_SCHEME_MKF_CKKS 2

@@ -394,6 +394,92 @@ __device__ __forceinline__ void ext_rows(u64 (&a0)[16], u64 *park1, const u64 *d
     if constexpr (NK > 1) store_row_quad<LOGN>(a1, park1, tid);
 }
 
+// ext_rows with carry-save accumulation (Acc3): the 16 values are walked in
+// chunks (four values for one key, two for two keys) with the digits inside, so
+// that only a chunk's three-word sums are live; each sum of ndig products key * digit is reduced once
+// (redc128), to the same canonical residue as the sum of the products' MRed
+// values.  For moduli where those sums fit (ext_cs_ok).
+template <int LOGN, int NK>
+__device__ __forceinline__ void ext_rows_cs(u64 (&a0)[16], u64 *park1, const u64 *d, size_t dstride,
+                                            const u64 *k0, const u64 *k1, size_t kstride, int ndig, int tid,
+                                            const ModConst &m) {
+    typedef Quad<LOGN> Qd;
+    // values per chunk: the sums of one key for four values, or of two keys for two
+    constexpr int CH = NK > 1 ? 2 : 4;
+    // finished values wait in this lane's LDS words (the LDS is free until the
+    // inverse transform), so that no more than a chunk is live in registers
+    u64 *park0 = g_lds;
+#pragma unroll
+    for (int c = 0; c < 16; c += CH) {
+        Acc3 s0[CH], s1[CH];
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+            acc3_zero(s0[e]);
+            if (NK > 1) acc3_zero(s1[e]);
+        }
+        // digits two at a time (all their loads issued before the first product),
+        // then an odd last one
+        auto digits = [&](int i, auto npair) {
+            constexpr int NP = decltype(npair)::value;
+            __builtin_amdgcn_sched_barrier(0);
+            u64v2 xd[NP][CH / 2], yd[NP][CH / 2], zd[NP][CH / 2];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                const gu64 *dr = uniform_ptr(d + (i + j) * dstride), *k0r = uniform_ptr(k0 + (i + j) * kstride);
+#pragma unroll
+                for (int e = 0; e < CH; e += 2) {
+                    const int off = Qd::elem(tid, c + e);
+                    xd[j][e / 2] = *(const gv2 *)(dr + off);
+                    yd[j][e / 2] = *(const gv2 *)(k0r + off);
+                    if (NK > 1) zd[j][e / 2] = *(const gv2 *)(uniform_ptr(k1 + (i + j) * kstride) + off);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int e = 0; e < CH; e += 2) {
+                    acc3_mac(s0[e], yd[j][e / 2].x, xd[j][e / 2].x);
+                    acc3_mac(s0[e + 1], yd[j][e / 2].y, xd[j][e / 2].y);
+                    if (NK > 1) {
+                        acc3_mac(s1[e], zd[j][e / 2].x, xd[j][e / 2].x);
+                        acc3_mac(s1[e + 1], zd[j][e / 2].y, xd[j][e / 2].y);
+                    }
+                }
+        };
+        int i = 0;
+        for (; i + 1 < ndig; i += 2) digits(i, std::integral_constant<int, 2>());
+        if (i < ndig) digits(i, std::integral_constant<int, 1>());
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+            u64 hi, lo;
+            acc3_fold(s0[e], hi, lo);
+            park0[(c + e) * Qd::T + tid] = redc128(hi, lo, m.q, m.qinv);
+        }
+        if constexpr (NK > 1) {
+            gu64 *p1 = uniform_ptr(park1);
+#pragma unroll
+            for (int e = 0; e < CH; e += 2) {
+                u64 hi, lo;
+                u64v2 w;
+                acc3_fold(s1[e], hi, lo);
+                w.x = redc128(hi, lo, m.q, m.qinv);
+                acc3_fold(s1[e + 1], hi, lo);
+                w.y = redc128(hi, lo, m.q, m.qinv);
+                *(gv2 *)(p1 + Qd::elem(tid, c + e)) = w;
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) a0[e] = park0[e * Qd::T + tid];
+    __syncthreads();  // before the transform's first exchange writes other lanes' words
+}
+
+// Whether ext_rows_cs may take a row of modulus q: ndig products of a canonical
+// key word and a digit word -- below 68q when the digit rows are kept unreduced
+// (lazy_digits_ok), else canonical -- fit the carry-save sum, which also keeps
+// the sum's high word below q for redc128.
+__device__ __forceinline__ bool ext_cs_ok(u64 q, int ndig);
+
 // Whether ext_rows may accumulate lazily: ndig sums of values below 2q stay
 // below 2^64, and reduce_big's quotient estimate needs q >= 2^32.
 __device__ __forceinline__ bool ext_lazy_ok(u64 q, int ndig) {
@@ -408,6 +494,10 @@ __device__ __forceinline__ bool ext_lazy_ok(u64 q, int ndig) {
 // moduli below 2^61 -- 68q < 2^63.1 holds for every q the unreduced transform
 // takes (fwd_lazy_ok: q < 2^57).
 __device__ __forceinline__ bool lazy_digits_ok(u64 q) { return fwd_lazy_ok(q); }
+
+__device__ __forceinline__ bool ext_cs_ok(u64 q, int ndig) {
+    return ndig > 0 && acc3_fits((u64)ndig, q - 1, lazy_digits_ok(q) ? 68 * q : q - 1);
+}
 
 // ext_rows in FP64 (modulus below 2^50): a0[e] = sum_i MRed(k0_i[e], d_i[e]) as a
 // double congruent residue (|a0| < 2.6q), computed as (sum_i k0_i d_i) 2^-64; with

@@ -244,6 +244,9 @@ extern "C" int mkhip_set_tuning(mkhip_ctx *c, int knob, int value) {
     case MKHIP_TUNE_XCD_EXT:
         c->T.xcd_ext = value ? 1 : 0;
         break;
+    case MKHIP_TUNE_EXT_CS:
+        c->T.ext_cs = value ? 1 : 0;
+        break;
     default:
         return MKHIP_ERR_ARG;
     }
@@ -279,6 +282,9 @@ extern "C" int mkhip_get_tuning(const mkhip_ctx *c, int knob, int *value) {
         return MKHIP_OK;
     case MKHIP_TUNE_XCD_EXT:
         *value = c->T.xcd_ext;
+        return MKHIP_OK;
+    case MKHIP_TUNE_EXT_CS:
+        *value = c->T.ext_cs;
         return MKHIP_OK;
     default:
         return MKHIP_ERR_ARG;

@@ -142,6 +142,47 @@ __device__ __forceinline__ u64 mred_lazy(u64 a, u64 b, u64 q, u64 qinv) {
     return hi + q - mqh;
 }
 
+// Carry-save sum of 128-bit products.  With a = a1 2^32 + a0 and b = b1 2^32 + b0,
+// each product adds lo32(a0 b0) to L, hi32(a0 b0) + a0 b1 + a1 b0 to M and a1 b1
+// to H, so that the sum of the products is L + 2^32 M + 2^64 H with no carry
+// passed between the words: one multiply and three multiply-adds per product
+// (a Montgomery reduction per product costs about three times that).  Exact
+// while M + hi32(L) stays below 2^64 (acc3_fits).
+struct Acc3 {
+    u64 L, M, H;
+};
+
+__device__ __forceinline__ void acc3_zero(Acc3 &A) { A.L = A.M = A.H = 0; }
+
+__device__ __forceinline__ void acc3_mac(Acc3 &A, u64 a, u64 b) {
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    const u64 p00 = (u64)a0 * b0;
+    A.L += (u32)p00;
+    A.M = mad32(a0, b1, mad32(a1, b0, A.M + (p00 >> 32)));
+    A.H = mad32(a1, b1, A.H);
+}
+
+// Whether n products of a <= amax and b <= bmax fit: each adds below
+// 2^32 (1 + a1 + b1) to M and below 2^32 to L.
+MKHIP_HD bool acc3_fits(u64 n, u64 amax, u64 bmax) {
+    const unsigned __int128 per = ((unsigned __int128)((amax >> 32) + (bmax >> 32) + 1) << 32) + 1;
+    return (unsigned __int128)n * per < ((unsigned __int128)1 << 64);
+}
+
+// The accumulated sum as (hi, lo), exact.
+__device__ __forceinline__ void acc3_fold(const Acc3 &A, u64 &hi, u64 &lo) {
+    const u64 m = A.M + (A.L >> 32);
+    lo = (m << 32) | (u32)A.L;
+    hi = A.H + (m >> 32);
+}
+
+// (hi 2^64 + lo) 2^-64 mod q, canonical, for hi < q (Lattigo's MRed on a 128-bit
+// value: lo - lo(m q) = 0 with m = lo q^-1, so the quotient is hi - hi(m q)).
+__device__ __forceinline__ u64 redc128(u64 hi, u64 lo, u64 q, u64 qinv) {
+    const u64 r = hi + q - mulhi64(lo * qinv, q);
+    return r >= q ? r - q : r;
+}
+
 MKHIP_HD u64 add_mod(u64 a, u64 b, u64 q) {
     u64 s = a + b;
     return s >= q ? s - q : s;

@@ -51,6 +51,8 @@ struct DevTables {
     int digit_pre;            // MKF CRT digits: digit-only ModUp part once per digit (a tuning knob)
     int xcd_ext;              // 1: external-product kernels (k_ext, k_mac2) in XCD-contiguous
                               // order, a tuning knob separate from xcd
+    int ext_cs;               // 1: external-product MACs summed in carry-save form (Acc3)
+                              // where the sums fit, a tuning knob
 };
 
 // N = 2^14 external products over half-limb workgroups: a plan that emits KEXT

@@ -87,6 +87,49 @@ def test_external_product_parity(setup):
     assert np.array_equal(got, exp)
 
 
+def test_external_product_carry_save_agrees(setup):
+    """External-product MACs summed in carry-save form (the default) give the words
+    of every product Montgomery-reduced, with keys and digits at q - 1 (the
+    largest sums) and uniform, for the primitive and for mul+relin with keys of
+    q - 1 words; the primitive also against the oracle."""
+    name, ctx, ref = setup
+    mk = _mk()
+    g = O.Prng(17)
+    level = ref.L - 1
+    beta = ctx.beta
+    N = ref.N
+    qcol = np.array(ref.QP, dtype=np.uint64)[:, None]
+    top = np.stack([np.broadcast_to(qcol - 1, (len(ref.QP), N)) for _ in range(beta)]).copy()
+    D_top = top[:, : level + 1 + len(ref.P)].copy()
+    D_top[:, level + 1:] = top[:, len(ref.Q):]
+    D_rnd = ref.decompose(g.uniform(ref.Q, N), level)
+    key_rnd = np.stack([g.uniform(ref.QP, N) for _ in range(beta)])
+    cases = [(D_top, top), (D_rnd, top), (D_top, key_rnd), (D_rnd, key_rnd)]
+    pids = [0, 1]
+    for pid in pids:
+        ctx.register_party(pid, top, top, top)
+    ctx.set_crs_u(top)
+    op0 = np.stack([g.uniform(ref.Q, N) for _ in range(3)])
+    op1 = np.stack([g.uniform(ref.Q, N) for _ in range(3)])
+    scale = float(ref.Q[-1])
+    got = {}
+    try:
+        for cs in (1, 0):
+            ctx.set_tuning(mk.TUNE_EXT_CS, cs)
+            got[cs] = ([ctx.external_product(D, k, level) for D, k in cases],
+                       ctx.kklss_ckks_mulrelin(op0, pids, op1, pids, scale, scale, scale))
+    finally:
+        ctx.set_tuning(mk.TUNE_EXT_CS, 1)
+    for a, b in zip(got[1][0], got[0][0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(got[1][1][0], got[0][1][0])
+    for (D, k), a in zip(cases, got[1][0]):
+        assert np.array_equal(a, ref.external_product(D, k, level))
+    keys = {p: [top, top, top] for p in pids}
+    exp, _ = _oracle_mulrelin(ref, op0, pids, op1, pids, keys, top, (ref.L - 1) - got[1][1][2])
+    assert np.array_equal(got[1][1][0], exp)
+
+
 def _keys(ctx, ref, parties, seed):
     g = O.Prng(seed)
     beta = ctx.beta
