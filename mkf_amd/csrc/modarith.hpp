@@ -92,6 +92,21 @@ __device__ __forceinline__ u64 shoup_lazy4(u64 a, u64 w, u64 wp, u64 q) {
     return ((u64)hi << 32) | (u32)lo;
 }
 
+// u + shoup_lazy4(a, w, wp, q) mod 2^64, with u taken as the addend of the first
+// multiply-add of the product's low word (so that a forward butterfly's sum
+// x + w y costs no addition of its own).
+__device__ __forceinline__ u64 shoup_lazy4_add(u64 a, u64 w, u64 wp, u64 q, u64 u) {
+    const u64 qh = mulhi64_approx(a, wp);
+    const u64 nq = 0 - q;
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), w0 = (u32)w, w1 = (u32)(w >> 32);
+    const u32 h0 = (u32)qh, h1 = (u32)(qh >> 32), n0 = (u32)nq, n1 = (u32)(nq >> 32);
+    const u64 lo = (u64)h0 * n0 + ((u64)a0 * w0 + u);
+    u32 c0 = a0 * w1, c1 = a1 * w0, c2 = h0 * n1, c3 = h1 * n0;
+    asm("" : "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3));
+    const u32 hi = (u32)(lo >> 32) + c0 + c1 + c2 + c3;
+    return ((u64)hi << 32) | (u32)lo;
+}
+
 MKHIP_HD u64 shoup(u64 a, u64 w, u64 wp, u64 q) {
     u64 r = shoup_lazy(a, w, wp, q);
     return r >= q ? r - q : r;

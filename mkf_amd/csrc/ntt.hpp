@@ -25,11 +25,13 @@ namespace mkhip {
 // ---- butterflies ---------------------------------------------------------
 
 // q4 = 4q.  In: x, y in [0, 8q).  Out: x, y in [0, 8q).
+// The sum u + t is formed inside the product (shoup_lazy4_add) and the
+// difference as 2u + 4q - (u + t): one shift-add and one subtraction.
 __device__ __forceinline__ void fwd_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q4, u64 q) {
-    u64 u = x >= q4 ? x - q4 : x;          // [0, 4q)
-    u64 t = shoup_lazy4(y, w, wp, q);      // [0, 4q)
-    x = u + t;                             // [0, 8q)
-    y = u + q4 - t;                        // (0, 8q)
+    const u64 u = x >= q4 ? x - q4 : x;           // [0, 4q)
+    const u64 s = shoup_lazy4_add(y, w, wp, q, u);  // u + t, t in [0, 4q): [0, 8q)
+    x = s;
+    y = (u << 1) + q4 - s;                        // u + 4q - t in (0, 8q)
 }
 
 // Unreduced form for moduli with headroom (q < 2^57): no conditional subtraction;
@@ -37,9 +39,9 @@ __device__ __forceinline__ void fwd_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q4, 
 // stages from inputs < 8q stay below 68q < 2^64.
 __device__ __forceinline__ void fwd_bfly_lazy(u64 &x, u64 &y, u64 w, u64 wp, u64 q4, u64 q) {
     const u64 u = x;
-    const u64 t = shoup_lazy4(y, w, wp, q);  // [0, 4q)
-    x = u + t;
-    y = u + q4 - t;
+    const u64 s = shoup_lazy4_add(y, w, wp, q, u);  // u + t, t in [0, 4q)
+    x = s;
+    y = (u << 1) + q4 - s;                          // u + 4q - t
 }
 
 // The moduli whose forward transforms run unreduced (and end with reduce_big).
@@ -128,9 +130,11 @@ struct FwdPass {
             // b is the same on every lane of a wave when TL is a multiple of 64:
             // the twiddles then come through scalar loads
             const int b = (TL % 64 == 0) ? __builtin_amdgcn_readfirstlane(col / TL) : col / TL;
+            // no scheduling barrier between the stages: the scheduler interleaves
+            // the end of one stage with the start of the next (measured faster in
+            // the decompositions; the inverse passes keep theirs)
 #pragma unroll
             for (int u = 0; u < NS; ++u) {
-                __builtin_amdgcn_sched_barrier(0);
                 const int half = R >> (u + 1);
 #pragma unroll
                 for (int grp = 0; grp < (1 << u); ++grp) {
@@ -299,7 +303,6 @@ __device__ __forceinline__ void ntt_fwd(u64 (&v)[16], u64 *s, int tid, const ulo
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 v[i] = reduce_big(v[i], q, mu);
-                __builtin_amdgcn_sched_barrier(0);
             }
         }
     } else {
