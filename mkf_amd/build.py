@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmkhip.so")
-SOURCES = ["kernels.hip", "kernels_mkf.hip", "kernels_bfv.hip", "kernels_fused.hip", "kernels_galois.hip",
+SOURCES = ["kernels.hip", "kernels_ext.hip", "kernels_mkf.hip", "kernels_bfv.hip", "kernels_fused.hip", "kernels_galois.hip",
            "kernels_keygen.hip", "kernels_poly.hip", "mkhip.hip", "mkhip_plan.hip", "mkhip_kklss.hip", "mkhip_mkf.hip", "mkhip_bfv.hip",
            "mkhip_rotate.hip", "mkhip_keygen.hip", "mkhip_poly.hip", "mkhip_batch.hip",
            "mkhip_server.hip", "mkhip_peaks.hip", "mkhip_noalloc.hip"]
@@ -33,6 +33,12 @@ FLAGS = [
 ]
 
 
+# per-file flags: the tensor product, external products and ModDowns schedule
+# better with the max-ILP strategy (measured on MI355X), the transforms of the
+# decompositions worse
+FILE_FLAGS = {"kernels_ext.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
+
+
 def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -48,7 +54,8 @@ def build(verbose=False, jobs=None):
     os.makedirs(OBJDIR, exist_ok=True)
     objs = [os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
     compile_flags = [f for f in FLAGS if f != "-shared"]
-    cmds = [[HIPCC] + compile_flags + ["-c", os.path.join(CSRC, s), "-o", o] for s, o in zip(SOURCES, objs)]
+    cmds = [[HIPCC] + compile_flags + FILE_FLAGS.get(s, []) + ["-c", os.path.join(CSRC, s), "-o", o]
+            for s, o in zip(SOURCES, objs)]
     jobs = jobs or min(len(cmds), max(1, (os.cpu_count() or 2) - 1), 8)
     with ThreadPoolExecutor(jobs) as ex:
         results = list(ex.map(lambda c: _run(c, verbose), cmds))
