@@ -434,8 +434,15 @@ __device__ __forceinline__ void ext_rows_cs(u64 (&a0)[16], u64 *park1, const u64
 #pragma unroll
                 for (int e = 0; e < CH; e += 2) {
                     const int off = Qd::elem(tid, c + e);
-                    // digit words are read once: non-temporal, as they were stored
-                    xd[j][e / 2] = __builtin_nontemporal_load((const gv2 *)(dr + off));
+                    // one key: a chunk covers whole 32-byte segments of the digit
+                    // rows, each read once (non-temporal, as they were stored);
+                    // two keys: a chunk reads half a segment, whose other half the
+                    // next chunk reads from L2 (and the Galois plans read one
+                    // decomposition for several rotations)
+                    if constexpr (NK == 1)
+                        xd[j][e / 2] = __builtin_nontemporal_load((const gv2 *)(dr + off));
+                    else
+                        xd[j][e / 2] = *(const gv2 *)(dr + off);
                     yd[j][e / 2] = *(const gv2 *)(k0r + off);
                     if (NK > 1) zd[j][e / 2] = *(const gv2 *)(uniform_ptr(k1 + (i + j) * kstride) + off);
                 }
@@ -478,6 +485,17 @@ __device__ __forceinline__ void ext_rows_cs(u64 (&a0)[16], u64 *park1, const u64
 #pragma unroll
     for (int e = 0; e < 16; ++e) a0[e] = park0[e * Qd::T + tid];
     __syncthreads();  // before the transform's first exchange writes other lanes' words
+}
+
+// Whether ext_rows_cs's chunks cover whole 32-byte segments of the rows (one
+// key: four values; two keys: two values, a whole segment only where the
+// transform's Quad layout holds pairs, at 2^13 per workgroup).  Two keys over
+// half segments measured slower than the reduced products (MKF switching-key
+// MACs at 2^14 1.25 -> 1.9 ms; hoisted rotations 1.26 -> 1.5 ms), one key or
+// whole segments faster (KKLSS external products 1.51 -> 1.30 and 1.16 -> 1.04 ms).
+template <int LS, int NK>
+__device__ constexpr bool ext_cs_geometry() {
+    return NK == 1 || Quad<LS>::R == 2;
 }
 
 // Whether ext_rows_cs may take a row of modulus q: ndig products of a canonical

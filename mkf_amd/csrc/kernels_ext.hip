@@ -90,7 +90,7 @@ __global__ void __launch_bounds__((Geo<LOGN, SPLIT>::TH), (LOGN == 14 && SPLIT ?
     } else {
         const u64 *dr = Dd.D + (size_t)r * N + hoff, *k0 = Dd.key[0] + (size_t)ri * N + hoff;
         const u64 *k1 = NKEY > 1 ? Dd.key[1] + (size_t)ri * N + hoff : nullptr;
-        if (T.ext_cs && ext_cs_ok(m.q, beta))
+        if (ext_cs_geometry<Gm::LS, NKEY>() && T.ext_cs && ext_cs_ok(m.q, beta))
             ext_rows_cs<Gm::LS, NKEY>(a0, o1, dr, (size_t)rows * N, k0, k1, kstride, beta, tid, m);
         else if (ext_lazy_ok(m.q, beta))
             ext_rows<Gm::LS, NKEY, true>(a0, o1, dr, (size_t)rows * N, k0, k1, kstride, beta, tid, m);

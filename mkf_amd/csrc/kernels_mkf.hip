@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
     u64 *o0 = Dd.acc[0] + (size_t)r * N + hoff, *o1 = Dd.acc[1] + (size_t)r * N + hoff;
     // swk digit b: output 0 at (2b) kstride, output 1 at (2b+1) kstride
     const u64 *dr = Dd.D + (size_t)r * N + hoff, *k0 = Dd.swk + (size_t)ri * N + hoff, *k1 = k0 + kstride;
-    if (T.ext_cs && ext_cs_ok(m.q, beta))
+    if (ext_cs_geometry<Gm::LS, 2>() && T.ext_cs && ext_cs_ok(m.q, beta))
         ext_rows_cs<Gm::LS, 2>(a0, o1, dr, (size_t)rows * N, k0, k1, 2 * kstride, beta, tid, m);
     else if (ext_lazy_ok(m.q, beta))
         ext_rows<Gm::LS, 2, true>(a0, o1, dr, (size_t)rows * N, k0, k1, 2 * kstride, beta, tid, m);
