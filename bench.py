@@ -39,7 +39,7 @@ ALL_SCHEMES = ["kklss_ckks", "mkf_ckks", "kklss_bfv", "mkf_bfv"]
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic_kklss_ckks_mulrelin_n14_l8_k4_b64.json")
 # the library's tuning knobs by the names the bench reports them under
 TUNING_KNOBS = {"fused_decomp": 1, "xcd_remap": 2, "decomp_order": 3, "fp64": 4, "split_limb": 5, "split_ext": 6,
-                "digit_pre": 7, "xcd_ext": 8, "ext_cs": 9}
+                "digit_pre": 7, "xcd_ext": 8, "ext_cs": 9, "inv_lazy": 10}
 
 
 def parse_tuning(text):
@@ -310,7 +310,8 @@ def run_scheme(scheme, args, rng):
         "tuning": {"fused_decomp": w.ctx.get_tuning(1), "xcd_remap": w.ctx.get_tuning(2),
                    "decomp_order": w.ctx.get_tuning(3), "split_limb": w.ctx.get_tuning(5),
                    "split_ext": w.ctx.get_tuning(6), "digit_pre": w.ctx.get_tuning(7),
-                   "xcd_ext": w.ctx.get_tuning(8), "ext_cs": w.ctx.get_tuning(9)},
+                   "xcd_ext": w.ctx.get_tuning(8), "ext_cs": w.ctx.get_tuning(9),
+                   "inv_lazy": w.ctx.get_tuning(10)},
         "shape": {"logN": w.logN, "L": w.L, "K": w.K, "k": w.k, "alpha": w.ctx.alpha, "beta": w.beta,
                   "nres": w.nres, "t": w.t},
     }
@@ -673,7 +674,8 @@ def run_uneven(args, rng):
                                  "tuning": {"fused_decomp": ctx.get_tuning(1), "xcd_remap": ctx.get_tuning(2),
                                             "decomp_order": ctx.get_tuning(3), "split_limb": ctx.get_tuning(5),
                                             "split_ext": ctx.get_tuning(6), "digit_pre": ctx.get_tuning(7),
-                                            "xcd_ext": ctx.get_tuning(8), "ext_cs": ctx.get_tuning(9)}}
+                                            "xcd_ext": ctx.get_tuning(8), "ext_cs": ctx.get_tuning(9),
+                                            "inv_lazy": ctx.get_tuning(10)}}
     batch.finalize()
     for _ in range(args.warmup):
         batch.submit()

@@ -496,6 +496,10 @@ int mkhip_register_mkf_group_keygen(mkhip_ctx *ctx, int group, int k, const uint
  *                            products kept as an exact carry-save 128-bit sum and reduced
  *                            once (1, default, for rows whose sums fit) or every product
  *                            Montgomery-reduced (0); the same words
+ *   MKHIP_TUNE_INV_LAZY      inverse transforms of limbs with 2^32 <= q < 2^57: sums of
+ *                            the Gentleman-Sande butterflies left unreduced, the values
+ *                            brought below q only when the next pass could overflow
+ *                            (1, default) or reduced every stage (0); the same words
  * A knob applies to plans built after it is set: a plan keeps the settings it was
  * built with for every later launch, graph capture and stage timing.  mkhip_autotune plans and times
  * every applicable combination of the decomposition knobs on the caller's batch, then
@@ -512,6 +516,7 @@ int mkhip_register_mkf_group_keygen(mkhip_ctx *ctx, int group, int k, const uint
 #define MKHIP_TUNE_DIGIT_PRE 7
 #define MKHIP_TUNE_XCD_EXT 8
 #define MKHIP_TUNE_EXT_CS 9
+#define MKHIP_TUNE_INV_LAZY 10
 #define MKHIP_SCHEME_KKLSS_CKKS 0
 #define MKHIP_SCHEME_KKLSS_BFV 1
 #define MKHIP_SCHEME_MKF_CKKS 2

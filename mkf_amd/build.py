@@ -33,10 +33,11 @@ FLAGS = [
 ]
 
 
-# per-file flags: the tensor product, external products and ModDowns schedule
-# better with the max-ILP strategy (measured on MI355X), the transforms of the
-# decompositions worse
-FILE_FLAGS = {"kernels_ext.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
+# per-file compiler flags (none at present: LLVM's max-ILP scheduling strategy
+# measured 4-5 % faster for the external products of kernels_ext.hip with
+# per-stage reduced inverse transforms, but with the unreduced inverse
+# transforms it makes them spill, and the default then measures faster)
+FILE_FLAGS = {}
 
 
 def _run(cmd, verbose):

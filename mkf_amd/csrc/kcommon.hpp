@@ -284,14 +284,15 @@ __device__ __forceinline__ void fwd_transform_row_fp(double (&v)[16], const u64 
 // Inverse transform of this workgroup's elements (v[e] = element h NS + tid + e TH,
 // values < 4q) with the N^-1 (or t N^-1) scaling given; for a split limb the last
 // stage and the scaling are left to k_intt_finish.
+// mu = floor(2^64 / q): the unreduced butterflies where the modulus allows them.
 template <int LOGN, bool SPLIT = (LOGN == 15)>
 __device__ __forceinline__ void inv_transform(u64 (&v)[16], int tid, int h, const ulonglong2 *itw, u64 q,
-                                              u64 ninv, u64 ninvp) {
+                                              u64 ninv, u64 ninvp, u64 mu = 0) {
     typedef Geo<LOGN, SPLIT> Gm;
     if constexpr (Gm::split)
-        ntt_inv<Gm::LS>(v, g_lds, tid, itw, q, ninv, ninvp, 2 + h, false);
+        ntt_inv<Gm::LS>(v, g_lds, tid, itw, q, ninv, ninvp, 2 + h, false, mu);
     else
-        ntt_inv<LOGN>(v, g_lds, tid, itw, q, ninv, ninvp, 1, true);
+        ntt_inv<LOGN>(v, g_lds, tid, itw, q, ninv, ninvp, 1, true, mu);
 }
 
 // inv_transform choosing the FP64 arithmetic for moduli below 2^50 (a wave-uniform
@@ -308,7 +309,7 @@ __device__ __forceinline__ void inv_transform_sel(u64 (&v)[16], int tid, int h, 
             ntt_inv_fp<LOGN>(v, g_lds, tid, itwd, fp_mod(m), fp_from_u64(tscale ? m.tninv : m.ninv), 1, true);
     } else {
         inv_transform<LOGN, SPLIT>(v, tid, h, opaque(T.itw + (size_t)ri * Gm::N), m.q, tscale ? m.tninv : m.ninv,
-                                   tscale ? m.tninvp : m.ninvp);
+                                   tscale ? m.tninvp : m.ninvp, T.inv_lazy ? m.mu : 0);
     }
 }
 

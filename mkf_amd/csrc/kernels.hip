@@ -6,7 +6,7 @@
 //                    keyswitch_hoisted.go:84-123)
 //
 // The tensor product, the external products and the ModDowns are in
-// kernels_ext.hip (built with another scheduling strategy).
+// kernels_ext.hip.
 //
 // Every kernel keeps one RNS limb of one polynomial per workgroup (N/16 threads,
 // 16 coefficients per thread) or one coefficient per thread, reads and writes

@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(Geo<LOGN>::TH)
         }
         v[e] = x;
     }
-    inv_transform<LOGN>(v, tid, h, T.itw + (size_t)ri * N, m.q, m.tninv, m.tninvp);
+    inv_transform<LOGN>(v, tid, h, T.itw + (size_t)ri * N, m.q, m.tninv, m.tninvp, T.inv_lazy ? m.mu : 0);
     store_row<Gm::LS>(v, Dd.out + off, tid);
 }
 

@@ -5,10 +5,6 @@
 //                    (keyswitch_hoisted.go:11-45)
 //   k_moddown        ModDown Q u P -> Q, accumulation into the output, and the
 //                    fused rescale (basis_extension.go:192-232, mkckks/evaluator.go:360-401)
-//
-// This file is compiled with the AMDGPU max-ILP scheduling strategy (mkf_amd/build.py),
-// which measured faster for these kernels than the default (and slower for the
-// decompositions in kernels.hip and kernels_fused.hip).
 #include "kcommon.hpp"
 
 namespace mkhip {

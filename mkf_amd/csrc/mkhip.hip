@@ -339,6 +339,7 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
     T.digit_pre = 0;
     T.xcd_ext = 1;
     T.ext_cs = 1;
+    T.inv_lazy = 1;
     T.logN = logN;
     T.N = N;
     T.L = L;

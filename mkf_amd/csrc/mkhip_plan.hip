@@ -247,6 +247,9 @@ extern "C" int mkhip_set_tuning(mkhip_ctx *c, int knob, int value) {
     case MKHIP_TUNE_EXT_CS:
         c->T.ext_cs = value ? 1 : 0;
         break;
+    case MKHIP_TUNE_INV_LAZY:
+        c->T.inv_lazy = value ? 1 : 0;
+        break;
     default:
         return MKHIP_ERR_ARG;
     }
@@ -285,6 +288,9 @@ extern "C" int mkhip_get_tuning(const mkhip_ctx *c, int knob, int *value) {
         return MKHIP_OK;
     case MKHIP_TUNE_EXT_CS:
         *value = c->T.ext_cs;
+        return MKHIP_OK;
+    case MKHIP_TUNE_INV_LAZY:
+        *value = c->T.inv_lazy;
         return MKHIP_OK;
     default:
         return MKHIP_ERR_ARG;

@@ -55,6 +55,20 @@ __device__ __forceinline__ void inv_bfly(u64 &x, u64 &y, u64 w, u64 wp, u64 q4, 
     y = shoup_lazy4(u + q4 - v, w, wp, q); // [0, 4q)
 }
 
+// Unreduced inverse butterfly: x, y < B (c = B, 2B <= 2^64, B >= 2q).  Out:
+// x + y < 2B and (x + B - y) w lazily in [0, 4q) <= 2B -- no conditional
+// subtraction.
+__device__ __forceinline__ void inv_bfly_lazy(u64 &x, u64 &y, u64 w, u64 wp, u64 c, u64 q) {
+    const u64 u = x, v = y;
+    x = u + v;
+    y = shoup_lazy4(u + c - v, w, wp, q);
+}
+
+// The moduli whose inverse transforms may run unreduced (ntt_inv with mu != 0):
+// reduce_big needs q >= 2^32, and a pass of up to four stages from values below
+// q must stay below 2^64.
+__device__ __forceinline__ bool inv_lazy_ok(u64 q) { return q >= (1ull << 32) && q < (1ull << 57); }
+
 // [0, 8q) -> [0, q)
 __device__ __forceinline__ u64 reduce_8q(u64 x, u64 q) {
     const u64 q4 = q << 2, q2 = q << 1;
@@ -156,6 +170,7 @@ struct FwdPass {
 // An inverse pass (U0, NS) covers inverse stages U0..U0+NS-1 (t = 2^U0 .. 2^(U0+NS-1)).
 template <int LOGN, int U0, int NS>
 struct InvPass {
+    static constexpr int STAGES = NS;
     static constexpr int N = 1 << LOGN;
     static constexpr int T = N / 16;
     static constexpr int TF = 1 << U0;
@@ -170,8 +185,11 @@ struct InvPass {
     __host__ __device__ static constexpr int base(int tid) { return (tid / TF) * R * TF + tid % TF; }
     __host__ __device__ static constexpr int off(int i) { return (i / R) * T * R + (i % R) * TF; }
     static constexpr bool pad_ok = pad_split_ok(TF, T, R);
+    // LAZY: unreduced butterflies (inv_bfly_lazy); the values enter below bnd,
+    // stage u's below bnd 2^u, and leave below bnd 2^NS (bnd 2^(NS-1) <= 2^63)
+    template <bool LAZY = false>
     __device__ static __forceinline__ void run(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ itw, u64 q,
-                                               int hh) {
+                                               int hh, u64 bnd = 0) {
         const u64 q4 = q << 2;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
@@ -189,7 +207,10 @@ struct InvPass {
 #pragma unroll
                     for (int kk = 0; kk < half; ++kk) {
                         const int k0 = grp * 2 * half + kk;
-                        inv_bfly(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, q4, q);
+                        if (LAZY)
+                            inv_bfly_lazy(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, bnd << u, q);
+                        else
+                            inv_bfly(v[c * R + k0], v[c * R + k0 + half], w.x, w.y, q4, q);
                     }
                 }
             }
@@ -326,18 +347,53 @@ __device__ __forceinline__ void ntt_fwd_any(u64 (&v)[16], u64 *s, int tid, const
 // canonical values in Linear layout.  With hh = 2 + h and scale = false it runs
 // every stage of a limb of 2^(LOGN+1) that stays inside half h and leaves the
 // values in [0, 4q); inv_last_stage finishes the limb.
+// Unreduced inverse (mu = floor(2^64 / q) given and inv_lazy_ok(q)): the
+// Gentleman-Sande sums are not reduced; the bound of the values is tracked
+// per pass (wave-uniform) and, when the pass's last stage could exceed 2^64,
+// the values are first brought to [0, q) (reduce_big).  The scaling takes any
+// value below 2^64 (exact Shoup quotient); without it (split limbs) the values
+// are reduced at the end, so that the finishing stage gets them below 4q.
+template <class Pass>
+__device__ __forceinline__ void inv_pass_lazy(u64 (&v)[16], int tid, const ulonglong2 *__restrict__ itw, u64 q,
+                                              u64 mu, int hh, u64 &bnd) {
+    if (bnd > (~0ull >> Pass::STAGES)) {  // bnd 2^(STAGES-1) > 2^63
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = reduce_big(v[i], q, mu);
+        // 4q, not q: a stage's differences leave below 4q whatever its inputs,
+        // so a bound below 2q would not hold after the pass's first stage
+        bnd = q << 2;
+    }
+    Pass::template run<true>(v, tid, itw, q, hh, bnd);
+    bnd <<= Pass::STAGES;
+}
+
 template <int LOGN>
 __device__ __forceinline__ void ntt_inv(u64 (&v)[16], u64 *s, int tid, const ulonglong2 *__restrict__ itw,
-                                        u64 q, u64 ninv, u64 ninvp, int hh = 1, bool scale = true) {
+                                        u64 q, u64 ninv, u64 ninvp, int hh = 1, bool scale = true, u64 mu = 0) {
     typedef NttPlan<LOGN> P;
     static_assert(P::I3::TF * 16 == (1 << LOGN), "last inverse pass must be the linear layout");
-    P::I0::run(v, tid, itw, q, hh);
-    lds_exchange<typename P::I0, typename P::I1>(v, s, tid);
-    P::I1::run(v, tid, itw, q, hh);
-    lds_exchange<typename P::I1, typename P::I2>(v, s, tid);
-    P::I2::run(v, tid, itw, q, hh);
-    lds_exchange<typename P::I2, typename P::I3>(v, s, tid);
-    P::I3::run(v, tid, itw, q, hh);
+    if (mu != 0 && inv_lazy_ok(q)) {
+        u64 bnd = q << 2;
+        inv_pass_lazy<typename P::I0>(v, tid, itw, q, mu, hh, bnd);
+        lds_exchange<typename P::I0, typename P::I1>(v, s, tid);
+        inv_pass_lazy<typename P::I1>(v, tid, itw, q, mu, hh, bnd);
+        lds_exchange<typename P::I1, typename P::I2>(v, s, tid);
+        inv_pass_lazy<typename P::I2>(v, tid, itw, q, mu, hh, bnd);
+        lds_exchange<typename P::I2, typename P::I3>(v, s, tid);
+        inv_pass_lazy<typename P::I3>(v, tid, itw, q, mu, hh, bnd);
+        if (!scale) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v[i] = reduce_big(v[i], q, mu);
+        }
+    } else {
+        P::I0::run(v, tid, itw, q, hh);
+        lds_exchange<typename P::I0, typename P::I1>(v, s, tid);
+        P::I1::run(v, tid, itw, q, hh);
+        lds_exchange<typename P::I1, typename P::I2>(v, s, tid);
+        P::I2::run(v, tid, itw, q, hh);
+        lds_exchange<typename P::I2, typename P::I3>(v, s, tid);
+        P::I3::run(v, tid, itw, q, hh);
+    }
     if (scale) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) v[i] = shoup(v[i], ninv, ninvp, q);

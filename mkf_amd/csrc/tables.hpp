@@ -53,6 +53,7 @@ struct DevTables {
                               // order, a tuning knob separate from xcd
     int ext_cs;               // 1: external-product MACs summed in carry-save form (Acc3)
                               // where the sums fit, a tuning knob
+    int inv_lazy;             // 1: unreduced inverse transforms for 2^32 <= q < 2^57 (ntt_inv)
 };
 
 // N = 2^14 external products over half-limb workgroups: a plan that emits KEXT

@@ -130,6 +130,38 @@ def test_external_product_carry_save_agrees(setup):
     assert np.array_equal(got[1][1][0], exp)
 
 
+def test_unreduced_inverse_transforms_agree(setup):
+    """Inverse transforms with unreduced Gentleman-Sande sums (the default for
+    2^32 <= q < 2^57) give the words of the per-stage reduced form and of the
+    oracle, on rows of q - 1 (the largest sums) and uniform rows of every
+    modulus, and inside mul+relin (tensor product, external products)."""
+    name, ctx, ref = setup
+    mk = _mk()
+    g = O.Prng(18)
+    QP = ref.QP
+    rows = list(range(len(QP)))
+    top = np.stack([np.full(ref.N, q - 1, dtype=np.uint64) for q in QP])
+    rnd = g.uniform(QP, ref.N)
+    keys, u = _keys(ctx, ref, [0, 1], seed=19)
+    op0 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
+    op1 = np.stack([g.uniform(ref.Q, ref.N) for _ in range(3)])
+    scale = float(ref.Q[-1])
+    got = {}
+    try:
+        for lazy in (1, 0):
+            ctx.set_tuning(mk.TUNE_INV_LAZY, lazy)
+            got[lazy] = ([ctx.ntt(x, rows, inverse=True) for x in (top, rnd)],
+                         ctx.kklss_ckks_mulrelin(op0, [0, 1], op1, [0, 1], scale, scale, scale))
+    finally:
+        ctx.set_tuning(mk.TUNE_INV_LAZY, 1)
+    for x, a, b in zip((top, rnd), got[1][0], got[0][0]):
+        assert np.array_equal(a, b)
+        assert np.array_equal(a, np.stack([ref.ntt(x[i], i, inverse=True) for i in rows]))
+    assert np.array_equal(got[1][1][0], got[0][1][0])
+    exp, _ = _oracle_mulrelin(ref, op0, [0, 1], op1, [0, 1], keys, u, (ref.L - 1) - got[1][1][2])
+    assert np.array_equal(got[1][1][0], exp)
+
+
 def _keys(ctx, ref, parties, seed):
     g = O.Prng(seed)
     beta = ctx.beta
