@@ -140,22 +140,25 @@ def stage_alg_bytes(B, k, L, K, beta, N, W=8):
 
 
 def stage_valu_work(B, k, L, K, beta, N):
-    """Modular-arithmetic work per launch of the transform-bearing KKLSS stages:
-    (forward butterflies on Q rows -- the unreduced form --, butterflies of the
-    reduced form -- forward on P rows, every inverse butterfly --, Montgomery
-    products of the MACs).  A limb transform is N/2 log2(N) butterflies."""
+    """Modular-arithmetic work per launch of the transform-bearing KKLSS stages, by
+    the form the kernels use at the headline shape (Q primes below 2^57, P primes
+    of 60 bits): (forward butterflies on Q rows -- unreduced --, butterflies of
+    the reduced form -- forward and inverse on P rows --, inverse butterflies on
+    Q rows -- unreduced --, Montgomery products -- the fused x / y MAC, the
+    tensor products --, products summed in carry-save form -- the external
+    products).  A limb transform is N/2 log2(N) butterflies."""
     rows = L + K
     nb = N // 2 * (N.bit_length() - 1)
     dig = 2 * k * beta
     return {
-        "decompose_mac": (dig * L * nb * B, dig * K * nb * B, dig * rows * N * B),
-        "decompose": (dig * L * nb * B, dig * K * nb * B, 0),
-        "mac_xy": (0, 0, dig * rows * N * B),
-        "tensor_ntt": (2 * L * nb * B, 0, 0),
-        "tensor": (0, (1 + k) * L * nb * B, (1 + 2 * k) * L * N * B),
-        "ext_xy": (0, 2 * k * rows * nb * B, dig * rows * N * B),
-        "decompose_t": (k * beta * L * nb * B, k * beta * K * nb * B, 0),
-        "ext_vu": (0, 2 * k * rows * nb * B, dig * rows * N * B),
+        "decompose_mac": (dig * L * nb * B, dig * K * nb * B, 0, dig * rows * N * B, 0),
+        "decompose": (dig * L * nb * B, dig * K * nb * B, 0, 0, 0),
+        "mac_xy": (0, 0, 0, dig * rows * N * B, 0),
+        "tensor_ntt": (2 * L * nb * B, 0, 0, 0, 0),
+        "tensor": (0, 0, (1 + k) * L * nb * B, (1 + 2 * k) * L * N * B, 0),
+        "ext_xy": (0, 2 * k * K * nb * B, 2 * k * L * nb * B, 0, dig * rows * N * B),
+        "decompose_t": (k * beta * L * nb * B, k * beta * K * nb * B, 0, 0, 0),
+        "ext_vu": (0, 2 * k * K * nb * B, 2 * k * L * nb * B, 0, dig * rows * N * B),
     }
 
 
@@ -173,7 +176,8 @@ def measure_peaks(iters=5):
                "copy_stream_gbs": round(stream_copy, 1)}
         for name, kind in (("mul_lo_u32_per_s", mkf_amd.PROBE_MUL32), ("mad_u64_u32_per_s", mkf_amd.PROBE_MAD64),
                            ("bfly_lazy_per_s", mkf_amd.PROBE_BFLY_LAZY), ("bfly_reduced_per_s", mkf_amd.PROBE_BFLY_RED),
-                           ("mred_add_per_s", mkf_amd.PROBE_MRED)):
+                           ("mred_add_per_s", mkf_amd.PROBE_MRED), ("acc3_mac_per_s", mkf_amd.PROBE_ACC3),
+                           ("ibfly_lazy_per_s", mkf_amd.PROBE_IBFLY_LAZY)):
             out[name] = float("%.4g" % ctx.probe_peak(kind, 4096, iters))
     finally:
         ctx.close()
@@ -1067,19 +1071,24 @@ def main():
             roofline["measured_copy_gbs"] = peaks["copy_gbs"]
             roofline["frac_of_measured_copy"] = round(achieved / peaks["copy_gbs"], 4)
 
+            rates = ("bfly_lazy_per_s", "bfly_reduced_per_s", "ibfly_lazy_per_s", "mred_add_per_s",
+                     "acc3_mac_per_s")
+
             def valu_frac(stage, ms):
-                lz, rd, mr = work[stage]
-                t_min = lz / peaks["bfly_lazy_per_s"] + rd / peaks["bfly_reduced_per_s"] + mr / peaks["mred_add_per_s"]
+                t_min = sum(n_ / peaks[r_] for n_, r_ in zip(work[stage], rates))
                 return t_min / (ms * 1e-3)
 
-            # the integer-VALU roofline: the stage's butterflies and Montgomery products
-            # at the measured in-register rates of the same primitives, over its time
+            # the integer-VALU roofline: the stage's butterflies and products at the
+            # measured in-register rates of the same primitives, over its time
             if names[dom] in work:
-                lz, rd, mr = work[names[dom]]
+                lz, rd, il, mr, cs = work[names[dom]]
                 roofline["valu"] = {
                     "bound": "int32-valu", "butterflies_unreduced": lz, "butterflies_reduced": rd,
-                    "mred_products": mr, "peak_bfly_unreduced_per_s": peaks["bfly_lazy_per_s"],
-                    "peak_bfly_reduced_per_s": peaks["bfly_reduced_per_s"], "peak_mred_per_s": peaks["mred_add_per_s"],
+                    "inv_butterflies_unreduced": il, "mred_products": mr, "acc3_products": cs,
+                    "peak_bfly_unreduced_per_s": peaks["bfly_lazy_per_s"],
+                    "peak_bfly_reduced_per_s": peaks["bfly_reduced_per_s"],
+                    "peak_ibfly_unreduced_per_s": peaks["ibfly_lazy_per_s"], "peak_mred_per_s": peaks["mred_add_per_s"],
+                    "peak_acc3_per_s": peaks["acc3_mac_per_s"],
                     "frac": round(valu_frac(names[dom], stage_ms[dom]), 4)}
             roofline["stages_valu_frac"] = {n_: round(valu_frac(n_, stage_ms[i_]), 4)
                                             for i_, n_ in enumerate(names) if n_ in work and stage_ms[i_] > 0}

@@ -123,7 +123,9 @@ int mkhip_sync(mkhip_ctx *ctx);
  * copy (several non-temporal 16-byte loads in flight a lane); the others count operations per second of work held
  * in registers, `size` steps per lane (0: 4096) -- v_mul_lo_u32, v_mad_u64_u32,
  * forward NTT butterflies (unreduced form of moduli below 2^57 / reduced form of
- * moduli up to 2^61) and Montgomery products with a modular add.
+ * moduli up to 2^61), Montgomery products with a modular add, products summed in
+ * carry-save form (MKHIP_PROBE_ACC3, the external products' MAC) and unreduced
+ * inverse butterflies (MKHIP_PROBE_IBFLY_LAZY).
  */
 #define MKHIP_PROBE_COPY 0
 #define MKHIP_PROBE_MUL32 1
@@ -132,6 +134,8 @@ int mkhip_sync(mkhip_ctx *ctx);
 #define MKHIP_PROBE_BFLY_RED 4
 #define MKHIP_PROBE_MRED 5
 #define MKHIP_PROBE_COPY_STREAM 6
+#define MKHIP_PROBE_ACC3 7
+#define MKHIP_PROBE_IBFLY_LAZY 8
 int mkhip_probe_peak(mkhip_ctx *ctx, int kind, size_t size, int iters, double *rate);
 
 /*

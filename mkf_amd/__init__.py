@@ -58,8 +58,8 @@ EXPORTS = [
 BASIS_STANDARD, BASIS_CHEBYSHEV = 0, 1
 TUNE_FUSED_DECOMP, TUNE_XCD_REMAP, TUNE_DECOMP_ORDER, TUNE_FP64, TUNE_SPLIT_LIMB, TUNE_SPLIT_EXT, TUNE_DIGIT_PRE, \
     TUNE_XCD_EXT, TUNE_EXT_CS, TUNE_INV_LAZY = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
-PROBE_COPY, PROBE_MUL32, PROBE_MAD64, PROBE_BFLY_LAZY, PROBE_BFLY_RED, PROBE_MRED, PROBE_COPY_STREAM = \
-    0, 1, 2, 3, 4, 5, 6
+PROBE_COPY, PROBE_MUL32, PROBE_MAD64, PROBE_BFLY_LAZY, PROBE_BFLY_RED, PROBE_MRED, PROBE_COPY_STREAM, PROBE_ACC3, \
+    PROBE_IBFLY_LAZY = 0, 1, 2, 3, 4, 5, 6, 7, 8
 SCHEME_IDS = {"kklss": 0, "bfv": 1, "mkf": 2, "mkf_bfv": 3, "rotate": 4, "mkf_rotate": 5}
 MAX_ROTATIONS = 16
 CONJUGATE = -2

@@ -98,6 +98,63 @@ __global__ void __launch_bounds__(kProbeThreads) k_probe_bfly(u64 w, u64 wp, u64
     if (s == 0x9e3779b97f4a7c15ull) sink[blockIdx.x] = s;
 }
 
+// Unreduced inverse butterflies (inv_bfly_lazy) on 16 values a lane, the values
+// brought below q every twelfth step: the bound 4q 2^11 of the last step stays
+// below 2^63 for the 50-bit probe modulus, as in the transforms, which reduce
+// once in their 13 or 14 stages
+__global__ void __launch_bounds__(kProbeThreads) k_probe_ibfly(u64 w, u64 wp, u64 q, u64 mu, int iters,
+                                                               u64 *sink) {
+    u64 v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = (threadIdx.x * 16 + j) % q;
+    u64 bnd = q << 2;
+    for (int it = 0; it < iters; ++it) {
+        if (it % 12 == 0) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = reduce_big(v[j], q, mu);
+            bnd = q << 2;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) inv_bfly_lazy(v[j], v[j + 8], w, wp, bnd, q);
+        bnd <<= 1;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    u64 s = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) s ^= v[j];
+    if (s == 0x9e3779b97f4a7c15ull) sink[blockIdx.x] = s;
+}
+
+// Products of a key word and a digit word summed in carry-save form (acc3_mac),
+// eight sums a lane, folded and reduced every eighth step as the external
+// products do for beta = 8
+__global__ void __launch_bounds__(kProbeThreads) k_probe_acc3(u64 b, u64 q, u64 qinv, int iters, u64 *sink) {
+    u64 a[8];
+    Acc3 acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        a[j] = (threadIdx.x * 8 + j + 1) % q;
+        acc3_zero(acc[j]);
+    }
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc3_mac(acc[j], a[j], b + it);
+        if ((it & 7) == 7) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                u64 hi, lo;
+                acc3_fold(acc[j], hi, lo);
+                a[j] = redc128(hi, lo, q, qinv);
+                acc3_zero(acc[j]);
+            }
+        }
+    }
+    u64 s = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s ^= a[j] ^ acc[j].L ^ acc[j].M ^ acc[j].H;
+    if (s == 0x9e3779b97f4a7c15ull) sink[blockIdx.x] = s;
+}
+
 __global__ void __launch_bounds__(kProbeThreads) k_probe_mred(u64 b, u64 q, u64 qinv, int iters, u64 *sink) {
     u64 acc[8], a[8];
 #pragma unroll
@@ -177,6 +234,16 @@ extern "C" int mkhip_probe_peak(mkhip_ctx *c, int kind, size_t size, int iters, 
             hipLaunchKernelGGL(k_probe_mred, dim3(grid), dim3(kProbeThreads), 0, st, h_mform(12345, q50), q50,
                                h_qinv(q50), inner, (u64 *)a);
             break;
+        case MKHIP_PROBE_ACC3:
+            hipLaunchKernelGGL(k_probe_acc3, dim3(grid), dim3(kProbeThreads), 0, st, h_mform(12345, q50), q50,
+                               h_qinv(q50), inner, (u64 *)a);
+            break;
+        case MKHIP_PROBE_IBFLY_LAZY: {
+            const u64 w = h_powmod(3, 12345, q50);
+            hipLaunchKernelGGL(k_probe_ibfly, dim3(grid), dim3(kProbeThreads), 0, st, w, h_shoup_companion(w, q50),
+                               q50, (u64)((((unsigned __int128)1) << 64) / q50), inner, (u64 *)a);
+            break;
+        }
         default:
             return hipErrorInvalidValue;
         }
@@ -189,10 +256,12 @@ extern "C" int mkhip_probe_peak(mkhip_ctx *c, int kind, size_t size, int iters, 
     case MKHIP_PROBE_MUL32:
     case MKHIP_PROBE_MAD64:
     case MKHIP_PROBE_MRED:
+    case MKHIP_PROBE_ACC3:
         work = lanes * inner * 8;
         break;
     case MKHIP_PROBE_BFLY_LAZY:
     case MKHIP_PROBE_BFLY_RED:
+    case MKHIP_PROBE_IBFLY_LAZY:
         work = lanes * inner * 8;
         break;
     default:

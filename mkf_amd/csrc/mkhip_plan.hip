@@ -367,6 +367,16 @@ extern "C" int mkhip_autotune(mkhip_ctx *c, int scheme, int nops, const void *op
             cands.push_back(flip);
         }
         double t = 0;
+        if (ci >= nfirst) {
+            // a flip is compared with the best re-timed just before it, so that a
+            // lucky first timing of the best (or a drift of the clocks since)
+            // does not decide a choice between forms a few % apart
+            const int s = time_cand(cands[best], &best_ms);
+            if (s != MKHIP_OK) {
+                apply({old_fused, old_order, old_xcd, old_split, old_split_ext, old_pre, old_xe});
+                return s;
+            }
+        }
         const int s = time_cand(cands[ci], &t);
         if (s != MKHIP_OK) {
             apply({old_fused, old_order, old_xcd, old_split, old_split_ext, old_pre, old_xe});
