@@ -811,11 +811,11 @@ def _serve_run(classes, args, rng, rate, duration, streams, nparty, kmin, kmax):
     return res
 
 
-def _serve_sweep(args, rng, specs, nparty, kmin, kmax, loads, burst):
+def _serve_sweep(args, rng, specs, nparty, kmin, kmax, loads, burst, streams):
     t_setup = time.perf_counter()
     classes = _serve_setup(args, rng, specs, nparty, kmax)
     setup_s = time.perf_counter() - t_setup
-    run = lambda rate, dur: _serve_run(classes, args, rng, rate, dur, args.serve_streams, nparty, kmin, kmax)
+    run = lambda rate, dur: _serve_run(classes, args, rng, rate, dur, streams, nparty, kmin, kmax)
     # capacity: a saturating burst (every request at once), after a warm-up burst
     run(1e6, min(burst, 64) / 1e6)
     sat = run(1e6, burst / 1e6)
@@ -825,7 +825,7 @@ def _serve_sweep(args, rng, specs, nparty, kmin, kmax, loads, burst):
         sweep[-1]["load"] = f
     res = {"classes": [{"scheme": c["scheme"] + ("_ckks" if c["scheme"] == "kklss" else ""), "logN": c["logN"],
                         "L": c["L"], "K": c["K"]} for c in classes],
-           "parties": nparty, "k": [kmin, kmax], "streams": args.serve_streams, "max_batch": args.serve_max_batch,
+           "parties": nparty, "k": [kmin, kmax], "streams": streams, "max_batch": args.serve_max_batch,
            "mem_gb": args.serve_mem_gb, "saturated": sat, "sweep": sweep, "setup_s": round(setup_s, 1)}
     for c in classes:
         for o in c["outs"]:
@@ -840,14 +840,14 @@ def run_serving(args, rng):
     k uniform in 2..16, N in {2^14, 2^15}, operands from registered parties whose
     keys stay resident; continuous batching, admission by device memory."""
     return _serve_sweep(args, rng, SERVE_CLASSES, args.serve_parties, 2, args.serve_kmax, args.serve_load,
-                        args.serve_burst)
+                        args.serve_burst, args.serve_streams)
 
 
 def run_serving_headline(args, rng):
     """The serving loop at the headline shape (KKLSS MK-CKKS, N=2^14, L=8, K=2,
     k=4): Poisson arrivals at fractions of the saturated rate."""
     return _serve_sweep(args, rng, [("kklss", 14, 8, 2)], 4, 4, 4, args.serve_headline_load,
-                        args.serve_headline_burst)
+                        args.serve_headline_burst, args.serve_headline_streams)
 
 
 def _cpu_threads(requested):
@@ -988,7 +988,11 @@ def main():
                     help="requests of the saturating burst at the headline shape")
     ap.add_argument("--serve-parties", type=int, default=32)
     ap.add_argument("--serve-kmax", type=int, default=16)
-    ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--serve-streams", type=int, default=4,
+                    help="server streams (batches in flight) of the mixed workload")
+    ap.add_argument("--serve-headline-streams", type=int, default=2,
+                    help="server streams at the headline shape (2 measured the best p99 there; the mixed "
+                         "workload's large ops saturate higher with 4)")
     ap.add_argument("--serve-max-batch", type=int, default=32)
     ap.add_argument("--serve-mem-gb", type=float, default=64.0)
     ap.add_argument("--rot-batch", type=int, default=16, help="ops per launch of the rotation legs (0: skip)")
