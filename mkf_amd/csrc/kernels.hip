@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(256)
         u64 hi = 0, lo = 0;
         for (int p = 0; p < Md.k; ++p) mac128(hi, lo, G(Md.key[p])[koff], G(Md.D)[p * dstride + doff]);
         // MForm(sum_p MRed(key_p, D_p)); full-level key layout, like the keys
-        G(Md.xy)[koff] = mred(redc_sum(hi, lo, m), m.r2, m.q, m.qinv);
+        G(Md.xy)[koff] = shoup(redc_sum(hi, lo, m), m.r1, m.r1p, m.q);
     }
 }
 

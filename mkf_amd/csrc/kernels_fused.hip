@@ -63,10 +63,11 @@ __device__ __forceinline__ void decomp_mac_row(const DecompDesc &Dd, int i, int 
             }
         }
     }
-    // MForm of the sum; an unreduced sum below 64 q keeps acc r2 < q 2^64, so the
-    // result is canonical either way
+    // MForm of the sum (below 64 q): a Shoup multiplication by 2^64 mod q, exact
+    // for any operand below 2^64 and a third the instructions of a Montgomery
+    // product by 2^128 mod q
 #pragma unroll
-    for (int e = 0; e < 16; ++e) acc[e] = mred(acc[e], m.r2, m.q, m.qinv);
+    for (int e = 0; e < 16; ++e) acc[e] = shoup(acc[e], m.r1, m.r1p, m.q);
     store_row_quad<Gm::LS>(acc, Dd.xy + krow, tid);
 }
 

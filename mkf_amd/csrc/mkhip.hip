@@ -190,6 +190,8 @@ extern "C" int mkhip_ctx_create(mkhip_ctx **out, int logN, const uint64_t *q, in
         m.qinv = h_qinv(qi);
         m.mu = (u64)((((u128)1) << 64) / qi);
         m.r2 = (u64)((((u128)1 << 64) % qi) * (u128)(((u128)1 << 64) % qi) % qi);
+        m.r1 = (u64)(((u128)1 << 64) % qi);
+        m.r1p = h_shoup_companion(m.r1, qi);
         m.ninv = h_inv((u64)N, qi);
         m.ninvp = h_shoup_companion(m.ninv, qi);
         m.tninv = h_mulmod(t % qi, m.ninv, qi);

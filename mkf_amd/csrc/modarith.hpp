@@ -211,6 +211,7 @@ struct ModConst {
     u64 qinv;   // q^-1 mod 2^64 (Montgomery)
     u64 mu;     // floor(2^64 / q) (Barrett)
     u64 r2;     // 2^128 mod q: mred(x, r2) = x * 2^64 mod q (MForm)
+    u64 r1, r1p;  // 2^64 mod q and its Shoup companion: shoup(x, r1, r1p) = MForm(x) for any x < 2^64
     u64 ninv;   // N^-1 mod q (canonical) for this modulus at the context's N
     u64 ninvp;  // Shoup companion of ninv
     u64 tninv;  // t * N^-1 mod q (BFV: Quantize's MulScalar(t) folded into the INTT)
