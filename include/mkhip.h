@@ -575,6 +575,11 @@ void mkhip_batch_destroy(mkhip_batch *batch);
  * (L - nres) x N for KKLSS, ku = |ids0 u ids1| in ascending id order; (1 + k) x
  * (L - nres) x N for MKF).  in0, in1 and out must stay valid until the request
  * completed; pinned buffers (mkhip_host_alloc) make the copies asynchronous.
+ * With on_device = 1, in0, in1 and out are device pointers instead (ciphertexts
+ * that stay in HBM between operations, such as the results of earlier ones): the
+ * batch reads and writes them in place, with no copies, and takes pool memory
+ * for workspace and descriptors only; latency then ends when the result is in
+ * device memory.
  * Contexts attached to a server must not be used by other threads while it runs.
  */
 typedef struct mkhip_server mkhip_server;
@@ -591,6 +596,7 @@ typedef struct {
     int k0, k1;                                      /* KKLSS party ids per operand */
     int group;                                       /* MKF party group             */
     int nres;                                        /* CKKS rescales; 0 for BFV    */
+    int on_device;                                   /* 1: in0, in1, out in HBM     */
 } mkhip_request;
 typedef struct {
     uint64_t completed, batches, max_batch;

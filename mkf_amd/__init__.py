@@ -105,6 +105,7 @@ class Request(ctypes.Structure):
         ("cls", ctypes.c_int), ("in0", ctypes.c_void_p), ("in1", ctypes.c_void_p), ("out", ctypes.c_void_p),
         ("ids0", ctypes.c_int * MAX_PARTIES), ("ids1", ctypes.c_int * MAX_PARTIES),
         ("k0", ctypes.c_int), ("k1", ctypes.c_int), ("group", ctypes.c_int), ("nres", ctypes.c_int),
+        ("on_device", ctypes.c_int),
     ]
 
 
@@ -1023,8 +1024,16 @@ class Server:
         for a in (in0, in1, out):
             if a.dtype != np.uint64 or not a.flags["C_CONTIGUOUS"]:
                 raise TypeError("expected C-contiguous uint64 arrays")
+        return self._enqueue(cls, in0.ctypes.data, in1.ctypes.data, out.ctypes.data, ids0, ids1, group, nres, 0)
+
+    def enqueue_device(self, cls, in0, in1, out, ids0=(), ids1=(), group=0, nres=0):
+        """A request whose operands and result are device buffers (ctx.alloc
+        pointers): served in place, with no host copies."""
+        return self._enqueue(cls, int(in0), int(in1), int(out), ids0, ids1, group, nres, 1)
+
+    def _enqueue(self, cls, in0, in1, out, ids0, ids1, group, nres, on_device):
         r = Request()
-        r.cls, r.in0, r.in1, r.out = cls, in0.ctypes.data, in1.ctypes.data, out.ctypes.data
+        r.cls, r.in0, r.in1, r.out, r.on_device = cls, in0, in1, out, on_device
         for j, v in enumerate(ids0):
             r.ids0[j] = v
         for j, v in enumerate(ids1):

@@ -91,6 +91,9 @@ struct Req {
     Clock::time_point t_in, t_start, t_done;
     int status = 1;  // 1 pending, 0 ok, < 0 error
     size_t in0_w = 0, in1_w = 0, out_w = 0;
+    // words of the request's operands and result the batch block holds (0 for
+    // device-resident requests, served in place)
+    size_t io_w() const { return r.on_device ? 0 : in0_w + in1_w + out_w; }
 };
 
 struct Class {
@@ -154,15 +157,30 @@ std::vector<char> make_ops(const Class &C, const std::vector<Req *> &reqs, size_
         p = p ? p + w : nullptr;
         return r ? r : (u64 *)(uintptr_t)256;  // placeholder for size queries
     };
+    // a host request's operands and result in the batch block, a device request's
+    // where they are
+    auto place = [&](const Req *q, u64 **op0, u64 **op1, u64 **out) {
+        if (q->r.on_device) {
+            *op0 = (u64 *)q->r.in0;
+            *op1 = (u64 *)q->r.in1;
+            *out = q->r.out;
+        } else {
+            *op0 = adv(q->in0_w);
+            *op1 = adv(q->in1_w);
+            *out = adv(q->out_w);
+        }
+    };
     if (is_kklss(C.scheme)) {
         v.resize(n * sizeof(mkhip_kklss_op));
         mkhip_kklss_op *ops = (mkhip_kklss_op *)v.data();
         for (size_t i = 0; i < n; ++i) {
             const Req *q = reqs[i];
             mkhip_kklss_op o = {};
-            o.op0 = adv(q->in0_w);
-            o.op1 = adv(q->in1_w);
-            o.out = adv(q->out_w);
+            u64 *o0, *o1, *oo;
+            place(q, &o0, &o1, &oo);
+            o.op0 = o0;
+            o.op1 = o1;
+            o.out = oo;
             for (int j = 0; j < q->r.k0; ++j) o.ids0[j] = q->r.ids0[j];
             for (int j = 0; j < q->r.k1; ++j) o.ids1[j] = q->r.ids1[j];
             o.k0 = q->r.k0;
@@ -177,9 +195,11 @@ std::vector<char> make_ops(const Class &C, const std::vector<Req *> &reqs, size_
         for (size_t i = 0; i < n; ++i) {
             const Req *q = reqs[i];
             mkhip_mkf_op o = {};
-            o.left = adv(q->in0_w);
-            o.right = adv(q->in1_w);
-            o.out = adv(q->out_w);
+            u64 *o0, *o1, *oo;
+            place(q, &o0, &o1, &oo);
+            o.left = o0;
+            o.right = o1;
+            o.out = oo;
             o.group = q->r.group;
             o.level = c->L - 1;
             o.nres = q->r.nres;
@@ -192,7 +212,7 @@ std::vector<char> make_ops(const Class &C, const std::vector<Req *> &reqs, size_
 // Device bytes of a batch over reqs[0..n): operands + outputs + workspace + descriptors.
 int batch_bytes(const Class &C, const std::vector<Req *> &reqs, size_t n, size_t *io, size_t *ws, size_t *desc) {
     size_t w = 0;
-    for (size_t i = 0; i < n; ++i) w += reqs[i]->in0_w + reqs[i]->in1_w + reqs[i]->out_w;
+    for (size_t i = 0; i < n; ++i) w += reqs[i]->io_w();
     *io = w * 8;
     std::vector<char> ops = make_ops(C, reqs, n, nullptr);
     PlanMem q;
@@ -292,9 +312,10 @@ bool launch_batch(mkhip_server *S, Class &C, Slot &sl) {
         const size_t cap = std::max<size_t>(desc, 1 << 20);
         if (hipHostMalloc(&sl.desc_host, cap, hipHostMallocDefault) == hipSuccess) sl.desc_host_bytes = cap;
     }
-    // operands in
+    // operands in (host requests; device requests are read in place)
     u64 *p = d_io;
     for (Req *q : sl.reqs) {
+        if (q->r.on_device) continue;
         if (st == MKHIP_OK)
             st = hip_status(hipMemcpyAsync(p, q->r.in0, q->in0_w * 8, hipMemcpyHostToDevice, sl.st));
         p += q->in0_w;
@@ -315,6 +336,7 @@ bool launch_batch(mkhip_server *S, Class &C, Slot &sl) {
     // results out
     p = d_io;
     for (Req *q : sl.reqs) {
+        if (q->r.on_device) continue;
         p += q->in0_w + q->in1_w;
         if (st == MKHIP_OK) st = hip_status(hipMemcpyAsync(q->r.out, p, q->out_w * 8, hipMemcpyDeviceToHost, sl.st));
         p += q->out_w;

@@ -158,3 +158,38 @@ def test_admission_by_device_memory(classes):
     status, _, _ = s.wait(_submit(s, r))
     assert status == -7  # MKHIP_ERR_OOM
     s.close()
+
+
+def test_device_resident_requests_in_a_mix(classes):
+    """Requests whose operands and result stay in HBM (served in place, no host
+    copies) give the host requests' words, batched with host requests of both
+    schemes; two are checked against the oracle."""
+    s, cl = _server(classes, streams=2, max_batch=8, mem_bytes=2 << 30)
+    g = O.Prng(95)
+    rng = np.random.default_rng(95)
+    probes = [_request("ckks", cl["ckks"], classes["ckks"][1], g, [0, 1, 3], [1, 2]),
+              _request("bfv", cl["bfv"], classes["bfv"][1], g, [2, 4], [0, 4, 5])]
+    dev = []
+    for r in probes:
+        ctx = classes[r["kind"]][0]
+        d = [ctx.alloc(a.nbytes) for a in (r["in0"], r["in1"], r["out"])]
+        ctx.h2d(d[0], r["in0"])
+        ctx.h2d(d[1], r["in1"])
+        dev.append((ctx, d))
+    mix = _mix(cl, classes, g, rng, 10)
+    tickets = [_submit(s, r) for r in mix[:5]]
+    for r, (ctx, d) in zip(probes, dev):
+        tickets.append(s.enqueue_device(r["cls"], d[0], d[1], d[2], ids0=r["ids0"], ids1=r["ids1"],
+                                        nres=r["nres"]))
+    tickets += [_submit(s, r) for r in mix[5:]]
+    for t in tickets:
+        assert s.wait(t)[0] == 0
+    for r, (ctx, d) in zip(probes, dev):
+        got = ctx.d2h(np.zeros_like(r["out"]), d[2])
+        assert np.array_equal(got, _expected(classes, r)), r["kind"]
+        for p in d:
+            ctx.free(p)
+    for r in mix[:2]:
+        assert np.array_equal(r["out"], _expected(classes, r)), r["kind"]
+    assert s.stats()["max_batch"] >= 2
+    s.close()
