@@ -30,6 +30,17 @@ import time
 
 import numpy as np
 
+# Each server stream needs a hardware queue of its own: HIP's default of 4 queues
+# per process makes the serving legs' streams share queues, and a batch then waits
+# behind another stream's batch (p99 of 30-120 ms at loads where 16 queues give
+# 5-8 ms).  Raised before the HIP runtime starts; a larger setting is kept.
+try:
+    _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+except ValueError:
+    _hwq = 4
+if _hwq < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -841,7 +852,8 @@ def _serve_sweep(args, rng, specs, nparty, kmin, kmax, loads, burst, streams, on
     res = {"classes": [{"scheme": c["scheme"] + ("_ckks" if c["scheme"] == "kklss" else ""), "logN": c["logN"],
                         "L": c["L"], "K": c["K"]} for c in classes],
            "parties": nparty, "k": [kmin, kmax], "streams": streams, "max_batch": args.serve_max_batch,
-           "mem_gb": args.serve_mem_gb, "saturated": sat, "sweep": sweep, "setup_s": round(setup_s, 1),
+           "mem_gb": args.serve_mem_gb, "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+           "saturated": sat, "sweep": sweep, "setup_s": round(setup_s, 1),
            "operands": "device" if on_device else "host"}
     for c in classes:
         for o in c["outs"]:
@@ -867,7 +879,8 @@ def run_serving_headline(args, rng, on_device=False):
     k=4): Poisson arrivals at fractions of the saturated rate; on_device: the
     requests' operands and results stay in HBM."""
     return _serve_sweep(args, rng, [("kklss", 14, 8, 2)], 4, 4, 4, args.serve_headline_load,
-                        args.serve_headline_burst, args.serve_headline_streams, on_device)
+                        args.serve_headline_burst,
+                        args.serve_device_streams if on_device else args.serve_headline_streams, on_device)
 
 
 def _cpu_threads(requested):
@@ -1010,9 +1023,11 @@ def main():
     ap.add_argument("--serve-kmax", type=int, default=16)
     ap.add_argument("--serve-streams", type=int, default=4,
                     help="server streams (batches in flight) of the mixed workload")
-    ap.add_argument("--serve-headline-streams", type=int, default=2,
-                    help="server streams at the headline shape (2 measured the best p99 there; the mixed "
-                         "workload's large ops saturate higher with 4)")
+    ap.add_argument("--serve-headline-streams", type=int, default=8,
+                    help="server streams at the headline shape, host operands (8 measured the most ops/s "
+                         "under p99 10 ms with 16 hardware queues)")
+    ap.add_argument("--serve-device-streams", type=int, default=4,
+                    help="server streams at the headline shape, device-resident operands")
     ap.add_argument("--serve-max-batch", type=int, default=32)
     ap.add_argument("--serve-mem-gb", type=float, default=64.0)
     ap.add_argument("--rot-batch", type=int, default=16, help="ops per launch of the rotation legs (0: skip)")
