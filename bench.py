@@ -348,7 +348,7 @@ def run_scheme(scheme, args, rng):
             res["plain_host"] = plain_host_rate(w, rng, args.plain_ops)
     if scheme == "kklss_ckks" and args.fp64_leg:
         # the same batch with the opt-in FP64 transforms for moduli below 2^50 (the
-        # same words; the north-star arithmetic is the integer form above)
+        # same words; the reported value is the integer form above)
         import mkf_amd
 
         w.ctx.set_tuning(mkf_amd.TUNE_FP64, 1)
