@@ -1,5 +1,6 @@
 """Host-side logic of bench.py (no GPU): the algorithmic-byte accounting of the
-headline plan, the reference's stage names and the mkfbench CSV rows."""
+headline plan, the reference's stage names, the mkfbench CSV rows and the
+hardware-queue setting of the serving legs."""
 import os
 import sys
 
@@ -40,3 +41,20 @@ def test_mkfbench_rows_follow_the_reference_header():
     parts = sum(r[c] for c in ("tensor_ns", "decomposition_ns", "external_product_ns", "modup_moddown_rescale_ns",
                                "memory_copy_ns", "other_ns"))
     assert abs(parts - r["total_ns"]) <= 6  # per-op rounding of each column
+
+
+def test_hardware_queues_raised_for_the_server_streams():
+    """bench.py gives each server stream a hardware queue of its own: a setting
+    below 16 is raised to 16 before the HIP runtime starts, a larger one kept."""
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = "import os, bench; print(os.environ['GPU_MAX_HW_QUEUES'])"
+    for given, expect in (("4", "16"), ("32", "32"), (None, "16")):
+        env = dict(os.environ)
+        env.pop("GPU_MAX_HW_QUEUES", None)
+        if given is not None:
+            env["GPU_MAX_HW_QUEUES"] = given
+        out = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True,
+                             check=True)
+        assert out.stdout.strip() == expect
