@@ -1006,8 +1006,9 @@ def main():
     ap.add_argument("--uneven-ops", type=int, default=256,
                     help="ops of the uneven batch (k in {2,4,8,16}, N in {2^13,2^14,2^15}); 0 skips it")
     ap.add_argument("--uneven-steps", type=int, default=5)
-    ap.add_argument("--uneven-budget-gb", type=float, default=24.0,
-                    help="workspace budget of each ring degree's branch of the uneven batch")
+    ap.add_argument("--uneven-budget-gb", type=float, default=64.0,
+                    help="workspace budget of each ring degree's branch of the uneven batch (64: the "
+                         "N=2^15 branch in about 3 chunks, 92 GB in all)")
     ap.add_argument("--uneven-tune-ops", type=int, default=16,
                     help="ops of each branch the autotuner times")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
